@@ -1,0 +1,261 @@
+/*
+ * upic.h -- C ABI of the MI355X-native per-timestep PIC hot path.
+ *
+ * Drop-in boundary for the three teaching codes of sgong11/Unstaggered_PIC:
+ *   P1 = Project1/ec_pic_serial_demo.cpp       (non-relativistic EC, global Picard)
+ *   P2 = Project2/spectral_pic_part3.cpp       (explicit BDF spectral GM push)
+ *   P3 = Project3/ec_pic_nyquist_projected.cpp (relativistic GM-HC-CN, nested solve)
+ *
+ * The reference has no FFI layer: its call surface is a set of file-static C++
+ * functions called by each code's step driver.  Every entry point below names
+ * the reference function it replaces (file:line).  Conventions of this ABI:
+ *   - no exceptions cross it; every call returns a upic_status;
+ *   - particle arrays are structure-of-arrays, component-major:
+ *       x[0..np) = x, x[np..2np) = y, x[2np..3np) = z   ("3*np" arrays);
+ *   - mesh vector fields are component-major (3*N), mesh index (i*ny+j)*nz+k,
+ *     exactly the reference's Grid::index (P1:381-383, P3:1450-1452);
+ *   - "dev" pointers are device (HBM) pointers; entry points that take host
+ *     arrays say so.  All work is enqueued on the context's HIP stream;
+ *     results are complete after upic_ctx_sync() or any call that returns
+ *     host data.
+ *   - numbers are FP64 unless the name says otherwise.
+ */
+#ifndef UPIC_H
+#define UPIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  Mirror the reference error conventions:
+ * std::runtime_error for bad input (P3:1339-1433) -> UPIC_EINVAL,
+ * NonlinearSolveFailure (P3:199-202)           -> UPIC_ENONLINEAR,
+ * SelectiveParticleRefinementNeeded (P3:252-260) -> UPIC_EREFINE. */
+typedef enum upic_status {
+    UPIC_OK = 0,
+    UPIC_EINVAL = 1,
+    UPIC_ENONLINEAR = 2,
+    UPIC_EREFINE = 3,
+    UPIC_EDEVICE = 4,
+    UPIC_ENOMEM = 5
+} upic_status;
+
+/* Which reference code's conventions and step driver to use. */
+typedef enum upic_code {
+    UPIC_CODE_P1 = 1, /* grid origin -L/2, Nyquist symbol kept (P1:386-392)   */
+    UPIC_CODE_P2 = 2, /* grid origin 0, Nyquist symbol zeroed (P2:320-330)    */
+    UPIC_CODE_P3 = 3  /* grid origin -L/2, even Nyquist zeroed (P3:1485-1504) */
+} upic_code;
+
+/* Per-particle failure reasons of the P3 inner solve (P3:204-211). */
+typedef enum upic_particle_failure {
+    UPIC_FAIL_NONE = 0,
+    UPIC_FAIL_NONFINITE_MAP = 1,
+    UPIC_FAIL_SUPERLUMINAL_MAP = 2,
+    UPIC_FAIL_NONFINITE_ITERATE = 3,
+    UPIC_FAIL_SUPERLUMINAL_ITERATE = 4,
+    UPIC_FAIL_MAXIMUM_ITERATIONS = 5
+} upic_particle_failure;
+
+/* Particle/grid precision of a simulation handle. */
+typedef enum upic_precision {
+    UPIC_PREC_FP64 = 0,  /* everything FP64 (the reference's arithmetic)            */
+    UPIC_PREC_MIXED = 1  /* FP32 particle push / LDS deposit tiles, FP64 grids, FFT,
+                            diagnostics and convergence tests                      */
+} upic_precision;
+
+/* Numerical parameters of one run.  Field names follow the reference decks
+ * (P1 Config 233-303, P2 Config 174-231, P3 Config 837-990). */
+typedef struct upic_params {
+    int32_t code;                 /* upic_code                                   */
+    int32_t nx, ny, nz;           /* powers of two (P1:395-396, P3:1521-1523)     */
+    double Lx, Ly, Lz;
+    double dt;
+    int32_t spline_order;         /* 1..4 (P1, P3); P2 is fixed quadratic         */
+    int32_t orbit_quad_order;     /* Gauss points per orbit segment               */
+    int32_t split_orbit_at_knots; /* 1 = split orbit at B-spline knots            */
+    int32_t nyquist_projection;   /* P3 only (P3:885)                             */
+    double c;                     /* P1/P2 light speed; P3 kappa                  */
+    double eps0;                  /* P1/P2 eps0; P3: 1/sigma1 is NOT used -- see sigma1 */
+    double mu0;                   /* P1/P2 mu0                                    */
+    double sigma1, sigma2;        /* P3 source coefficients (P3:955-957)          */
+    /* P1 global Picard (P1:280-283) */
+    double picard_tol;
+    int32_t picard_max_iter;
+    int32_t final_consistency_sweep;
+    /* P3 outer current solve (P3:891-901) */
+    int32_t nonlinear_solver_picard; /* 0 = anderson_quasi_newton, 1 = picard     */
+    int32_t nonlinear_max_iter;
+    double nonlinear_rtol, nonlinear_atol;
+    int32_t anderson_memory, anderson_start;
+    double anderson_damping, anderson_regularization, anderson_growth_limit;
+    /* P3 inner per-particle solve (P3:917-920) */
+    double local_rtol, local_atol;
+    int32_t local_max_iter;
+    double local_damping;
+    /* P3 selective refinement (P3:927-931) */
+    int32_t selective_particle_fallback;
+    int32_t selective_particle_max_substeps;
+    /* P2 (P2:174-231) */
+    int32_t bdf_order;            /* 1 or 2                                       */
+    int32_t charge_conserving;    /* 1 = continuity rho, 0 = naive redeposit      */
+    /* GPU-only controls (no reference counterpart) */
+    int32_t precision;            /* upic_precision                               */
+    int32_t sort_interval;        /* cell-sort particles every k steps (0 = never) */
+    int32_t learned_guess;        /* P3: MLP initial guess for the inner solve    */
+    int32_t reserved[5];
+} upic_params;
+
+/* Orbit splitting statistics (PathStats, P1:771-776, P3:2509-2514). */
+typedef struct upic_path_stats {
+    int64_t n_particles;
+    int64_t crossing_particles;
+    int64_t total_segments;
+    int32_t max_segments;
+    int32_t pad;
+} upic_path_stats;
+
+/* Outcome of one time step (P1 PicardHistoryEntry 1503-1507, P3
+ * NonlinearSolveInfo 5493-5519). */
+typedef struct upic_step_info {
+    int32_t iterations;              /* P1 picard_iterations / P3 nonlinear_iterations */
+    int32_t converged;
+    double current_error;            /* P1 relRMS(J - J_prev) of the last iterate      */
+    double velocity_error;           /* P1 relRMS(v_next - v_guess)                    */
+    /* P3 outer solve */
+    int32_t hit_max_iter;
+    int32_t anderson_restarts;
+    int32_t anderson_steps;
+    int32_t picard_fallback_steps;
+    int32_t backtracked_steps;
+    int32_t map_evaluations;
+    double minimum_update_alpha;
+    double final_rms_residual;
+    double final_max_particle_residual;
+    double final_absolute_rms_residual;
+    double estimated_map_contraction;
+    /* P3 inner solve */
+    int32_t local_iterations_max;
+    int32_t failed_particles;
+    double local_iterations_mean;
+    double local_scaled_residual_max;
+    int64_t local_iterations_total;  /* summed over all map evaluations of the step   */
+    upic_path_stats path;
+} upic_step_info;
+
+/* Per-step diagnostics (P1 Diagnostics 1037-1055, P3 Diagnostics 3469-3499,
+ * P2 Diagnostics 882-896). */
+typedef struct upic_diagnostics {
+    double gauss_rms, gauss_max, gauge_rms, gauge_max;
+    double divE_rms, rho_rms, psi_over_c2_rms, divA_rms;
+    double field_energy, kinetic_energy, total_energy;
+    double rho_mode, E_mode_re, E_mode_im, E_mode_abs;
+    double Bz_y_mode_re, Bz_y_mode_im, Bz_y_mode_abs;
+    double mean_rho;
+    /* P2 */
+    double continuity_rms, Bz_l2, B_l2, E_l2, rho_l2, J_l2, max_speed, mean_gamma;
+} upic_diagnostics;
+
+/* ------------------------------------------------------------------------- */
+/* Context: device, stream, grid symbols, quadrature.                        */
+/* ------------------------------------------------------------------------- */
+typedef struct upic_ctx upic_ctx;
+
+/* Creates a context on the current HIP device (make_grid P1:394-418,
+ * P3:1520-1546; gauss_legendre_01 P1:739-763).  Validates the deck values the
+ * reference validates (P3:1339-1433). */
+upic_status upic_ctx_create(const upic_params* params, upic_ctx** out);
+upic_status upic_ctx_destroy(upic_ctx* ctx);
+upic_status upic_ctx_sync(upic_ctx* ctx);
+/* Last error message of this context (static storage, valid until next call). */
+const char* upic_last_error(const upic_ctx* ctx);
+/* Gauss-Legendre nodes/weights on [0,1] used by the context (host arrays of
+ * orbit_quad_order entries). */
+upic_status upic_quadrature(const upic_ctx* ctx, double* s, double* w);
+
+/* ---- Group A: kernel-level entry points (device pointers) --------------- */
+
+/* path_average_deposit_current (P1:883-906, P3:2784-2857).  J is overwritten
+ * (3*N).  stats may be NULL (host pointer). */
+upic_status upic_deposit_current_orbit(upic_ctx* ctx, const double* x0, const double* dx,
+                                       const double* vbar, const double* q, int64_t np,
+                                       double* J, upic_path_stats* stats);
+/* path_average_gather_vector (P1:858-881, P3:2689-2713). out is 3*np. */
+upic_status upic_gather_orbit(upic_ctx* ctx, const double* x0, const double* dx,
+                              const double* field3, int64_t np, double* out);
+/* orbit_discrete_gradient_A(_interval) (P1:908-934, P3:2958-2984).
+ * theta(s) = theta0 + (theta1-theta0) s; D is 9*np, D[(a*3+b)*np + p] = dA_a/dx_b. */
+upic_status upic_orbit_grad_A(upic_ctx* ctx, const double* x0, const double* dx,
+                              const double* A_n, const double* A_np1, double theta0,
+                              double theta1, int64_t np, double* D);
+/* gather_vector_at over particles (P1:666-678, P3:2267-2346). out is 3*np. */
+upic_status upic_gather_point(upic_ctx* ctx, const double* pos, const double* field3,
+                              int64_t np, double* out);
+/* deposit_scalar_to_mesh (P1:617-623, P3:2117-2126); mesh (N) overwritten. */
+upic_status upic_deposit_scalar_point(upic_ctx* ctx, const double* pos, const double* values,
+                                      int64_t np, int divide_by_dV, double* mesh);
+/* fft3d on interleaved complex data (2*N doubles) (P1:456-485, P3:1614-1656,
+ * P2:406-424); inverse scales by 1/N. */
+upic_status upic_fft(upic_ctx* ctx, double* data, int inverse);
+/* spectral_gradient_scalar / spectral_divergence_vector / spectral_curl
+ * (P1:505-546, P3:1721-1795). */
+upic_status upic_spectral_gradient(upic_ctx* ctx, const double* f, double* out3);
+upic_status upic_spectral_divergence(upic_ctx* ctx, const double* V3, double* out);
+upic_status upic_spectral_curl(upic_ctx* ctx, const double* A3, double* out3);
+/* continuity_charge_update (P1:995-1001, P3:3288-3296). */
+upic_status upic_continuity(upic_ctx* ctx, const double* rho_n, const double* J3, double dt,
+                            double* rho_out);
+/* cn_potential_update (P1:977-993, P3:3249-3270).  P1 uses c/eps0/mu0,
+ * P3 uses kappa/sigma1/sigma2 from the context parameters. */
+upic_status upic_cn_potentials(upic_ctx* ctx, const double* phi, const double* psi,
+                               const double* A3, const double* U3, const double* rho_mid,
+                               const double* J3, double dt, double* phi_out, double* psi_out,
+                               double* A_out3, double* U_out3);
+/* project_nyquist (P3:2044-2078) on ncomp consecutive fields. */
+upic_status upic_project_nyquist(upic_ctx* ctx, double* f, int ncomp);
+/* solve_periodic_poisson_for_phi (P1:959-968, P3:3196-3206). */
+upic_status upic_poisson(upic_ctx* ctx, const double* rho, double* phi);
+
+/* ---- Group B: per-step drivers on a device-resident simulation ---------- */
+typedef struct upic_sim upic_sim;
+
+/* Host-side particle/mesh state, reference State layout (P1:970-975,
+ * P3:3208-3222; P2 ParticleSet/FieldState).  Particle arrays are 3*np SoA;
+ * any pointer may be NULL where the code does not use it. */
+typedef struct upic_host_state {
+    int64_t np;
+    double *x, *v, *v_prev, *P, *vbar_last; /* 3*np each                          */
+    double *q, *m;                         /* np each                            */
+    double *phi, *psi, *rho;               /* N each                             */
+    double *A, *U, *J;                     /* 3*N each                           */
+    /* P2 history (BDF, newest first): 5*N each for phi and rho, 5*3*N for A.
+     * fixed_ion_rho is N.  (P2:787-810, 1126-1131) */
+    double *phi_hist, *A_hist, *rho_hist, *fixed_ion_rho;
+} upic_host_state;
+
+upic_status upic_sim_create(upic_ctx* ctx, int64_t np, upic_sim** out);
+upic_status upic_sim_destroy(upic_sim* sim);
+upic_status upic_sim_upload(upic_sim* sim, const upic_host_state* st);
+upic_status upic_sim_download(upic_sim* sim, upic_host_state* st);
+/* P1 picard_step_orbit (P1:1509-1552). */
+upic_status upic_step_p1(upic_sim* sim, upic_step_info* info);
+/* P3 nonlinear_step_orbit (P3:6477-6526) -> nonlinear_step_uniform_subcycled
+ * (5878-6215) with the selective current map (5235-5431). */
+upic_status upic_step_p3(upic_sim* sim, upic_step_info* info);
+/* P2 one iteration of run_simulation's step loop (P2:1133-1189). */
+upic_status upic_step_p2(upic_sim* sim, upic_step_info* info);
+/* compute_diagnostics (P1:1057-1087, P3:3521-3574, P2:898-962) on the current
+ * state; mode = perturbation_mode. */
+upic_status upic_sim_diagnostics(upic_sim* sim, int mode, upic_diagnostics* out);
+/* Device pointers of the resident state (for zero-copy callers). */
+upic_status upic_sim_device_state(upic_sim* sim, upic_host_state* dev);
+/* Per-particle failure reasons of the last P3 map evaluation (np bytes, host). */
+upic_status upic_sim_failure_reasons(upic_sim* sim, uint8_t* reasons);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UPIC_H */

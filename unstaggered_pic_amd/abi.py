@@ -1,0 +1,96 @@
+"""ctypes mirror of the plain-data structs in include/upic.h.
+
+Only data layouts live here; the compute entry points are bound in
+:mod:`unstaggered_pic_amd.lib`.
+"""
+import ctypes as C
+
+UPIC_OK, UPIC_EINVAL, UPIC_ENONLINEAR, UPIC_EREFINE, UPIC_EDEVICE, UPIC_ENOMEM = range(6)
+STATUS_NAMES = {0: "UPIC_OK", 1: "UPIC_EINVAL", 2: "UPIC_ENONLINEAR", 3: "UPIC_EREFINE",
+                4: "UPIC_EDEVICE", 5: "UPIC_ENOMEM"}
+CODE_P1, CODE_P2, CODE_P3 = 1, 2, 3
+PREC_FP64, PREC_MIXED = 0, 1
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("code", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("Lx", C.c_double), ("Ly", C.c_double), ("Lz", C.c_double), ("dt", C.c_double),
+        ("spline_order", C.c_int32), ("orbit_quad_order", C.c_int32),
+        ("split_orbit_at_knots", C.c_int32), ("nyquist_projection", C.c_int32),
+        ("c", C.c_double), ("eps0", C.c_double), ("mu0", C.c_double),
+        ("sigma1", C.c_double), ("sigma2", C.c_double),
+        ("picard_tol", C.c_double), ("picard_max_iter", C.c_int32),
+        ("final_consistency_sweep", C.c_int32),
+        ("nonlinear_solver_picard", C.c_int32), ("nonlinear_max_iter", C.c_int32),
+        ("nonlinear_rtol", C.c_double), ("nonlinear_atol", C.c_double),
+        ("anderson_memory", C.c_int32), ("anderson_start", C.c_int32),
+        ("anderson_damping", C.c_double), ("anderson_regularization", C.c_double),
+        ("anderson_growth_limit", C.c_double),
+        ("local_rtol", C.c_double), ("local_atol", C.c_double),
+        ("local_max_iter", C.c_int32), ("local_damping", C.c_double),
+        ("selective_particle_fallback", C.c_int32),
+        ("selective_particle_max_substeps", C.c_int32),
+        ("bdf_order", C.c_int32), ("charge_conserving", C.c_int32),
+        ("precision", C.c_int32), ("sort_interval", C.c_int32), ("learned_guess", C.c_int32),
+        ("reserved", C.c_int32 * 5),
+    ]
+
+
+class PathStats(C.Structure):
+    _fields_ = [("n_particles", C.c_int64), ("crossing_particles", C.c_int64),
+                ("total_segments", C.c_int64), ("max_segments", C.c_int32), ("pad", C.c_int32)]
+
+
+class StepInfo(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32), ("converged", C.c_int32),
+        ("current_error", C.c_double), ("velocity_error", C.c_double),
+        ("hit_max_iter", C.c_int32), ("anderson_restarts", C.c_int32),
+        ("anderson_steps", C.c_int32), ("picard_fallback_steps", C.c_int32),
+        ("backtracked_steps", C.c_int32), ("map_evaluations", C.c_int32),
+        ("minimum_update_alpha", C.c_double), ("final_rms_residual", C.c_double),
+        ("final_max_particle_residual", C.c_double), ("final_absolute_rms_residual", C.c_double),
+        ("estimated_map_contraction", C.c_double),
+        ("local_iterations_max", C.c_int32), ("failed_particles", C.c_int32),
+        ("local_iterations_mean", C.c_double), ("local_scaled_residual_max", C.c_double),
+        ("local_iterations_total", C.c_int64),
+        ("path", PathStats),
+    ]
+
+
+class Diagnostics(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "gauss_rms", "gauss_max", "gauge_rms", "gauge_max",
+        "divE_rms", "rho_rms", "psi_over_c2_rms", "divA_rms",
+        "field_energy", "kinetic_energy", "total_energy",
+        "rho_mode", "E_mode_re", "E_mode_im", "E_mode_abs",
+        "Bz_y_mode_re", "Bz_y_mode_im", "Bz_y_mode_abs", "mean_rho",
+        "continuity_rms", "Bz_l2", "B_l2", "E_l2", "rho_l2", "J_l2", "max_speed", "mean_gamma")]
+
+
+class HostState(C.Structure):
+    _fields_ = [("np", C.c_int64)] + [(n, C.POINTER(C.c_double)) for n in (
+        "x", "v", "v_prev", "P", "vbar_last", "q", "m", "phi", "psi", "rho", "A", "U", "J",
+        "phi_hist", "A_hist", "rho_hist", "fixed_ion_rho")]
+
+
+def make_params(**kw):
+    """Params with the reference's deck defaults (P1 Config 233-303,
+    P3 Config 837-990), overridden by keyword arguments."""
+    p = Params()
+    defaults = dict(code=CODE_P1, nx=4, ny=4, nz=1, Lx=6.283185307179586, Ly=6.283185307179586,
+                    Lz=1.0, dt=0.00625, spline_order=2, orbit_quad_order=16, split_orbit_at_knots=1,
+                    nyquist_projection=0, c=1.0, eps0=1.0, mu0=1.0, sigma1=1.0, sigma2=1.0,
+                    picard_tol=1e-11, picard_max_iter=16, final_consistency_sweep=1,
+                    nonlinear_solver_picard=0, nonlinear_max_iter=32, nonlinear_rtol=1e-12,
+                    nonlinear_atol=1e-13, anderson_memory=4, anderson_start=2, anderson_damping=0.7,
+                    anderson_regularization=1e-12, anderson_growth_limit=1.2, local_rtol=1e-11,
+                    local_atol=1e-13, local_max_iter=24, local_damping=1.0,
+                    selective_particle_fallback=0, selective_particle_max_substeps=64,
+                    bdf_order=1, charge_conserving=1, precision=PREC_FP64, sort_interval=1,
+                    learned_guess=0)
+    defaults.update(kw)
+    for k, v in defaults.items():
+        setattr(p, k, v)
+    return p
