@@ -250,10 +250,44 @@ upic_status upic_step_p2(upic_sim* sim, upic_step_info* info);
 /* compute_diagnostics (P1:1057-1087, P3:3521-3574, P2:898-962) on the current
  * state; mode = perturbation_mode. */
 upic_status upic_sim_diagnostics(upic_sim* sim, int mode, upic_diagnostics* out);
+/* Initial mesh state and particle momenta from the uploaded x, v, q, m: the
+ * common tail of the reference initialisers (P1:1287-1303 / P3:3920-3937):
+ * rho = deposit(q)/dV + n0 with the mean removed, J = point current deposit,
+ * phi from Poisson, psi = A = U = 0, P = m v (P1) or m gamma v (P3), and the
+ * history velocity v_prev by the Taylor start-up (P1:1296-1303,
+ * P3:3423-3441). */
+upic_status upic_sim_init_fields(upic_sim* sim, double n0);
 /* Device pointers of the resident state (for zero-copy callers). */
 upic_status upic_sim_device_state(upic_sim* sim, upic_host_state* dev);
 /* Per-particle failure reasons of the last P3 map evaluation (np bytes, host). */
 upic_status upic_sim_failure_reasons(upic_sim* sim, uint8_t* reasons);
+
+/* ---- Particle loading for the synthetic benchmark workloads ------------- */
+/* No reference counterpart: the reference's initialisers (P1:1151-1304,
+ * P3:3689-4108) build quiet-start eigenmodes; the benchmark workloads need
+ * seeded random loads.  Generation is host-side C++ (splitmix64 + Box-Muller)
+ * and deterministic for a seed. */
+typedef enum upic_load_kind {
+    UPIC_LOAD_LANDAU = 1,   /* density 1 + alpha cos(k x) by inverse CDF, y uniform,
+                               Maxwellian velocities of width vth[d]            */
+    UPIC_LOAD_BIMAXWELL = 2,/* uniform density, Gaussian momenta u = gamma v / c of
+                               widths vth[d] (stays sub-luminal)                 */
+    UPIC_LOAD_BEAMS = 3     /* cell lattice + noise*dx jitter, two beams at +-v0 in x
+                               with Maxwellian spread vth[d]                    */
+} upic_load_kind;
+typedef struct upic_load_spec {
+    int32_t kind;           /* upic_load_kind                                  */
+    int32_t ppc;            /* particles per cell                              */
+    uint64_t seed;
+    double n0, alpha;
+    int32_t mode;           /* perturbation mode (k = 2 pi mode / Lx)          */
+    int32_t pad;
+    double vth[3];
+    double v0, noise;
+} upic_load_spec;
+/* Fills st->x, v, q, m (host arrays of st->np = nx*ny*nz*ppc particles) with
+ * q = -n0 Lx Ly Lz / np and m = |q|. */
+upic_status upic_load_particles(const upic_params* params, const upic_load_spec* spec, upic_host_state* st);
 
 #ifdef __cplusplus
 }

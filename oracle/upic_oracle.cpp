@@ -1634,6 +1634,43 @@ static void p1_init_tail(const Ctx& C, State& st, Real n0) {
     st.vbar_last = st.v;
 }
 
+// Tail of the P3 initialisers for given particles (P3:3920-3937): rho, J,
+// phi, P = m gamma v + q A (A = 0), projection (3398-3405) and the
+// relativistic history velocity (3423-3441).
+static void p3_init_tail(const Ctx& C, State& st, Real n0) {
+    const Grid& g = C.g;
+    const std::size_t np = st.x.size();
+    const Real kappa = C.p.c;
+    Field rho_e = deposit_scalar_to_mesh(st.x, st.q, g, C.order(), true);
+    st.rho.assign(g.N, 0.0);
+    for (int id = 0; id < g.N; ++id) st.rho[id] = rho_e[id] + n0;
+    subtract_mean(st.rho);
+    st.J = deposit_current_to_mesh(st.x, st.v, st.q, g, C.order());
+    st.phi = solve_periodic_poisson_for_phi(st.rho, g, C.p.sigma1, false, 1.0);
+    st.psi.assign(g.N, 0.0);
+    st.A = zero_vecfield(g.N);
+    st.U = zero_vecfield(g.N);
+    st.P.assign(np, {0, 0, 0});
+    for (std::size_t p = 0; p < np; ++p) st.P[p] = momentum_from_v(st.v[p], st.m[p], kappa);
+    if (g.nyquist_projection) {
+        project_nyquist(st.rho, g); project_nyquist(st.phi, g); project_nyquist(st.psi, g);
+        project_nyquist(st.A, g); project_nyquist(st.U, g); project_nyquist(st.J, g);
+        for (std::size_t p = 0; p < np; ++p)
+            st.P[p] = add3(momentum_from_v(st.v[p], st.m[p], kappa), mul3(st.q[p], gather_vector_at(st.A, st.x[p], g, C.order())));
+    }
+    const VecField E0 = compute_E(st, g);
+    const VecField B0 = spectral_curl(st.A, g);
+    st.vbar_last = st.v;
+    st.v_prev.assign(np, {0, 0, 0});
+    for (std::size_t p = 0; p < np; ++p) {
+        const Vec3 Ep = gather_vector_at(E0, st.x[p], g, C.order());
+        const Vec3 Bp = gather_vector_at(B0, st.x[p], g, C.order());
+        const Vec3 pn = momentum_from_v(st.v[p], st.m[p], kappa);
+        const Vec3 lor = mul3(st.q[p], add3(Ep, cross3(st.v[p], Bp)));
+        st.v_prev[p] = velocity_from_p(sub3(pn, mul3(C.p.dt, lor)), st.m[p], kappa);
+    }
+}
+
 // initialize_landau_damping_1d3d (P1:1214-1304)
 static State p1_init_landau(const Ctx& C, int nv, Real alpha, Real vth, int mode, Real n0) {
     const Grid& g = C.g;
@@ -1924,6 +1961,18 @@ int orc_run(const upic_params* p, upic_host_state* st_io, int n_steps, int mode,
             st = (p->code == 1) ? p1_step(C, st, &inf) : p3_step(C, st, &inf);
             row(n + 1, st, &inf);
         }
+        store_state(st, st_io);
+    });
+}
+// Initial mesh state, momenta and history velocity from given x, v, q, m
+// (tails of the P1 / P3 initialisers).
+int orc_init_fields(const upic_params* p, upic_host_state* st_io, double n0) {
+    return guarded([&] {
+        const Ctx C(*p);
+        State st;
+        st.x = load3(st_io->x, st_io->np); st.v = load3(st_io->v, st_io->np);
+        st.q.assign(st_io->q, st_io->q + st_io->np); st.m.assign(st_io->m, st_io->m + st_io->np);
+        if (p->code == 1) p1_init_tail(C, st, n0); else p3_init_tail(C, st, n0);
         store_state(st, st_io);
     });
 }

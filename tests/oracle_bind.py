@@ -73,7 +73,7 @@ class Oracle:
                      "orc_spectral_gradient", "orc_spectral_divergence", "orc_spectral_curl", "orc_continuity",
                      "orc_cn_potentials", "orc_project_nyquist", "orc_poisson", "orc_p3_local_solve", "orc_step",
                      "orc_run", "orc_diagnostics", "orc_p1_step_diagnostics", "orc_p1_init", "orc_p2_run",
-                     "orc_mt19937_uniform"):
+                     "orc_mt19937_uniform", "orc_init_fields"):
             getattr(L, name).restype = C.c_int
 
     def check(self, rc):
@@ -216,6 +216,13 @@ class Oracle:
         st = State(h.np, self._N(p))
         h = st.host()
         self.check(self.L.orc_p1_init(*args, C.byref(h)))
+        return st
+
+    def init_fields(self, p, x, v, q, m, n0=1.0):
+        st = State(len(q), self._N(p))
+        st.x[:] = x; st.v[:] = v; st.q[:] = q; st.m[:] = m
+        h = st.host()
+        self.check(self.L.orc_init_fields(C.byref(p), C.byref(h), C.c_double(n0)))
         return st
 
     def p2_run(self, p, st, fixed_ion_rho, n_steps):

@@ -75,6 +75,23 @@ class HostState(C.Structure):
         "phi_hist", "A_hist", "rho_hist", "fixed_ion_rho")]
 
 
+LOAD_LANDAU, LOAD_BIMAXWELL, LOAD_BEAMS = 1, 2, 3
+
+
+class LoadSpec(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("ppc", C.c_int32), ("seed", C.c_uint64), ("n0", C.c_double),
+                ("alpha", C.c_double), ("mode", C.c_int32), ("pad", C.c_int32), ("vth", C.c_double * 3),
+                ("v0", C.c_double), ("noise", C.c_double)]
+
+
+def make_load(kind, ppc, seed=12345, n0=1.0, alpha=0.0, mode=1, vth=(0.0, 0.0, 0.0), v0=0.0, noise=0.0):
+    s = LoadSpec()
+    s.kind, s.ppc, s.seed, s.n0, s.alpha, s.mode = kind, ppc, seed, n0, alpha, mode
+    s.vth[:] = list(vth)
+    s.v0, s.noise = v0, noise
+    return s
+
+
 def make_params(**kw):
     """Params with the reference's deck defaults (P1 Config 233-303,
     P3 Config 837-990), overridden by keyword arguments."""

@@ -1,0 +1,194 @@
+// api_internal.hpp -- host-side internals shared by the C-ABI translation
+// units: error type, device allocation, the context object.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/upic.h"
+#include "kernels.hpp"
+#include "particles.hpp"
+
+using namespace upic;
+
+namespace upic_host {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess) throw Error(UPIC_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+inline T* dalloc(size_t n) {
+    T* p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T));
+    if (e != hipSuccess) throw Error(UPIC_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    return p;
+}
+template <class T>
+inline void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+inline bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// Gauss-Legendre on [0,1] (gauss_legendre_01, P1:739-763), evaluated in long
+// double so the nodes are correctly rounded.
+inline void gauss_legendre_01(int n, double* s, double* w) {
+    const long double PI = 3.141592653589793238462643383279502884L;
+    const int m = (n + 1) / 2;
+    for (int i = 0; i < m; ++i) {
+        long double x = cosl(PI * ((long double)i + 0.75L) / ((long double)n + 0.5L));
+        long double dp = 0.0L;
+        for (int it = 0; it < 100; ++it) {
+            long double p0 = 1.0L, p1 = x;
+            for (int k = 2; k <= n; ++k) {
+                const long double pk = ((2.0L * k - 1.0L) * x * p1 - (k - 1.0L) * p0) / (long double)k;
+                p0 = p1;
+                p1 = pk;
+            }
+            if (n == 1) { p1 = x; p0 = 1.0L; }
+            dp = (long double)n * (x * p1 - p0) / (x * x - 1.0L);
+            const long double dx = -p1 / dp;
+            x += dx;
+            if (fabsl(dx) < 1.0e-19L) break;
+        }
+        {
+            long double p0 = 1.0L, p1 = x;
+            for (int k = 2; k <= n; ++k) {
+                const long double pk = ((2.0L * k - 1.0L) * x * p1 - (k - 1.0L) * p0) / (long double)k;
+                p0 = p1;
+                p1 = pk;
+            }
+            dp = (long double)n * (x * p1 - p0) / (x * x - 1.0L);
+        }
+        const long double wl = 2.0L / ((1.0L - x * x) * dp * dp);
+        s[i] = (double)(0.5L * (1.0L - x));
+        w[i] = (double)(0.5L * wl);
+        s[n - 1 - i] = (double)(0.5L * (1.0L + x));
+        w[n - 1 - i] = (double)(0.5L * wl);
+    }
+}
+
+}  // namespace upic_host
+using namespace upic_host;
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct upic_ctx {
+    upic_params p{};
+    GridDev g{};
+    QuadDev q{};
+    OrbitOpt opt{};
+    SpecArgs sp{};
+    FFTPlan plan{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::string err;
+    double* dred = nullptr;               // small device reduction area (64 doubles)
+    unsigned long long* dstats = nullptr; // 8 counters
+    double* hred = nullptr;               // pinned mirror
+    unsigned long long* hstats = nullptr;
+    // group-A workspace, grown on demand
+    double* work = nullptr;
+    size_t work_n = 0;
+
+    int N() const { return g.N; }
+    double* workspace(size_t n) {
+        if (n > work_n) {
+            dfree(work);
+            work = dalloc<double>(n);
+            work_n = n;
+        }
+        return work;
+    }
+    void sync() { HIPCHK(hipStreamSynchronize(stream)); }
+    double c2() const { return p.c * p.c; }
+    double phi_src() const { return p.code == 3 ? p.sigma1 : 1.0 / p.eps0; }
+    double A_src() const { return p.code == 3 ? p.sigma2 : p.mu0; }
+};
+
+namespace upic_host {
+
+inline void validate(const upic_params& p) {
+    if (p.code < 1 || p.code > 3) throw Error(UPIC_EINVAL, "code must be 1, 2 or 3");
+    if (!is_pow2(p.nx) || !is_pow2(p.ny) || !is_pow2(p.nz))
+        throw Error(UPIC_EINVAL, "the in-file radix-2 FFT requires nx, ny, nz to be powers of two");
+    if (p.nx > 2048 || p.ny > 2048 || p.nz > 2048) throw Error(UPIC_EINVAL, "grid lines longer than 2048 are not supported");
+    if (p.code == 2 && p.nz != 1) throw Error(UPIC_EINVAL, "P2 is a 2-D code: nz must be 1");
+    if (p.code != 2 && (p.spline_order < 1 || p.spline_order > 4)) throw Error(UPIC_EINVAL, "spline_order must be 1, 2, 3, or 4");
+    if (p.code != 2 && (p.orbit_quad_order < 1 || p.orbit_quad_order > kMaxQuad))
+        throw Error(UPIC_EINVAL, "orbit_quad_order must be in [1, 32]");
+    if (!(p.dt > 0.0)) throw Error(UPIC_EINVAL, "time stepping requires dt>0");
+    if (!(p.Lx > 0.0) || !(p.Ly > 0.0) || !(p.Lz > 0.0)) throw Error(UPIC_EINVAL, "domain lengths must be positive");
+    if (p.code == 3) {  // P3:1349-1433
+        if (!(p.c > 0.0) || !(p.sigma1 > 0.0) || !(p.sigma2 > 0.0))
+            throw Error(UPIC_EINVAL, "nondimensional coefficients require kappa>0, sigma1>0, sigma2>0");
+        if (!(p.nonlinear_rtol > 0.0) || !(p.nonlinear_atol > 0.0) || p.nonlinear_max_iter <= 0)
+            throw Error(UPIC_EINVAL, "nonlinear controls require nonlinear_rtol>0, nonlinear_atol>0, and nonlinear_max_iter>0");
+        if (p.anderson_memory < 1 || p.anderson_start < 0)
+            throw Error(UPIC_EINVAL, "Anderson controls require anderson_memory>=1 and anderson_start>=0");
+        if (p.anderson_memory > 8) throw Error(UPIC_EINVAL, "anderson_memory above 8 is not supported");
+        if (!(p.anderson_damping > 0.0) || !(p.anderson_damping <= 1.0))
+            throw Error(UPIC_EINVAL, "anderson_damping must satisfy 0 < damping <= 1");
+        if (!(p.anderson_regularization >= 0.0) || !(p.anderson_growth_limit > 1.0))
+            throw Error(UPIC_EINVAL, "Anderson controls require regularization>=0 and growth_limit>1");
+        if (!(p.local_rtol > 0.0) || !(p.local_atol > 0.0) || p.local_max_iter < 1 || !(p.local_damping > 0.0) ||
+            !(p.local_damping <= 1.0))
+            throw Error(UPIC_EINVAL, "local particle solve requires positive rtol/atol/max_iter and 0<damping<=1");
+        const int ms = p.selective_particle_max_substeps;
+        if (ms < 1 || ms > 64 || (ms & (ms - 1)) != 0)
+            throw Error(UPIC_EINVAL, "selective particle fallback requires a power-of-two max_substeps in [1,64]");
+        const double compat = p.c * p.c * p.sigma2;
+        if (std::abs(p.sigma1 - compat) / std::max(1.0, std::abs(p.sigma1)) > 1.0e-10)
+            throw Error(UPIC_EINVAL, "Maxwell-compatible nondimensionalization requires sigma1 = kappa^2*sigma2");
+    } else {
+        if (!(p.c > 0.0) || !(p.eps0 > 0.0) || !(p.mu0 > 0.0)) throw Error(UPIC_EINVAL, "c, eps0 and mu0 must be positive");
+    }
+    if (p.code == 1 && p.picard_max_iter < 1) throw Error(UPIC_EINVAL, "picard_max_iter must be positive");
+    if (p.code == 2 && p.bdf_order != 1 && p.bdf_order != 2) throw Error(UPIC_EINVAL, "method must be bdf1 or bdf2");
+    if (p.precision != UPIC_PREC_FP64 && p.precision != UPIC_PREC_MIXED) throw Error(UPIC_EINVAL, "unknown precision");
+}
+
+template <class F>
+int guarded(upic_ctx* c, F&& f) {
+    try {
+        f();
+        if (c) c->err.clear();
+        return UPIC_OK;
+    } catch (const Error& e) {
+        if (c) c->err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        if (c) c->err = e.what();
+        return UPIC_EINVAL;
+    }
+}
+
+// real fields -> spectra; spectra -> real (destroys the spectra)
+inline void to_spectra(upic_ctx* c, const double* re, double* zh, int batch) {
+    launch_real_to_complex(c->stream, re, zh, (int64_t)batch * c->N());
+    launch_fft(c->stream, c->plan, zh, batch, false);
+}
+inline void from_spectra(upic_ctx* c, double* zh, double* re, int batch) {
+    launch_fft(c->stream, c->plan, zh, batch, true);
+    launch_complex_real(c->stream, zh, re, (int64_t)batch * c->N());
+}
+
+}  // namespace upic_host

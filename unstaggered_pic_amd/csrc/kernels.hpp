@@ -1,0 +1,175 @@
+// kernels.hpp -- argument blocks and launch entry points of the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_common.hpp"
+
+namespace upic {
+
+struct OrbitOpt;
+
+// P1 particle sweep arguments (all device pointers, SoA 3*np vectors).
+struct P1Args {
+    int64_t np;
+    double dt;
+    const double *x, *v, *v_prev, *P, *q, *m;
+    const double* vg;             // current velocity iterate v_guess
+    const double *gpm, *A_n, *A_end;
+    double* vg_out;               // next iterate
+    double* J_out;                // current of the next map evaluation (3*N, pre-zeroed)
+    double* red;                  // [3] velocity-error sums
+    unsigned long long* stats;    // [3] path statistics
+    double *x_out, *v_out, *P_out, *vprev_out, *vbar_out;  // final mode
+};
+
+// P3 per-particle map arguments.
+struct P3Args {
+    int64_t np;
+    double dt, kappa, rtol, atol, damping;
+    int max_iter;
+    int write_out;
+    const double *x, *P, *v, *q, *m;
+    const int8_t* layout;         // substeps per particle (NULL = 1)
+    const double* guess;          // optional initial guess of vbar (NULL = v^n, P3:5318)
+    const double *A_n, *A_end, *gpm;
+    double* J_out;                // deposit target (3*N, pre-zeroed) or NULL
+    double *x_out, *P_out, *v_out, *vbar_out;
+    uint8_t* reason;              // per-particle failure reason (optional)
+    double* red;                  // [2]: sum of local iterations, failed particles
+    double* redmax;               // [2]: max local iterations, max final scaled residual
+    unsigned long long* stats;
+};
+
+// P2 explicit step arguments.
+struct P2Args {
+    int64_t np;
+    int nx, ny;
+    double Lx, Ly, dx, dy, dA, dt, c;
+    double *x, *v, *v_prev, *P;
+    const double *q, *m;
+    double* J;                    // 3*N deposit target
+    double* rho;                  // naive charge deposit target or NULL
+    const double* F;              // 11 gathered scalar fields (push)
+};
+
+// particle_kernels.hip
+void launch_deposit_orbit(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const double* x0,
+                          const double* dx, const double* vbar, const double* qa, int64_t np, double* J,
+                          unsigned long long* stats);
+void launch_gather_orbit(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const double* x0,
+                         const double* dx, const double* F, int64_t np, double* out);
+void launch_grad_A(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const double* x0,
+                   const double* dx, const double* A0, const double* A1, double th0, double th1, int p1_theta,
+                   int64_t np, double* out);
+void launch_gather_point(hipStream_t s, const GridDev& g, const double* pos, const double* F, int64_t np, double* out);
+void launch_deposit_scalar_point(hipStream_t s, const GridDev& g, const double* pos, const double* val, int64_t np,
+                                 double scale, double* mesh);
+void launch_p1_deposit_guess(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const P1Args& a);
+void launch_p1_push(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const P1Args& a, bool final_);
+void launch_p3_map(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const P3Args& a);
+void launch_p2_advance_deposit(hipStream_t s, const P2Args& a);
+void launch_p2_push(hipStream_t s, const P2Args& a);
+
+// field_kernels.hip ---------------------------------------------------------
+struct FFTPlan {
+    int n[3];                 // nx, ny, nz
+    double* tw[3];            // per-axis twiddles: cos/sin interleaved, n/2 entries
+};
+// In-place complex FFT over all three axes (z, y, x order as P1:456-485) of
+// `batch` consecutive fields of N complex values each.
+void launch_fft(hipStream_t s, const FFTPlan& plan, double* data, int batch, bool inverse);
+// re/im packing
+void launch_real_to_complex(hipStream_t s, const double* re, double* z, int64_t n);
+void launch_complex_real(hipStream_t s, const double* z, double* re, int64_t n);
+// y = a*x + b*y over n doubles; x may be NULL (then y = b*y)
+void launch_axpby(hipStream_t s, int64_t n, double a, const double* x, double b, double* y);
+void launch_copy(hipStream_t s, int64_t n, const double* x, double* y);
+void launch_zero(hipStream_t s, int64_t n, double* y);
+void launch_fill(hipStream_t s, int64_t n, double v, double* y);
+
+// Spectral symbols: wave numbers of mode (i,j,k) for the grid code.
+struct SpecArgs {
+    int nx, ny, nz, N, code, projection;
+    double L[3];
+};
+
+// Spectral field solve of one map evaluation in k-space (P1:1473-1485,
+// P3:5265-5285).  Inputs are complex spectra (2*N doubles each):
+//   Jh[3], rho_n_h, phi_n_h, psi_n_h, A_n_h[3], U_n_h[3].
+// Outputs (complex): rho_end_h, phi_end_h, psi_end_h, A_end_h[3], U_end_h[3],
+// gpm_h[3] = i k (phi_n + phi_end)/2.
+struct SolveArgs {
+    SpecArgs sp;
+    double dt, c2, phi_src, A_src;  // c^2, source scales (1/eps0 | sigma1, mu0 | sigma2)
+    const double *Jh, *rho_n_h, *phi_n_h, *psi_n_h, *A_n_h, *U_n_h;
+    double *rho_end_h, *phi_end_h, *psi_end_h, *A_end_h, *U_end_h, *gpm_h;
+};
+void launch_spectral_solve(hipStream_t s, const SolveArgs& a);
+// out_h[d] = i k_d f_h (gradient of a scalar spectrum), d = 0..2
+void launch_spectral_grad(hipStream_t s, const SpecArgs& sp, const double* fh, double* outh3);
+// out_h = i k . V_h
+void launch_spectral_div(hipStream_t s, const SpecArgs& sp, const double* Vh3, double* outh);
+// out_h = i k x A_h
+void launch_spectral_curl(hipStream_t s, const SpecArgs& sp, const double* Ah3, double* outh3);
+// Poisson in k-space (P1:959-968 / P3:3196-3206): phi_h = src * rho_h / k^2
+void launch_spectral_poisson(hipStream_t s, const SpecArgs& sp, double src, const double* rhoh, double* phih);
+// CN wave step on one pair (cn_fft_wave_step, P1:940-957) in k-space
+void launch_spectral_cn(hipStream_t s, const SpecArgs& sp, double dt, double c2, double src, const double* uh,
+                        const double* vh, const double* Sh, double* uoh, double* voh);
+// Zero mode 0 of `batch` spectra (mean removal)
+void launch_zero_mean_mode(hipStream_t s, double* zh, int batch, int N);
+// Continuity in k-space: rho_h = rho_n_h - dt i k.J_h (mode 0 zeroed)
+void launch_spectral_continuity(hipStream_t s, const SpecArgs& sp, double dt, const double* rhonh, const double* Jh3,
+                                double* rhoh);
+// Real-space Nyquist projector (project_nyquist, P3:2044-2061) on `batch` fields
+void launch_project_nyquist(hipStream_t s, const SpecArgs& sp, double* f, int batch);
+// Subtract the mean of each of `batch` real fields (subtract_mean, P1:215-219)
+void launch_subtract_mean(hipStream_t s, double* f, int batch, int N, double* scratch);
+
+// Sum reductions: out[k] = sum_i x_k[i]*y_k[i] for several pairs; generic
+// block partials + atomics into out (pre-zeroed).
+void launch_dot_many(hipStream_t s, int64_t n, int npairs, const double* const* xs, const double* const* ys,
+                     double* out);
+// Relative-RMS pieces of two vectors: out += {sum (a-b)^2, sum a^2, sum b^2}
+void launch_diff_norms(hipStream_t s, int64_t n, const double* a, const double* b, double* out3);
+// P3 residual norms on node vectors (nonlinear_residual_norms, P3:5576-5610):
+// map, guess are 3*N component-major; out (pre-zeroed):
+//   [0] sum |r|^2, [1] sum |map|^2, [2] sum |guess|^2,
+//   [3] max |r|/max(floor, vs), [4] max |r|/(atol + rtol vs), [5] nonfinite count
+void launch_residual_norms(hipStream_t s, int N, const double* map, const double* guess, double floor_, double rtol,
+                           double atol, double* out);
+// Energy-type reductions over grid fields: out += {sum f^2 ...}
+void launch_sum_squares(hipStream_t s, int64_t n, const double* f, double* out);
+void launch_sum(hipStream_t s, int64_t n, const double* f, double* out);
+void launch_max_abs(hipStream_t s, int64_t n, const double* f, double* out);
+
+// Particle sums for diagnostics: kinetic energy (P1 nonrelativistic /
+// P3 relativistic), P2 max speed and mean gamma.
+void launch_kinetic(hipStream_t s, int64_t np, const double* v, const double* m, double c, int relativistic,
+                    double* out3);
+
+// Cell sort of particles (counting sort by cell of x): keys, histogram, scan,
+// permutation, and the gather of SoA arrays through it.
+void launch_cell_keys(hipStream_t s, const GridDev& g, int64_t np, const double* x, int* keys, int* counts);
+void launch_scan_counts(hipStream_t s, int ncell, int* counts, int* offsets);
+void launch_scatter_perm(hipStream_t s, int64_t np, const int* keys, int* cursor, int* perm);
+void launch_permute(hipStream_t s, int64_t np, int ncomp, const double* src, double* dst, const int* perm);
+void launch_permute_i64(hipStream_t s, int64_t np, const int64_t* src, int64_t* dst, const int* perm);
+
+}  // namespace upic
+
+namespace upic {
+// out = sum_k c_k f_k over n doubles (up to 8 inputs; f_k may alias out)
+struct LinComb { int k; double c[8]; const double* f[8]; };
+void launch_lincomb(hipStream_t s, int64_t n, const LinComb& lc, double* out);
+// z_h /= (a + k^2)  (BDF wave solve denominator, P2:843-846)
+void launch_spectral_div_shift(hipStream_t s, const SpecArgs& sp, double a, double* zh, int batch);
+// Initial momenta and history velocity of the initialisers:
+//   P1 (P1:1293-1303): P = m v, v_prev = v - dt (q/m) E(x)
+//   P3 (P3:3931-3441): P = m gamma v + q A(x), v_prev from p - dt q (E + v x B)
+void launch_init_momenta(hipStream_t s, const GridDev& g, int64_t np, int code, double kappa, double dt,
+                         const double* x, const double* v, const double* q, const double* m, const double* E,
+                         const double* B, const double* A, double* P, double* v_prev, double* vbar_last);
+}  // namespace upic

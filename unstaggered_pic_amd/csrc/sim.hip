@@ -1,0 +1,1147 @@
+// sim.hip -- device-resident simulation and the per-step drivers:
+//   P1 picard_step_orbit           (Project1/ec_pic_serial_demo.cpp:1509-1552)
+//   P3 nonlinear_step_orbit        (Project3/ec_pic_nyquist_projected.cpp:6477-6526,
+//                                   5878-6215, 5235-5431)
+//   P2 run_simulation step body    (Project2/spectral_pic_part3.cpp:1133-1189)
+// and the per-step diagnostics (P1:1057-1087, P3:3521-3574).
+#include "api_internal.hpp"
+
+struct upic_sim {
+    upic_ctx* c = nullptr;
+    int64_t np = 0;
+    // particle state, SoA FP64
+    double *x = nullptr, *v = nullptr, *v_prev = nullptr, *P = nullptr, *vbar_last = nullptr;
+    double *q = nullptr, *m = nullptr;
+    int64_t* id = nullptr;  // original index of each stored particle (the cell sort permutes)
+    // mesh state, real
+    double *phi = nullptr, *psi = nullptr, *rho = nullptr, *A = nullptr, *U = nullptr, *J = nullptr;
+    // spectra of the state at step start: phi, psi, rho, A(3), U(3) -> 9 complex fields
+    double* S0 = nullptr;
+    // spectra of one map evaluation: J(3), rho_e, phi_e, psi_e, A_e(3), U_e(3), gpm(3) -> 15
+    double* S1 = nullptr;
+    double* zs = nullptr;      // complex scratch, 16 fields
+    double* AG = nullptr;      // real A_end (3N) | gpm (3N)
+    double* Jb[2] = {nullptr, nullptr};
+    double* gtmp = nullptr;    // real scratch 12N
+    // particle scratch
+    double *vg[2] = {nullptr, nullptr};
+    double *xo = nullptr, *vo = nullptr, *Po = nullptr, *vbo = nullptr;
+    uint8_t* reason = nullptr;
+    int8_t* layout = nullptr;
+    // P3 outer-solve vectors (3N each)
+    double *guess = nullptr, *resid = nullptr, *cand = nullptr, *trial = nullptr, *Jacc = nullptr, *Jtry = nullptr,
+           *Jfield = nullptr;
+    double* xh[9] = {};
+    double* rh[9] = {};
+    double* dX[8] = {};
+    double* dR[8] = {};
+    // cell sort
+    int *keys = nullptr, *counts = nullptr, *offsets = nullptr, *perm = nullptr;
+    double* ptmp = nullptr;   // 3*np
+    double* ptmp1 = nullptr;  // np
+    int64_t* idtmp = nullptr;
+    // P2 histories (real), newest first
+    double* phi_hist[5] = {};
+    double* A_hist[5] = {};
+    double* rho_hist[5] = {};
+    double* fixed_ion = nullptr;
+    int64_t steps = 0;
+    std::vector<uint8_t> host_reason;
+};
+
+namespace {
+
+void sim_free(upic_sim* s) {
+    double** dbl[] = {&s->x, &s->v, &s->v_prev, &s->P, &s->vbar_last, &s->q, &s->m, &s->phi, &s->psi, &s->rho,
+                      &s->A, &s->U, &s->J, &s->S0, &s->S1, &s->zs, &s->AG, &s->Jb[0], &s->Jb[1], &s->gtmp,
+                      &s->vg[0], &s->vg[1], &s->xo, &s->vo, &s->Po, &s->vbo, &s->guess, &s->resid, &s->cand,
+                      &s->trial, &s->Jacc, &s->Jtry, &s->Jfield, &s->ptmp, &s->ptmp1, &s->fixed_ion};
+    for (double** p : dbl) dfree(*p);
+    for (int i = 0; i < 9; ++i) { dfree(s->xh[i]); dfree(s->rh[i]); }
+    for (int i = 0; i < 8; ++i) { dfree(s->dX[i]); dfree(s->dR[i]); }
+    for (int i = 0; i < 5; ++i) { dfree(s->phi_hist[i]); dfree(s->A_hist[i]); dfree(s->rho_hist[i]); }
+    dfree(s->id); dfree(s->idtmp); dfree(s->reason); dfree(s->layout);
+    dfree(s->keys); dfree(s->counts); dfree(s->offsets); dfree(s->perm);
+}
+
+inline double* Zf(double* base, int k, int N) { return base + (size_t)2 * N * k; }
+
+// ---------------------------------------------------------------------------
+// Shared pieces
+// ---------------------------------------------------------------------------
+
+// Spectra of the step-start state (the reference transforms the same real
+// fields: make_wave_spectral_state P3:3098-3109, cn_fft_wave_step P1:940-957).
+void state_spectra(upic_sim* s) {
+    upic_ctx* c = s->c;
+    const int N = c->N();
+    to_spectra(c, s->phi, Zf(s->S0, 0, N), 1);
+    to_spectra(c, s->psi, Zf(s->S0, 1, N), 1);
+    to_spectra(c, s->rho, Zf(s->S0, 2, N), 1);
+    to_spectra(c, s->A, Zf(s->S0, 3, N), 3);
+    to_spectra(c, s->U, Zf(s->S0, 6, N), 3);
+}
+
+// Field solve of one map evaluation from a real current (3N): continuity,
+// CN potentials, grad phi_mid.  Leaves the end-of-step spectra in S1 and the
+// real A_end and grad phi_mid in AG.
+void field_solve(upic_sim* s, const double* Jreal) {
+    upic_ctx* c = s->c;
+    const int N = c->N();
+    to_spectra(c, Jreal, Zf(s->S1, 0, N), 3);
+    SolveArgs a;
+    a.sp = c->sp;
+    a.dt = c->p.dt;
+    a.c2 = c->c2();
+    a.phi_src = c->phi_src();
+    a.A_src = c->A_src();
+    a.Jh = Zf(s->S1, 0, N);
+    a.phi_n_h = Zf(s->S0, 0, N);
+    a.psi_n_h = Zf(s->S0, 1, N);
+    a.rho_n_h = Zf(s->S0, 2, N);
+    a.A_n_h = Zf(s->S0, 3, N);
+    a.U_n_h = Zf(s->S0, 6, N);
+    a.rho_end_h = Zf(s->S1, 3, N);
+    a.phi_end_h = Zf(s->S1, 4, N);
+    a.psi_end_h = Zf(s->S1, 5, N);
+    a.A_end_h = Zf(s->S1, 6, N);
+    a.U_end_h = Zf(s->S1, 9, N);
+    a.gpm_h = Zf(s->S1, 12, N);
+    launch_spectral_solve(c->stream, a);
+    // A_end and gpm are adjacent in S1 (fields 6..8 and 12..14): copy both to scratch
+    launch_copy(c->stream, 6ll * N, Zf(s->S1, 6, N), Zf(s->zs, 0, N));
+    launch_copy(c->stream, 6ll * N, Zf(s->S1, 12, N), Zf(s->zs, 3, N));
+    from_spectra(c, s->zs, s->AG, 6);
+}
+
+// End-of-step real mesh state from the last field solve.
+void commit_fields(upic_sim* s, const double* Jfinal) {
+    upic_ctx* c = s->c;
+    const int N = c->N();
+    launch_copy(c->stream, 2ll * N * 3, Zf(s->S1, 3, N), Zf(s->zs, 0, N));  // rho_e, phi_e, psi_e
+    launch_copy(c->stream, 6ll * N, Zf(s->S1, 9, N), Zf(s->zs, 3, N));      // U_e
+    from_spectra(c, s->zs, s->gtmp, 6);
+    launch_copy(c->stream, N, s->gtmp, s->rho);
+    launch_copy(c->stream, N, s->gtmp + N, s->phi);
+    launch_copy(c->stream, N, s->gtmp + 2 * N, s->psi);
+    launch_copy(c->stream, 3ll * N, s->gtmp + 3 * N, s->U);
+    launch_copy(c->stream, 3ll * N, s->AG, s->A);
+    if (Jfinal != s->J) launch_copy(c->stream, 3ll * N, Jfinal, s->J);
+}
+
+void zero_red(upic_ctx* c, int n = 16) {
+    launch_zero(c->stream, n, c->dred);
+    HIPCHK(hipMemsetAsync(c->dstats, 0, 3 * sizeof(unsigned long long), c->stream));
+}
+void read_red(upic_ctx* c, int n = 16) {
+    HIPCHK(hipMemcpyAsync(c->hred, c->dred, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(c->hstats, c->dstats, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+    c->sync();
+}
+void fill_path(upic_ctx* c, int64_t np, upic_path_stats& ps) {
+    ps.n_particles = np;
+    ps.crossing_particles = (int64_t)c->hstats[0];
+    ps.total_segments = (int64_t)c->hstats[1];
+    ps.max_segments = std::max<int32_t>(1, (int32_t)c->hstats[2]);
+    ps.pad = 0;
+}
+
+// Counting sort of the particles by the cell that contains them.
+void sort_particles(upic_sim* s) {
+    upic_ctx* c = s->c;
+    const int64_t np = s->np;
+    const int ncell = c->N();
+    if (np == 0) return;
+    HIPCHK(hipMemsetAsync(s->counts, 0, ncell * sizeof(int), c->stream));
+    launch_cell_keys(c->stream, c->g, np, s->x, s->keys, s->counts);
+    launch_scan_counts(c->stream, ncell, s->counts, s->offsets);
+    launch_scatter_perm(c->stream, np, s->keys, s->offsets, s->perm);
+    double** arr3[] = {&s->x, &s->v, &s->v_prev, &s->P, &s->vbar_last};
+    for (double** a : arr3) {
+        launch_permute(c->stream, np, 3, *a, s->ptmp, s->perm);
+        std::swap(*a, s->ptmp);
+    }
+    double** arr1[] = {&s->q, &s->m};
+    for (double** a : arr1) {
+        launch_permute(c->stream, np, 1, *a, s->ptmp1, s->perm);
+        std::swap(*a, s->ptmp1);
+    }
+    launch_permute_i64(c->stream, np, s->id, s->idtmp, s->perm);
+    std::swap(s->id, s->idtmp);
+}
+
+void maybe_sort(upic_sim* s) {
+    const int k = s->c->p.sort_interval;
+    if (k > 0 && (s->steps % k) == 0) sort_particles(s);
+}
+
+// ---------------------------------------------------------------------------
+// P1 step (picard_step_orbit, P1:1509-1552)
+// ---------------------------------------------------------------------------
+void p1_step(upic_sim* s, upic_step_info* info) {
+    upic_ctx* c = s->c;
+    const upic_params& p = c->p;
+    const int N = c->N();
+    const int64_t np = s->np;
+    maybe_sort(s);
+    state_spectra(s);
+    P1Args a{};
+    a.np = np; a.dt = p.dt;
+    a.x = s->x; a.v = s->v; a.v_prev = s->v_prev; a.P = s->P; a.q = s->q; a.m = s->m;
+    a.A_n = s->A; a.A_end = s->AG; a.gpm = s->AG + 3 * (size_t)N;
+    a.red = c->dred; a.stats = c->dstats;
+    // J_0 from v_guess = 2 v^n - v^{n-1}
+    int cur = 0;
+    launch_zero(c->stream, 3ll * N, s->Jb[cur]);
+    zero_red(c);
+    a.vg_out = s->vg[cur];
+    a.J_out = s->Jb[cur];
+    launch_p1_deposit_guess(c->stream, c->g, c->q, c->opt, a);
+    int iters = 0;
+    double cur_err = std::numeric_limits<double>::infinity(), vel_err = cur_err;
+    const double inf = std::numeric_limits<double>::infinity();
+    bool have_prev = false;
+    upic_path_stats ps{};
+    for (int it = 0; it < p.picard_max_iter; ++it) {
+        const int nxt = 1 - cur;
+        field_solve(s, s->Jb[cur]);
+        zero_red(c);
+        if (have_prev) launch_diff_norms(c->stream, 3ll * N, s->Jb[cur], s->Jb[nxt], c->dred + 4);
+        launch_zero(c->stream, 3ll * N, s->Jb[nxt]);
+        a.vg = s->vg[cur];
+        a.vg_out = s->vg[nxt];
+        a.J_out = s->Jb[nxt];
+        launch_p1_push(c->stream, c->g, c->q, c->opt, a, false);
+        HIPCHK(hipGetLastError());
+        read_red(c, 8);
+        ++iters;
+        const double* h = c->hred;
+        const double cnt = 3.0 * (double)np;
+        // relative_rms_error_vec_particles (P1:196-202)
+        vel_err = std::sqrt(h[0] / cnt) / (0.5 * (std::sqrt(h[1] / cnt) + std::sqrt(h[2] / cnt)) + 1.0e-300);
+        if (have_prev) {  // relative_rms_error_vecfield (P1:204-213)
+            const double cf = 3.0 * (double)N;
+            cur_err = std::sqrt(h[4] / cf) / (0.5 * (std::sqrt(h[5] / cf) + std::sqrt(h[6] / cf)) + 1.0e-300);
+        } else {
+            cur_err = inf;
+        }
+        fill_path(c, np, ps);
+        have_prev = true;
+        cur = nxt;
+        if (it > 0 && std::max(cur_err, vel_err) < p.picard_tol) break;
+    }
+    // Here Jb[cur] holds the current of map(v_next) and vg[cur] = v_next.
+    const double* Jfinal;
+    if (p.final_consistency_sweep) {
+        field_solve(s, s->Jb[cur]);
+        a.vg = s->vg[cur];
+        Jfinal = s->Jb[cur];
+    } else {
+        // the state of the last evaluated map: fields of J_prev, iterate vg[prev]
+        const int prev = 1 - cur;
+        field_solve(s, s->Jb[prev]);
+        a.vg = s->vg[prev];
+        Jfinal = s->Jb[prev];
+    }
+    a.x_out = s->xo; a.v_out = s->vo; a.P_out = s->Po; a.vprev_out = s->vbo /*temp*/; a.vbar_out = s->vbar_last;
+    launch_p1_push(c->stream, c->g, c->q, c->opt, a, true);
+    // the old velocity becomes v_prev; swap buffers in
+    std::swap(s->x, s->xo);
+    std::swap(s->P, s->Po);
+    std::swap(s->v_prev, s->vbo);  // vbo now holds the stale v_prev
+    std::swap(s->v, s->vo);
+    commit_fields(s, Jfinal);
+    HIPCHK(hipGetLastError());
+    c->sync();
+    ++s->steps;
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->iterations = iters;
+        info->converged = (std::max(cur_err, vel_err) < p.picard_tol) ? 1 : 0;
+        info->current_error = cur_err;
+        info->velocity_error = vel_err;
+        info->path = ps;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P3 step (nonlinear_step_uniform_subcycled with a selective layout)
+// ---------------------------------------------------------------------------
+struct MapOut {
+    double iter_sum = 0.0, iter_max = 0.0, scaled_max = 0.0;
+    int64_t failed = 0;
+    upic_path_stats ps{};
+};
+
+struct RefineNeeded : Error {
+    std::vector<int64_t> failed;  // stored-order particle indices
+    RefineNeeded(const std::string& m, std::vector<int64_t> f) : Error(UPIC_EREFINE, m), failed(std::move(f)) {}
+};
+
+// evaluate_selective_current_map_at_layout (P3:5235-5431): guess (3N) ->
+// J_map (3N).  write_out stores the particle end state in xo/Po/vo/vbo.
+MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bool use_layout) {
+    upic_ctx* c = s->c;
+    const upic_params& p = c->p;
+    const int N = c->N();
+    launch_copy(c->stream, 3ll * N, guess, s->Jfield);
+    launch_project_nyquist(c->stream, c->sp, s->Jfield, 3);
+    field_solve(s, s->Jfield);
+    launch_zero(c->stream, 3ll * N, Jmap);
+    zero_red(c);
+    P3Args a{};
+    a.np = s->np; a.dt = p.dt; a.kappa = p.c; a.rtol = p.local_rtol; a.atol = p.local_atol;
+    a.damping = p.local_damping; a.max_iter = p.local_max_iter; a.write_out = write_out ? 1 : 0;
+    a.x = s->x; a.P = s->P; a.v = s->v; a.q = s->q; a.m = s->m;
+    a.layout = use_layout ? s->layout : nullptr;
+    a.guess = nullptr;
+    a.A_n = s->A; a.A_end = s->AG; a.gpm = s->AG + 3 * (size_t)N;
+    a.J_out = Jmap;
+    a.x_out = s->xo; a.P_out = s->Po; a.v_out = s->vo; a.vbar_out = s->vbo;
+    a.reason = s->reason;
+    a.red = c->dred; a.redmax = c->dred + 2; a.stats = c->dstats;
+    launch_p3_map(c->stream, c->g, c->q, c->opt, a);
+    launch_project_nyquist(c->stream, c->sp, Jmap, 3);
+    HIPCHK(hipGetLastError());
+    read_red(c, 4);
+    MapOut o;
+    o.iter_sum = c->hred[0];
+    o.failed = (int64_t)c->hred[1];
+    o.iter_max = c->hred[2];
+    o.scaled_max = c->hred[3];
+    fill_path(c, s->np, o.ps);
+    if (o.failed > 0) {
+        s->host_reason.resize((size_t)s->np);
+        HIPCHK(hipMemcpy(s->host_reason.data(), s->reason, (size_t)s->np, hipMemcpyDeviceToHost));
+        std::vector<int64_t> f;
+        for (int64_t i = 0; i < s->np; ++i)
+            if (s->host_reason[i] != 0) f.push_back(i);
+        throw RefineNeeded("selective local particle solve failed for " + std::to_string(f.size()) + " particles", f);
+    }
+    return o;
+}
+
+struct Norms {
+    double rms_relative, max_particle_relative, rms_absolute, rms_scaled, max_particle_scaled;
+    bool nonfinite;
+};
+// nonlinear_residual_norms (P3:5576-5610)
+Norms residual_norms(upic_sim* s, const double* map, const double* guess) {
+    upic_ctx* c = s->c;
+    const upic_params& p = c->p;
+    const int N = c->N();
+    const double floor_ = 64.0 * std::numeric_limits<double>::epsilon() * p.c;
+    launch_zero(c->stream, 8, c->dred + 8);
+    launch_residual_norms(c->stream, N, map, guess, floor_, p.nonlinear_rtol, p.nonlinear_atol, c->dred + 8);
+    HIPCHK(hipMemcpyAsync(c->hred + 8, c->dred + 8, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    c->sync();
+    const double* h = c->hred + 8;
+    const double cnt = 3.0 * (double)N;
+    Norms n;
+    n.rms_absolute = std::sqrt(h[0] / cnt);
+    const double vrms = 0.5 * (std::sqrt(h[1] / cnt) + std::sqrt(h[2] / cnt));
+    n.rms_relative = n.rms_absolute / (vrms + floor_);
+    n.rms_scaled = n.rms_absolute / (p.nonlinear_atol + p.nonlinear_rtol * vrms);
+    n.max_particle_relative = h[3];
+    n.max_particle_scaled = h[4];
+    n.nonfinite = h[5] != 0.0;
+    return n;
+}
+bool finite_vec(upic_sim* s, const double* v) {
+    // the nonfinite flag of the residual kernel on (v, v)
+    return !residual_norms(s, v, v).nonfinite;
+}
+
+// small dense solve with the reference's pivot rules (P3:5706-5742)
+bool solve_small_dense(std::vector<std::vector<double>> A, std::vector<double> b, std::vector<double>& x) {
+    const int n = (int)b.size();
+    double scale = 0.0;
+    for (const auto& row : A) for (double v : row) scale = std::max(scale, std::abs(v));
+    if (!(scale > 0.0) || !std::isfinite(scale)) return false;
+    double largest = 0.0, smallest = std::numeric_limits<double>::infinity();
+    for (int col = 0; col < n; ++col) {
+        int pr = col;
+        for (int row = col + 1; row < n; ++row) if (std::abs(A[row][col]) > std::abs(A[pr][col])) pr = row;
+        const double piv = std::abs(A[pr][col]);
+        if (!std::isfinite(piv) || piv <= 100.0 * std::numeric_limits<double>::epsilon() * scale) return false;
+        largest = std::max(largest, piv);
+        smallest = std::min(smallest, piv);
+        if (pr != col) { std::swap(A[pr], A[col]); std::swap(b[pr], b[col]); }
+        for (int row = col + 1; row < n; ++row) {
+            const double f = A[row][col] / A[col][col];
+            for (int j = col; j < n; ++j) A[row][j] -= f * A[col][j];
+            b[row] -= f * b[col];
+        }
+    }
+    if (smallest <= 1.0e-12 * largest) return false;
+    x.assign(n, 0.0);
+    for (int row = n - 1; row >= 0; --row) {
+        double rhs = b[row];
+        for (int j = row + 1; j < n; ++j) rhs -= A[row][j] * x[j];
+        x[row] = rhs / A[row][row];
+        if (!std::isfinite(x[row])) return false;
+    }
+    return true;
+}
+
+// Device-resident history of (guess, residual) pairs.
+struct History {
+    upic_sim* s;
+    int len = 0;
+    int slot[9];  // physical buffer of logical entry i
+    explicit History(upic_sim* ss) : s(ss) { for (int i = 0; i < 9; ++i) slot[i] = i; }
+    double* X(int i) { return s->xh[slot[i]]; }
+    double* Rr(int i) { return s->rh[slot[i]]; }
+    void clear() { len = 0; }
+    void push(const double* x, const double* r, int keep) {
+        upic_ctx* c = s->c;
+        const int64_t n = 3ll * c->N();
+        if (len == keep) {  // drop the oldest
+            const int t = slot[0];
+            for (int i = 0; i + 1 < len; ++i) slot[i] = slot[i + 1];
+            slot[len - 1] = t;
+            --len;
+        }
+        launch_copy(c->stream, n, x, s->xh[slot[len]]);
+        launch_copy(c->stream, n, r, s->rh[slot[len]]);
+        ++len;
+    }
+    void assign1(const double* x, const double* r) { len = 0; push(x, r, 9); }
+};
+
+// anderson_type2_candidate (P3:5763-5823)
+bool anderson_candidate(upic_sim* s, History& H, int memory, double damping, double reg, double* cand) {
+    upic_ctx* c = s->c;
+    const int64_t n = 3ll * c->N();
+    const int available = H.len - 1;
+    const int m = std::min(memory, available);
+    if (m <= 0) return false;
+    const int first = H.len - m - 1;
+    for (int j = 0; j < m; ++j) {
+        LinComb lx{2, {1.0, -1.0}, {H.X(first + j + 1), H.X(first + j)}};
+        launch_lincomb(c->stream, n, lx, s->dX[j]);
+        LinComb lr{2, {1.0, -1.0}, {H.Rr(first + j + 1), H.Rr(first + j)}};
+        launch_lincomb(c->stream, n, lr, s->dR[j]);
+    }
+    std::vector<const double*> xs, ys;
+    for (int i = 0; i < m; ++i) { xs.push_back(s->dR[i]); ys.push_back(H.Rr(H.len - 1)); }
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j <= i; ++j) { xs.push_back(s->dR[i]); ys.push_back(s->dR[j]); }
+    const int npairs = (int)xs.size();
+    launch_zero(c->stream, npairs, c->dred + 16);
+    launch_dot_many(c->stream, n, npairs, xs.data(), ys.data(), c->dred + 16);
+    HIPCHK(hipMemcpyAsync(c->hred + 16, c->dred + 16, npairs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    c->sync();
+    std::vector<std::vector<double>> gram(m, std::vector<double>(m, 0.0));
+    std::vector<double> rhs(m, 0.0);
+    double trace = 0.0;
+    int k = m;
+    for (int i = 0; i < m; ++i) rhs[i] = c->hred[16 + i];
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j <= i; ++j) {
+            const double v = c->hred[16 + k++];
+            gram[i][j] = v;
+            gram[j][i] = v;
+        }
+    for (int i = 0; i < m; ++i) trace += gram[i][i];
+    if (!(trace > 0.0) || !std::isfinite(trace)) return false;
+    std::vector<double> unreg;
+    if (!solve_small_dense(gram, rhs, unreg)) return false;
+    const double rs = trace / (double)m;
+    for (int i = 0; i < m; ++i) gram[i][i] += reg * rs;
+    std::vector<double> gamma;
+    if (!solve_small_dense(gram, rhs, gamma)) return false;
+    // cand = x_last + damping r_last - sum_j gamma_j (dX_j + damping dR_j)
+    LinComb lc{};
+    lc.k = 0;
+    lc.c[lc.k] = 1.0; lc.f[lc.k++] = H.X(H.len - 1);
+    lc.c[lc.k] = damping; lc.f[lc.k++] = H.Rr(H.len - 1);
+    for (int j = 0; j < m && lc.k + 2 <= 8; ++j) {
+        lc.c[lc.k] = -gamma[j]; lc.f[lc.k++] = s->dX[j];
+        lc.c[lc.k] = -gamma[j] * damping; lc.f[lc.k++] = s->dR[j];
+    }
+    launch_lincomb(c->stream, n, lc, cand);
+    if (m > 3) {  // memory 4: remaining terms
+        for (int j = 3; j < m; ++j) {
+            LinComb l2{3, {1.0, -gamma[j], -gamma[j] * damping}, {cand, s->dX[j], s->dR[j]}};
+            launch_lincomb(c->stream, n, l2, cand);
+        }
+    }
+    return true;
+}
+
+void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
+    upic_ctx* c = s->c;
+    const upic_params& p = c->p;
+    const int N = c->N();
+    const int64_t n3 = 3ll * N;
+    const bool picard = p.nonlinear_solver_picard != 0;
+    launch_copy(c->stream, n3, s->J, s->guess);
+    if (!finite_vec(s, s->guess)) throw Error(UPIC_ENONLINEAR, "nonfinite initial macro-current predictor");
+    History H(s);
+    double prev_rms = std::numeric_limits<double>::infinity();
+    bool converged = false, have_prefetched = false;
+    int restarts = 0, and_steps = 0, pic_steps = 0, backtracked = 0, maps = 0, iters = 0;
+    double min_alpha = 1.0, contraction = std::numeric_limits<double>::infinity();
+    int64_t local_total = 0;
+    // Jacc: current map of the accepted guess
+    auto eval = [&](const double* g, double* Jout, bool wo) {
+        ++maps;
+        MapOut o = p3_map(s, g, Jout, wo, use_layout);
+        local_total += (int64_t)o.iter_sum;
+        return o;
+    };
+    const int keep = p.anderson_memory + 1;
+    for (int it = 0; it < p.nonlinear_max_iter; ++it) {
+        if (have_prefetched) have_prefetched = false;
+        else eval(s->guess, s->Jacc, false);
+        LinComb lr{2, {1.0, -1.0}, {s->Jacc, s->guess}};
+        launch_lincomb(c->stream, n3, lr, s->resid);
+        const Norms nm = residual_norms(s, s->Jacc, s->guess);
+        if (nm.nonfinite) throw Error(UPIC_ENONLINEAR, "uniform particle map returned a nonfinite macro current");
+        iters = it + 1;
+        if (nm.rms_scaled <= 1.0 && nm.max_particle_scaled <= 1.0) { converged = true; break; }
+        if (!picard && std::isfinite(prev_rms) && nm.rms_relative > p.anderson_growth_limit * prev_rms) {
+            H.clear();
+            ++restarts;
+        }
+        H.push(s->guess, s->resid, keep);
+        bool used_anderson = false;
+        if (!picard && it >= p.anderson_start && H.len >= 2) {
+            used_anderson = anderson_candidate(s, H, p.anderson_memory, p.anderson_damping, p.anderson_regularization, s->cand);
+            if (!used_anderson || !finite_vec(s, s->cand)) {
+                H.assign1(s->guess, s->resid);
+                ++restarts;
+                used_anderson = false;
+            }
+        }
+        // backtracking (P3:6044-6084); accepted guess -> trial, its map -> Jtry
+        auto backtrack = [&](const double* full, double& alpha_out) {
+            for (int bt = 0; bt <= 6; ++bt) {
+                const double alpha = std::ldexp(1.0, -bt);
+                // guess + alpha (full - guess)
+                LinComb lt{3, {1.0, alpha, -alpha}, {s->guess, full, s->guess}};
+                launch_lincomb(c->stream, n3, lt, s->trial);
+                if (!finite_vec(s, s->trial)) continue;
+                eval(s->trial, s->Jtry, false);
+                const Norms tn = residual_norms(s, s->Jtry, s->trial);
+                const bool dec = tn.rms_scaled <= (1.0 - 1.0e-4 * alpha) * nm.rms_scaled;
+                const bool safe = tn.max_particle_scaled <= p.anderson_growth_limit * nm.max_particle_scaled;
+                if (dec && safe) { alpha_out = alpha; return true; }
+            }
+            return false;
+        };
+        bool accepted = false, as_anderson = false;
+        double alpha = 0.0;
+        if (used_anderson) {
+            accepted = backtrack(s->cand, alpha);
+            as_anderson = accepted;
+            if (!accepted) { H.assign1(s->guess, s->resid); ++restarts; }
+        }
+        if (!accepted) {
+            const double damping = picard ? 1.0 : p.anderson_damping;
+            LinComb lp{2, {1.0, damping}, {s->guess, s->resid}};
+            launch_lincomb(c->stream, n3, lp, s->cand);
+            accepted = backtrack(s->cand, alpha);
+            as_anderson = false;
+        }
+        if (!accepted) throw Error(UPIC_ENONLINEAR, "outer current Anderson/Picard backtracking failed at the prescribed macro dt");
+        if (as_anderson) ++and_steps; else ++pic_steps;
+        if (alpha < 1.0) { ++backtracked; min_alpha = std::min(min_alpha, alpha); }
+        prev_rms = nm.rms_relative;
+        std::swap(s->guess, s->trial);
+        std::swap(s->Jacc, s->Jtry);
+        have_prefetched = true;
+    }
+    if (!converged) throw Error(UPIC_ENONLINEAR, "outer macro-current solve failed after " + std::to_string(iters) + " iterations");
+    const MapOut fo = eval(s->guess, s->Jacc, true);
+    const Norms fn = residual_norms(s, s->Jacc, s->guess);
+    const bool ok = !fn.nonfinite && fn.rms_scaled <= 1.0 && fn.max_particle_scaled <= 1.0;
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->iterations = iters;
+        info->converged = ok ? 1 : 0;
+        info->anderson_restarts = restarts;
+        info->anderson_steps = and_steps;
+        info->picard_fallback_steps = pic_steps;
+        info->backtracked_steps = backtracked;
+        info->map_evaluations = maps;
+        info->minimum_update_alpha = min_alpha;
+        info->final_rms_residual = fn.rms_relative;
+        info->final_max_particle_residual = fn.max_particle_relative;
+        info->final_absolute_rms_residual = fn.rms_absolute;
+        info->estimated_map_contraction = contraction;
+        info->local_iterations_max = (int32_t)fo.iter_max;
+        info->local_iterations_mean = s->np ? fo.iter_sum / (double)s->np : 0.0;
+        info->local_scaled_residual_max = fo.scaled_max;
+        info->local_iterations_total = local_total;
+        info->path = fo.ps;
+    }
+    if (!ok) throw Error(UPIC_ENONLINEAR, "final outer macro-current consistency evaluation failed at the prescribed macro dt");
+    // commit: v_prev <- v, particle outputs in, mesh state from the final map
+    std::swap(s->v_prev, s->v);   // v_prev = old v
+    std::swap(s->v, s->vo);       // v = new v ; vo holds stale v_prev
+    std::swap(s->x, s->xo);
+    std::swap(s->P, s->Po);
+    std::swap(s->vbar_last, s->vbo);
+    commit_fields(s, s->Jacc);
+    c->sync();
+}
+
+// nonlinear_step_orbit (P3:6477-6526) with the selective fallback (6328-6454)
+void p3_step(upic_sim* s, upic_step_info* info) {
+    upic_ctx* c = s->c;
+    maybe_sort(s);
+    state_spectra(s);
+    try {
+        p3_step_layout(s, false, info);
+    } catch (const RefineNeeded& e) {
+        if (!c->p.selective_particle_fallback)
+            throw Error(UPIC_ENONLINEAR, std::string("inner particle solve failed over full dt; time and dt unchanged: ") + e.what());
+        std::vector<int8_t> lay((size_t)s->np, 1);
+        std::vector<int64_t> failed = e.failed;
+        for (;;) {
+            for (int64_t i : failed) {
+                if (lay[i] >= c->p.selective_particle_max_substeps)
+                    throw Error(UPIC_ENONLINEAR, "selective particle fallback reached h=dt/" +
+                                                     std::to_string(c->p.selective_particle_max_substeps) +
+                                                     " with particles still unconverged; macro dt was not changed");
+            }
+            for (int64_t i : failed) lay[i] = (int8_t)std::min<int>(c->p.selective_particle_max_substeps, 2 * lay[i]);
+            HIPCHK(hipMemcpy(s->layout, lay.data(), (size_t)s->np, hipMemcpyHostToDevice));
+            try {
+                p3_step_layout(s, true, info);
+                break;
+            } catch (const RefineNeeded& e2) {
+                failed = e2.failed;
+            }
+        }
+    }
+    ++s->steps;
+}
+
+// ---------------------------------------------------------------------------
+// P2 step (P2:1133-1189)
+// ---------------------------------------------------------------------------
+void p2_step(upic_sim* s, upic_step_info* info) {
+    upic_ctx* c = s->c;
+    const upic_params& p = c->p;
+    const int N = c->N();
+    const int64_t np = s->np;
+    const double dt = p.dt;
+    const std::vector<double> a = p.bdf_order == 2 ? std::vector<double>{1.5, -2.0, 0.5} : std::vector<double>{1.0, -1.0};
+    std::vector<double> b(2 * a.size() - 1, 0.0);  // convolve_coefficients (P2:779-785)
+    for (size_t i = 0; i < a.size(); ++i) for (size_t j = 0; j < a.size(); ++j) b[i + j] += a[i] * a[j];
+    P2Args pa{};
+    pa.np = np; pa.nx = p.nx; pa.ny = p.ny; pa.Lx = p.Lx; pa.Ly = p.Ly;
+    pa.dx = p.Lx / p.nx; pa.dy = p.Ly / p.ny; pa.dA = pa.dx * pa.dy; pa.dt = dt; pa.c = p.c;
+    pa.x = s->x; pa.v = s->v; pa.v_prev = s->v_prev; pa.P = s->P; pa.q = s->q; pa.m = s->m;
+    double* Jn = s->Jb[0];
+    double* rho_new = s->Jb[1];  // first N doubles
+    launch_zero(c->stream, 3ll * N, Jn);
+    pa.J = Jn;
+    pa.rho = nullptr;
+    if (!p.charge_conserving) {
+        launch_copy(c->stream, N, s->fixed_ion, rho_new);
+        pa.rho = rho_new;
+    }
+    launch_p2_advance_deposit(c->stream, pa);  // positions (P2:1050-1055), J (747-755)
+    if (p.charge_conserving) {  // update_charge_from_continuity (P2:865-880)
+        to_spectra(c, Jn, s->zs, 3);
+        launch_spectral_div(c->stream, c->sp, s->zs, Zf(s->zs, 3, N));
+        from_spectra(c, Zf(s->zs, 3, N), s->gtmp, 1);
+        LinComb lc{};
+        lc.k = 0;
+        lc.c[lc.k] = -dt / a[0]; lc.f[lc.k++] = s->gtmp;
+        for (size_t l = 1; l < a.size(); ++l) { lc.c[lc.k] = -a[l] / a[0]; lc.f[lc.k++] = s->rho_hist[l - 1]; }
+        launch_lincomb(c->stream, N, lc, rho_new);
+    }
+    // BDF wave solves (solve_bdf_wave_scalar P2:829-848) by linearity on the
+    // combined real right-hand side, one transform pair per field.
+    const double scale = 1.0 / (p.c * p.c * dt * dt);
+    auto bdf = [&](double src_scale, const double* src, double* const* hist, double* outre, double* zh) {
+        LinComb lc{};
+        lc.k = 0;
+        lc.c[lc.k] = src_scale; lc.f[lc.k++] = src;
+        for (size_t l = 1; l < b.size(); ++l) { lc.c[lc.k] = -scale * b[l]; lc.f[lc.k++] = hist[l - 1]; }
+        launch_lincomb(c->stream, N, lc, outre);
+        to_spectra(c, outre, zh, 1);
+        launch_spectral_div_shift(c->stream, c->sp, scale * b[0], zh, 1);
+    };
+    // new phi and A spectra in zs fields 0..3
+    double* phi_new = s->phi_hist[4];  // recycled below
+    double* A_new = s->A_hist[4];
+    bdf(1.0 / p.eps0, rho_new, s->phi_hist, phi_new, Zf(s->zs, 0, N));
+    for (int comp = 0; comp < 3; ++comp) {
+        double* Ah[5];
+        for (int l = 0; l < 5; ++l) Ah[l] = s->A_hist[l] + (size_t)comp * N;
+        bdf(p.mu0, Jn + (size_t)comp * N, Ah, A_new + (size_t)comp * N, Zf(s->zs, 1 + comp, N));
+    }
+    // gathered fields: A (3), grad phi (x,y), grad A (3 x 2)  -> 11 real fields
+    double* F = s->gtmp + N;  // 11N after the divergence scratch
+    double* z2 = Zf(s->zs, 4, N);  // 8 complex scratch fields
+    launch_spectral_grad(c->stream, c->sp, Zf(s->zs, 0, N), z2);  // phi_x, phi_y, (phi_z = 0)
+    for (int comp = 0; comp < 3; ++comp) launch_spectral_grad(c->stream, c->sp, Zf(s->zs, 1 + comp, N), Zf(z2, 2 + 2 * comp, N));
+    // layout in z2: phi_x phi_y | Ax_x Ax_y | Ay_x Ay_y | Az_x Az_y (z components overwritten)
+    // keep the order by writing each gradient's z slot into the next one's x slot: grad writes 3 fields,
+    // the next call overwrites the previous z slot, which is zero in 2-D.
+    from_spectra(c, z2, F + 3ll * N, 8);
+    from_spectra(c, Zf(s->zs, 1, N), F, 3);
+    from_spectra(c, Zf(s->zs, 0, N), phi_new, 1);  // real phi^{n+1}
+    // A_new real was overwritten with its BDF rhs; recompute from F (A real)
+    launch_copy(c->stream, 3ll * N, F, A_new);
+    pa.F = F;
+    launch_p2_push(c->stream, pa);
+    // history shift (P2:1185-1188)
+    double* ph = s->phi_hist[4];
+    double* Ah = s->A_hist[4];
+    double* rh = s->rho_hist[4];
+    for (int l = 4; l > 0; --l) { s->phi_hist[l] = s->phi_hist[l - 1]; s->A_hist[l] = s->A_hist[l - 1]; s->rho_hist[l] = s->rho_hist[l - 1]; }
+    s->phi_hist[0] = ph;
+    s->A_hist[0] = Ah;
+    s->rho_hist[0] = rh;
+    launch_copy(c->stream, N, rho_new, rh);
+    launch_copy(c->stream, N, ph, s->phi);
+    launch_copy(c->stream, 3ll * N, Ah, s->A);
+    launch_copy(c->stream, N, rh, s->rho);
+    launch_copy(c->stream, 3ll * N, Jn, s->J);
+    HIPCHK(hipGetLastError());
+    c->sync();
+    ++s->steps;
+    if (info) std::memset(info, 0, sizeof(*info));
+}
+
+// compute_diagnostics of P2 (P2:898-962) after a step: the BDF derivatives
+// use hist[0] as the new level and hist[1..] as the pre-step history; the
+// initial state uses BDF1 over duplicated histories (P2:1121-1129).
+void p2_diagnostics(upic_sim* s, upic_diagnostics* out) {
+    upic_ctx* c = s->c;
+    const upic_params& p = c->p;
+    const int N = c->N();
+    const double dt = p.dt;
+    const std::vector<double> a = (s->steps == 0 || p.bdf_order == 1) ? std::vector<double>{1.0, -1.0}
+                                                                      : std::vector<double>{1.5, -2.0, 0.5};
+    double* r = s->gtmp;  // 12N real scratch
+    double* z = s->zs;    // 16 complex
+    double* d = c->dred + 32;
+    launch_zero(c->stream, 24, d);
+    auto deriv = [&](double* const* hist, size_t off, double* outre) {
+        LinComb lc{};
+        lc.k = 0;
+        for (size_t l = 0; l < a.size(); ++l) { lc.c[lc.k] = a[l] / dt; lc.f[lc.k++] = hist[l] + off; }
+        launch_lincomb(c->stream, N, lc, outre);
+    };
+    // phi_t (r0), A_t (r1..3)
+    deriv(s->phi_hist, 0, r);
+    for (int k = 0; k < 3; ++k) deriv(s->A_hist, (size_t)k * N, r + (size_t)(1 + k) * N);
+    // spectra: phi (z0), A (z1..3)
+    to_spectra(c, s->phi_hist[0], Zf(z, 0, N), 1);
+    to_spectra(c, s->A_hist[0], Zf(z, 1, N), 3);
+    launch_spectral_div(c->stream, c->sp, Zf(z, 1, N), Zf(z, 4, N));       // divA
+    launch_spectral_curl(c->stream, c->sp, Zf(z, 1, N), Zf(z, 5, N));      // B (5..7)
+    launch_spectral_grad(c->stream, c->sp, Zf(z, 0, N), Zf(z, 8, N));      // grad phi (8..10)
+    from_spectra(c, Zf(z, 4, N), r + 4 * (size_t)N, 1);                    // divA real
+    from_spectra(c, Zf(z, 5, N), r + 5 * (size_t)N, 3);                    // B real (5..7)
+    from_spectra(c, Zf(z, 8, N), r + 8 * (size_t)N, 2);                    // grad phi x,y (8, 9)
+    {   // gauge = phi_t / c^2 + divA  -> r10
+        LinComb g{2, {1.0 / (p.c * p.c), 1.0}, {r, r + 4 * (size_t)N}};
+        launch_lincomb(c->stream, N, g, r + 10 * (size_t)N);
+        launch_sum_squares(c->stream, N, r + 10 * (size_t)N, d + 0);
+    }
+    // E = -grad phi - A_t  (stored in r1..3, z component has no phi gradient)
+    for (int k = 0; k < 3; ++k) {
+        if (k < 2) {
+            LinComb e{2, {-1.0, -1.0}, {r + (size_t)(8 + k) * N, r + (size_t)(1 + k) * N}};
+            launch_lincomb(c->stream, N, e, r + (size_t)(1 + k) * N);
+        } else {
+            launch_axpby(c->stream, N, 0.0, nullptr, -1.0, r + 3 * (size_t)N);
+        }
+    }
+    launch_sum_squares(c->stream, 3ll * N, r + (size_t)N, d + 1);         // sum E^2
+    to_spectra(c, r + (size_t)N, Zf(z, 8, N), 3);
+    launch_spectral_div(c->stream, c->sp, Zf(z, 8, N), Zf(z, 11, N));
+    from_spectra(c, Zf(z, 11, N), r + 8 * (size_t)N, 1);                   // divE
+    {   // gauss = divE - rho / eps0
+        LinComb g{2, {1.0, -1.0 / p.eps0}, {r + 8 * (size_t)N, s->rho_hist[0]}};
+        launch_lincomb(c->stream, N, g, r + 9 * (size_t)N);
+        launch_sum_squares(c->stream, N, r + 9 * (size_t)N, d + 2);
+    }
+    // continuity = rho_t + divJ
+    deriv(s->rho_hist, 0, r);
+    to_spectra(c, s->J, Zf(z, 0, N), 3);
+    launch_spectral_div(c->stream, c->sp, Zf(z, 0, N), Zf(z, 3, N));
+    from_spectra(c, Zf(z, 3, N), r + 4 * (size_t)N, 1);
+    {
+        LinComb g{2, {1.0, 1.0}, {r, r + 4 * (size_t)N}};
+        launch_lincomb(c->stream, N, g, r + 10 * (size_t)N);
+        launch_sum_squares(c->stream, N, r + 10 * (size_t)N, d + 3);
+    }
+    launch_sum_squares(c->stream, N, r + 7 * (size_t)N, d + 4);           // Bz^2
+    launch_sum_squares(c->stream, 3ll * N, r + 5 * (size_t)N, d + 5);     // B^2
+    launch_sum_squares(c->stream, N, s->rho_hist[0], d + 6);
+    launch_sum_squares(c->stream, 3ll * N, s->J, d + 7);
+    launch_kinetic(c->stream, s->np, s->v, s->m, p.c, 2, d + 8);        // [8] kin, [9] gamma sum, [10] max speed
+    HIPCHK(hipMemcpyAsync(c->hred, d, 12 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    c->sync();
+    const double* h = c->hred;
+    const double dA = (p.Lx / p.nx) * (p.Ly / p.ny);
+    upic_diagnostics& o = *out;
+    std::memset(&o, 0, sizeof(o));
+    o.gauge_rms = std::sqrt(h[0] / N);
+    o.gauss_rms = std::sqrt(h[2] / N);
+    o.continuity_rms = std::sqrt(h[3] / N);
+    o.Bz_l2 = std::sqrt(h[4] / N);
+    o.B_l2 = std::sqrt(h[5] / N);
+    o.E_l2 = std::sqrt(h[1] / N);
+    o.rho_l2 = std::sqrt(h[6] / N);
+    o.J_l2 = std::sqrt(h[7] / N);
+    o.field_energy = 0.5 * p.eps0 * (h[1] * dA) + 0.5 / p.mu0 * (h[5] * dA);
+    o.kinetic_energy = h[8];
+    o.total_energy = o.field_energy + o.kinetic_energy;
+    o.max_speed = h[10];
+    o.mean_gamma = s->np ? h[9] / (double)s->np : 0.0;
+}
+
+}  // namespace
+
+extern "C" {
+
+upic_status upic_sim_create(upic_ctx* c, int64_t np, upic_sim** out) {
+    if (!c || !out || np < 0) return UPIC_EINVAL;
+    *out = nullptr;
+    upic_sim* s = new upic_sim();
+    const int rc = guarded(c, [&] {
+        if (np > (int64_t)1 << 31) throw Error(UPIC_EINVAL, "at most 2^31 particles per simulation");
+        s->c = c;
+        s->np = np;
+        const int N = c->N();
+        const size_t n3 = 3 * (size_t)np;
+        s->x = dalloc<double>(n3); s->v = dalloc<double>(n3); s->v_prev = dalloc<double>(n3);
+        s->P = dalloc<double>(n3); s->vbar_last = dalloc<double>(n3);
+        s->q = dalloc<double>(np); s->m = dalloc<double>(np);
+        s->id = dalloc<int64_t>(np); s->idtmp = dalloc<int64_t>(np);
+        s->phi = dalloc<double>(N); s->psi = dalloc<double>(N); s->rho = dalloc<double>(N);
+        s->A = dalloc<double>(3 * (size_t)N); s->U = dalloc<double>(3 * (size_t)N); s->J = dalloc<double>(3 * (size_t)N);
+        s->S0 = dalloc<double>(2 * (size_t)N * 9);
+        s->S1 = dalloc<double>(2 * (size_t)N * 15);
+        s->zs = dalloc<double>(2 * (size_t)N * 16);
+        s->AG = dalloc<double>(6 * (size_t)N);
+        s->gtmp = dalloc<double>(12 * (size_t)N);
+        s->Jb[0] = dalloc<double>(3 * (size_t)N);
+        s->Jb[1] = dalloc<double>(3 * (size_t)N);
+        s->xo = dalloc<double>(n3); s->vo = dalloc<double>(n3); s->Po = dalloc<double>(n3); s->vbo = dalloc<double>(n3);
+        s->ptmp = dalloc<double>(n3);
+        s->ptmp1 = dalloc<double>(np);
+        s->keys = dalloc<int>(np); s->perm = dalloc<int>(np);
+        s->counts = dalloc<int>(N); s->offsets = dalloc<int>(N);
+        if (c->p.code == 1) {
+            s->vg[0] = dalloc<double>(n3);
+            s->vg[1] = dalloc<double>(n3);
+        }
+        if (c->p.code == 3) {
+            s->reason = dalloc<uint8_t>(np);
+            s->layout = dalloc<int8_t>(np);
+            double** v3[] = {&s->guess, &s->resid, &s->cand, &s->trial, &s->Jacc, &s->Jtry, &s->Jfield};
+            for (double** p : v3) *p = dalloc<double>(3 * (size_t)N);
+            for (int i = 0; i <= c->p.anderson_memory; ++i) {
+                s->xh[i] = dalloc<double>(3 * (size_t)N);
+                s->rh[i] = dalloc<double>(3 * (size_t)N);
+            }
+            for (int i = 0; i < c->p.anderson_memory; ++i) {
+                s->dX[i] = dalloc<double>(3 * (size_t)N);
+                s->dR[i] = dalloc<double>(3 * (size_t)N);
+            }
+        }
+        if (c->p.code == 2) {
+            for (int l = 0; l < 5; ++l) {
+                s->phi_hist[l] = dalloc<double>(N);
+                s->A_hist[l] = dalloc<double>(3 * (size_t)N);
+                s->rho_hist[l] = dalloc<double>(N);
+            }
+            s->fixed_ion = dalloc<double>(N);
+        }
+        // identity ids
+        std::vector<int64_t> ids((size_t)np);
+        for (int64_t i = 0; i < np; ++i) ids[i] = i;
+        HIPCHK(hipMemcpy(s->id, ids.data(), np * sizeof(int64_t), hipMemcpyHostToDevice));
+        launch_zero(c->stream, n3, s->vbar_last);
+    });
+    if (rc != UPIC_OK) {
+        sim_free(s);
+        delete s;
+        return (upic_status)rc;
+    }
+    *out = s;
+    return UPIC_OK;
+}
+
+upic_status upic_sim_destroy(upic_sim* s) {
+    if (!s) return UPIC_OK;
+    if (s->c && s->c->stream) (void)hipStreamSynchronize(s->c->stream);
+    sim_free(s);
+    delete s;
+    return UPIC_OK;
+}
+
+upic_status upic_sim_upload(upic_sim* s, const upic_host_state* st) {
+    if (!s || !st || st->np != s->np) return UPIC_EINVAL;
+    upic_ctx* c = s->c;
+    return (upic_status)guarded(c, [&] {
+        const int N = c->N();
+        const size_t n3 = 3 * (size_t)s->np * sizeof(double), n1 = (size_t)s->np * sizeof(double);
+        auto up = [&](double* d, const double* h, size_t bytes) {
+            if (h) HIPCHK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
+        };
+        up(s->x, st->x, n3); up(s->v, st->v, n3); up(s->v_prev, st->v_prev, n3); up(s->P, st->P, n3);
+        up(s->vbar_last, st->vbar_last, n3); up(s->q, st->q, n1); up(s->m, st->m, n1);
+        up(s->phi, st->phi, N * sizeof(double)); up(s->psi, st->psi, N * sizeof(double)); up(s->rho, st->rho, N * sizeof(double));
+        up(s->A, st->A, 3 * N * sizeof(double)); up(s->U, st->U, 3 * N * sizeof(double)); up(s->J, st->J, 3 * N * sizeof(double));
+        if (c->p.code == 2) {
+            for (int l = 0; l < 5; ++l) {
+                if (st->phi_hist) up(s->phi_hist[l], st->phi_hist + (size_t)l * N, N * sizeof(double));
+                if (st->A_hist) up(s->A_hist[l], st->A_hist + (size_t)l * 3 * N, 3 * N * sizeof(double));
+                if (st->rho_hist) up(s->rho_hist[l], st->rho_hist + (size_t)l * N, N * sizeof(double));
+            }
+            up(s->fixed_ion, st->fixed_ion_rho, N * sizeof(double));
+        }
+        std::vector<int64_t> ids((size_t)s->np);
+        for (int64_t i = 0; i < s->np; ++i) ids[i] = i;
+        HIPCHK(hipMemcpy(s->id, ids.data(), s->np * sizeof(int64_t), hipMemcpyHostToDevice));
+        s->steps = 0;
+    });
+}
+
+upic_status upic_sim_download(upic_sim* s, upic_host_state* st) {
+    if (!s || !st || st->np != s->np) return UPIC_EINVAL;
+    upic_ctx* c = s->c;
+    return (upic_status)guarded(c, [&] {
+        c->sync();
+        const int N = c->N();
+        const int64_t np = s->np;
+        std::vector<int64_t> ids((size_t)np);
+        HIPCHK(hipMemcpy(ids.data(), s->id, np * sizeof(int64_t), hipMemcpyDeviceToHost));
+        std::vector<double> tmp(3 * (size_t)np);
+        // particles back in their original order
+        auto down = [&](const double* d, double* h, int ncomp) {
+            if (!h) return;
+            HIPCHK(hipMemcpy(tmp.data(), d, ncomp * np * sizeof(double), hipMemcpyDeviceToHost));
+            for (int cc = 0; cc < ncomp; ++cc)
+                for (int64_t i = 0; i < np; ++i) h[cc * np + ids[i]] = tmp[cc * np + i];
+        };
+        down(s->x, st->x, 3); down(s->v, st->v, 3); down(s->v_prev, st->v_prev, 3); down(s->P, st->P, 3);
+        down(s->vbar_last, st->vbar_last, 3); down(s->q, st->q, 1); down(s->m, st->m, 1);
+        auto dm = [&](const double* d, double* h, size_t n) {
+            if (h) HIPCHK(hipMemcpy(h, d, n * sizeof(double), hipMemcpyDeviceToHost));
+        };
+        dm(s->phi, st->phi, N); dm(s->psi, st->psi, N); dm(s->rho, st->rho, N);
+        dm(s->A, st->A, 3 * (size_t)N); dm(s->U, st->U, 3 * (size_t)N); dm(s->J, st->J, 3 * (size_t)N);
+        if (c->p.code == 2) {
+            for (int l = 0; l < 5; ++l) {
+                if (st->phi_hist) dm(s->phi_hist[l], st->phi_hist + (size_t)l * N, N);
+                if (st->A_hist) dm(s->A_hist[l], st->A_hist + (size_t)l * 3 * N, 3 * (size_t)N);
+                if (st->rho_hist) dm(s->rho_hist[l], st->rho_hist + (size_t)l * N, N);
+            }
+        }
+    });
+}
+
+upic_status upic_step_p1(upic_sim* s, upic_step_info* info) {
+    if (!s) return UPIC_EINVAL;
+    if (s->c->p.code != 1) { s->c->err = "upic_step_p1 needs a P1 context"; return UPIC_EINVAL; }
+    return (upic_status)guarded(s->c, [&] { p1_step(s, info); });
+}
+upic_status upic_step_p3(upic_sim* s, upic_step_info* info) {
+    if (!s) return UPIC_EINVAL;
+    if (s->c->p.code != 3) { s->c->err = "upic_step_p3 needs a P3 context"; return UPIC_EINVAL; }
+    return (upic_status)guarded(s->c, [&] { p3_step(s, info); });
+}
+upic_status upic_step_p2(upic_sim* s, upic_step_info* info) {
+    if (!s) return UPIC_EINVAL;
+    if (s->c->p.code != 2) { s->c->err = "upic_step_p2 needs a P2 context"; return UPIC_EINVAL; }
+    return (upic_status)guarded(s->c, [&] { p2_step(s, info); });
+}
+
+upic_status upic_sim_init_fields(upic_sim* s, double n0) {
+    if (!s) return UPIC_EINVAL;
+    upic_ctx* c = s->c;
+    return (upic_status)guarded(c, [&] {
+        const upic_params& p = c->p;
+        const int N = c->N();
+        const int64_t np = s->np;
+        if (p.code == 2) {  // run_simulation set-up (P2:1119-1131): rho0 (naive), J0, phi0, A = 0, histories
+            P2Args pa{};
+            pa.np = np; pa.nx = p.nx; pa.ny = p.ny; pa.Lx = p.Lx; pa.Ly = p.Ly;
+            pa.dx = p.Lx / p.nx; pa.dy = p.Ly / p.ny; pa.dA = pa.dx * pa.dy; pa.dt = 0.0; pa.c = p.c;
+            pa.x = s->x; pa.v = s->v; pa.v_prev = s->v_prev; pa.P = s->P; pa.q = s->q; pa.m = s->m;
+            launch_copy(c->stream, N, s->fixed_ion, s->rho);
+            launch_zero(c->stream, 3ll * N, s->J);
+            pa.J = s->J;
+            pa.rho = s->rho;
+            launch_p2_advance_deposit(c->stream, pa);
+            to_spectra(c, s->rho, Zf(s->zs, 0, N), 1);
+            launch_spectral_poisson(c->stream, c->sp, 1.0 / p.eps0, Zf(s->zs, 0, N), Zf(s->zs, 1, N));
+            from_spectra(c, Zf(s->zs, 1, N), s->phi, 1);
+            launch_zero(c->stream, 3ll * N, s->A);
+            for (int l = 0; l < 5; ++l) {
+                launch_copy(c->stream, N, s->phi, s->phi_hist[l]);
+                launch_copy(c->stream, N, s->rho, s->rho_hist[l]);
+                launch_zero(c->stream, 3ll * N, s->A_hist[l]);
+            }
+            HIPCHK(hipGetLastError());
+            c->sync();
+            s->steps = 0;
+            return;
+        }
+        // rho = deposit(q)/dV (projected) + n0, mean removed
+        launch_zero(c->stream, N, s->rho);
+        launch_deposit_scalar_point(c->stream, c->g, s->x, s->q, np, 1.0 / c->g.dV, s->rho);
+        launch_project_nyquist(c->stream, c->sp, s->rho, 1);
+        {
+            launch_fill(c->stream, N, n0, s->gtmp);
+            LinComb l2{2, {1.0, 1.0}, {s->rho, s->gtmp}};
+            launch_lincomb(c->stream, N, l2, s->rho);
+        }
+        launch_subtract_mean(c->stream, s->rho, 1, N, c->dred + 60);
+        // point current: an orbit of zero length integrates to the point value
+        launch_zero(c->stream, 3 * np, s->xo);
+        launch_zero(c->stream, 3ll * N, s->J);
+        launch_deposit_orbit(c->stream, c->g, c->q, c->opt, s->x, s->xo, s->v, s->q, np, s->J, nullptr);
+        launch_project_nyquist(c->stream, c->sp, s->J, 3);
+        // phi
+        to_spectra(c, s->rho, Zf(s->zs, 0, N), 1);
+        launch_spectral_poisson(c->stream, c->sp, c->phi_src(), Zf(s->zs, 0, N), Zf(s->zs, 1, N));
+        launch_copy(c->stream, 2ll * N, Zf(s->zs, 1, N), Zf(s->zs, 2, N));
+        from_spectra(c, Zf(s->zs, 2, N), s->phi, 1);
+        launch_zero(c->stream, N, s->psi);
+        launch_zero(c->stream, 3ll * N, s->A);
+        launch_zero(c->stream, 3ll * N, s->U);
+        // E = -grad phi (U = 0), B = 0
+        launch_spectral_grad(c->stream, c->sp, Zf(s->zs, 1, N), Zf(s->zs, 3, N));
+        from_spectra(c, Zf(s->zs, 3, N), s->gtmp, 3);
+        launch_axpby(c->stream, 3ll * N, 0.0, nullptr, -1.0, s->gtmp);
+        launch_zero(c->stream, 6ll * N, s->gtmp + 3ll * N);  // B = 0 | A = 0
+        launch_init_momenta(c->stream, c->g, np, p.code, p.c, p.dt, s->x, s->v, s->q, s->m, s->gtmp,
+                            s->gtmp + 3ll * N, s->gtmp + 6ll * N, s->P, s->v_prev, s->vbar_last);
+        HIPCHK(hipGetLastError());
+        c->sync();
+        s->steps = 0;
+    });
+}
+
+upic_status upic_sim_device_state(upic_sim* s, upic_host_state* d) {
+    if (!s || !d) return UPIC_EINVAL;
+    std::memset(d, 0, sizeof(*d));
+    d->np = s->np;
+    d->x = s->x; d->v = s->v; d->v_prev = s->v_prev; d->P = s->P; d->vbar_last = s->vbar_last;
+    d->q = s->q; d->m = s->m;
+    d->phi = s->phi; d->psi = s->psi; d->rho = s->rho; d->A = s->A; d->U = s->U; d->J = s->J;
+    d->fixed_ion_rho = s->fixed_ion;
+    return UPIC_OK;
+}
+
+upic_status upic_sim_failure_reasons(upic_sim* s, uint8_t* reasons) {
+    if (!s || !reasons || !s->reason) return UPIC_EINVAL;
+    return (upic_status)guarded(s->c, [&] {
+        s->c->sync();
+        std::vector<int64_t> ids((size_t)s->np);
+        HIPCHK(hipMemcpy(ids.data(), s->id, s->np * sizeof(int64_t), hipMemcpyDeviceToHost));
+        std::vector<uint8_t> r((size_t)s->np);
+        HIPCHK(hipMemcpy(r.data(), s->reason, (size_t)s->np, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < s->np; ++i) reasons[ids[i]] = r[i];
+    });
+}
+
+// compute_diagnostics (P1:1057-1087, P3:3521-3574) on the resident state.
+upic_status upic_sim_diagnostics(upic_sim* s, int mode, upic_diagnostics* out) {
+    if (!s || !out) return UPIC_EINVAL;
+    upic_ctx* c = s->c;
+    return (upic_status)guarded(c, [&] {
+        if (c->p.code == 2) { p2_diagnostics(s, out); return; }
+        const upic_params& p = c->p;
+        const int N = c->N();
+        const int64_t n3 = 3ll * N;
+        double* z = s->zs;        // 12 complex fields
+        double* r = s->gtmp;      // 12 real fields
+        double* d = c->dred + 32; // 24 accumulators
+        launch_zero(c->stream, 24, d);
+        // E = -grad phi - U
+        to_spectra(c, s->phi, Zf(z, 0, N), 1);
+        launch_spectral_grad(c->stream, c->sp, Zf(z, 0, N), Zf(z, 1, N));
+        from_spectra(c, Zf(z, 1, N), r, 3);
+        for (int k = 0; k < 3; ++k) {
+            LinComb lc{2, {-1.0, -1.0}, {r + (size_t)k * N, s->U + (size_t)k * N}};
+            launch_lincomb(c->stream, N, lc, r + (size_t)k * N);
+        }
+        // E spectra (kept: E_x mode), divE
+        to_spectra(c, r, Zf(z, 1, N), 3);
+        launch_spectral_div(c->stream, c->sp, Zf(z, 1, N), Zf(z, 4, N));
+        launch_sum_squares(c->stream, n3, r, d + 0);  // sum E^2
+        from_spectra(c, Zf(z, 4, N), r + 3 * (size_t)N, 1);  // divE real
+        const double s1 = p.code == 3 ? p.sigma1 : 1.0 / p.eps0;
+        {
+            LinComb g{2, {1.0, -s1}, {r + 3 * (size_t)N, s->rho}};
+            if (p.code == 1) { g.c[1] = -1.0 / p.eps0; }
+            launch_lincomb(c->stream, N, g, r + 4 * (size_t)N);  // gauss
+        }
+        launch_sum_squares(c->stream, N, r + 4 * (size_t)N, d + 1);
+        launch_max_abs(c->stream, N, r + 4 * (size_t)N, d + 2);
+        launch_sum_squares(c->stream, N, r + 3 * (size_t)N, d + 3);  // divE
+        launch_sum_squares(c->stream, N, s->rho, d + 4);
+        // divA, gauge, B
+        to_spectra(c, s->A, Zf(z, 4, N), 3);
+        launch_spectral_div(c->stream, c->sp, Zf(z, 4, N), Zf(z, 7, N));
+        launch_spectral_curl(c->stream, c->sp, Zf(z, 4, N), Zf(z, 8, N));  // B spectra in 8..10
+        from_spectra(c, Zf(z, 7, N), r + 5 * (size_t)N, 1);  // divA real
+        {
+            LinComb g{2, {1.0 / (p.c * p.c), 1.0}, {s->psi, r + 5 * (size_t)N}};
+            launch_lincomb(c->stream, N, g, r + 6 * (size_t)N);  // gauge
+        }
+        launch_sum_squares(c->stream, N, r + 6 * (size_t)N, d + 5);
+        launch_max_abs(c->stream, N, r + 6 * (size_t)N, d + 6);
+        launch_sum_squares(c->stream, N, s->psi, d + 7);
+        launch_sum_squares(c->stream, N, r + 5 * (size_t)N, d + 8);
+        // Bz spectrum mode before the inverse transform
+        const int imode = ((mode % p.nx) + p.nx) % p.nx;
+        const int jmode = ((mode % p.ny) + p.ny) % p.ny;
+        const int idx = imode * p.ny * p.nz, idy = jmode * p.nz;
+        double modes[6];
+        HIPCHK(hipMemcpyAsync(&c->hred[40], Zf(z, 10, N) + 2 * (size_t)idy, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(&c->hred[42], Zf(z, 1, N) + 2 * (size_t)idx, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        from_spectra(c, Zf(z, 8, N), r + 7 * (size_t)N, 3);  // B real
+        launch_sum_squares(c->stream, n3, r + 7 * (size_t)N, d + 9);
+        // rho spectrum mode, mean
+        to_spectra(c, s->rho, Zf(z, 0, N), 1);
+        HIPCHK(hipMemcpyAsync(&c->hred[44], Zf(z, 0, N) + 2 * (size_t)idx, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        launch_sum(c->stream, N, s->rho, d + 10);
+        launch_kinetic(c->stream, s->np, s->v, s->m, p.c, p.code == 3 ? 1 : 0, d + 11);
+        HIPCHK(hipMemcpyAsync(c->hred, d, 16 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        c->sync();
+        std::memcpy(modes, &c->hred[40], sizeof(modes));
+        const double* h = c->hred;
+        upic_diagnostics& o = *out;
+        std::memset(&o, 0, sizeof(o));
+        const double invN = 1.0 / (double)N;
+        o.gauss_rms = std::sqrt(h[1] / N);
+        o.gauss_max = h[2];
+        o.divE_rms = std::sqrt(h[3] / N);
+        o.rho_rms = std::abs(s1) * std::sqrt(h[4] / N);
+        o.gauge_rms = std::sqrt(h[5] / N);
+        o.gauge_max = h[6];
+        o.psi_over_c2_rms = std::sqrt(h[7] / N) / (p.c * p.c);
+        o.divA_rms = std::sqrt(h[8] / N);
+        if (p.code == 3) o.field_energy = (0.5 / p.sigma1 * h[0] + 0.5 / p.sigma2 * h[9]) * c->g.dV;
+        else o.field_energy = (0.5 * p.eps0 * h[0] + 0.5 / p.mu0 * h[9]) * c->g.dV;
+        o.kinetic_energy = h[11];
+        o.total_energy = o.field_energy + o.kinetic_energy;
+        o.Bz_y_mode_re = p.code == 3 ? modes[0] * invN : 0.0;
+        o.Bz_y_mode_im = p.code == 3 ? modes[1] * invN : 0.0;
+        o.Bz_y_mode_abs = p.code == 3 ? std::hypot(modes[0], modes[1]) * invN : 0.0;
+        o.E_mode_re = modes[2] * invN;
+        o.E_mode_im = modes[3] * invN;
+        o.E_mode_abs = std::hypot(modes[2], modes[3]) * invN;
+        o.rho_mode = std::hypot(modes[4], modes[5]) * invN;
+        o.mean_rho = h[10] / N;
+    });
+}
+
+}  // extern "C"

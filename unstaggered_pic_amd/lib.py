@@ -1,0 +1,289 @@
+"""ctypes binding of libupic.so (the HIP library) and a thin host wrapper
+around the device-resident simulation handle.
+
+The product path runs only on the HIP library: importing this module without
+the built library, or using it without a HIP device, raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import Diagnostics, HostState, LoadSpec, Params, PathStats, StepInfo, STATUS_NAMES
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libupic.so")
+
+dp = C.POINTER(C.c_double)
+vp = C.c_void_p
+
+
+class UpicError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    """Load libupic.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library missing: {LIB_PATH} (run python -m unstaggered_pic_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        L.upic_last_error.restype = C.c_char_p
+        L.upic_last_error.argtypes = [vp]
+        sigs = {
+            "upic_ctx_create": [C.POINTER(Params), C.POINTER(vp)],
+            "upic_ctx_destroy": [vp], "upic_ctx_sync": [vp],
+            "upic_quadrature": [vp, dp, dp],
+            "upic_deposit_current_orbit": [vp, vp, vp, vp, vp, C.c_int64, vp, C.POINTER(PathStats)],
+            "upic_gather_orbit": [vp, vp, vp, vp, C.c_int64, vp],
+            "upic_orbit_grad_A": [vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_int64, vp],
+            "upic_gather_point": [vp, vp, vp, C.c_int64, vp],
+            "upic_deposit_scalar_point": [vp, vp, vp, C.c_int64, C.c_int, vp],
+            "upic_fft": [vp, vp, C.c_int],
+            "upic_spectral_gradient": [vp, vp, vp], "upic_spectral_divergence": [vp, vp, vp],
+            "upic_spectral_curl": [vp, vp, vp],
+            "upic_continuity": [vp, vp, vp, C.c_double, vp],
+            "upic_cn_potentials": [vp, vp, vp, vp, vp, vp, vp, C.c_double, vp, vp, vp, vp],
+            "upic_project_nyquist": [vp, vp, C.c_int], "upic_poisson": [vp, vp, vp],
+            "upic_sim_create": [vp, C.c_int64, C.POINTER(vp)], "upic_sim_destroy": [vp],
+            "upic_sim_upload": [vp, C.POINTER(HostState)], "upic_sim_download": [vp, C.POINTER(HostState)],
+            "upic_step_p1": [vp, C.POINTER(StepInfo)], "upic_step_p2": [vp, C.POINTER(StepInfo)],
+            "upic_step_p3": [vp, C.POINTER(StepInfo)],
+            "upic_sim_diagnostics": [vp, C.c_int, C.POINTER(Diagnostics)],
+            "upic_sim_device_state": [vp, C.POINTER(HostState)],
+            "upic_sim_failure_reasons": [vp, C.POINTER(C.c_uint8)],
+            "upic_sim_init_fields": [vp, C.c_double],
+            "upic_load_particles": [C.POINTER(Params), C.POINTER(LoadSpec), C.POINTER(HostState)],
+        }
+        for name, args in sigs.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    """Names declared in include/upic.h (checked against the library)."""
+    hdr = os.path.join(os.path.dirname(HERE), "include", "upic.h")
+    import re
+    names = []
+    for line in open(hdr):
+        m = re.match(r"^(?:upic_status|const char\*)\s+(upic_\w+)\(", line)
+        if m:
+            names.append(m.group(1))
+    return names
+
+
+class Context:
+    """Grid, quadrature, stream (upic_ctx)."""
+
+    def __init__(self, params: Params):
+        import torch  # device memory comes from torch's HIP allocator
+        if not torch.cuda.is_available():
+            raise RuntimeError("no HIP device: the unstaggered_pic_amd hot path needs an MI355X")
+        self.torch = torch
+        self.params = params
+        self.h = vp()
+        self._check(lib().upic_ctx_create(C.byref(params), C.byref(self.h)), None)
+
+    def _check(self, rc, h="self"):
+        if rc != 0:
+            hh = self.h if h == "self" else None
+            raise UpicError(rc, lib().upic_last_error(hh).decode())
+
+    def sync(self):
+        self._check(lib().upic_ctx_sync(self.h))
+
+    def quadrature(self):
+        n = self.params.orbit_quad_order
+        s, w = np.zeros(n), np.zeros(n)
+        self._check(lib().upic_quadrature(self.h, s.ctypes.data_as(dp), w.ctypes.data_as(dp)))
+        return s, w
+
+    def close(self):
+        if self.h:
+            lib().upic_ctx_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- group A on torch device tensors --------------------------------
+    def dev(self, a):
+        t = self.torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).to("cuda")
+        return t
+
+    @staticmethod
+    def ptr(t):
+        return C.c_void_p(t.data_ptr()) if t is not None else None
+
+    def N(self):
+        p = self.params
+        return p.nx * p.ny * p.nz
+
+    def empty(self, n):
+        return self.torch.empty(n, dtype=self.torch.float64, device="cuda")
+
+    def deposit_current_orbit(self, x0, dx, vbar, q):
+        n = q.numel()
+        J = self.empty(3 * self.N())
+        st = PathStats()
+        self._check(lib().upic_deposit_current_orbit(self.h, self.ptr(x0), self.ptr(dx), self.ptr(vbar), self.ptr(q),
+                                                     n, self.ptr(J), C.byref(st)))
+        return J, st
+
+    def gather_orbit(self, x0, dx, f3):
+        n = x0.numel() // 3
+        out = self.empty(3 * n)
+        self._check(lib().upic_gather_orbit(self.h, self.ptr(x0), self.ptr(dx), self.ptr(f3), n, self.ptr(out)))
+        self.sync()
+        return out
+
+    def orbit_grad_A(self, x0, dx, An, Anp1, th0=0.0, th1=1.0):
+        n = x0.numel() // 3
+        out = self.empty(9 * n)
+        self._check(lib().upic_orbit_grad_A(self.h, self.ptr(x0), self.ptr(dx), self.ptr(An), self.ptr(Anp1), th0, th1,
+                                            n, self.ptr(out)))
+        self.sync()
+        return out
+
+    def gather_point(self, pos, f3):
+        n = pos.numel() // 3
+        out = self.empty(3 * n)
+        self._check(lib().upic_gather_point(self.h, self.ptr(pos), self.ptr(f3), n, self.ptr(out)))
+        self.sync()
+        return out
+
+    def deposit_scalar_point(self, pos, values, divide=True):
+        out = self.empty(self.N())
+        self._check(lib().upic_deposit_scalar_point(self.h, self.ptr(pos), self.ptr(values), values.numel(), int(divide),
+                                                    self.ptr(out)))
+        self.sync()
+        return out
+
+    def fft(self, z, inverse=False):
+        self._check(lib().upic_fft(self.h, self.ptr(z), int(inverse)))
+        self.sync()
+        return z
+
+    def _unary(self, name, inp, nout):
+        out = self.empty(nout)
+        self._check(getattr(lib(), name)(self.h, self.ptr(inp), self.ptr(out)))
+        self.sync()
+        return out
+
+    def spectral_gradient(self, f):
+        return self._unary("upic_spectral_gradient", f, 3 * self.N())
+
+    def spectral_divergence(self, V):
+        return self._unary("upic_spectral_divergence", V, self.N())
+
+    def spectral_curl(self, A):
+        return self._unary("upic_spectral_curl", A, 3 * self.N())
+
+    def poisson(self, rho):
+        return self._unary("upic_poisson", rho, self.N())
+
+    def continuity(self, rho, J, dt):
+        out = self.empty(self.N())
+        self._check(lib().upic_continuity(self.h, self.ptr(rho), self.ptr(J), dt, self.ptr(out)))
+        self.sync()
+        return out
+
+    def cn_potentials(self, phi, psi, A, U, rho_mid, J, dt):
+        N = self.N()
+        o = [self.empty(N), self.empty(N), self.empty(3 * N), self.empty(3 * N)]
+        self._check(lib().upic_cn_potentials(self.h, *[self.ptr(t) for t in (phi, psi, A, U, rho_mid, J)], dt,
+                                             *[self.ptr(t) for t in o]))
+        self.sync()
+        return o
+
+    def project_nyquist(self, f, ncomp=1):
+        f = f.clone()
+        self._check(lib().upic_project_nyquist(self.h, self.ptr(f), ncomp))
+        self.sync()
+        return f
+
+
+class Simulation:
+    """Device-resident particle and mesh state with the per-step drivers."""
+
+    FIELDS_P = ("x", "v", "v_prev", "P", "vbar_last")
+
+    def __init__(self, ctx: Context, np_):
+        self.ctx = ctx
+        self.np = np_
+        self.h = vp()
+        ctx._check(lib().upic_sim_create(ctx.h, np_, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().upic_sim_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _host(self, st):
+        h = HostState()
+        h.np = self.np
+        for f in ("x", "v", "v_prev", "P", "vbar_last", "q", "m", "phi", "psi", "rho", "A", "U", "J",
+                  "phi_hist", "A_hist", "rho_hist", "fixed_ion_rho"):
+            a = getattr(st, f, None)
+            setattr(h, f, a.ctypes.data_as(dp) if a is not None else None)
+        return h
+
+    def upload(self, st):
+        h = self._host(st)
+        self.ctx._check(lib().upic_sim_upload(self.h, C.byref(h)))
+
+    def download(self, st):
+        h = self._host(st)
+        self.ctx._check(lib().upic_sim_download(self.h, C.byref(h)))
+        return st
+
+    def step(self):
+        code = self.ctx.params.code
+        info = StepInfo()
+        f = {1: lib().upic_step_p1, 2: lib().upic_step_p2, 3: lib().upic_step_p3}[code]
+        self.ctx._check(f(self.h, C.byref(info)))
+        return info
+
+    def diagnostics(self, mode=1):
+        d = Diagnostics()
+        self.ctx._check(lib().upic_sim_diagnostics(self.h, mode, C.byref(d)))
+        return d
+
+    def init_fields(self, n0=1.0):
+        self.ctx._check(lib().upic_sim_init_fields(self.h, n0))
+
+    def failure_reasons(self):
+        r = np.zeros(self.np, np.uint8)
+        self.ctx._check(lib().upic_sim_failure_reasons(self.h, r.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return r
+
+
+def load_particles(params: Params, spec, np_):
+    """Seeded particle load (host C++): returns x, v, q, m numpy arrays."""
+    x, v = np.zeros(3 * np_), np.zeros(3 * np_)
+    q, m = np.zeros(np_), np.zeros(np_)
+    h = HostState()
+    h.np = np_
+    h.x, h.v, h.q, h.m = (a.ctypes.data_as(dp) for a in (x, v, q, m))
+    rc = lib().upic_load_particles(C.byref(params), C.byref(spec), C.byref(h))
+    if rc != 0:
+        raise UpicError(rc, "upic_load_particles rejected its arguments")
+    return x, v, q, m
