@@ -1,0 +1,197 @@
+"""Benchmark of the per-timestep PIC hot path on one MI355X.
+
+Workload (the Weibel configuration the headline metric is quoted on):
+P3 form (relativistic GM-HC-CN, outer Anderson on the mesh current, inner
+per-particle Picard), 2-D periodic 256 x 256 grid on [0, 20 pi)^2, 256
+electrons per cell (16,777,216 particles), bi-Maxwellian momenta
+u_th = (0.05, 0.2, 0.2) c, uniform density, cubic B-splines, omega_p dt =
+0.05, Nyquist projection on, tolerances of the reference's Weibel manuscript
+deck (Project3/inputs/manuscript/weibel_main_dt02_T100.txt).  Particles are
+generated by the library's seeded loader (synthetic, seed 12345).
+
+One step = one upic_step_p3 call (a full nonlinear time step).  The timed
+region covers exactly --steps steps bracketed by device synchronisation;
+inputs are resident in HBM before it starts.
+
+Prints one JSON line (see the keys below).  --gpus N with N > 1 exits: this
+build runs on one GPU.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, vendor figure
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector, vendor figure
+
+
+def workload_params(nx=256, ppc=256, precision=0):
+    from unstaggered_pic_amd.abi import CODE_P3, make_params
+    L = 20.0 * np.pi
+    return make_params(code=CODE_P3, nx=nx, ny=nx, nz=1, Lx=L, Ly=L, Lz=1.0, dt=0.05, spline_order=3,
+                       orbit_quad_order=4, split_orbit_at_knots=1, nyquist_projection=1, c=1.0, sigma1=1.0,
+                       sigma2=1.0, nonlinear_solver_picard=0, nonlinear_max_iter=80, nonlinear_rtol=1e-11,
+                       nonlinear_atol=1e-13, anderson_memory=4, anderson_start=2, anderson_damping=0.5,
+                       anderson_regularization=1e-12, anderson_growth_limit=1.2, local_rtol=1e-12,
+                       local_atol=1e-14, local_max_iter=64, local_damping=1.0, precision=precision,
+                       sort_interval=1)
+
+
+def workload_load(ppc, seed=12345):
+    from unstaggered_pic_amd.abi import LOAD_BIMAXWELL, make_load
+    return make_load(LOAD_BIMAXWELL, ppc, seed=seed, n0=1.0, vth=(0.05, 0.2, 0.2))
+
+
+class HostArrays:
+    def __init__(self, np_, N):
+        self.x, self.v = np.zeros(3 * np_), np.zeros(3 * np_)
+        self.v_prev, self.P, self.vbar_last = np.zeros(3 * np_), np.zeros(3 * np_), np.zeros(3 * np_)
+        self.q, self.m = np.zeros(np_), np.zeros(np_)
+        self.phi, self.psi, self.rho = np.zeros(N), np.zeros(N), np.zeros(N)
+        self.A, self.U, self.J = np.zeros(3 * N), np.zeros(3 * N), np.zeros(3 * N)
+
+
+def cpu_baseline(nx, dt_budget=30.0):
+    """The reference's own FP64 step (compiled from its sources into
+    oracle/_ref/libref_p3.so) -- or, where that is absent, the oracle port --
+    timed on the host for one step of the same deck at 1 particle per cell."""
+    from unstaggered_pic_amd import lib
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_bind import Oracle, State, ptr
+    ppc = 1
+    p = workload_params(nx=nx, ppc=ppc)
+    np_ = nx * nx * ppc
+    x, v, q, m = lib.load_particles(p, workload_load(ppc), np_)
+    orc = Oracle()
+    st = orc.init_fields(p, x, v, q, m, 1.0)
+    refso = os.path.join(ROOT, "oracle", "_ref", "libref_p3.so")
+    kind = "port"
+    t0 = time.perf_counter()
+    if os.path.exists(refso):
+        L = C.CDLL(refso)
+        out = State(st.np, st.N)
+        it = C.c_int32()
+        info = np.zeros(3)
+        t0 = time.perf_counter()
+        rc = L.ref_p3_step(C.byref(p), C.byref(st.host()), C.byref(out.host()), C.byref(it), ptr(info))
+        el = time.perf_counter() - t0
+        if rc == 0:
+            kind = "reference"
+        else:
+            el = None
+    else:
+        el = None
+    if kind == "port":
+        t0 = time.perf_counter()
+        orc.step(p, st)
+        el = time.perf_counter() - t0
+    return {"value": np_ / el, "unit": "particle-steps/s", "cores": 1, "kind": kind,
+            "sample": f"one step of the same deck on the {nx}x{nx} grid at 1 particle/cell ({np_} particles), "
+                      f"serial FP64, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--ppc", type=int, default=256)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel event timing")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from unstaggered_pic_amd import lib
+    prec = 0 if args.precision == "fp64" else 1
+    p = workload_params(args.nx, args.ppc, prec)
+    N = p.nx * p.ny
+    np_ = N * args.ppc
+    x, v, q, m = lib.load_particles(p, workload_load(args.ppc), np_)
+    ctx = lib.Context(p)
+    sim = lib.Simulation(ctx, np_)
+    h = HostArrays(np_, N)
+    h.x[:], h.v[:], h.q[:], h.m[:] = x, v, q, m
+    sim.upload(h)
+    del h, x, v, q, m
+    sim.init_fields(1.0)
+    d0 = sim.diagnostics(1)
+    for _ in range(args.warmup):
+        sim.step()
+    ctx.set_timing(not args.no_timing)
+    infos = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        infos.append(sim.step())
+    ctx.sync()
+    el = time.perf_counter() - t0
+    launches, kms = ctx.timing("p3_map")
+    d1 = sim.diagnostics(1)
+    value = np_ * args.steps / el
+    # algorithmic HBM bytes of one p3_map launch: per particle the FP64 state
+    # it reads (x, P, v: 72 B; q, m: 16 B) -- the write-out of the end state
+    # (x, P, v, vbar_macro: 96 B) happens in one launch per step -- plus the
+    # mesh fields read (A_n, A_end, grad phi_mid: 9 N doubles) and the current
+    # written (3 N doubles).
+    maps = sum(i.map_evaluations for i in infos)
+    bytes_per_launch = np_ * 88.0 + (np_ * 96.0 * args.steps) / max(maps, 1) + 12.0 * 8 * N
+    avg_ms = kms / max(launches, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if launches else None
+    # FP64 operations of one inner iteration at r = 3, nq = 4 (counted from the
+    # kernel source: per Gauss point 2 axis stencils + 16 nodes x (3 gather +
+    # 12 D_A) FMAs; per iteration the end-point gather and the HC push).
+    segs = np.mean([i.path.total_segments / max(i.path.n_particles, 1) for i in infos])
+    flops_iter = segs * 4 * (2 * 4 * 24 + 16 * (2 * 15 + 6 * 2 + 3)) + 16 * 6 + 250
+    local_total = sum(i.local_iterations_total for i in infos)
+    compute_tflops = (flops_iter * local_total / args.steps) * (launches / max(launches, 1)) / (
+        kms * 1e-3 / args.steps) / 1e12 if launches else None
+    out = {
+        "metric": "particle-steps/s on 1 MI355X, mixed+learned vs plain FP64; rel. energy drift",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == 0 else "f32/f64",
+        "data": "synthetic (seeded bi-Maxwellian load, splitmix64 seed 12345)",
+        "config": {"workload": f"weibel_p3_{p.nx}x{p.ny}_ppc{args.ppc}_cubic_dt0.05", "particles": np_,
+                   "grid": [p.nx, p.ny], "spline_order": 3, "orbit_quad_order": 4,
+                   "precision": args.precision, "form": "P3 nested solve (Anderson outer, per-particle Picard inner)"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "kernel": "k_p3_map", "avg_launch_ms": avg_ms, "launches": launches,
+                     "bytes_per_launch": bytes_per_launch},
+        "compute_roofline": {"bound": "fp64_valu", "achieved": compute_tflops, "peak": FP64_PEAK_TFLOPS,
+                             "unit": "TFLOP/s",
+                             "frac": (compute_tflops / FP64_PEAK_TFLOPS) if compute_tflops else None,
+                             "flops_per_inner_iteration": flops_iter},
+        "physics": {"rel_total_energy_change": (d1.total_energy - d0.total_energy) / abs(d0.total_energy),
+                    "outer_iterations": [i.iterations for i in infos],
+                    "map_evaluations": [i.map_evaluations for i in infos],
+                    "mean_inner_iterations": float(np.mean([i.local_iterations_mean for i in infos])),
+                    "avg_path_segments": float(segs)},
+    }
+    sim.close()
+    ctx.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.nx)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

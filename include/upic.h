@@ -172,9 +172,25 @@ upic_status upic_ctx_destroy(upic_ctx* ctx);
 upic_status upic_ctx_sync(upic_ctx* ctx);
 /* Last error message of this context (static storage, valid until next call). */
 const char* upic_last_error(const upic_ctx* ctx);
+/* Kernel timing with HIP events on the context stream (no reference
+ * counterpart; the reference only times whole steps, P3:145-194).  When
+ * enabled, each launch of a timed kernel family ("p1_push", "p1_push_final",
+ * "p1_deposit_guess", "p3_map", "p2_deposit", "p2_push") is bracketed by
+ * events and synchronised; upic_ctx_timing returns the accumulated launches
+ * and milliseconds since the last upic_ctx_set_timing call. */
+upic_status upic_ctx_set_timing(upic_ctx* ctx, int enable);
+upic_status upic_ctx_timing(upic_ctx* ctx, const char* name, int64_t* launches, double* total_ms);
 /* Gauss-Legendre nodes/weights on [0,1] used by the context (host arrays of
  * orbit_quad_order entries). */
 upic_status upic_quadrature(const upic_ctx* ctx, double* s, double* w);
+
+/* ---- Device memory plumbing for FFI callers without a HIP allocator ------ */
+/* (no reference counterpart) */
+upic_status upic_device_count(int* count);
+upic_status upic_device_alloc(size_t bytes, void** ptr);
+upic_status upic_device_free(void* ptr);
+upic_status upic_memcpy_htod(void* dst, const void* src, size_t bytes);
+upic_status upic_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 
 /* ---- Group A: kernel-level entry points (device pointers) --------------- */
 

@@ -25,14 +25,14 @@ def gpu():
 
 
 def rel(a, b):
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
+    a = np.asarray(a)
+    b = np.asarray(b)
     s = max(np.max(np.abs(b)), 1e-300)
     return float(np.max(np.abs(a - b)) / s)
 
 
 def host(t):
-    return t.detach().cpu().numpy()
+    return t.numpy()
 
 
 def random_particles(rng, p, n, vscale):

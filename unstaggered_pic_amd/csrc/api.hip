@@ -52,6 +52,8 @@ upic_status upic_ctx_create(const upic_params* params, upic_ctx** out) {
             c->plan.tw[d] = dalloc<double>(n);
             HIPCHK(hipMemcpy(c->plan.tw[d], tw.data(), n * sizeof(double), hipMemcpyHostToDevice));
         }
+        HIPCHK(hipEventCreate(&c->ev0));
+        HIPCHK(hipEventCreate(&c->ev1));
         c->dred = dalloc<double>(64);
         c->dstats = dalloc<unsigned long long>(8);
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hred), 64 * sizeof(double), hipHostMallocDefault));
@@ -75,6 +77,8 @@ upic_status upic_ctx_destroy(upic_ctx* c) {
     dfree(c->work);
     if (c->hred) (void)hipHostFree(c->hred);
     if (c->hstats) (void)hipHostFree(c->hstats);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return UPIC_OK;
@@ -85,6 +89,21 @@ upic_status upic_ctx_sync(upic_ctx* c) {
     return (upic_status)guarded(c, [&] { c->sync(); });
 }
 
+upic_status upic_ctx_set_timing(upic_ctx* c, int enable) {
+    if (!c) return UPIC_EINVAL;
+    c->timing = enable != 0;
+    c->timers.clear();
+    return UPIC_OK;
+}
+
+upic_status upic_ctx_timing(upic_ctx* c, const char* name, int64_t* launches, double* total_ms) {
+    if (!c || !name || !launches || !total_ms) return UPIC_EINVAL;
+    auto it = c->timers.find(name);
+    *launches = it == c->timers.end() ? 0 : it->second.launches;
+    *total_ms = it == c->timers.end() ? 0.0 : it->second.ms;
+    return UPIC_OK;
+}
+
 const char* upic_last_error(const upic_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
 
 upic_status upic_quadrature(const upic_ctx* c, double* s, double* w) {
@@ -92,6 +111,30 @@ upic_status upic_quadrature(const upic_ctx* c, double* s, double* w) {
     std::memcpy(s, c->q.s, c->q.n * sizeof(double));
     std::memcpy(w, c->q.w, c->q.n * sizeof(double));
     return UPIC_OK;
+}
+
+// ---- device memory plumbing ---------------------------------------------
+upic_status upic_device_count(int* count) {
+    if (!count) return UPIC_EINVAL;
+    *count = 0;
+    if (hipGetDeviceCount(count) != hipSuccess) { *count = 0; (void)hipGetLastError(); }
+    return UPIC_OK;
+}
+upic_status upic_device_alloc(size_t bytes, void** ptr) {
+    if (!ptr) return UPIC_EINVAL;
+    *ptr = nullptr;
+    if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) { (void)hipGetLastError(); return UPIC_ENOMEM; }
+    return UPIC_OK;
+}
+upic_status upic_device_free(void* ptr) {
+    if (ptr && hipFree(ptr) != hipSuccess) return UPIC_EDEVICE;
+    return UPIC_OK;
+}
+upic_status upic_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? UPIC_OK : UPIC_EDEVICE;
+}
+upic_status upic_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? UPIC_OK : UPIC_EDEVICE;
 }
 
 // ---- Group A --------------------------------------------------------------

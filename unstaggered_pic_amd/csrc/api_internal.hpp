@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -108,6 +109,11 @@ struct upic_ctx {
     // group-A workspace, grown on demand
     double* work = nullptr;
     size_t work_n = 0;
+    // per-kernel-family timing with HIP events on `stream`
+    struct KTimer { int64_t launches = 0; double ms = 0.0; };
+    bool timing = false;
+    std::map<std::string, KTimer> timers;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     int N() const { return g.N; }
     double* workspace(size_t n) {
@@ -179,6 +185,22 @@ int guarded(upic_ctx* c, F&& f) {
         if (c) c->err = e.what();
         return UPIC_EINVAL;
     }
+}
+
+// Runs `launch` between two events on the context stream when timing is on
+// and accumulates the elapsed time under `name`.
+template <class F>
+void timed(upic_ctx* c, const char* name, F&& launch) {
+    if (!c->timing) { launch(); return; }
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    launch();
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    auto& t = c->timers[name];
+    t.launches += 1;
+    t.ms += ms;
 }
 
 // real fields -> spectra; spectra -> real (destroys the spectra)

@@ -140,6 +140,13 @@ __device__ __forceinline__ int mode_number(int i, int n, int code) {
 }
 __device__ __forceinline__ bool even_nyq(int i, int n) { return n > 1 && (n % 2 == 0) && i == n / 2; }
 
+// k[d] is the first-derivative symbol, k2 the Laplacian symbol.  P1 keeps
+// the Nyquist wave number +n/2 in k2 (P1:386-392); its first derivatives
+// are always followed by a transform back to a real grid function
+// (ifft_real, P1:494-499), which removes the imaginary Nyquist component --
+// the same as a zero derivative symbol there.  Using the zero symbol keeps
+// chained k-space operations (divergence -> continuity -> wave solve)
+// identical to the reference's real-space round trips.
 struct KVec {
     double k[3], k2;
     bool excluded;
@@ -148,10 +155,15 @@ __device__ __forceinline__ KVec kvec(const SpecArgs& sp, int id) {
     const int i = id / (sp.ny * sp.nz), j = (id / sp.nz) % sp.ny, kk = id % sp.nz;
     const double twopi = 6.283185307179586476925286766559;
     KVec o;
-    o.k[0] = twopi * (double)mode_number(i, sp.nx, sp.code) / sp.L[0];
-    o.k[1] = twopi * (double)mode_number(j, sp.ny, sp.code) / sp.L[1];
-    o.k[2] = (sp.code == 2) ? 0.0 : twopi * (double)mode_number(kk, sp.nz, sp.code) / sp.L[2];
-    o.k2 = o.k[0] * o.k[0] + o.k[1] * o.k[1] + o.k[2] * o.k[2];
+    const int mi = mode_number(i, sp.nx, sp.code), mj = mode_number(j, sp.ny, sp.code);
+    const int mk = (sp.code == 2) ? 0 : mode_number(kk, sp.nz, sp.code);
+    const double ki = twopi * (double)mi / sp.L[0];
+    const double kj = twopi * (double)mj / sp.L[1];
+    const double kz = (sp.code == 2) ? 0.0 : twopi * (double)mk / sp.L[2];
+    o.k2 = ki * ki + kj * kj + kz * kz;
+    o.k[0] = even_nyq(i, sp.nx) ? 0.0 : ki;
+    o.k[1] = even_nyq(j, sp.ny) ? 0.0 : kj;
+    o.k[2] = even_nyq(kk, sp.nz) ? 0.0 : kz;
     o.excluded = sp.projection && (even_nyq(i, sp.nx) || even_nyq(j, sp.ny) || even_nyq(kk, sp.nz));
     return o;
 }

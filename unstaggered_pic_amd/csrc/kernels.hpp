@@ -17,6 +17,7 @@ struct P1Args {
     const double *x, *v, *v_prev, *P, *q, *m;
     const double* vg;             // current velocity iterate v_guess
     const double *gpm, *A_n, *A_end;
+    const double* NR;             // 2-D packed node records (grad phi_mid, A^n, A^{n+1})
     double* vg_out;               // next iterate
     double* J_out;                // current of the next map evaluation (3*N, pre-zeroed)
     double* red;                  // [3] velocity-error sums
@@ -34,6 +35,7 @@ struct P3Args {
     const int8_t* layout;         // substeps per particle (NULL = 1)
     const double* guess;          // optional initial guess of vbar (NULL = v^n, P3:5318)
     const double *A_n, *A_end, *gpm;
+    const double* NR;             // 2-D packed node records
     double* J_out;                // deposit target (3*N, pre-zeroed) or NULL
     double *x_out, *P_out, *v_out, *vbar_out;
     uint8_t* reason;              // per-particle failure reason (optional)
@@ -71,6 +73,9 @@ void launch_p1_push(hipStream_t s, const GridDev& g, const QuadDev& q, const Orb
 void launch_p3_map(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const P3Args& a);
 void launch_p2_advance_deposit(hipStream_t s, const P2Args& a);
 void launch_p2_push(hipStream_t s, const P2Args& a);
+
+// 2-D node records: NR[8*id + ...] = grad phi_mid x, y; A^n x, y, z; A^{n+1} x, y, z
+void launch_pack_nodes(hipStream_t s, int N, const double* gpm, const double* A_n, const double* A_end, double* NR);
 
 // field_kernels.hip ---------------------------------------------------------
 struct FFTPlan {

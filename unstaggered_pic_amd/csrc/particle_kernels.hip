@@ -5,6 +5,7 @@
 // instantiated for spline degree R = 1..4 and for 2-D (nz = 1) or 3-D grids.
 #include "kernels.hpp"
 #include "particles.hpp"
+#include "orbit2d.hpp"
 
 namespace upic {
 
@@ -118,6 +119,9 @@ __global__ __launch_bounds__(256) void k_deposit_scalar_point(GridDev g, const d
 template <int R, bool Z3>
 __global__ __launch_bounds__(256) void k_p1_deposit_guess(GridDev g, QuadDev q, OrbitOpt opt, P1Args a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double tile_mem[kTile * kTile * 3];
+    DepTile T{};
+    if constexpr (!Z3) T = tile_begin<R>(g, tile_mem, a.x, a.np, (int64_t)blockIdx.x * blockDim.x);
     int nseg = 0;
     if (p < a.np) {
         double X[3], V[3], Vp[3], G[3], Vb[3], D[3];
@@ -129,8 +133,10 @@ __global__ __launch_bounds__(256) void k_p1_deposit_guess(GridDev g, QuadDev q, 
             D[d] = a.dt * Vb[d];
         }
         st3(a.vg_out, a.np, p, G);
-        nseg = deposit_orbit_one<R, Z3>(g, q, opt, X, D, Vb, a.q[p], a.J_out);
+        if constexpr (Z3) nseg = deposit_orbit_one<R, Z3>(g, q, opt, X, D, Vb, a.q[p], a.J_out);
+        else nseg = deposit_orbit_2d<R>(g, q, opt, X, D, Vb, a.q[p], T, a.J_out);
     }
+    if constexpr (!Z3) tile_end(g, T, a.J_out);
     path_stats_add(nseg, a.stats);
 }
 
@@ -142,6 +148,9 @@ __global__ __launch_bounds__(256) void k_p1_deposit_guess(GridDev g, QuadDev q, 
 template <int R, bool Z3, bool FINAL>
 __global__ __launch_bounds__(256) void k_p1_push(GridDev g, QuadDev q, OrbitOpt opt, P1Args a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double tile_mem[kTile * kTile * 3];
+    DepTile T{};
+    if constexpr (!Z3 && !FINAL) T = tile_begin<R>(g, tile_mem, a.x, a.np, (int64_t)blockIdx.x * blockDim.x);
     double err[3] = {0.0, 0.0, 0.0};
     int nseg = 0;
     if (p < a.np) {
@@ -158,9 +167,14 @@ __global__ __launch_bounds__(256) void k_p1_push(GridDev g, QuadDev q, OrbitOpt 
         Xe[1] = wrap1(X[1] + Dx[1], g.L[1]);
         Xe[2] = wrap1(X[2] + Dx[2], g.L[2]);
         double gpb[3], DA[3][3];
-        orbit_gather_grad<R, Z3, true, true>(g, q, opt, X, Dx, a.gpm, a.A_n, a.A_end, 0.0, 1.0, true, gpb, DA);
         double Ae[3];
-        gather3<R, Z3>(g, a.A_end, Xe, Ae);
+        if constexpr (Z3) {
+            orbit_gather_grad<R, Z3, true, true>(g, q, opt, X, Dx, a.gpm, a.A_n, a.A_end, 0.0, 1.0, true, gpb, DA);
+            gather3<R, Z3>(g, a.A_end, Xe, Ae);
+        } else {
+            orbit_gg_2d<R>(g, q, opt, X, Dx, a.NR, 0.0, 1.0, true, gpb, DA);
+            gather_A_rec<R>(g, a.NR, 1, Xe, Ae);
+        }
         double Pe[3], Vn[3];
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
@@ -190,9 +204,11 @@ __global__ __launch_bounds__(256) void k_p1_push(GridDev g, QuadDev q, OrbitOpt 
                 Vb2[d] = 0.5 * (Vn[d] + V[d]);
                 D2[d] = a.dt * Vb2[d];
             }
-            nseg = deposit_orbit_one<R, Z3>(g, q, opt, X, D2, Vb2, qp, a.J_out);
+            if constexpr (Z3) nseg = deposit_orbit_one<R, Z3>(g, q, opt, X, D2, Vb2, qp, a.J_out);
+            else nseg = deposit_orbit_2d<R>(g, q, opt, X, D2, Vb2, qp, T, a.J_out);
         }
     }
+    if constexpr (!Z3 && !FINAL) tile_end(g, T, a.J_out);
     if constexpr (!FINAL) {
         block_reduce_add<3>(err, a.red);
         path_stats_add(nseg, a.stats);
@@ -249,6 +265,19 @@ __device__ __forceinline__ void gather_time_interp(const GridDev& g, const doubl
     for (int d = 0; d < 3; ++d) out[d] = (1.0 - theta) * a0[d] + theta * a1[d];
 }
 
+// 2-D time-interpolated gather from the node records
+template <int R>
+__device__ __forceinline__ void gather_interp_rec(const GridDev& g, const double* NR, double theta, const double x[3],
+                                                  double out[3]) {
+    if (theta == 0.0) { gather_A_rec<R>(g, NR, 0, x, out); return; }
+    if (theta == 1.0) { gather_A_rec<R>(g, NR, 1, x, out); return; }
+    double a0[3], a1[3];
+    gather_A_rec<R>(g, NR, 0, x, a0);
+    gather_A_rec<R>(g, NR, 1, x, a1);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[d] = (1.0 - theta) * a0[d] + theta * a1[d];
+}
+
 struct LocalMap {
     double vbar[3], P_end[3], v_end[3];
 };
@@ -264,10 +293,15 @@ __device__ __forceinline__ void p3_local_map(const GridDev& g, const QuadDev& qd
     const double dxp[3] = {h * gs[0], h * gs[1], h * gs[2]};
     const double x1[3] = {wrap1(x0[0] + dxp[0], g.L[0]), wrap1(x0[1] + dxp[1], g.L[1]), wrap1(x0[2] + dxp[2], g.L[2])};
     double A1p[3];
-    gather_time_interp<R, Z3>(g, a.A_n, a.A_end, th1, x1, A1p);
-    const double Aep[3] = {0.5 * (A0p[0] + A1p[0]), 0.5 * (A0p[1] + A1p[1]), 0.5 * (A0p[2] + A1p[2])};
     double gpb[3], D[3][3];
-    orbit_gather_grad<R, Z3, true, true>(g, qd, opt, x0, dxp, a.gpm, a.A_n, a.A_end, th0, th1, false, gpb, D);
+    if constexpr (Z3) {
+        gather_time_interp<R, Z3>(g, a.A_n, a.A_end, th1, x1, A1p);
+        orbit_gather_grad<R, Z3, true, true>(g, qd, opt, x0, dxp, a.gpm, a.A_n, a.A_end, th0, th1, false, gpb, D);
+    } else {
+        gather_interp_rec<R>(g, a.NR, th1, x1, A1p);
+        orbit_gg_2d<R>(g, qd, opt, x0, dxp, a.NR, th0, th1, false, gpb, D);
+    }
+    const double Aep[3] = {0.5 * (A0p[0] + A1p[0]), 0.5 * (A0p[1] + A1p[1]), 0.5 * (A0p[2] + A1p[2])};
     double Dv[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) Dv[i] = D[i][0] * gs[0] + D[i][1] * gs[1] + D[i][2] * gs[2];
@@ -313,6 +347,9 @@ __device__ __forceinline__ void p3_local_map(const GridDev& g, const QuadDev& qd
 template <int R, bool Z3>
 __global__ __launch_bounds__(256) void k_p3_map(GridDev g, QuadDev qd, OrbitOpt opt, P3Args a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double tile_mem[kTile * kTile * 3];
+    DepTile T{};
+    if constexpr (!Z3) T = tile_begin<R>(g, tile_mem, a.x, a.np, (int64_t)blockIdx.x * blockDim.x);
     // red: [0] sum of local iterations, [1] failed particles; redm: [0] max
     // local iterations, [1] max final scaled residual
     double rsum[2] = {0.0, 0.0}, rmax[2] = {0.0, 0.0};
@@ -327,7 +364,8 @@ __global__ __launch_bounds__(256) void k_p3_map(GridDev g, QuadDev qd, OrbitOpt 
         for (int s = 0; s < nsub && reason == 0; ++s) {
             const double th0 = (double)s / (double)nsub, th1 = (double)(s + 1) / (double)nsub;
             double A0p[3];
-            gather_time_interp<R, Z3>(g, a.A_n, a.A_end, th0, xpos, A0p);
+            if constexpr (Z3) gather_time_interp<R, Z3>(g, a.A_n, a.A_end, th0, xpos, A0p);
+            else gather_interp_rec<R>(g, a.NR, th0, xpos, A0p);
             const double pst[3] = {Pc[0] - qp * A0p[0], Pc[1] - qp * A0p[1], Pc[2] - qp * A0p[2]};
             double gs[3] = {vc[0], vc[1], vc[2]};
             if (a.guess && nsub == 1) ld3(a.guess, a.np, p, gs);
@@ -364,7 +402,10 @@ __global__ __launch_bounds__(256) void k_p3_map(GridDev g, QuadDev qd, OrbitOpt 
             rmax[1] = fmax(rmax[1], scaled);
             if (!conv) break;
             const double dxp[3] = {h * vacc[0], h * vacc[1], h * vacc[2]};
-            if (a.J_out) nseg_total += deposit_orbit_one<R, Z3>(g, qd, opt, xpos, dxp, vacc, qp / (double)nsub, a.J_out);
+            if (a.J_out) {
+                if constexpr (Z3) nseg_total += deposit_orbit_one<R, Z3>(g, qd, opt, xpos, dxp, vacc, qp / (double)nsub, a.J_out);
+                else nseg_total += deposit_orbit_2d<R>(g, qd, opt, xpos, dxp, vacc, qp / (double)nsub, T, a.J_out);
+            }
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 xpos[d] = wrap1(xpos[d] + dxp[d], g.L[d]);
@@ -381,6 +422,9 @@ __global__ __launch_bounds__(256) void k_p3_map(GridDev g, QuadDev qd, OrbitOpt 
             st3(a.v_out, a.np, p, vc);
             st3(a.vbar_out, a.np, p, vmac);
         }
+    }
+    if constexpr (!Z3) {
+        if (a.J_out) tile_end(g, T, a.J_out);
     }
     block_reduce_add<2>(rsum, a.red);
     block_reduce_max<2>(rmax, a.redmax);
@@ -599,5 +643,21 @@ void launch_init_momenta(hipStream_t s, const GridDev& g, int64_t np, int code, 
                          const double* B, const double* A, double* P, double* v_prev, double* vbar_last) {
     if (np <= 0) return;
     UPIC_DISPATCH(g.order, g.nz > 1, hipLaunchKernelGGL((k_init_momenta<R, Z>), dim3(nblk(np)), dim3(256), 0, s, g, np, code, kappa, dt, x, v, q, m, E, B, A, P, v_prev, vbar_last));
+}
+}  // namespace upic
+
+namespace upic {
+__global__ void k_pack_nodes(int N, const double* __restrict__ gpm, const double* __restrict__ A0,
+                             const double* __restrict__ A1, double* __restrict__ NR) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= N) return;
+    double2* o = reinterpret_cast<double2*>(NR + (size_t)8 * id);
+    o[0] = make_double2(gpm[id], gpm[N + id]);
+    o[1] = make_double2(A0[id], A0[N + id]);
+    o[2] = make_double2(A0[2 * N + id], A1[id]);
+    o[3] = make_double2(A1[N + id], A1[2 * N + id]);
+}
+void launch_pack_nodes(hipStream_t s, int N, const double* gpm, const double* A_n, const double* A_end, double* NR) {
+    hipLaunchKernelGGL(k_pack_nodes, dim3((N + 255) / 256), dim3(256), 0, s, N, gpm, A_n, A_end, NR);
 }
 }  // namespace upic

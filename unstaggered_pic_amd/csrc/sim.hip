@@ -21,6 +21,7 @@ struct upic_sim {
     double* S1 = nullptr;
     double* zs = nullptr;      // complex scratch, 16 fields
     double* AG = nullptr;      // real A_end (3N) | gpm (3N)
+    double* NR = nullptr;      // 2-D packed node records (8N)
     double* Jb[2] = {nullptr, nullptr};
     double* gtmp = nullptr;    // real scratch 12N
     // particle scratch
@@ -53,7 +54,7 @@ namespace {
 
 void sim_free(upic_sim* s) {
     double** dbl[] = {&s->x, &s->v, &s->v_prev, &s->P, &s->vbar_last, &s->q, &s->m, &s->phi, &s->psi, &s->rho,
-                      &s->A, &s->U, &s->J, &s->S0, &s->S1, &s->zs, &s->AG, &s->Jb[0], &s->Jb[1], &s->gtmp,
+                      &s->A, &s->U, &s->J, &s->S0, &s->S1, &s->zs, &s->AG, &s->NR, &s->Jb[0], &s->Jb[1], &s->gtmp,
                       &s->vg[0], &s->vg[1], &s->xo, &s->vo, &s->Po, &s->vbo, &s->guess, &s->resid, &s->cand,
                       &s->trial, &s->Jacc, &s->Jtry, &s->Jfield, &s->ptmp, &s->ptmp1, &s->fixed_ion};
     for (double** p : dbl) dfree(*p);
@@ -112,6 +113,7 @@ void field_solve(upic_sim* s, const double* Jreal) {
     launch_copy(c->stream, 6ll * N, Zf(s->S1, 6, N), Zf(s->zs, 0, N));
     launch_copy(c->stream, 6ll * N, Zf(s->S1, 12, N), Zf(s->zs, 3, N));
     from_spectra(c, s->zs, s->AG, 6);
+    if (c->g.nz == 1) launch_pack_nodes(c->stream, N, s->AG + 3 * (size_t)N, s->A, s->AG, s->NR);
 }
 
 // End-of-step real mesh state from the last field solve.
@@ -189,6 +191,7 @@ void p1_step(upic_sim* s, upic_step_info* info) {
     a.np = np; a.dt = p.dt;
     a.x = s->x; a.v = s->v; a.v_prev = s->v_prev; a.P = s->P; a.q = s->q; a.m = s->m;
     a.A_n = s->A; a.A_end = s->AG; a.gpm = s->AG + 3 * (size_t)N;
+    a.NR = s->NR;
     a.red = c->dred; a.stats = c->dstats;
     // J_0 from v_guess = 2 v^n - v^{n-1}
     int cur = 0;
@@ -196,7 +199,7 @@ void p1_step(upic_sim* s, upic_step_info* info) {
     zero_red(c);
     a.vg_out = s->vg[cur];
     a.J_out = s->Jb[cur];
-    launch_p1_deposit_guess(c->stream, c->g, c->q, c->opt, a);
+    timed(c, "p1_deposit_guess", [&] { launch_p1_deposit_guess(c->stream, c->g, c->q, c->opt, a); });
     int iters = 0;
     double cur_err = std::numeric_limits<double>::infinity(), vel_err = cur_err;
     const double inf = std::numeric_limits<double>::infinity();
@@ -211,7 +214,7 @@ void p1_step(upic_sim* s, upic_step_info* info) {
         a.vg = s->vg[cur];
         a.vg_out = s->vg[nxt];
         a.J_out = s->Jb[nxt];
-        launch_p1_push(c->stream, c->g, c->q, c->opt, a, false);
+        timed(c, "p1_push", [&] { launch_p1_push(c->stream, c->g, c->q, c->opt, a, false); });
         HIPCHK(hipGetLastError());
         read_red(c, 8);
         ++iters;
@@ -244,7 +247,7 @@ void p1_step(upic_sim* s, upic_step_info* info) {
         Jfinal = s->Jb[prev];
     }
     a.x_out = s->xo; a.v_out = s->vo; a.P_out = s->Po; a.vprev_out = s->vbo /*temp*/; a.vbar_out = s->vbar_last;
-    launch_p1_push(c->stream, c->g, c->q, c->opt, a, true);
+    timed(c, "p1_push_final", [&] { launch_p1_push(c->stream, c->g, c->q, c->opt, a, true); });
     // the old velocity becomes v_prev; swap buffers in
     std::swap(s->x, s->xo);
     std::swap(s->P, s->Po);
@@ -296,11 +299,12 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     a.layout = use_layout ? s->layout : nullptr;
     a.guess = nullptr;
     a.A_n = s->A; a.A_end = s->AG; a.gpm = s->AG + 3 * (size_t)N;
+    a.NR = s->NR;
     a.J_out = Jmap;
     a.x_out = s->xo; a.P_out = s->Po; a.v_out = s->vo; a.vbar_out = s->vbo;
     a.reason = s->reason;
     a.red = c->dred; a.redmax = c->dred + 2; a.stats = c->dstats;
-    launch_p3_map(c->stream, c->g, c->q, c->opt, a);
+    timed(c, "p3_map", [&] { launch_p3_map(c->stream, c->g, c->q, c->opt, a); });
     launch_project_nyquist(c->stream, c->sp, Jmap, 3);
     HIPCHK(hipGetLastError());
     read_red(c, 4);
@@ -645,7 +649,7 @@ void p2_step(upic_sim* s, upic_step_info* info) {
         launch_copy(c->stream, N, s->fixed_ion, rho_new);
         pa.rho = rho_new;
     }
-    launch_p2_advance_deposit(c->stream, pa);  // positions (P2:1050-1055), J (747-755)
+    timed(c, "p2_deposit", [&] { launch_p2_advance_deposit(c->stream, pa); });  // positions (P2:1050-1055), J (747-755)
     if (p.charge_conserving) {  // update_charge_from_continuity (P2:865-880)
         to_spectra(c, Jn, s->zs, 3);
         launch_spectral_div(c->stream, c->sp, s->zs, Zf(s->zs, 3, N));
@@ -691,7 +695,7 @@ void p2_step(upic_sim* s, upic_step_info* info) {
     // A_new real was overwritten with its BDF rhs; recompute from F (A real)
     launch_copy(c->stream, 3ll * N, F, A_new);
     pa.F = F;
-    launch_p2_push(c->stream, pa);
+    timed(c, "p2_push", [&] { launch_p2_push(c->stream, pa); });
     // history shift (P2:1185-1188)
     double* ph = s->phi_hist[4];
     double* Ah = s->A_hist[4];
@@ -826,6 +830,7 @@ upic_status upic_sim_create(upic_ctx* c, int64_t np, upic_sim** out) {
         s->S1 = dalloc<double>(2 * (size_t)N * 15);
         s->zs = dalloc<double>(2 * (size_t)N * 16);
         s->AG = dalloc<double>(6 * (size_t)N);
+        s->NR = dalloc<double>(8 * (size_t)N);
         s->gtmp = dalloc<double>(12 * (size_t)N);
         s->Jb[0] = dalloc<double>(3 * (size_t)N);
         s->Jb[1] = dalloc<double>(3 * (size_t)N);

@@ -59,6 +59,11 @@ def lib():
             "upic_sim_device_state": [vp, C.POINTER(HostState)],
             "upic_sim_failure_reasons": [vp, C.POINTER(C.c_uint8)],
             "upic_sim_init_fields": [vp, C.c_double],
+            "upic_ctx_set_timing": [vp, C.c_int],
+            "upic_device_count": [C.POINTER(C.c_int)], "upic_device_alloc": [C.c_size_t, C.POINTER(vp)],
+            "upic_device_free": [vp], "upic_memcpy_htod": [vp, vp, C.c_size_t],
+            "upic_memcpy_dtoh": [vp, vp, C.c_size_t],
+            "upic_ctx_timing": [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)],
             "upic_load_particles": [C.POINTER(Params), C.POINTER(LoadSpec), C.POINTER(HostState)],
         }
         for name, args in sigs.items():
@@ -81,14 +86,56 @@ def exported_symbols():
     return names
 
 
+def device_count():
+    n = C.c_int()
+    lib().upic_device_count(C.byref(n))
+    return n.value
+
+
+class DeviceArray:
+    """A float64 buffer in HBM, allocated through the library."""
+
+    def __init__(self, n):
+        self.n = int(n)
+        self.p = vp()
+        rc = lib().upic_device_alloc(8 * max(self.n, 1), C.byref(self.p))
+        if rc != 0:
+            raise UpicError(rc, "device allocation failed")
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = cls(a.size)
+        if a.size:
+            lib().upic_memcpy_htod(d.p, a.ctypes.data_as(vp), 8 * a.size)
+        return d
+
+    def numpy(self):
+        out = np.empty(self.n)
+        if self.n:
+            rc = lib().upic_memcpy_dtoh(out.ctypes.data_as(vp), self.p, 8 * self.n)
+            if rc != 0:
+                raise UpicError(rc, "device copy failed")
+        return out
+
+    def numel(self):
+        return self.n
+
+    def __del__(self):
+        try:
+            if self.p:
+                lib().upic_device_free(self.p)
+                self.p = vp()
+        except Exception:
+            pass
+
+
 class Context:
     """Grid, quadrature, stream (upic_ctx)."""
 
     def __init__(self, params: Params):
-        import torch  # device memory comes from torch's HIP allocator
-        if not torch.cuda.is_available():
+        if device_count() < 1:
             raise RuntimeError("no HIP device: the unstaggered_pic_amd hot path needs an MI355X")
-        self.torch = torch
         self.params = params
         self.h = vp()
         self._check(lib().upic_ctx_create(C.byref(params), C.byref(self.h)), None)
@@ -100,6 +147,14 @@ class Context:
 
     def sync(self):
         self._check(lib().upic_ctx_sync(self.h))
+
+    def set_timing(self, on=True):
+        self._check(lib().upic_ctx_set_timing(self.h, int(on)))
+
+    def timing(self, name):
+        n, ms = C.c_int64(), C.c_double()
+        self._check(lib().upic_ctx_timing(self.h, name.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
 
     def quadrature(self):
         n = self.params.orbit_quad_order
@@ -118,21 +173,21 @@ class Context:
         except Exception:
             pass
 
-    # --- group A on torch device tensors --------------------------------
-    def dev(self, a):
-        t = self.torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).to("cuda")
-        return t
+    # --- group A on DeviceArrays --------------------------------------
+    @staticmethod
+    def dev(a):
+        return DeviceArray.from_host(a)
 
     @staticmethod
     def ptr(t):
-        return C.c_void_p(t.data_ptr()) if t is not None else None
+        return t.p if t is not None else None
 
     def N(self):
         p = self.params
         return p.nx * p.ny * p.nz
 
     def empty(self, n):
-        return self.torch.empty(n, dtype=self.torch.float64, device="cuda")
+        return DeviceArray(n)
 
     def deposit_current_orbit(self, x0, dx, vbar, q):
         n = q.numel()
@@ -209,7 +264,7 @@ class Context:
         return o
 
     def project_nyquist(self, f, ncomp=1):
-        f = f.clone()
+        f = DeviceArray.from_host(f.numpy())
         self._check(lib().upic_project_nyquist(self.h, self.ptr(f), ncomp))
         self.sync()
         return f
