@@ -30,18 +30,22 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, vendor figure
 FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector, vendor figure
+FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector, vendor figure
 
 
 def workload_params(nx=256, ppc=256, precision=0):
-    from unstaggered_pic_amd.abi import CODE_P3, make_params
+    """The Weibel deck; precision 1 selects the mixed mode with its FP32
+    tolerance floors (inner and outer relative tolerances 1e-6)."""
+    from unstaggered_pic_amd.abi import CODE_P3, make_params, with_mixed_precision
     L = 20.0 * np.pi
-    return make_params(code=CODE_P3, nx=nx, ny=nx, nz=1, Lx=L, Ly=L, Lz=1.0, dt=0.05, spline_order=3,
+    p = make_params(code=CODE_P3, nx=nx, ny=nx, nz=1, Lx=L, Ly=L, Lz=1.0, dt=0.05, spline_order=3,
                        orbit_quad_order=4, split_orbit_at_knots=1, nyquist_projection=1, c=1.0, sigma1=1.0,
                        sigma2=1.0, nonlinear_solver_picard=0, nonlinear_max_iter=80, nonlinear_rtol=1e-11,
                        nonlinear_atol=1e-13, anderson_memory=4, anderson_start=2, anderson_damping=0.5,
                        anderson_regularization=1e-12, anderson_growth_limit=1.2, local_rtol=1e-12,
-                       local_atol=1e-14, local_max_iter=64, local_damping=1.0, precision=precision,
+                       local_atol=1e-14, local_max_iter=64, local_damping=1.0, precision=0,
                        sort_interval=1)
+    return with_mixed_precision(p) if precision == 1 else p
 
 
 def workload_load(ppc, seed=12345):
@@ -136,7 +140,8 @@ def main():
         infos.append(sim.step())
     ctx.sync()
     el = time.perf_counter() - t0
-    launches, kms = ctx.timing("p3_map")
+    kname = "p3_map" if prec == 0 else "p3_map_mixed"
+    launches, kms = ctx.timing(kname)
     d1 = sim.diagnostics(1)
     value = np_ * args.steps / el
     # algorithmic HBM bytes of one p3_map launch: per particle the FP64 state
@@ -144,8 +149,14 @@ def main():
     # (x, P, v, vbar_macro: 96 B) happens in one launch per step -- plus the
     # mesh fields read (A_n, A_end, grad phi_mid: 9 N doubles) and the current
     # written (3 N doubles).
+    # Mixed mode: reads cell (4 B), offsets, P, v (3 x 12 B), q, m (8 B);
+    # writes cell, offsets, P, v, vbar_macro (52 B); FP32 node records
+    # (8 floats per node) read, FP64 current written.
     maps = sum(i.map_evaluations for i in infos)
-    bytes_per_launch = np_ * 88.0 + (np_ * 96.0 * args.steps) / max(maps, 1) + 12.0 * 8 * N
+    if prec == 0:
+        bytes_per_launch = np_ * 88.0 + (np_ * 96.0 * args.steps) / max(maps, 1) + 12.0 * 8 * N
+    else:
+        bytes_per_launch = np_ * 48.0 + (np_ * 52.0 * args.steps) / max(maps, 1) + (32.0 + 24.0) * N
     avg_ms = kms / max(launches, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if launches else None
     # FP64 operations of one inner iteration at r = 3, nq = 4 (counted from the
@@ -174,11 +185,13 @@ def main():
                    "precision": args.precision, "form": "P3 nested solve (Anderson outer, per-particle Picard inner)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "kernel": "k_p3_map", "avg_launch_ms": avg_ms, "launches": launches,
+                     "kernel": "k_p3_map" if prec == 0 else "k_p3_map_mixed", "avg_launch_ms": avg_ms, "launches": launches,
                      "bytes_per_launch": bytes_per_launch},
-        "compute_roofline": {"bound": "fp64_valu", "achieved": compute_tflops, "peak": FP64_PEAK_TFLOPS,
+        "compute_roofline": {"bound": "fp64_valu" if prec == 0 else "fp32_valu", "achieved": compute_tflops,
+                             "peak": FP64_PEAK_TFLOPS if prec == 0 else FP32_PEAK_TFLOPS,
                              "unit": "TFLOP/s",
-                             "frac": (compute_tflops / FP64_PEAK_TFLOPS) if compute_tflops else None,
+                             "frac": (compute_tflops / (FP64_PEAK_TFLOPS if prec == 0 else FP32_PEAK_TFLOPS))
+                             if compute_tflops else None,
                              "flops_per_inner_iteration": flops_iter},
         "physics": {"rel_total_energy_change": (d1.total_energy - d0.total_energy) / abs(d0.total_energy),
                     "outer_iterations": [i.iterations for i in infos],

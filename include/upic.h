@@ -63,8 +63,11 @@ typedef enum upic_particle_failure {
 /* Particle/grid precision of a simulation handle. */
 typedef enum upic_precision {
     UPIC_PREC_FP64 = 0,  /* everything FP64 (the reference's arithmetic)            */
-    UPIC_PREC_MIXED = 1  /* FP32 particle push / LDS deposit tiles, FP64 grids, FFT,
-                            diagnostics and convergence tests                      */
+    UPIC_PREC_MIXED = 1  /* P3 on 2-D grids: particles held as an int32 cell index
+                            plus FP32 in-cell offsets, FP32 momenta/velocities;
+                            FP32 orbit push and LDS deposit tiles flushed into
+                            FP64 mesh currents; FP64 grids, FFT, diagnostics
+                            and convergence tests                              */
 } upic_precision;
 
 /* Numerical parameters of one run.  Field names follow the reference decks
@@ -273,7 +276,10 @@ upic_status upic_sim_diagnostics(upic_sim* sim, int mode, upic_diagnostics* out)
  * history velocity v_prev by the Taylor start-up (P1:1296-1303,
  * P3:3423-3441). */
 upic_status upic_sim_init_fields(upic_sim* sim, double n0);
-/* Device pointers of the resident state (for zero-copy callers). */
+/* Device pointers of the resident state (for zero-copy callers).  With
+ * UPIC_PREC_MIXED the particle pointers are an FP64 view rebuilt from the
+ * FP32 state by this call; writes to them reach the simulation only through
+ * upic_sim_upload. */
 upic_status upic_sim_device_state(upic_sim* sim, upic_host_state* dev);
 /* Per-particle failure reasons of the last P3 map evaluation (np bytes, host). */
 upic_status upic_sim_failure_reasons(upic_sim* sim, uint8_t* reasons);

@@ -111,3 +111,20 @@ def make_params(**kw):
     for k, v in defaults.items():
         setattr(p, k, v)
     return p
+
+
+# Tolerance floors of the mixed-precision mode: an FP32 orbit map resolves
+# the orbit-averaged velocity to ~1e-7 relative, so the inner and outer
+# fixed-point tests need relative tolerances of about 1e-6 to be reachable.
+MIXED_TOLERANCE_FLOORS = dict(local_rtol=1e-6, local_atol=1e-9, nonlinear_rtol=1e-6, nonlinear_atol=1e-9)
+
+
+def with_mixed_precision(p):
+    """Copy of Params `p` in the mixed-precision mode with its tolerances
+    raised to the FP32 floors (values already above a floor are kept)."""
+    q = Params()
+    C.memmove(C.byref(q), C.byref(p), C.sizeof(Params))
+    q.precision = PREC_MIXED
+    for k, v in MIXED_TOLERANCE_FLOORS.items():
+        setattr(q, k, max(getattr(q, k), v))
+    return q

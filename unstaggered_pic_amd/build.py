@@ -10,7 +10,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libupic.so")
-SOURCES = ["particle_kernels.hip", "field_kernels.hip", "api.hip", "sim.hip", "init.hip"]
+SOURCES = ["particle_kernels.hip", "mixed.hip", "field_kernels.hip", "api.hip", "sim.hip", "init.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-std=c++17", "-O3", "-fPIC", "--offload-arch=gfx950", "-munsafe-fp-atomics",
@@ -20,31 +20,41 @@ FLAGS = [
 ]
 
 
-def _compile(src):
-    obj = os.path.join(CSRC, "_build", src + ".o")
+def _compile(src, defines=(), tag=""):
+    obj = os.path.join(CSRC, "_build" + tag, src + ".o")
     os.makedirs(os.path.dirname(obj), exist_ok=True)
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *FLAGS, *[f"-D{d}" for d in defines], "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
     return obj, r.stderr
 
 
-def build(verbose=False):
+def build(verbose=False, defines=(), out=None):
+    """Compile and link the library.  `defines` (e.g. ["UPIC_CHUNK=2"]) and
+    `out` build a tuning variant next to the default library; the loader
+    picks a variant up through the UPIC_LIBRARY environment variable."""
+    out = out or OUT
+    tag = "" if out == OUT else "_" + os.path.basename(out).replace(".", "_")
     with ThreadPoolExecutor(max_workers=min(4, os.cpu_count() or 1)) as ex:
-        results = list(ex.map(_compile, SOURCES))
+        results = list(ex.map(lambda s: _compile(s, defines, tag), SOURCES))
     for _, err in results:
         if verbose and err.strip():
             print(err, file=sys.stderr)
     objs = [o for o, _ in results]
-    tmp = OUT + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
-    os.replace(tmp, OUT)
-    return OUT
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-D", dest="defines", action="append", default=[])
+    ap.add_argument("-o", dest="out", default=None)
+    a = ap.parse_args()
+    print(build(verbose=True, defines=a.defines, out=a.out and os.path.abspath(a.out)))

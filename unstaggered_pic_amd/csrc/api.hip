@@ -30,6 +30,7 @@ upic_status upic_ctx_create(const upic_params* params, upic_ctx** out) {
         g.order = p.code == 2 ? 2 : p.spline_order;
         c->q.n = p.code == 2 ? 1 : p.orbit_quad_order;
         gauss_legendre_01(c->q.n, c->q.s, c->q.w);
+        c->qf = fp32_rule(c->q);
         c->opt.split = p.split_orbit_at_knots ? 1 : 0;
         c->opt.wrap_path = p.code == 1 ? 1 : 0;
         SpecArgs& sp = c->sp;

@@ -92,10 +92,47 @@ using namespace upic_host;
 // ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
+// FP32 copy of a Gauss-Legendre rule on [0, 1] that keeps the rule's
+// symmetry exactly: the upper nodes are rounded and the lower ones are their
+// exact mirrors 1 - s (Sterbenz), weights are mirrored, and the largest
+// weight absorbs the rounding so that the weights sum to 1 in FP32.  Plain
+// rounding leaves the first moment sum w s off 1/2 by ~1e-8, a systematic
+// shift of every orbit average that shows up as an energy drift.
+inline QuadF fp32_rule(const QuadDev& q) {
+    QuadF r{};
+    r.n = q.n <= 8 ? q.n : 0;
+    const int n = r.n;
+    for (int i = n / 2; i < n; ++i) {  // s >= 1/2
+        r.s[i] = (float)q.s[i];
+        r.w[i] = (float)q.w[i];
+    }
+    for (int i = 0; i < n / 2; ++i) {
+        r.s[i] = 1.0f - r.s[n - 1 - i];
+        r.w[i] = r.w[n - 1 - i];
+    }
+    if (n % 2 == 1) r.s[n / 2] = 0.5f;
+    // fix the total in FP32: adjust the central weight(s)
+    const int mid = n / 2;
+    for (int pass = 0; pass < 4 && n > 0; ++pass) {
+        float tot = 0.0f;
+        for (int i = 0; i < n; ++i) tot += r.w[i];
+        if (tot == 1.0f) break;
+        if (n % 2 == 1) {
+            r.w[mid] += 1.0f - tot;
+        } else {
+            const float d = 0.5f * (1.0f - tot);
+            r.w[mid] += d;
+            r.w[mid - 1] = r.w[mid];
+        }
+    }
+    return r;
+}
+
 struct upic_ctx {
     upic_params p{};
     GridDev g{};
     QuadDev q{};
+    QuadF qf{};          // FP32 copy of the rule (mixed precision)
     OrbitOpt opt{};
     SpecArgs sp{};
     FFTPlan plan{};
@@ -170,6 +207,12 @@ inline void validate(const upic_params& p) {
     if (p.code == 1 && p.picard_max_iter < 1) throw Error(UPIC_EINVAL, "picard_max_iter must be positive");
     if (p.code == 2 && p.bdf_order != 1 && p.bdf_order != 2) throw Error(UPIC_EINVAL, "method must be bdf1 or bdf2");
     if (p.precision != UPIC_PREC_FP64 && p.precision != UPIC_PREC_MIXED) throw Error(UPIC_EINVAL, "unknown precision");
+    if (p.precision == UPIC_PREC_MIXED) {
+        if (p.code != 3 || p.nz != 1)
+            throw Error(UPIC_EINVAL, "mixed precision is implemented for the P3 step on 2-D grids (nz = 1)");
+        if (p.orbit_quad_order > 8) throw Error(UPIC_EINVAL, "mixed precision supports orbit_quad_order <= 8");
+        if (p.nx * p.ny >= (1 << 30)) throw Error(UPIC_EINVAL, "mixed precision needs nx*ny < 2^30");
+    }
 }
 
 template <class F>

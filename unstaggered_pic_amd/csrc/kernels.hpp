@@ -77,6 +77,51 @@ void launch_p2_push(hipStream_t s, const P2Args& a);
 // 2-D node records: NR[8*id + ...] = grad phi_mid x, y; A^n x, y, z; A^{n+1} x, y, z
 void launch_pack_nodes(hipStream_t s, int N, const double* gpm, const double* A_n, const double* A_end, double* NR);
 
+// mixed.hip: FP32 particle state of the mixed-precision P3 path (2-D) -----
+// Gauss-Legendre rule on [0, 1] in FP32 (at most 8 points).
+struct QuadF {
+    int n;
+    float s[8], w[8];
+};
+// Device arrays of the mixed particle state: cell = i*ny + j; off = in-cell
+// offsets (x, y in cell units, z in units of Lz), SoA 3*np; P, v, v_prev,
+// vbar SoA 3*np; q, m per particle.
+struct MixedState {
+    int* cell;
+    float* off;
+    float *P, *v, *vp, *vb, *q, *m;
+};
+struct P3MArgs {
+    int64_t np;
+    double dt, kappa, rtol, atol, damping;
+    int max_iter, write_out;
+    float inv_dV, inv_Lz;
+    const int* cell;
+    const float *off, *P, *v, *q, *m;
+    const int8_t* layout;          // substeps per particle (NULL = 1)
+    const float* guess;            // optional initial guess of vbar (NULL = v^n)
+    const float4* NR;              // FP32 node records, two float4 per node
+    double* J_out;                 // FP64 mesh current (3*N, pre-zeroed) or NULL
+    int* cell_out;
+    float *off_out, *P_out, *v_out, *vbar_out;
+    uint8_t* reason;
+    double* red;                   // [2]: sum of local iterations, failed particles
+    double* redmax;                // [2]: max local iterations, max final scaled residual
+    unsigned long long* stats;
+};
+void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& a);
+// NRf[8*id + ...] = FP32 copy of the FP64 node records
+void launch_pack_nodes_f32(hipStream_t s, int N, const double* gpm, const double* A_n, const double* A_end, float* NR);
+// also replaces q and m by their FP32 values (one charge and mass for all parts)
+void launch_mixed_compress(hipStream_t s, const GridDev& g, int64_t np, const double* x, const double* P,
+                           const double* v, const double* vp, const double* vb, double* q, double* m,
+                           const MixedState& ms);
+void launch_mixed_expand(hipStream_t s, const GridDev& g, int64_t np, const MixedState& ms, double* x, double* P,
+                         double* v, double* vp, double* vb);
+void launch_mixed_keys(hipStream_t s, int64_t np, const int* cell, int* keys, int* counts);
+void launch_permute_f32(hipStream_t s, int64_t np, int ncomp, const float* src, float* dst, const int* perm);
+void launch_permute_i32(hipStream_t s, int64_t np, const int* src, int* dst, const int* perm);
+
 // field_kernels.hip ---------------------------------------------------------
 struct FFTPlan {
     int n[3];                 // nx, ny, nz

@@ -1,0 +1,800 @@
+// mixed.hip -- the mixed-precision particle path of the P3 step (2-D, nz = 1).
+//
+// Particle state is held as an int32 cell index c = i*ny + j plus in-cell
+// offsets t = (u - i, v - j, w) in cell units (u = (x + Lx/2)/hx; w is the z
+// position in units of Lz, the single z cell), momenta P, velocities v and
+// orbit-averaged velocities in FP32.  The orbit integrals, the GM-HC map and
+// the deposit run in FP32 on cell-relative coordinates; node records are an
+// FP32 copy of the FP64 mesh fields; the deposit accumulates in an FP32 LDS
+// tile that is flushed into the FP64 mesh current with FP64 atomics.  The
+// inner convergence test, every reduction and the field solve stay in FP64.
+//
+// The arithmetic follows the FP64 path (particle_kernels.hip, orbit2d.hpp):
+//   evaluate_local_particle_substep   P3:4843-4884
+//   solve_local_particle_substep      P3:4930-4983
+//   path_average_gather_and_gradient  P3:3016-3071
+//   path_average_deposit_current      P3:2784-2857
+//   path_breakpoints_for_particle     P3:2555-2606
+// with positions relative to the particle's start cell, so that FP32 keeps
+// ~1e-7 of a cell of resolution anywhere on the grid.
+#include "kernels.hpp"
+#include "particles.hpp"
+
+namespace upic {
+
+namespace {
+
+__device__ __forceinline__ int wrapn(int i, int n) {
+    if ((unsigned)i >= (unsigned)n) {
+        i %= n;
+        if (i < 0) i += n;
+    }
+    return i;
+}
+__device__ __forceinline__ int wrap1n(int i, int n) { return i >= n ? i - n : i; }
+
+// Centred B-spline of degree R and its derivative (P3:1856-1947), FP32.
+template <int R>
+__device__ __forceinline__ void bsp_f(float r, float& w, float& dw) {
+    if constexpr (R == 1) {
+        if (r >= -1.0f && r < 0.0f) { w = 1.0f + r; dw = 1.0f; }
+        else if (r >= 0.0f && r < 1.0f) { w = 1.0f - r; dw = -1.0f; }
+        else { w = 0.0f; dw = 0.0f; }
+    } else {
+        const float a = fabsf(r);
+        const float sg = (r < 0.0f) ? -1.0f : 1.0f;
+        if constexpr (R == 2) {
+            if (a < 0.5f) { w = 0.75f - a * a; dw = -2.0f * r; }
+            else if (a < 1.5f) { const float z = 1.5f - a; w = 0.5f * z * z; dw = -sg * z; }
+            else { w = 0.0f; dw = 0.0f; }
+        } else if constexpr (R == 3) {
+            if (a < 1.0f) { w = (2.0f / 3.0f) - a * a + 0.5f * a * a * a; dw = sg * (-2.0f * a + 1.5f * a * a); }
+            else if (a < 2.0f) { const float z = 2.0f - a; w = z * z * z * (1.0f / 6.0f); dw = -0.5f * sg * z * z; }
+            else { w = 0.0f; dw = 0.0f; }
+        } else {
+            if (a < 0.5f) {
+                const float a2 = a * a;
+                w = 115.0f / 192.0f - (5.0f / 8.0f) * a2 + 0.25f * a2 * a2;
+                dw = sg * (-(5.0f / 4.0f) * a + a * a * a);
+            } else if (a < 1.5f) {
+                w = 55.0f / 96.0f + (5.0f / 24.0f) * a - 1.25f * a * a + (5.0f / 6.0f) * a * a * a -
+                    (1.0f / 6.0f) * a * a * a * a;
+                dw = sg * (5.0f / 24.0f - 2.5f * a + 2.5f * a * a - (2.0f / 3.0f) * a * a * a);
+            } else if (a < 2.5f) {
+                const float z = 2.5f - a;
+                const float z2 = z * z;
+                w = z2 * z2 * (1.0f / 24.0f);
+                dw = -sg * z * z * z * (1.0f / 6.0f);
+            } else { w = 0.0f; dw = 0.0f; }
+        }
+    }
+}
+
+// One axis of the stencil at cell-relative coordinate t: nodes at cell
+// offsets fl + a, fl = floor(t - (R-1)/2) (axis_stencil, P3:1971-1986).
+template <int R>
+struct AxF {
+    int fl;
+    float w[R + 1], dw[R + 1];  // dw already divided by h
+};
+// The last weight is the complement of the others and the last derivative
+// weight minus their sum: the FP32 closed forms round with a bias (2/3 rounds
+// up, so the cubic weights sum to 1 + 4e-8 on average), and a partition of
+// unity off by a bias scales every gather and deposit coherently, which
+// shows up as an energy drift; the complement leaves only unbiased rounding.
+template <int R>
+__device__ __forceinline__ void axis_f(float t, float ih, AxF<R>& s) {
+    const float f = floorf(t - 0.5f * (float)(R - 1));
+    s.fl = (int)f;
+    const float r0 = t - f;
+    float sw = 0.0f, sdw = 0.0f;
+#pragma unroll
+    for (int a = 0; a < R; ++a) {
+        float w, dw;
+        bsp_f<R>(r0 - (float)a, w, dw);
+        s.w[a] = w;
+        s.dw[a] = dw;
+        sw += w;
+        sdw += dw;
+    }
+    s.w[R] = 1.0f - sw;
+    s.dw[R] = -sdw;
+#pragma unroll
+    for (int a = 0; a <= R; ++a) s.dw[a] *= ih;
+}
+
+// Knot crossings of the two in-plane axes in increasing s (the merge of
+// path_breakpoints_for_particle, P3:2555-2606).  The FP32 filter and
+// de-duplication widths are 1e-6: a cut that close to another or to an end
+// changes the orbit integral by far less than FP32 resolution.
+struct CrossF {
+    float a0, du;
+    int m, m_end, step;
+};
+struct BreakF {
+    CrossF ax[2];
+    float last;
+};
+__device__ __forceinline__ void breakf_init(BreakF& bg, int order, float t0x, float t0y, float dux, float duy) {
+    bg.last = 0.0f;
+    const float off = 0.5f * (float)(order - 1);
+    const float t0[2] = {t0x, t0y}, du[2] = {dux, duy};
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        CrossF& c = bg.ax[d];
+        c.a0 = t0[d] - off;
+        c.du = du[d];
+        c.m = 0; c.m_end = 0; c.step = 1;
+        if (du[d] == 0.0f) continue;
+        const float a1 = c.a0 + du[d];
+        const float lo = fminf(c.a0, a1), hi = fmaxf(c.a0, a1);
+        const int first = (int)floorf(lo) + 1, last = (int)ceilf(hi) - 1;
+        if (last < first) continue;
+        if (du[d] > 0.0f) { c.m = first; c.m_end = last + 1; c.step = 1; }
+        else { c.m = last; c.m_end = first - 1; c.step = -1; }
+    }
+}
+__device__ __forceinline__ float breakf_next(BreakF& bg) {
+    const float tol = 1.0e-6f;
+    for (;;) {
+        const float s0 = (bg.ax[0].m == bg.ax[0].m_end) ? 2.0f : ((float)bg.ax[0].m - bg.ax[0].a0) / bg.ax[0].du;
+        const float s1 = (bg.ax[1].m == bg.ax[1].m_end) ? 2.0f : ((float)bg.ax[1].m - bg.ax[1].a0) / bg.ax[1].du;
+        const bool first = s0 <= s1;
+        const float s = first ? s0 : s1;
+        if (s > 1.5f) { bg.last = 1.0f; return 1.0f; }
+        if (first) bg.ax[0].m += bg.ax[0].step;
+        else bg.ax[1].m += bg.ax[1].step;
+        if (!(s > tol && s < 1.0f - tol)) continue;
+        if (fabsf(s - bg.last) > tol) { bg.last = s; return s; }
+    }
+}
+
+// Scheduling fence between stencil rows: keeps the compiler from hoisting
+// every node record load of the chunk at once (register pressure).
+#ifndef UPIC_MIXED_SCHED
+#define UPIC_MIXED_SCHED 1
+#endif
+#if UPIC_MIXED_SCHED
+#define UPIC_MIXED_FENCE() __builtin_amdgcn_sched_barrier(0)
+#else
+#define UPIC_MIXED_FENCE() ((void)0)
+#endif
+
+// FP32 node record: (gx, gy, A0x, A0y) (A0z, A1x, A1y, A1z)
+__device__ __forceinline__ void rec_f(const float4* __restrict__ NR, int id, float4& r0, float4& r1) {
+    r0 = NR[2 * id];
+    r1 = NR[2 * id + 1];
+}
+
+struct MGrid {
+    int nx, ny, N, order;
+    float ihx, ihy;  // 1 / h
+    float hx, hy;
+};
+
+// A^n (which = 0) or A^{n+1} (which = 1) at cell-relative (tx, ty) of cell (ci, cj)
+template <int R>
+__device__ __forceinline__ void gatherA_f(const MGrid& g, const float4* __restrict__ NR, int ci, int cj, float tx,
+                                          float ty, int which, float out[3]) {
+    AxF<R> sx, sy;
+    axis_f<R>(tx, g.ihx, sx);
+    axis_f<R>(ty, g.ihy, sy);
+    const int bx = wrapn(ci + sx.fl, g.nx), by = wrapn(cj + sy.fl, g.ny);
+    float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
+#pragma unroll
+    for (int a = 0; a <= R; ++a) {
+        const int ia = wrap1n(bx + a, g.nx);
+#pragma unroll
+        for (int b = 0; b <= R; ++b) {
+            const int ib = wrap1n(by + b, g.ny);
+            float4 r0, r1;
+            rec_f(NR, ia * g.ny + ib, r0, r1);
+            const float w = sx.w[a] * sy.w[b];
+            if (which) {
+                o0 += r1.y * w; o1 += r1.z * w; o2 += r1.w * w;
+            } else {
+                o0 += r0.z * w; o1 += r0.w * w; o2 += r1.x * w;
+            }
+        }
+    }
+    out[0] = o0; out[1] = o1; out[2] = o2;
+}
+
+template <int R>
+__device__ __forceinline__ void gather_interp_f(const MGrid& g, const float4* NR, int ci, int cj, float tx, float ty,
+                                                float theta, float out[3]) {
+    if (theta == 0.0f) { gatherA_f<R>(g, NR, ci, cj, tx, ty, 0, out); return; }
+    if (theta == 1.0f) { gatherA_f<R>(g, NR, ci, cj, tx, ty, 1, out); return; }
+    float a0[3], a1[3];
+    gatherA_f<R>(g, NR, ci, cj, tx, ty, 0, a0);
+    gatherA_f<R>(g, NR, ci, cj, tx, ty, 1, a1);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[d] = (1.0f - theta) * a0[d] + theta * a1[d];
+}
+
+// Fused orbit gather of grad phi_mid and the orbit gradient of A over the
+// path t(s) = t0 + s du (cell units), theta(s) = th0 + (th1 - th0) s.
+// Gauss points in chunks of four share one node loop while their stencil
+// base agrees (the quadrature folded into node weights, as in the FP64 path).
+template <int R>
+__device__ __forceinline__ int orbit_gg_f(const MGrid& g, const QuadF& q, bool split, const float4* __restrict__ NR,
+                                          int ci, int cj, float t0x, float t0y, float dux, float duy, float th0,
+                                          float th1, float gat[2], float D[3][2]) {
+    float g0 = 0.0f, g1 = 0.0f, d00 = 0.0f, d01 = 0.0f, d10 = 0.0f, d11 = 0.0f, d20 = 0.0f, d21 = 0.0f;
+    BreakF bg;
+    breakf_init(bg, R, t0x, t0y, split ? dux : 0.0f, split ? duy : 0.0f);
+    float sa = 0.0f;
+    int nseg = 0;
+    for (;;) {
+        const float sb = breakf_next(bg);
+        const float len = sb - sa;
+        if (len > 0.0f) {
+            for (int q0 = 0; q0 < q.n; q0 += 4) {
+                float wx[4][R + 1], dwx[4][R + 1], wy[4][R + 1], dwy[4][R + 1], wq[4], th[4];
+                int bx[4], by[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int iq = q0 + k < q.n ? q0 + k : q0;
+                    const float s = sa + len * q.s[iq];
+                    wq[k] = (q0 + k < q.n) ? len * q.w[iq] : 0.0f;
+                    AxF<R> sx, sy;
+                    axis_f<R>(t0x + s * dux, g.ihx, sx);
+                    axis_f<R>(t0y + s * duy, g.ihy, sy);
+                    bx[k] = sx.fl;
+                    by[k] = sy.fl;
+#pragma unroll
+                    for (int a = 0; a <= R; ++a) {
+                        wx[k][a] = sx.w[a]; dwx[k][a] = sx.dw[a];
+                        wy[k][a] = sy.w[a]; dwy[k][a] = sy.dw[a];
+                    }
+                    th[k] = th0 + (th1 - th0) * s;
+                }
+                const bool same = bx[1] == bx[0] && bx[2] == bx[0] && bx[3] == bx[0] && by[1] == by[0] &&
+                                  by[2] == by[0] && by[3] == by[0];
+                if (same) {
+                    const int ox = wrapn(ci + bx[0], g.nx), oy = wrapn(cj + by[0], g.ny);
+#pragma unroll
+                    for (int a = 0; a <= R; ++a) {
+                        const int ia = wrap1n(ox + a, g.nx);
+#pragma unroll
+                        for (int b = 0; b <= R; ++b) {
+                            const int ib = wrap1n(oy + b, g.ny);
+                            float W = 0.0f, Gx0 = 0.0f, Gx1 = 0.0f, Gy0 = 0.0f, Gy1 = 0.0f;
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) {
+                                const float ws = wq[k] * wy[k][b];
+                                const float sx_ = dwx[k][a] * ws;
+                                const float sy_ = wx[k][a] * wq[k] * dwy[k][b];
+                                W += wx[k][a] * ws;
+                                Gx1 += th[k] * sx_;
+                                Gx0 += sx_;
+                                Gy1 += th[k] * sy_;
+                                Gy0 += sy_;
+                            }
+                            Gx0 -= Gx1;
+                            Gy0 -= Gy1;
+                            float4 r0, r1;
+                            rec_f(NR, ia * g.ny + ib, r0, r1);
+                            g0 += W * r0.x;
+                            g1 += W * r0.y;
+                            d00 += r0.z * Gx0 + r1.y * Gx1;
+                            d01 += r0.z * Gy0 + r1.y * Gy1;
+                            d10 += r0.w * Gx0 + r1.z * Gx1;
+                            d11 += r0.w * Gy0 + r1.z * Gy1;
+                            d20 += r1.x * Gx0 + r1.w * Gx1;
+                            d21 += r1.x * Gy0 + r1.w * Gy1;
+                        }
+                        UPIC_MIXED_FENCE();
+                    }
+                } else {
+                    // the stencil changes inside the chunk (rare): point by
+                    // point, recomputing each point's stencil in a rolled loop
+#pragma unroll 1
+                    for (int k = 0; k < 4 && q0 + k < q.n; ++k) {
+                        const float s = sa + len * q.s[q0 + k];
+                        const float wk = len * q.w[q0 + k];
+                        const float thk = th0 + (th1 - th0) * s;
+                        AxF<R> sx, sy;
+                        axis_f<R>(t0x + s * dux, g.ihx, sx);
+                        axis_f<R>(t0y + s * duy, g.ihy, sy);
+                        const int ox = wrapn(ci + sx.fl, g.nx), oy = wrapn(cj + sy.fl, g.ny);
+#pragma unroll
+                        for (int a = 0; a <= R; ++a) {
+                            const int ia = wrap1n(ox + a, g.nx);
+#pragma unroll
+                            for (int b = 0; b <= R; ++b) {
+                                const int ib = wrap1n(oy + b, g.ny);
+                                const float W = wk * sx.w[a] * sy.w[b];
+                                const float sx_ = wk * sx.dw[a] * sy.w[b];
+                                const float sy_ = wk * sx.w[a] * sy.dw[b];
+                                const float Gx1 = thk * sx_, Gx0 = sx_ - Gx1;
+                                const float Gy1 = thk * sy_, Gy0 = sy_ - Gy1;
+                                float4 r0, r1;
+                                rec_f(NR, ia * g.ny + ib, r0, r1);
+                                g0 += W * r0.x;
+                                g1 += W * r0.y;
+                                d00 += r0.z * Gx0 + r1.y * Gx1;
+                                d01 += r0.z * Gy0 + r1.y * Gy1;
+                                d10 += r0.w * Gx0 + r1.z * Gx1;
+                                d11 += r0.w * Gy0 + r1.z * Gy1;
+                                d20 += r1.x * Gx0 + r1.w * Gx1;
+                                d21 += r1.x * Gy0 + r1.w * Gy1;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        ++nseg;
+        sa = sb;
+        if (sb >= 1.0f) break;
+    }
+    gat[0] = g0; gat[1] = g1;
+    D[0][0] = d00; D[0][1] = d01;
+    D[1][0] = d10; D[1][1] = d11;
+    D[2][0] = d20; D[2][1] = d21;
+    return nseg;
+}
+
+// FP32 LDS deposit tile of kTileM x kTileM nodes x 3 components; origin three
+// cells below the workgroup's first particle cell.
+constexpr int kTileM = 8;
+struct DepTileF {
+    float* t;
+    int ox, oy;
+};
+
+__device__ __forceinline__ void tilef_add(const MGrid& g, const DepTileF& T, double* __restrict__ J, int ia, int ib,
+                                          float j0, float j1, float j2) {
+    int dx = ia - T.ox;
+    if (dx < 0) dx += g.nx;
+    int dy = ib - T.oy;
+    if (dy < 0) dy += g.ny;
+    if (dx < kTileM && dy < kTileM) {
+        float* c = T.t + 3 * (dx * kTileM + dy);
+        atomicAdd(c, j0);
+        atomicAdd(c + 1, j1);
+        atomicAdd(c + 2, j2);
+    } else {
+        const int id = ia * g.ny + ib;
+        unsafeAtomicAdd(&J[id], (double)j0);
+        unsafeAtomicAdd(&J[g.N + id], (double)j1);
+        unsafeAtomicAdd(&J[2 * g.N + id], (double)j2);
+    }
+}
+
+// Orbit current deposit of one particle (weights per stencil window kept in
+// registers, flushed when the window moves).
+template <int R>
+__device__ __forceinline__ int deposit_orbit_f(const MGrid& g, const QuadF& q, bool split, int ci, int cj, float t0x,
+                                               float t0y, float dux, float duy, const float vel[3], float qs,
+                                               const DepTileF& T, double* __restrict__ J) {
+    float W[R + 1][R + 1];
+    int bx = 0, by = 0;
+    bool have = false;
+    auto flush = [&]() {
+        if (!have) return;
+        const int ox = wrapn(ci + bx, g.nx), oy = wrapn(cj + by, g.ny);
+#pragma unroll
+        for (int a = 0; a <= R; ++a) {
+            const int ia = wrap1n(ox + a, g.nx);
+#pragma unroll
+            for (int b = 0; b <= R; ++b) {
+                const int ib = wrap1n(oy + b, g.ny);
+                const float w = W[a][b] * qs;
+                tilef_add(g, T, J, ia, ib, w * vel[0], w * vel[1], w * vel[2]);
+            }
+        }
+        have = false;
+    };
+    BreakF bg;
+    breakf_init(bg, R, t0x, t0y, split ? dux : 0.0f, split ? duy : 0.0f);
+    float sa = 0.0f;
+    int nseg = 0;
+    for (;;) {
+        const float sb = breakf_next(bg);
+        const float len = sb - sa;
+        if (len > 0.0f) {
+            for (int iq = 0; iq < q.n; ++iq) {
+                const float s = sa + len * q.s[iq];
+                const float w = len * q.w[iq];
+                AxF<R> sx, sy;
+                axis_f<R>(t0x + s * dux, g.ihx, sx);
+                axis_f<R>(t0y + s * duy, g.ihy, sy);
+                if (!(have && sx.fl == bx && sy.fl == by)) {
+                    flush();
+                    bx = sx.fl;
+                    by = sy.fl;
+#pragma unroll
+                    for (int a = 0; a <= R; ++a)
+#pragma unroll
+                        for (int b = 0; b <= R; ++b) W[a][b] = 0.0f;
+                    have = true;
+                }
+#pragma unroll
+                for (int a = 0; a <= R; ++a) {
+                    const float wa = w * sx.w[a];
+#pragma unroll
+                    for (int b = 0; b <= R; ++b) W[a][b] += wa * sy.w[b];
+                }
+            }
+        }
+        ++nseg;
+        sa = sb;
+        if (sb >= 1.0f) break;
+    }
+    flush();
+    return nseg;
+}
+
+__device__ __forceinline__ DepTileF tilef_begin(const MGrid& g, float* lds, const int* __restrict__ cell, int64_t np,
+                                                int64_t p_first) {
+    __shared__ int org[2];
+    if (threadIdx.x == 0) {
+        int ox = 0, oy = 0;
+        if (p_first < np) {
+            const int c = cell[p_first];
+            ox = wrapn(c / g.ny - 3, g.nx);
+            oy = wrapn(c % g.ny - 3, g.ny);
+        }
+        org[0] = ox;
+        org[1] = oy;
+    }
+    for (int i = threadIdx.x; i < kTileM * kTileM * 3; i += blockDim.x) lds[i] = 0.0f;
+    __syncthreads();
+    return DepTileF{lds, org[0], org[1]};
+}
+
+__device__ __forceinline__ void tilef_end(const MGrid& g, const DepTileF& T, double* __restrict__ J) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < kTileM * kTileM * 3; i += blockDim.x) {
+        const float v = T.t[i];
+        if (v != 0.0f) {
+            const int comp = i % 3, cell = i / 3;
+            const int dx = cell / kTileM, dy = cell % kTileM;
+            if (dx >= g.nx || dy >= g.ny) continue;
+            const int ia = (T.ox + dx) % g.nx, ib = (T.oy + dy) % g.ny;
+            unsafeAtomicAdd(&J[(size_t)comp * g.N + ia * g.ny + ib], (double)v);
+        }
+    }
+}
+
+__device__ __forceinline__ float gamma_f(const float p[3], float mk) {
+    const float imk = 1.0f / mk;
+    const float u0 = p[0] * imk, u1 = p[1] * imk, u2 = p[2] * imk;
+    return sqrtf(1.0f + (u0 * u0 + u1 * u1 + u2 * u2));
+}
+
+// higuera_cary_rotate_with_K (P3:4636-4655), FP32.  The rotation is linear
+// and homogeneous in u = p/(m kappa), so it is applied to p directly; 1/(m
+// kappa) enters only the dimensionless gamma terms.  Rotating u and scaling
+// back by m kappa multiplies every momentum by m kappa * fl(1/(m kappa)),
+// a factor 1 +- 6e-8 that is the same for all particles of one mass: a
+// coherent shrink or growth of the kinetic energy every step.
+__device__ __forceinline__ void hc_rotate_f(const float pm[3], const float B[3], float q, float m, float kappa, float dt,
+                                            float out[3]) {
+    const float imk = 1.0f / (m * kappa);
+    const float u2 = (pm[0] * pm[0] + pm[1] * pm[1] + pm[2] * pm[2]) * (imk * imk);
+    const float gm2 = 1.0f + u2;
+    const float c = q * dt / (2.0f * m);
+    const float tau[3] = {c * B[0], c * B[1], c * B[2]};
+    const float tau2 = tau[0] * tau[0] + tau[1] * tau[1] + tau[2] * tau[2];
+    const float udt = (pm[0] * tau[0] + pm[1] * tau[1] + pm[2] * tau[2]) * imk;
+    const float sigma = gm2 - tau2;
+    const float ghc2 = 0.5f * (sigma + sqrtf(sigma * sigma + 4.0f * (tau2 + udt * udt)));
+    const float ghc = sqrtf(fmaxf(ghc2, 1.0f));
+    const float ig = 1.0f / ghc;
+    const float t[3] = {ig * tau[0], ig * tau[1], ig * tau[2]};
+    const float sc = 2.0f / (1.0f + (t[0] * t[0] + t[1] * t[1] + t[2] * t[2]));
+    const float sv[3] = {sc * t[0], sc * t[1], sc * t[2]};
+    const float up[3] = {pm[0] + (pm[1] * t[2] - pm[2] * t[1]), pm[1] + (pm[2] * t[0] - pm[0] * t[2]),
+                         pm[2] + (pm[0] * t[1] - pm[1] * t[0])};
+    out[0] = pm[0] + (up[1] * sv[2] - up[2] * sv[1]);
+    out[1] = pm[1] + (up[2] * sv[0] - up[0] * sv[2]);
+    out[2] = pm[2] + (up[0] * sv[1] - up[1] * sv[0]);
+}
+
+struct LocalMapF {
+    float vbar[3], P_end[3], v_end[3];
+};
+
+// evaluate_local_particle_substep (P3:4843-4884) in cell-relative FP32.
+template <int R>
+__device__ __forceinline__ void local_map_f(const MGrid& g, const QuadF& qd, bool split, const float4* NR,
+                                            const float gs[3], int ci, int cj, float tx, float ty, const float P0[3],
+                                            const float A0p[3], const float pst[3], float q, float m, float kappa,
+                                            float h, float th0, float th1, LocalMapF& out) {
+    const float dux = h * gs[0] * g.ihx, duy = h * gs[1] * g.ihy;
+    float A1p[3];
+    gather_interp_f<R>(g, NR, ci, cj, tx + dux, ty + duy, th1, A1p);
+    float gpb[2], D[3][2];
+    orbit_gg_f<R>(g, qd, split, NR, ci, cj, tx, ty, dux, duy, th0, th1, gpb, D);
+    const float Aep[3] = {0.5f * (A0p[0] + A1p[0]), 0.5f * (A0p[1] + A1p[1]), 0.5f * (A0p[2] + A1p[2])};
+    float Dv[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) Dv[i] = D[i][0] * gs[0] + D[i][1] * gs[1];
+    // K = D^T - D with the z column of D zero: B_eff = (K12, -K02, K01)
+    const float B[3] = {D[2][1], -D[2][0], D[1][0] - D[0][1]};
+    const float hq = 0.5f * q * h;
+    const float gp3[3] = {gpb[0], gpb[1], 0.0f};
+    float pm[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pm[d] = P0[d] - q * Aep[d] - hq * gp3[d] + hq * Dv[d];
+    float pp[3];
+    hc_rotate_f(pm, B, q, m, kappa, h, pp);
+    float pe[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float t = pp[d] + q * Aep[d] - hq * gp3[d] + hq * Dv[d];
+        out.P_end[d] = t;
+        pe[d] = t - q * A1p[d];
+    }
+    const float mk = m * kappa;
+    const float ge = gamma_f(pe, mk);
+    const float ive = 1.0f / (m * ge);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out.v_end[d] = ive * pe[d];
+    // kinetic_secant_velocity (P3:682-689)
+    const float gmi = gamma_f(pm, mk), gst = gamma_f(pst, mk), gpl = gamma_f(pp, mk);
+    const float c1 = 1.0f / (m * (gmi + gst)), c2 = 1.0f / (m * (ge + gpl));
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out.vbar[d] = 0.5f * (c1 * (pm[d] + pst[d]) + c2 * (pe[d] + pp[d]));
+}
+
+__device__ __forceinline__ void ld3f(const float* __restrict__ a, int64_t np, int64_t p, float o[3]) {
+    o[0] = a[p]; o[1] = a[np + p]; o[2] = a[2 * np + p];
+}
+__device__ __forceinline__ void st3f(float* __restrict__ a, int64_t np, int64_t p, const float v[3]) {
+    a[p] = v[0]; a[np + p] = v[1]; a[2 * np + p] = v[2];
+}
+__device__ __forceinline__ double norm3f(const float v[3]) {
+    const double a = v[0], b = v[1], c = v[2];
+    return sqrt(a * a + b * b + c * c);
+}
+__device__ __forceinline__ bool finite3f(const float v[3]) {
+    return isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]);
+}
+
+// advance a cell-relative coordinate by du cells and renormalise into [0, 1)
+__device__ __forceinline__ void move_axis(float& t, float du, int& c, int n) {
+    const float u = t + du;
+    const float f = floorf(u);
+    float r = u - f;
+    int k = (int)f;
+    if (r >= 1.0f) { r = 0.0f; k += 1; }
+    t = r;
+    c = wrapn(c + k, n);
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float tile_mem[kTileM * kTileM * 3];
+    const DepTileF T = tilef_begin(g, tile_mem, a.cell, a.np, (int64_t)blockIdx.x * blockDim.x);
+    double rsum[2] = {0.0, 0.0}, rmax[2] = {0.0, 0.0};
+    int nseg_total = 0;
+    if (p < a.np) {
+        const int nsub = a.layout ? (int)a.layout[p] : 1;
+        const float h = (float)(a.dt / (double)nsub);
+        const int c0 = a.cell[p];
+        int ci = c0 / g.ny, cj = c0 - (c0 / g.ny) * g.ny;
+        float tx = a.off[p], ty = a.off[a.np + p], tz = a.off[2 * a.np + p];
+        float Pc[3], vc[3];
+        ld3f(a.P, a.np, p, Pc);
+        ld3f(a.v, a.np, p, vc);
+        double vmac[3] = {0.0, 0.0, 0.0};
+        const float qp = a.q[p], mp = a.m[p];
+        const float kappa = (float)a.kappa;
+        int reason = 0;
+        for (int s = 0; s < nsub && reason == 0; ++s) {
+            const float th0 = (float)s / (float)nsub, th1 = (float)(s + 1) / (float)nsub;
+            float A0p[3];
+            gather_interp_f<R>(g, a.NR, ci, cj, tx, ty, th0, A0p);
+            const float pst[3] = {Pc[0] - qp * A0p[0], Pc[1] - qp * A0p[1], Pc[2] - qp * A0p[2]};
+            float gs[3] = {vc[0], vc[1], vc[2]};
+            if (a.guess && nsub == 1) ld3f(a.guess, a.np, p, gs);
+            LocalMapF acc;
+            float vacc[3];
+            bool conv = false;
+            int its = 0;
+            double scaled = 0.0;
+            reason = 5;  // maximum_iterations
+            for (int it = 0; it < a.max_iter; ++it) {
+                LocalMapF mpd;
+                local_map_f<R>(g, qd, split != 0, a.NR, gs, ci, cj, tx, ty, Pc, A0p, pst, qp, mp, kappa, h, th0, th1, mpd);
+                ++its;
+                if (!finite3f(mpd.vbar)) { reason = 1; break; }
+                const double nm = norm3f(mpd.vbar);
+                if (nm >= a.kappa) { reason = 2; break; }
+                const float res[3] = {mpd.vbar[0] - gs[0], mpd.vbar[1] - gs[1], mpd.vbar[2] - gs[2]};
+                const double scale = 0.5 * (nm + norm3f(gs));
+                scaled = norm3f(res) / (a.atol + a.rtol * scale);
+                if (scaled <= 1.0) {
+                    vacc[0] = gs[0]; vacc[1] = gs[1]; vacc[2] = gs[2];
+                    acc = mpd;
+                    conv = true;
+                    reason = 0;
+                    break;
+                }
+                const float damp = (float)a.damping;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) gs[d] = gs[d] + damp * res[d];
+                if (!finite3f(gs)) { reason = 3; break; }
+                if (norm3f(gs) >= a.kappa) { reason = 4; break; }
+            }
+            rsum[0] += (double)its;
+            rmax[0] = fmax(rmax[0], (double)its);
+            rmax[1] = fmax(rmax[1], scaled);
+            if (!conv) break;
+            const float dux = h * vacc[0] * g.ihx, duy = h * vacc[1] * g.ihy;
+            if (a.J_out)
+                nseg_total += deposit_orbit_f<R>(g, qd, split != 0, ci, cj, tx, ty, dux, duy, vacc,
+                                                 qp / (float)nsub * a.inv_dV, T, a.J_out);
+            move_axis(tx, dux, ci, g.nx);
+            move_axis(ty, duy, cj, g.ny);
+            int cz = 0;
+            move_axis(tz, h * vacc[2] * a.inv_Lz, cz, 1);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                Pc[d] = acc.P_end[d];
+                vc[d] = acc.v_end[d];
+                vmac[d] += (double)vacc[d] / (double)nsub;
+            }
+        }
+        if (a.reason) a.reason[p] = (uint8_t)reason;
+        if (reason != 0) rsum[1] = 1.0;
+        if (reason == 0 && a.write_out) {
+            a.cell_out[p] = ci * g.ny + cj;
+            a.off_out[p] = tx;
+            a.off_out[a.np + p] = ty;
+            a.off_out[2 * a.np + p] = tz;
+            st3f(a.P_out, a.np, p, Pc);
+            st3f(a.v_out, a.np, p, vc);
+            const float vm[3] = {(float)vmac[0], (float)vmac[1], (float)vmac[2]};
+            st3f(a.vbar_out, a.np, p, vm);
+        }
+    }
+    if (a.J_out) tilef_end(g, T, a.J_out);
+    block_reduce_add<2>(rsum, a.red);
+    block_reduce_max<2>(rmax, a.redmax);
+    path_stats_add(nseg_total, a.stats);
+}
+
+// FP32 node records from the FP64 fields
+__global__ void k_pack_nodes_f32(int N, const double* __restrict__ gpm, const double* __restrict__ A0,
+                                 const double* __restrict__ A1, float4* __restrict__ NR) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= N) return;
+    NR[2 * id] = make_float4((float)gpm[id], (float)gpm[N + id], (float)A0[id], (float)A0[N + id]);
+    NR[2 * id + 1] = make_float4((float)A0[2 * N + id], (float)A1[id], (float)A1[N + id], (float)A1[2 * N + id]);
+}
+
+// FP64 particle arrays -> mixed state.  q and m are rounded to FP32 and the
+// FP64 copies replaced by the rounded values: every particle shares one
+// charge and one mass, so a rounding left only on the FP32 side would be a
+// coherent bias between the FP32 push and the FP64 field solve and
+// diagnostics (an energy drift of order 1e-7 per step).
+__global__ void k_mixed_compress(GridDev g, int64_t np, const double* __restrict__ x, const double* __restrict__ P,
+                                 const double* __restrict__ v, const double* __restrict__ vp,
+                                 const double* __restrict__ vb, double* __restrict__ q, double* __restrict__ m,
+                                 MixedState s) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int c[2];
+    const int n[2] = {g.nx, g.ny};
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+        const double u = (wrap1(x[d * np + p], g.L[d]) + 0.5 * g.L[d]) / g.h[d];
+        int i = (int)floor(u);
+        float t = (float)(u - (double)i);
+        if (t >= 1.0f) { t = 0.0f; i += 1; }
+        if (i >= n[d]) i -= n[d];
+        if (i < 0) i += n[d];
+        c[d] = i;
+        s.off[d * np + p] = t;
+    }
+    {
+        float t = (float)((wrap1(x[2 * np + p], g.L[2]) + 0.5 * g.L[2]) / g.L[2]);
+        if (t >= 1.0f) t = 0.0f;
+        s.off[2 * np + p] = t;
+    }
+    s.cell[p] = c[0] * g.ny + c[1];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        s.P[d * np + p] = (float)P[d * np + p];
+        s.v[d * np + p] = (float)v[d * np + p];
+        s.vp[d * np + p] = (float)vp[d * np + p];
+        s.vb[d * np + p] = (float)vb[d * np + p];
+    }
+    const float qf = (float)q[p], mf = (float)m[p];
+    s.q[p] = qf;
+    s.m[p] = mf;
+    q[p] = (double)qf;
+    m[p] = (double)mf;
+}
+
+// mixed state -> FP64 particle arrays (x in [-L/2, L/2))
+__global__ void k_mixed_expand(GridDev g, int64_t np, MixedState s, double* __restrict__ x, double* __restrict__ P,
+                               double* __restrict__ v, double* __restrict__ vp, double* __restrict__ vb) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int c = s.cell[p];
+    const int ci = c / g.ny, cj = c - (c / g.ny) * g.ny;
+    x[p] = -0.5 * g.L[0] + ((double)ci + (double)s.off[p]) * g.h[0];
+    x[np + p] = -0.5 * g.L[1] + ((double)cj + (double)s.off[np + p]) * g.h[1];
+    x[2 * np + p] = -0.5 * g.L[2] + (double)s.off[2 * np + p] * g.L[2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        P[d * np + p] = (double)s.P[d * np + p];
+        v[d * np + p] = (double)s.v[d * np + p];
+        vp[d * np + p] = (double)s.vp[d * np + p];
+        vb[d * np + p] = (double)s.vb[d * np + p];
+    }
+}
+
+// counting-sort keys straight from the cell index
+__global__ void k_mixed_keys(int64_t np, const int* __restrict__ cell, int* __restrict__ keys, int* __restrict__ counts) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int k = cell[p];
+    keys[p] = k;
+    atomicAdd(&counts[k], 1);
+}
+
+template <class T>
+__global__ void k_permute_t(int64_t np, int ncomp, const T* __restrict__ src, T* __restrict__ dst,
+                            const int* __restrict__ perm) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const int64_t j = perm[i];
+    for (int c = 0; c < ncomp; ++c) dst[c * np + i] = src[c * np + j];
+}
+
+inline unsigned mblk(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& a) {
+    if (a.np <= 0) return;
+    MGrid mg;
+    mg.nx = g.nx; mg.ny = g.ny; mg.N = g.N; mg.order = g.order;
+    mg.hx = (float)g.h[0]; mg.hy = (float)g.h[1];
+    mg.ihx = (float)(1.0 / g.h[0]); mg.ihy = (float)(1.0 / g.h[1]);
+    switch (g.order) {
+        case 1: hipLaunchKernelGGL(k_p3_map_mixed<1>, dim3(mblk(a.np)), dim3(256), 0, s, mg, q, split, a); break;
+        case 2: hipLaunchKernelGGL(k_p3_map_mixed<2>, dim3(mblk(a.np)), dim3(256), 0, s, mg, q, split, a); break;
+        case 3: hipLaunchKernelGGL(k_p3_map_mixed<3>, dim3(mblk(a.np)), dim3(256), 0, s, mg, q, split, a); break;
+        case 4: hipLaunchKernelGGL(k_p3_map_mixed<4>, dim3(mblk(a.np)), dim3(256), 0, s, mg, q, split, a); break;
+        default: break;
+    }
+}
+
+void launch_pack_nodes_f32(hipStream_t s, int N, const double* gpm, const double* A_n, const double* A_end, float* NR) {
+    hipLaunchKernelGGL(k_pack_nodes_f32, dim3(mblk(N)), dim3(256), 0, s, N, gpm, A_n, A_end,
+                       reinterpret_cast<float4*>(NR));
+}
+
+void launch_mixed_compress(hipStream_t s, const GridDev& g, int64_t np, const double* x, const double* P,
+                           const double* v, const double* vp, const double* vb, double* q, double* m,
+                           const MixedState& ms) {
+    if (np > 0) hipLaunchKernelGGL(k_mixed_compress, dim3(mblk(np)), dim3(256), 0, s, g, np, x, P, v, vp, vb, q, m, ms);
+}
+
+void launch_mixed_expand(hipStream_t s, const GridDev& g, int64_t np, const MixedState& ms, double* x, double* P,
+                         double* v, double* vp, double* vb) {
+    if (np > 0) hipLaunchKernelGGL(k_mixed_expand, dim3(mblk(np)), dim3(256), 0, s, g, np, ms, x, P, v, vp, vb);
+}
+
+void launch_mixed_keys(hipStream_t s, int64_t np, const int* cell, int* keys, int* counts) {
+    if (np > 0) hipLaunchKernelGGL(k_mixed_keys, dim3(mblk(np)), dim3(256), 0, s, np, cell, keys, counts);
+}
+
+void launch_permute_f32(hipStream_t s, int64_t np, int ncomp, const float* src, float* dst, const int* perm) {
+    if (np > 0) hipLaunchKernelGGL(k_permute_t<float>, dim3(mblk(np)), dim3(256), 0, s, np, ncomp, src, dst, perm);
+}
+
+void launch_permute_i32(hipStream_t s, int64_t np, const int* src, int* dst, const int* perm) {
+    if (np > 0) hipLaunchKernelGGL(k_permute_t<int>, dim3(mblk(np)), dim3(256), 0, s, np, 1, src, dst, perm);
+}
+
+}  // namespace upic

@@ -139,21 +139,33 @@ __device__ __forceinline__ int orbit_gg_2d(const GridDev& g, const QuadDev& q, c
                         }
                     }
                 } else {
-                    // stencil changes inside the chunk: point by point
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) {
+                    // the stencil changes inside the chunk (rare): point by
+                    // point, recomputing each point's stencil in a rolled loop
+#pragma unroll 1
+                    for (int k = 0; k < 4 && q0 + k < q.n; ++k) {
+                        const double s = sa + len * q.s[q0 + k];
+                        const double wk = len * q.w[q0 + k];
+                        const double thk = p1_theta ? s : th0 + (th1 - th0) * s;
+                        double xq0 = x0[0] + s * dxp[0], xq1 = x0[1] + s * dxp[1];
+                        if (opt.wrap_path) {
+                            xq0 = wrap1(xq0, g.L[0]);
+                            xq1 = wrap1(xq1, g.L[1]);
+                        }
+                        AxisSt<R> sx, sy;
+                        axis_stencil<R>(wrap1(xq0, g.L[0]), g.nx, g.L[0], g.h[0], sx);
+                        axis_stencil<R>(wrap1(xq1, g.L[1]), g.ny, g.L[1], g.h[1], sy);
 #pragma unroll
                         for (int a = 0; a <= R; ++a) {
-                            const int ia = wrap_idx(bx[k] + a, g.nx);
+                            const int ia = wrap_idx(sx.base + a, g.nx);
 #pragma unroll
                             for (int b = 0; b <= R; ++b) {
-                                const int ib = wrap_idx(by[k] + b, g.ny);
-                                const double ws = wq[k] * wy[k][b];
-                                const double W = wx[k][a] * ws;
-                                const double sx_ = dwx[k][a] * ws;
-                                const double sy_ = wx[k][a] * wq[k] * dwy[k][b];
-                                const double Gx1 = th[k] * sx_, Gx0 = sx_ - Gx1;
-                                const double Gy1 = th[k] * sy_, Gy0 = sy_ - Gy1;
+                                const int ib = wrap_idx(sy.base + b, g.ny);
+                                const double ws = wk * sy.w[b];
+                                const double W = sx.w[a] * ws;
+                                const double sx_ = sx.dw[a] * ws;
+                                const double sy_ = sx.w[a] * wk * sy.dw[b];
+                                const double Gx1 = thk * sx_, Gx0 = sx_ - Gx1;
+                                const double Gy1 = thk * sy_, Gy0 = sy_ - Gy1;
                                 double r[8];
                                 load_rec(NR, ia * g.ny + ib, r);
                                 g0 += W * r[0];

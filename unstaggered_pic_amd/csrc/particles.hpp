@@ -272,4 +272,23 @@ __device__ __forceinline__ void block_reduce_max(double v[K], double* __restrict
     }
 }
 
+// Path statistics accumulator: [0] crossing particles, [1] total segments,
+// [2] max segments (PathStats, P1:771-776).
+__device__ __forceinline__ void path_stats_add(int nseg, unsigned long long* __restrict__ stats) {
+    if (!stats) return;
+    const int lane = threadIdx.x & 63;
+    unsigned long long cross = nseg > 1 ? 1ull : 0ull, segs = (unsigned long long)nseg;
+    int mx = nseg;
+    for (int o = 32; o > 0; o >>= 1) {
+        cross += __shfl_down(cross, o, 64);
+        segs += __shfl_down(segs, o, 64);
+        mx = max(mx, __shfl_down(mx, o, 64));
+    }
+    if (lane == 0) {
+        atomicAdd(&stats[0], cross);
+        atomicAdd(&stats[1], segs);
+        atomicMax(&stats[2], (unsigned long long)mx);
+    }
+}
+
 }  // namespace upic

@@ -48,7 +48,25 @@ struct upic_sim {
     double* fixed_ion = nullptr;
     int64_t steps = 0;
     std::vector<uint8_t> host_reason;
+    // mixed-precision particle state (precision = UPIC_PREC_MIXED): the map
+    // reads ms and writes mo; the FP64 arrays above are a view rebuilt on
+    // demand (diagnostics, download) while fp64_stale is set
+    bool mixed = false;
+    bool fp64_stale = false;
+    MixedState ms{}, mo{};
+    float* ftmp = nullptr;   // 3*np sort scratch
+    float* ftmp1 = nullptr;  // np sort scratch
+    int* itmp = nullptr;     // np sort scratch
+    float* NRf = nullptr;   // FP32 node records (8N)
+    double outer_atol = 0.0;  // absolute outer tolerance in force for this step
 };
+
+// Mixed precision: an FP32 map resolves each node current only to a fraction
+// of the current's overall scale, so the outer absolute tolerance is floored
+// at nonlinear_rtol times the step-start rms node current, and the final
+// consistency evaluation (which re-runs the map with a different FP32 atomic
+// summation order) is accepted within twice the tolerance.
+constexpr double kMixedFinalSlack = 2.0;
 
 namespace {
 
@@ -63,6 +81,27 @@ void sim_free(upic_sim* s) {
     for (int i = 0; i < 5; ++i) { dfree(s->phi_hist[i]); dfree(s->A_hist[i]); dfree(s->rho_hist[i]); }
     dfree(s->id); dfree(s->idtmp); dfree(s->reason); dfree(s->layout);
     dfree(s->keys); dfree(s->counts); dfree(s->offsets); dfree(s->perm);
+    float** fl[] = {&s->ms.off, &s->ms.P, &s->ms.v, &s->ms.vp, &s->ms.vb, &s->ms.q, &s->ms.m,
+                    &s->mo.off, &s->mo.P, &s->mo.v, &s->mo.vb, &s->ftmp, &s->ftmp1, &s->NRf};
+    for (float** p : fl) dfree(*p);
+    dfree(s->ms.cell); dfree(s->mo.cell); dfree(s->itmp);
+}
+
+// FP64 particle arrays -> mixed state
+void mixed_compress(upic_sim* s) {
+    upic_ctx* c = s->c;
+    launch_mixed_compress(c->stream, c->g, s->np, s->x, s->P, s->v, s->v_prev, s->vbar_last, s->q, s->m, s->ms);
+    HIPCHK(hipGetLastError());
+    s->fp64_stale = false;
+}
+
+// Bring the FP64 particle arrays up to date with the mixed state.
+void fp64_view(upic_sim* s) {
+    if (!s->mixed || !s->fp64_stale) return;
+    upic_ctx* c = s->c;
+    launch_mixed_expand(c->stream, c->g, s->np, s->ms, s->x, s->P, s->v, s->v_prev, s->vbar_last);
+    HIPCHK(hipGetLastError());
+    s->fp64_stale = false;
 }
 
 inline double* Zf(double* base, int k, int N) { return base + (size_t)2 * N * k; }
@@ -113,7 +152,8 @@ void field_solve(upic_sim* s, const double* Jreal) {
     launch_copy(c->stream, 6ll * N, Zf(s->S1, 6, N), Zf(s->zs, 0, N));
     launch_copy(c->stream, 6ll * N, Zf(s->S1, 12, N), Zf(s->zs, 3, N));
     from_spectra(c, s->zs, s->AG, 6);
-    if (c->g.nz == 1) launch_pack_nodes(c->stream, N, s->AG + 3 * (size_t)N, s->A, s->AG, s->NR);
+    if (s->mixed) launch_pack_nodes_f32(c->stream, N, s->AG + 3 * (size_t)N, s->A, s->AG, s->NRf);
+    else if (c->g.nz == 1) launch_pack_nodes(c->stream, N, s->AG + 3 * (size_t)N, s->A, s->AG, s->NR);
 }
 
 // End-of-step real mesh state from the last field solve.
@@ -155,6 +195,32 @@ void sort_particles(upic_sim* s) {
     const int ncell = c->N();
     if (np == 0) return;
     HIPCHK(hipMemsetAsync(s->counts, 0, ncell * sizeof(int), c->stream));
+    if (s->mixed) {
+        launch_mixed_keys(c->stream, np, s->ms.cell, s->keys, s->counts);
+        launch_scan_counts(c->stream, ncell, s->counts, s->offsets);
+        launch_scatter_perm(c->stream, np, s->keys, s->offsets, s->perm);
+        float** f3[] = {&s->ms.off, &s->ms.P, &s->ms.v, &s->ms.vp, &s->ms.vb};
+        for (float** a : f3) {
+            launch_permute_f32(c->stream, np, 3, *a, s->ftmp, s->perm);
+            std::swap(*a, s->ftmp);
+        }
+        float** f1[] = {&s->ms.q, &s->ms.m};
+        for (float** a : f1) {
+            launch_permute_f32(c->stream, np, 1, *a, s->ftmp1, s->perm);
+            std::swap(*a, s->ftmp1);
+        }
+        launch_permute_i32(c->stream, np, s->ms.cell, s->itmp, s->perm);
+        std::swap(s->ms.cell, s->itmp);
+        double** arr1[] = {&s->q, &s->m};
+        for (double** a : arr1) {
+            launch_permute(c->stream, np, 1, *a, s->ptmp1, s->perm);
+            std::swap(*a, s->ptmp1);
+        }
+        launch_permute_i64(c->stream, np, s->id, s->idtmp, s->perm);
+        std::swap(s->id, s->idtmp);
+        s->fp64_stale = true;  // the FP64 view is rebuilt from the permuted state
+        return;
+    }
     launch_cell_keys(c->stream, c->g, np, s->x, s->keys, s->counts);
     launch_scan_counts(c->stream, ncell, s->counts, s->offsets);
     launch_scatter_perm(c->stream, np, s->keys, s->offsets, s->perm);
@@ -292,6 +358,22 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     field_solve(s, s->Jfield);
     launch_zero(c->stream, 3ll * N, Jmap);
     zero_red(c);
+    if (s->mixed) {
+        P3MArgs a{};
+        a.np = s->np; a.dt = p.dt; a.kappa = p.c; a.rtol = p.local_rtol; a.atol = p.local_atol;
+        a.damping = p.local_damping; a.max_iter = p.local_max_iter; a.write_out = write_out ? 1 : 0;
+        a.inv_dV = (float)(1.0 / c->g.dV);
+        a.inv_Lz = (float)(1.0 / p.Lz);
+        a.cell = s->ms.cell; a.off = s->ms.off; a.P = s->ms.P; a.v = s->ms.v; a.q = s->ms.q; a.m = s->ms.m;
+        a.layout = use_layout ? s->layout : nullptr;
+        a.guess = nullptr;
+        a.NR = reinterpret_cast<const float4*>(s->NRf);
+        a.J_out = Jmap;
+        a.cell_out = s->mo.cell; a.off_out = s->mo.off; a.P_out = s->mo.P; a.v_out = s->mo.v; a.vbar_out = s->mo.vb;
+        a.reason = s->reason;
+        a.red = c->dred; a.redmax = c->dred + 2; a.stats = c->dstats;
+        timed(c, "p3_map_mixed", [&] { launch_p3_map_mixed(c->stream, c->g, c->qf, c->opt.split, a); });
+    } else {
     P3Args a{};
     a.np = s->np; a.dt = p.dt; a.kappa = p.c; a.rtol = p.local_rtol; a.atol = p.local_atol;
     a.damping = p.local_damping; a.max_iter = p.local_max_iter; a.write_out = write_out ? 1 : 0;
@@ -305,6 +387,7 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     a.reason = s->reason;
     a.red = c->dred; a.redmax = c->dred + 2; a.stats = c->dstats;
     timed(c, "p3_map", [&] { launch_p3_map(c->stream, c->g, c->q, c->opt, a); });
+    }
     launch_project_nyquist(c->stream, c->sp, Jmap, 3);
     HIPCHK(hipGetLastError());
     read_red(c, 4);
@@ -336,7 +419,8 @@ Norms residual_norms(upic_sim* s, const double* map, const double* guess) {
     const int N = c->N();
     const double floor_ = 64.0 * std::numeric_limits<double>::epsilon() * p.c;
     launch_zero(c->stream, 8, c->dred + 8);
-    launch_residual_norms(c->stream, N, map, guess, floor_, p.nonlinear_rtol, p.nonlinear_atol, c->dred + 8);
+    const double atol = s->outer_atol > 0.0 ? s->outer_atol : p.nonlinear_atol;
+    launch_residual_norms(c->stream, N, map, guess, floor_, p.nonlinear_rtol, atol, c->dred + 8);
     HIPCHK(hipMemcpyAsync(c->hred + 8, c->dred + 8, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     c->sync();
     const double* h = c->hred + 8;
@@ -345,7 +429,7 @@ Norms residual_norms(upic_sim* s, const double* map, const double* guess) {
     n.rms_absolute = std::sqrt(h[0] / cnt);
     const double vrms = 0.5 * (std::sqrt(h[1] / cnt) + std::sqrt(h[2] / cnt));
     n.rms_relative = n.rms_absolute / (vrms + floor_);
-    n.rms_scaled = n.rms_absolute / (p.nonlinear_atol + p.nonlinear_rtol * vrms);
+    n.rms_scaled = n.rms_absolute / (atol + p.nonlinear_rtol * vrms);
     n.max_particle_relative = h[3];
     n.max_particle_scaled = h[4];
     n.nonfinite = h[5] != 0.0;
@@ -557,10 +641,12 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
         std::swap(s->Jacc, s->Jtry);
         have_prefetched = true;
     }
-    if (!converged) throw Error(UPIC_ENONLINEAR, "outer macro-current solve failed after " + std::to_string(iters) + " iterations");
+    if (!converged)
+        throw Error(UPIC_ENONLINEAR, "outer macro-current solve failed after " + std::to_string(iters) + " iterations");
     const MapOut fo = eval(s->guess, s->Jacc, true);
     const Norms fn = residual_norms(s, s->Jacc, s->guess);
-    const bool ok = !fn.nonfinite && fn.rms_scaled <= 1.0 && fn.max_particle_scaled <= 1.0;
+    const double slack = s->mixed ? kMixedFinalSlack : 1.0;
+    const bool ok = !fn.nonfinite && fn.rms_scaled <= slack && fn.max_particle_scaled <= slack;
     if (info) {
         std::memset(info, 0, sizeof(*info));
         info->iterations = iters;
@@ -581,8 +667,23 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
         info->local_iterations_total = local_total;
         info->path = fo.ps;
     }
-    if (!ok) throw Error(UPIC_ENONLINEAR, "final outer macro-current consistency evaluation failed at the prescribed macro dt");
+    if (!ok)
+        throw Error(UPIC_ENONLINEAR, "final outer macro-current consistency evaluation failed at the prescribed macro dt"
+                                     " (scaled rms " + std::to_string(fn.rms_scaled) + ", scaled max " +
+                                     std::to_string(fn.max_particle_scaled) + ")");
     // commit: v_prev <- v, particle outputs in, mesh state from the final map
+    if (s->mixed) {
+        std::swap(s->ms.vp, s->ms.v);
+        std::swap(s->ms.v, s->mo.v);
+        std::swap(s->ms.cell, s->mo.cell);
+        std::swap(s->ms.off, s->mo.off);
+        std::swap(s->ms.P, s->mo.P);
+        std::swap(s->ms.vb, s->mo.vb);
+        s->fp64_stale = true;
+        commit_fields(s, s->Jacc);
+        c->sync();
+        return;
+    }
     std::swap(s->v_prev, s->v);   // v_prev = old v
     std::swap(s->v, s->vo);       // v = new v ; vo holds stale v_prev
     std::swap(s->x, s->xo);
@@ -597,6 +698,15 @@ void p3_step(upic_sim* s, upic_step_info* info) {
     upic_ctx* c = s->c;
     maybe_sort(s);
     state_spectra(s);
+    s->outer_atol = 0.0;
+    if (s->mixed) {
+        const int N = c->N();
+        launch_zero(c->stream, 1, c->dred + 20);
+        launch_sum_squares(c->stream, 3ll * N, s->J, c->dred + 20);
+        HIPCHK(hipMemcpyAsync(c->hred + 20, c->dred + 20, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        c->sync();
+        s->outer_atol = std::max(c->p.nonlinear_atol, c->p.nonlinear_rtol * std::sqrt(c->hred[20] / (double)N));
+    }
     try {
         p3_step_layout(s, false, info);
     } catch (const RefineNeeded& e) {
@@ -865,11 +975,29 @@ upic_status upic_sim_create(upic_ctx* c, int64_t np, upic_sim** out) {
             }
             s->fixed_ion = dalloc<double>(N);
         }
+        if (c->p.precision == UPIC_PREC_MIXED) {
+            s->mixed = true;
+            for (MixedState* m : {&s->ms, &s->mo}) {
+                m->cell = dalloc<int>(np);
+                m->off = dalloc<float>(n3);
+                m->P = dalloc<float>(n3);
+                m->v = dalloc<float>(n3);
+                m->vb = dalloc<float>(n3);
+            }
+            s->ms.vp = dalloc<float>(n3);
+            s->ms.q = dalloc<float>(np);
+            s->ms.m = dalloc<float>(np);
+            s->ftmp = dalloc<float>(n3);
+            s->ftmp1 = dalloc<float>(np);
+            s->itmp = dalloc<int>(np);
+            s->NRf = dalloc<float>(8 * (size_t)N);
+        }
         // identity ids
         std::vector<int64_t> ids((size_t)np);
         for (int64_t i = 0; i < np; ++i) ids[i] = i;
         HIPCHK(hipMemcpy(s->id, ids.data(), np * sizeof(int64_t), hipMemcpyHostToDevice));
         launch_zero(c->stream, n3, s->vbar_last);
+        if (s->mixed) mixed_compress(s);
     });
     if (rc != UPIC_OK) {
         sim_free(s);
@@ -892,6 +1020,7 @@ upic_status upic_sim_upload(upic_sim* s, const upic_host_state* st) {
     if (!s || !st || st->np != s->np) return UPIC_EINVAL;
     upic_ctx* c = s->c;
     return (upic_status)guarded(c, [&] {
+        fp64_view(s);  // arrays the caller leaves out keep their current values
         const int N = c->N();
         const size_t n3 = 3 * (size_t)s->np * sizeof(double), n1 = (size_t)s->np * sizeof(double);
         auto up = [&](double* d, const double* h, size_t bytes) {
@@ -913,6 +1042,7 @@ upic_status upic_sim_upload(upic_sim* s, const upic_host_state* st) {
         for (int64_t i = 0; i < s->np; ++i) ids[i] = i;
         HIPCHK(hipMemcpy(s->id, ids.data(), s->np * sizeof(int64_t), hipMemcpyHostToDevice));
         s->steps = 0;
+        if (s->mixed) mixed_compress(s);
     });
 }
 
@@ -920,6 +1050,7 @@ upic_status upic_sim_download(upic_sim* s, upic_host_state* st) {
     if (!s || !st || st->np != s->np) return UPIC_EINVAL;
     upic_ctx* c = s->c;
     return (upic_status)guarded(c, [&] {
+        fp64_view(s);
         c->sync();
         const int N = c->N();
         const int64_t np = s->np;
@@ -997,6 +1128,7 @@ upic_status upic_sim_init_fields(upic_sim* s, double n0) {
             s->steps = 0;
             return;
         }
+        fp64_view(s);
         // rho = deposit(q)/dV (projected) + n0, mean removed
         launch_zero(c->stream, N, s->rho);
         launch_deposit_scalar_point(c->stream, c->g, s->x, s->q, np, 1.0 / c->g.dV, s->rho);
@@ -1028,6 +1160,7 @@ upic_status upic_sim_init_fields(upic_sim* s, double n0) {
         launch_init_momenta(c->stream, c->g, np, p.code, p.c, p.dt, s->x, s->v, s->q, s->m, s->gtmp,
                             s->gtmp + 3ll * N, s->gtmp + 6ll * N, s->P, s->v_prev, s->vbar_last);
         HIPCHK(hipGetLastError());
+        if (s->mixed) mixed_compress(s);
         c->sync();
         s->steps = 0;
     });
@@ -1035,6 +1168,8 @@ upic_status upic_sim_init_fields(upic_sim* s, double n0) {
 
 upic_status upic_sim_device_state(upic_sim* s, upic_host_state* d) {
     if (!s || !d) return UPIC_EINVAL;
+    const int rc = guarded(s->c, [&] { fp64_view(s); });
+    if (rc != UPIC_OK) return (upic_status)rc;
     std::memset(d, 0, sizeof(*d));
     d->np = s->np;
     d->x = s->x; d->v = s->v; d->v_prev = s->v_prev; d->P = s->P; d->vbar_last = s->vbar_last;
@@ -1062,6 +1197,7 @@ upic_status upic_sim_diagnostics(upic_sim* s, int mode, upic_diagnostics* out) {
     upic_ctx* c = s->c;
     return (upic_status)guarded(c, [&] {
         if (c->p.code == 2) { p2_diagnostics(s, out); return; }
+        fp64_view(s);
         const upic_params& p = c->p;
         const int N = c->N();
         const int64_t n3 = 3ll * N;

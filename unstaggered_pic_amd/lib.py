@@ -12,7 +12,8 @@ import numpy as np
 from .abi import Diagnostics, HostState, LoadSpec, Params, PathStats, StepInfo, STATUS_NAMES
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libupic.so")
+# UPIC_LIBRARY selects a tuning variant built with build.py -o
+LIB_PATH = os.environ.get("UPIC_LIBRARY") or os.path.join(HERE, "libupic.so")
 
 dp = C.POINTER(C.c_double)
 vp = C.c_void_p
