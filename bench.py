@@ -9,9 +9,17 @@ u_th = (0.05, 0.2, 0.2) c, uniform density, cubic B-splines, omega_p dt =
 deck (Project3/inputs/manuscript/weibel_main_dt02_T100.txt).  Particles are
 generated by the library's seeded loader (synthetic, seed 12345).
 
+Forms: the plain FP64 step, and the mixed-precision step (FP32 particle
+state, push and deposit tiles; FP64 mesh, FFT and convergence tests, inner
+and outer relative tolerances 1e-6) with the learned initial guess of the
+inner solve, whose network is trained on the device on rows the first
+--collect-steps steps of the same run produce.  The default runs both and
+reports the mixed + learned form as `value`, the plain FP64 form beside it.
+
 One step = one upic_step_p3 call (a full nonlinear time step).  The timed
 region covers exactly --steps steps bracketed by device synchronisation;
-inputs are resident in HBM before it starts.
+inputs are resident in HBM before it starts; data collection and training
+happen before it and are reported separately.
 
 Prints one JSON line (see the keys below).  --gpus N with N > 1 exits: this
 build runs on one GPU.
@@ -101,24 +109,9 @@ def cpu_baseline(nx, dt_budget=30.0):
                       f"serial FP64, {el:.1f} s"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--nx", type=int, default=256)
-    ap.add_argument("--ppc", type=int, default=256)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel event timing")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
-              file=sys.stderr)
-        sys.exit(2)
-
+def run_form(args, prec, learned):
+    """Load, initialise and time one form of the step; returns a dict."""
     from unstaggered_pic_amd import lib
-    prec = 0 if args.precision == "fp64" else 1
     p = workload_params(args.nx, args.ppc, prec)
     N = p.nx * p.ny
     np_ = N * args.ppc
@@ -131,10 +124,30 @@ def main():
     del h, x, v, q, m
     sim.init_fields(1.0)
     d0 = sim.diagnostics(1)
+    extra = {}
+    if learned:
+        # training rows from the first collect_steps steps of the run itself
+        stride = max(1, np_ // args.rows_per_step)
+        per_step = (np_ + stride - 1) // stride
+        sim.collect_guess_data(stride, per_step * args.collect_steps)
+        t0 = time.perf_counter()
+        for _ in range(args.collect_steps):
+            sim.step()
+        ctx.sync()
+        t_collect = time.perf_counter() - t0
+        sim.collect_guess_data(0, 0)
+        t0 = time.perf_counter()
+        losses = sim.train_guess(epochs=args.train_epochs, batch=4096, lr=2e-3, seed=5)
+        t_train = time.perf_counter() - t0
+        extra = {"collect_steps": args.collect_steps, "collect_seconds": t_collect, "training_rows":
+                 per_step * args.collect_steps, "train_epochs": args.train_epochs, "train_seconds": t_train,
+                 "train_loss_first_last": [float(losses[0]), float(losses[-1])]}
     for _ in range(args.warmup):
         sim.step()
     ctx.set_timing(not args.no_timing)
+    d_start = sim.diagnostics(1)
     infos = []
+    ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         infos.append(sim.step())
@@ -143,15 +156,53 @@ def main():
     kname = "p3_map" if prec == 0 else "p3_map_mixed"
     launches, kms = ctx.timing(kname)
     d1 = sim.diagnostics(1)
-    value = np_ * args.steps / el
-    # algorithmic HBM bytes of one p3_map launch: per particle the FP64 state
-    # it reads (x, P, v: 72 B; q, m: 16 B) -- the write-out of the end state
-    # (x, P, v, vbar_macro: 96 B) happens in one launch per step -- plus the
-    # mesh fields read (A_n, A_end, grad phi_mid: 9 N doubles) and the current
-    # written (3 N doubles).
-    # Mixed mode: reads cell (4 B), offsets, P, v (3 x 12 B), q, m (8 B);
-    # writes cell, offsets, P, v, vbar_macro (52 B); FP32 node records
-    # (8 floats per node) read, FP64 current written.
+    sim.close()
+    ctx.close()
+    return {"p": p, "np": np_, "N": N, "elapsed": el, "infos": infos, "launches": launches, "kms": kms,
+            "kname": kname, "d0": d0, "d_start": d_start, "d1": d1, "extra": extra}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--ppc", type=int, default=256)
+    ap.add_argument("--form", choices=["both", "fp64", "mixed", "learned"], default="both",
+                    help="both: plain FP64 and mixed + learned guess; value is the last")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default=None, help="alias of --form fp64|mixed")
+    ap.add_argument("--collect-steps", type=int, default=40)
+    ap.add_argument("--rows-per-step", type=int, default=16384)
+    ap.add_argument("--train-epochs", type=int, default=15)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel event timing")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+    if args.precision:
+        args.form = args.precision
+    forms = {"both": [(0, False), (1, True)], "fp64": [(0, False)], "mixed": [(1, False)],
+             "learned": [(1, True)]}[args.form]
+    runs = []
+    for prec, learned in forms:
+        runs.append(run_form(args, prec, learned))
+        r = runs[-1]
+        print(f"# form prec={prec} learned={learned}: {1e3 * r['elapsed'] / args.steps:.2f} ms/step",
+              file=sys.stderr, flush=True)
+    r = runs[-1]
+    prec = 1 if forms[-1][0] == 1 else 0
+    learned = forms[-1][1]
+    np_, N, infos = r["np"], r["N"], r["infos"]
+    value = np_ * args.steps / r["elapsed"]
+    launches, kms = r["launches"], r["kms"]
+    # algorithmic HBM bytes of one map launch.  FP64: reads x, P, v (72 B),
+    # q, m (16 B) per particle, writes x, P, v, vbar_macro (96 B) once per
+    # step, reads A_n, A_end, grad phi_mid (9 N doubles), writes J (3 N).
+    # Mixed: reads cell (4 B), offsets, P, v (3 x 12 B), q, m (8 B); writes
+    # cell, offsets, P, v, vbar_macro (52 B) once per step; reads the FP32
+    # node records (32 N B), writes the FP64 current (24 N B).
     maps = sum(i.map_evaluations for i in infos)
     if prec == 0:
         bytes_per_launch = np_ * 88.0 + (np_ * 96.0 * args.steps) / max(maps, 1) + 12.0 * 8 * N
@@ -159,14 +210,24 @@ def main():
         bytes_per_launch = np_ * 48.0 + (np_ * 52.0 * args.steps) / max(maps, 1) + (32.0 + 24.0) * N
     avg_ms = kms / max(launches, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if launches else None
-    # FP64 operations of one inner iteration at r = 3, nq = 4 (counted from the
+    # operations of one inner iteration at r = 3, nq = 4 (counted from the
     # kernel source: per Gauss point 2 axis stencils + 16 nodes x (3 gather +
     # 12 D_A) FMAs; per iteration the end-point gather and the HC push).
     segs = np.mean([i.path.total_segments / max(i.path.n_particles, 1) for i in infos])
     flops_iter = segs * 4 * (2 * 4 * 24 + 16 * (2 * 15 + 6 * 2 + 3)) + 16 * 6 + 250
     local_total = sum(i.local_iterations_total for i in infos)
-    compute_tflops = (flops_iter * local_total / args.steps) * (launches / max(launches, 1)) / (
-        kms * 1e-3 / args.steps) / 1e12 if launches else None
+    peak_c = FP64_PEAK_TFLOPS if prec == 0 else FP32_PEAK_TFLOPS
+    compute_tflops = (flops_iter * local_total / args.steps) / (kms * 1e-3 / args.steps) / 1e12 if launches else None
+
+    def phys(rr):
+        d0, ds, d1 = rr["d0"], rr["d_start"], rr["d1"]
+        return {"rel_total_energy_change_timed": (d1.total_energy - ds.total_energy) / abs(ds.total_energy),
+                "rel_total_energy_change_since_init": (d1.total_energy - d0.total_energy) / abs(d0.total_energy),
+                "outer_iterations": [i.iterations for i in rr["infos"]],
+                "map_evaluations": [i.map_evaluations for i in rr["infos"]],
+                "mean_inner_iterations": float(np.mean([i.local_iterations_mean for i in rr["infos"]]))}
+
+    form_name = {(0, False): "plain FP64", (1, False): "mixed precision", (1, True): "mixed precision + learned guess"}
     out = {
         "metric": "particle-steps/s on 1 MI355X, mixed+learned vs plain FP64; rel. energy drift",
         "value": value,
@@ -174,33 +235,35 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * el / args.steps,
+        "ms_per_step": 1e3 * r["elapsed"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if prec == 0 else "f32/f64",
         "data": "synthetic (seeded bi-Maxwellian load, splitmix64 seed 12345)",
-        "config": {"workload": f"weibel_p3_{p.nx}x{p.ny}_ppc{args.ppc}_cubic_dt0.05", "particles": np_,
-                   "grid": [p.nx, p.ny], "spline_order": 3, "orbit_quad_order": 4,
-                   "precision": args.precision, "form": "P3 nested solve (Anderson outer, per-particle Picard inner)"},
+        "config": {"workload": f"weibel_p3_{args.nx}x{args.nx}_ppc{args.ppc}_cubic_dt0.05", "particles": np_,
+                   "grid": [args.nx, args.nx], "spline_order": 3, "orbit_quad_order": 4,
+                   "form": form_name[(prec, learned)],
+                   "solver": "P3 nested solve (Anderson outer, per-particle Picard inner)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "kernel": "k_p3_map" if prec == 0 else "k_p3_map_mixed", "avg_launch_ms": avg_ms, "launches": launches,
-                     "bytes_per_launch": bytes_per_launch},
+                     "kernel": "k_p3_map" if prec == 0 else "k_p3_map_mixed", "avg_launch_ms": avg_ms,
+                     "launches": launches, "bytes_per_launch": bytes_per_launch},
         "compute_roofline": {"bound": "fp64_valu" if prec == 0 else "fp32_valu", "achieved": compute_tflops,
-                             "peak": FP64_PEAK_TFLOPS if prec == 0 else FP32_PEAK_TFLOPS,
-                             "unit": "TFLOP/s",
-                             "frac": (compute_tflops / (FP64_PEAK_TFLOPS if prec == 0 else FP32_PEAK_TFLOPS))
-                             if compute_tflops else None,
+                             "peak": peak_c, "unit": "TFLOP/s",
+                             "frac": (compute_tflops / peak_c) if compute_tflops else None,
                              "flops_per_inner_iteration": flops_iter},
-        "physics": {"rel_total_energy_change": (d1.total_energy - d0.total_energy) / abs(d0.total_energy),
-                    "outer_iterations": [i.iterations for i in infos],
-                    "map_evaluations": [i.map_evaluations for i in infos],
-                    "mean_inner_iterations": float(np.mean([i.local_iterations_mean for i in infos])),
-                    "avg_path_segments": float(segs)},
+        "physics": phys(r),
     }
-    sim.close()
-    ctx.close()
+    out["physics"]["avg_path_segments"] = float(segs)
+    if r["extra"]:
+        out["learned_guess"] = r["extra"]
+    if len(runs) > 1:
+        r0 = runs[0]
+        v0 = r0["np"] * args.steps / r0["elapsed"]
+        out["plain_fp64"] = {"value": v0, "ms_per_step": 1e3 * r0["elapsed"] / args.steps,
+                             "avg_launch_ms": r0["kms"] / max(r0["launches"], 1), "physics": phys(r0)}
+        out["speedup_vs_plain_fp64"] = value / v0
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.nx)
     print(json.dumps(out))

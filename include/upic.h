@@ -108,7 +108,7 @@ typedef struct upic_params {
     /* GPU-only controls (no reference counterpart) */
     int32_t precision;            /* upic_precision                               */
     int32_t sort_interval;        /* cell-sort particles every k steps (0 = never) */
-    int32_t learned_guess;        /* P3: MLP initial guess for the inner solve    */
+    int32_t learned_guess;        /* P3 mixed: initial guess mode (upic_guess_mode) */
     int32_t reserved[5];
 } upic_params;
 
@@ -283,6 +283,44 @@ upic_status upic_sim_init_fields(upic_sim* sim, double n0);
 upic_status upic_sim_device_state(upic_sim* sim, upic_host_state* dev);
 /* Per-particle failure reasons of the last P3 map evaluation (np bytes, host). */
 upic_status upic_sim_failure_reasons(upic_sim* sim, uint8_t* reasons);
+
+/* ---- Learned initial guess of the P3 inner solve (mixed precision) ----- */
+/* No reference counterpart: the reference starts every per-particle Picard
+ * solve (solve_local_particle_substep, P3:4930-4983) from v^n (P3:5318).
+ * These entry points change only that starting iterate; the iteration to
+ * the unchanged tolerance decides convergence.  Modes: v^n; the physics
+ * predictor v^n + (h/2) q/(m gamma)(F - v (v.F)/c^2), F = E + v x B at the
+ * start point; the predictor plus a 12 -> 64 -> 64 -> 3 SiLU network of the
+ * features (v, E, B, (v . grad) E), BF16 on MFMA with FP32 accumulation,
+ * fused into the push kernel.  The network is trained on the device on rows
+ * (features, converged correction) the solver itself produces. */
+#define UPIC_MLP_NPARAM 5214  /* [in_mean 12|in_scale 12|out_scale 3|W1 64x12|b1 64|W2 64x64|b2 64|W3 3x64|b3 3] */
+typedef enum upic_guess_mode {
+    UPIC_GUESS_PLAIN = 0,
+    UPIC_GUESS_PREDICTOR = 1,
+    UPIC_GUESS_LEARNED = 2
+} upic_guess_mode;
+/* Collect one training row per `stride` stored particles from the final map
+ * evaluation of each following step, up to `capacity` rows (stride 0 stops). */
+upic_status upic_sim_collect_guess_data(upic_sim* sim, int64_t stride, int64_t capacity);
+/* Rows collected so far; copies min(n, max_rows) rows of 15 floats
+ * (12 features, 3 target components) to `rows` when it is not NULL. */
+upic_status upic_sim_guess_data(upic_sim* sim, float* rows, int64_t max_rows, int64_t* n_rows);
+/* Train the network on the collected rows (Adam, minibatches of `batch`,
+ * `epochs` passes), install it and switch to UPIC_GUESS_LEARNED.  Writes the
+ * mean minibatch loss of each epoch to loss_per_epoch when not NULL. */
+upic_status upic_sim_train_guess(upic_sim* sim, int32_t epochs, int32_t batch, double lr, uint64_t seed,
+                                 double* loss_per_epoch);
+upic_status upic_sim_set_guess_mode(upic_sim* sim, int32_t mode);
+upic_status upic_sim_guess_params(upic_sim* sim, float* params);          /* UPIC_MLP_NPARAM floats out */
+upic_status upic_sim_set_guess_params(upic_sim* sim, const float* params); /* and in (installs the network) */
+/* Standalone network kernels on device buffers: the MFMA inference the push
+ * kernel runs (X: n rows of 12 features -> Y: n rows of 3, scaled outputs)
+ * and the FP32 training loss/gradient of rows of 15 floats (grad: UPIC_MLP_NPARAM
+ * floats, entries before W1 left zero; loss: host double). */
+upic_status upic_mlp_forward(upic_ctx* ctx, const float* params, const float* X, int64_t n, float* Y);
+upic_status upic_mlp_loss_grad(upic_ctx* ctx, const float* params, const float* rows, int64_t n, float* grad,
+                               double* loss);
 
 /* ---- Particle loading for the synthetic benchmark workloads ------------- */
 /* No reference counterpart: the reference's initialisers (P1:1151-1304,

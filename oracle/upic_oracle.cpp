@@ -12,8 +12,8 @@
 //   P3 = Project3/ec_pic_nyquist_projected.cpp
 // in their general 3-D form (nz >= 1), with the reference's summation orders,
 // FFT twiddle recurrence and long-double accumulations, so that its results
-// agree with the reference to roundoff.  Pinning of each part is listed in
-// the parity section of DESIGN.md and exercised by tests/test_oracle_*.py.
+// agree with the reference to roundoff.  Each part is pinned by the checks in
+// tests/test_oracle.py (golden vectors, closed forms, brute force).
 //
 // Layout at the C boundary: the upic.h conventions (particle arrays 3*np
 // component-major, vector fields 3*N component-major).
@@ -2126,4 +2126,94 @@ int orc_mt19937_uniform(uint32_t seed, double lo, double hi, int64_t n, double* 
     return 0;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Learned-guess network (no reference counterpart): FP64 restatement of the
+// 12 -> 64 -> 64 -> 3 SiLU MLP of the library (parameter layout of
+// UPIC_MLP_NPARAM in upic.h) and of its training loss and gradient.
+//   x = (f - in_mean) * in_scale; h1 = silu(W1 x + b1); h2 = silu(W2 h1 + b2);
+//   y = W3 h2 + b3; output = y * out_scale.
+//   loss = sum_b sum_c (y_bc - t_bc / out_scale_c)^2 / (3 n)
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kIn = 12, kH = 64, kOut = 3;
+constexpr int oMean = 0, oScale = 12, oOut = 24, oW1 = 27, oB1 = oW1 + kH * kIn, oW2 = oB1 + kH, oB2 = oW2 + kH * kH,
+              oW3 = oB2 + kH, oB3 = oW3 + kOut * kH, oN = oB3 + kOut;
+inline double sig(double z) { return 1.0 / (1.0 + std::exp(-z)); }
+struct MlpPass {
+    double x[kIn], z1[kH], h1[kH], z2[kH], h2[kH], y[kOut];
+};
+void mlp_fwd(const float* P, const float* f, MlpPass& s) {
+    for (int k = 0; k < kIn; ++k) s.x[k] = ((double)f[k] - P[oMean + k]) * (double)P[oScale + k];
+    for (int i = 0; i < kH; ++i) {
+        double a = P[oB1 + i];
+        for (int k = 0; k < kIn; ++k) a += (double)P[oW1 + i * kIn + k] * s.x[k];
+        s.z1[i] = a;
+        s.h1[i] = a * sig(a);
+    }
+    for (int i = 0; i < kH; ++i) {
+        double a = P[oB2 + i];
+        for (int k = 0; k < kH; ++k) a += (double)P[oW2 + i * kH + k] * s.h1[k];
+        s.z2[i] = a;
+        s.h2[i] = a * sig(a);
+    }
+    for (int c = 0; c < kOut; ++c) {
+        double a = P[oB3 + c];
+        for (int k = 0; k < kH; ++k) a += (double)P[oW3 + c * kH + k] * s.h2[k];
+        s.y[c] = a;
+    }
+}
+}  // namespace
+
+extern "C" {
+int orc_mlp_forward(const float* P, const float* X, int64_t n, double* Y) {
+    MlpPass s;
+    for (int64_t i = 0; i < n; ++i) {
+        mlp_fwd(P, X + 12 * i, s);
+        for (int c = 0; c < kOut; ++c) Y[3 * i + c] = s.y[c] * (double)P[oOut + c];
+    }
+    return 0;
+}
+
+int orc_mlp_loss_grad(const float* P, const float* rows, int64_t n, double* G, double* loss) {
+    for (int i = 0; i < oN; ++i) G[i] = 0.0;
+    double L = 0.0;
+    MlpPass s;
+    for (int64_t b = 0; b < n; ++b) {
+        const float* r = rows + 15 * b;
+        mlp_fwd(P, r, s);
+        double dy[kOut];
+        for (int c = 0; c < kOut; ++c) {
+            const double e = s.y[c] - (double)r[12 + c] / (double)P[oOut + c];
+            L += e * e;
+            dy[c] = 2.0 * e / (3.0 * (double)n);
+            G[oB3 + c] += dy[c];
+            for (int k = 0; k < kH; ++k) G[oW3 + c * kH + k] += dy[c] * s.h2[k];
+        }
+        double dz2[kH];
+        for (int k = 0; k < kH; ++k) {
+            double dh = 0.0;
+            for (int c = 0; c < kOut; ++c) dh += (double)P[oW3 + c * kH + k] * dy[c];
+            const double sg = sig(s.z2[k]);
+            dz2[k] = dh * sg * (1.0 + s.z2[k] * (1.0 - sg));
+        }
+        double dh1[kH] = {};
+        for (int i = 0; i < kH; ++i) {
+            G[oB2 + i] += dz2[i];
+            for (int k = 0; k < kH; ++k) {
+                G[oW2 + i * kH + k] += dz2[i] * s.h1[k];
+                dh1[k] += (double)P[oW2 + i * kH + k] * dz2[i];
+            }
+        }
+        for (int i = 0; i < kH; ++i) {
+            const double sg = sig(s.z1[i]);
+            const double d = dh1[i] * sg * (1.0 + s.z1[i] * (1.0 - sg));
+            G[oB1 + i] += d;
+            for (int k = 0; k < kIn; ++k) G[oW1 + i * kIn + k] += d * s.x[k];
+        }
+    }
+    *loss = L / (3.0 * (double)n);
+    return 0;
+}
 }  // extern "C"

@@ -73,7 +73,7 @@ class Oracle:
                      "orc_spectral_gradient", "orc_spectral_divergence", "orc_spectral_curl", "orc_continuity",
                      "orc_cn_potentials", "orc_project_nyquist", "orc_poisson", "orc_p3_local_solve", "orc_step",
                      "orc_run", "orc_diagnostics", "orc_p1_step_diagnostics", "orc_p1_init", "orc_p2_run",
-                     "orc_mt19937_uniform", "orc_init_fields"):
+                     "orc_mt19937_uniform", "orc_init_fields", "orc_mlp_forward", "orc_mlp_loss_grad"):
             getattr(L, name).restype = C.c_int
 
     def check(self, rc):
@@ -202,6 +202,22 @@ class Oracle:
         h = st.host()
         self.check(self.L.orc_diagnostics(C.byref(p), C.byref(h), mode, C.byref(d)))
         return d
+
+    def mlp_forward(self, P, X):
+        P = np.ascontiguousarray(P, np.float32)
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, 12)
+        Y = np.zeros((X.shape[0], 3))
+        self.check(self.L.orc_mlp_forward(ptr(P, C.c_float), ptr(X, C.c_float), C.c_int64(X.shape[0]), ptr(Y)))
+        return Y
+
+    def mlp_loss_grad(self, P, rows):
+        P = np.ascontiguousarray(P, np.float32)
+        rows = np.ascontiguousarray(rows, np.float32).reshape(-1, 15)
+        G = np.zeros(P.size)
+        loss = np.zeros(1)
+        self.check(self.L.orc_mlp_loss_grad(ptr(P, C.c_float), ptr(rows, C.c_float), C.c_int64(rows.shape[0]),
+                                            ptr(G), ptr(loss)))
+        return G, float(loss[0])
 
     def p1_step_diagnostics(self, p, s0, s1):
         out = np.zeros(17)

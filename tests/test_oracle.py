@@ -325,3 +325,69 @@ def test_oracle_matches_compiled_reference_p3(orc, ref):
         assert oi.iterations == it.value
         for f in ("x", "v", "P", "phi", "psi", "A", "U", "J", "rho"):
             assert np.array_equal(getattr(out, f), getattr(so, f)), f
+
+
+# ---------------------------------------------------------------------------
+# Learned-guess network (no reference counterpart): the oracle's FP64 forward
+# and training gradient, pinned by a closed form and by finite differences.
+# ---------------------------------------------------------------------------
+MLP_N = 5214
+W1, B1, W2, B2, W3, B3 = 27, 795, 859, 4955, 5019, 5211
+
+
+def mlp_random_params(rng, scale=0.3):
+    P = np.zeros(MLP_N, np.float32)
+    P[0:12] = rng.normal(size=12) * 0.1                   # in_mean
+    P[12:24] = 1.0 + rng.random(12)                       # in_scale
+    P[24:27] = 0.5 + rng.random(3)                        # out_scale
+    P[W1:] = rng.normal(size=MLP_N - W1) * scale
+    return P
+
+
+def test_mlp_forward_closed_form(orc):
+    """Single active path: y_0 = out_scale_0 * (b3_0 + w3 silu(w2 silu(w1 x_0
+    + b1) + b2)), other outputs = out_scale_c * b3_c."""
+    silu = lambda z: z / (1.0 + np.exp(-z))
+    P = np.zeros(MLP_N, np.float32)
+    P[12:24] = 1.0
+    P[0] = 0.25
+    P[12] = 2.0
+    P[24:27] = (3.0, 0.5, 2.0)
+    P[W1 + 0] = 1.5
+    P[B1 + 0] = -0.2
+    P[W2 + 0] = -0.75
+    P[B2 + 0] = 0.1
+    P[W3 + 0] = 2.0
+    P[B3:B3 + 3] = (0.3, -0.4, 0.05)
+    X = np.zeros((4, 12), np.float32)
+    X[:, 0] = (-1.0, 0.0, 0.5, 2.0)
+    Y = orc.mlp_forward(P, X)
+    x = (X[:, 0].astype(np.float64) - 0.25) * 2.0
+    f32 = lambda a: float(np.float32(a))
+    h1 = silu(f32(1.5) * x + f32(-0.2))
+    h2 = silu(f32(-0.75) * h1 + f32(0.1))
+    np.testing.assert_allclose(Y[:, 0], 3.0 * (f32(0.3) + 2.0 * h2), rtol=1e-14)
+    np.testing.assert_allclose(Y[:, 1], 0.5 * f32(-0.4), rtol=1e-14)
+    np.testing.assert_allclose(Y[:, 2], 2.0 * f32(0.05), rtol=1e-14)
+
+
+def test_mlp_gradient_finite_differences(orc):
+    """d loss / d theta against central differences of the oracle's own loss
+    (parameters perturbed in FP32-representable steps)."""
+    rng = np.random.default_rng(7)
+    P = mlp_random_params(rng)
+    rows = np.zeros((9, 15), np.float32)
+    rows[:, :12] = rng.normal(size=(9, 12))
+    rows[:, 12:] = rng.normal(size=(9, 3))
+    G, L0 = orc.mlp_loss_grad(P, rows)
+    picks = list(rng.choice(np.arange(W1, MLP_N), size=40, replace=False)) + [W1, B1 + 3, W2 + 100, B2, W3 + 7, B3 + 2]
+    for i in picks:
+        h = np.float32(2.0 ** -10) * max(np.float32(1.0), abs(P[i]))
+        Pp, Pm = P.copy(), P.copy()
+        Pp[i] += h
+        Pm[i] -= h
+        dh = float(Pp[i]) - float(Pm[i])
+        fd = (orc.mlp_loss_grad(Pp, rows)[1] - orc.mlp_loss_grad(Pm, rows)[1]) / dh
+        assert abs(fd - G[i]) <= 1e-5 * max(1.0, abs(G[i])), (i, fd, G[i])
+    assert L0 > 0.0
+    assert np.all(G[:W1] == 0.0)

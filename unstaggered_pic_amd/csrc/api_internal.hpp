@@ -207,6 +207,9 @@ inline void validate(const upic_params& p) {
     if (p.code == 1 && p.picard_max_iter < 1) throw Error(UPIC_EINVAL, "picard_max_iter must be positive");
     if (p.code == 2 && p.bdf_order != 1 && p.bdf_order != 2) throw Error(UPIC_EINVAL, "method must be bdf1 or bdf2");
     if (p.precision != UPIC_PREC_FP64 && p.precision != UPIC_PREC_MIXED) throw Error(UPIC_EINVAL, "unknown precision");
+    if (p.learned_guess < 0 || p.learned_guess > 2) throw Error(UPIC_EINVAL, "learned_guess must be 0, 1 or 2");
+    if (p.learned_guess > 0 && p.precision != UPIC_PREC_MIXED)
+        throw Error(UPIC_EINVAL, "the initial-guess modes are implemented for the mixed-precision P3 step");
     if (p.precision == UPIC_PREC_MIXED) {
         if (p.code != 3 || p.nz != 1)
             throw Error(UPIC_EINVAL, "mixed precision is implemented for the P3 step on 2-D grids (nz = 1)");

@@ -108,6 +108,11 @@ struct P3MArgs {
     double* red;                   // [2]: sum of local iterations, failed particles
     double* redmax;                // [2]: max local iterations, max final scaled residual
     unsigned long long* stats;
+    // initial guess: 0 = v^n, 1 = physics predictor, 2 = predictor + network
+    int guess_mode;
+    const float* mlp;              // network inference blob (guess_mode 2)
+    float* train;                  // training rows (15 floats) or NULL
+    int64_t train_stride, train_offset, train_cap;
 };
 void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& a);
 // NRf[8*id + ...] = FP32 copy of the FP64 node records
@@ -121,6 +126,20 @@ void launch_mixed_expand(hipStream_t s, const GridDev& g, int64_t np, const Mixe
 void launch_mixed_keys(hipStream_t s, int64_t np, const int* cell, int* keys, int* counts);
 void launch_permute_f32(hipStream_t s, int64_t np, int ncomp, const float* src, float* dst, const int* perm);
 void launch_permute_i32(hipStream_t s, int64_t np, const int* src, int* dst, const int* perm);
+
+// mlp.hip: the learned-guess network (layout in mlp.hpp) -------------------
+// Training rows are 15 floats: 12 features, then the 3 target components.
+// moments of the rows -> normalisation entries of P (acc: 27 doubles, pre-zeroed)
+void launch_mlp_moments(hipStream_t s, const float* rows, int64_t n, double* acc, float* P);
+// loss and gradient (G pre-zeroed, loss pre-zeroed) of rows idx[0..B) (idx NULL: rows 0..B);
+// act: scratch of mlp_grad_scratch_floats(B) floats
+void launch_mlp_grad(hipStream_t s, const float* P, const float* rows, const int* idx, int B, float* G, double* loss,
+                     float* act);
+size_t mlp_grad_scratch_floats(int B);
+void launch_mlp_adam(hipStream_t s, float* P, const float* G, float* M, float* V, float lr, int t);
+void launch_mlp_pack(hipStream_t s, const float* P, float* blob);
+void launch_mlp_forward(hipStream_t s, const float* blob, const float* X, int64_t n, float* Y);
+void launch_mlp_batch_idx(hipStream_t s, uint64_t seed, uint64_t step, int B, int64_t n, int* idx);
 
 // field_kernels.hip ---------------------------------------------------------
 struct FFTPlan {

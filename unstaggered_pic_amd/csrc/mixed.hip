@@ -18,6 +18,7 @@
 // with positions relative to the particle's start cell, so that FP32 keeps
 // ~1e-7 of a cell of resolution anywhere on the grid.
 #include "kernels.hpp"
+#include "mlp.hpp"
 #include "particles.hpp"
 
 namespace upic {
@@ -566,11 +567,95 @@ __device__ __forceinline__ void move_axis(float& t, float du, int& c, int n) {
     c = wrapn(c + k, n);
 }
 
+// Initial guess of the inner solve (no counterpart in the reference, which
+// starts from v^n, P3:5318).  Gathered at the start point: E = -grad phi_mid
+// - (A^{n+1} - A^n)/h, B = curl (A^n + A^{n+1})/2 and the convective
+// derivative (v . grad) E; the physics predictor of the orbit-averaged
+// velocity is half a relativistic Lorentz step,
+//   delta = (h/2) q/(m gamma) (F - v (v.F)/kappa^2),  F = E + v x B,
+// and the network (mlp.hpp) adds a learned correction from the 12 features
+// (v, E, B, (v . grad) E).
 template <int R>
+__device__ __forceinline__ void guess_features(const MGrid& g, const float4* __restrict__ NR, int ci, int cj, float tx,
+                                               float ty, const float v[3], float q, float m, float kappa, float h,
+                                               float f[12], float dphys[3]) {
+    AxF<R> sx, sy;
+    axis_f<R>(tx, g.ihx, sx);
+    axis_f<R>(ty, g.ihy, sy);
+    const int bx = wrapn(ci + sx.fl, g.nx), by = wrapn(cj + sy.fl, g.ny);
+    float gp0 = 0.0f, gp1 = 0.0f, dg00 = 0.0f, dg01 = 0.0f, dg10 = 0.0f, dg11 = 0.0f;
+    float dA[3] = {0.0f, 0.0f, 0.0f}, ddA[3][2] = {}, dAm[3][2] = {};
+#pragma unroll
+    for (int a = 0; a <= R; ++a) {
+        const int ia = wrap1n(bx + a, g.nx);
+#pragma unroll
+        for (int b = 0; b <= R; ++b) {
+            const int ib = wrap1n(by + b, g.ny);
+            float4 r0, r1;
+            rec_f(NR, ia * g.ny + ib, r0, r1);
+            const float W = sx.w[a] * sy.w[b], Wx = sx.dw[a] * sy.w[b], Wy = sx.w[a] * sy.dw[b];
+            gp0 += W * r0.x;
+            gp1 += W * r0.y;
+            dg00 += Wx * r0.x; dg01 += Wy * r0.x;
+            dg10 += Wx * r0.y; dg11 += Wy * r0.y;
+            const float da[3] = {r1.y - r0.z, r1.z - r0.w, r1.w - r1.x};
+            const float am[3] = {0.5f * (r0.z + r1.y), 0.5f * (r0.w + r1.z), 0.5f * (r1.x + r1.w)};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                dA[c] += W * da[c];
+                ddA[c][0] += Wx * da[c];
+                ddA[c][1] += Wy * da[c];
+                dAm[c][0] += Wx * am[c];
+                dAm[c][1] += Wy * am[c];
+            }
+        }
+    }
+    const float ih = 1.0f / h;
+    const float E[3] = {-gp0 - dA[0] * ih, -gp1 - dA[1] * ih, -dA[2] * ih};
+    const float Bm[3] = {dAm[2][1], -dAm[2][0], dAm[1][0] - dAm[0][1]};
+    const float vgE[3] = {-(v[0] * dg00 + v[1] * dg01) - (v[0] * ddA[0][0] + v[1] * ddA[0][1]) * ih,
+                          -(v[0] * dg10 + v[1] * dg11) - (v[0] * ddA[1][0] + v[1] * ddA[1][1]) * ih,
+                          -(v[0] * ddA[2][0] + v[1] * ddA[2][1]) * ih};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        f[c] = v[c];
+        f[3 + c] = E[c];
+        f[6 + c] = Bm[c];
+        f[9 + c] = vgE[c];
+    }
+    const float ik2 = 1.0f / (kappa * kappa);
+    const float v2 = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * ik2;
+    const float gam = 1.0f / sqrtf(fmaxf(1.0f - v2, 1.0e-12f));
+    const float F[3] = {E[0] + (v[1] * Bm[2] - v[2] * Bm[1]), E[1] + (v[2] * Bm[0] - v[0] * Bm[2]),
+                        E[2] + (v[0] * Bm[1] - v[1] * Bm[0])};
+    const float vF = (v[0] * F[0] + v[1] * F[1] + v[2] * F[2]) * ik2;
+    const float coef = 0.5f * h * q / (m * gam);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dphys[c] = coef * (F[c] - v[c] * vF);
+}
+
+// GM: initial guess of the inner solve, 0 = v^n (the reference's), 1 =
+// physics predictor, 2 = predictor + network; COLLECT: write (features,
+// converged correction) training rows in the write-out pass.
+template <int R, int GM, bool COLLECT>
 __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ float tile_mem[kTileM * kTileM * 3];
+    extern __shared__ unsigned char mlp_lds[];
     const DepTileF T = tilef_begin(g, tile_mem, a.cell, a.np, (int64_t)blockIdx.x * blockDim.x);
+    // features and guess at the start point (all lanes: the network runs on
+    // whole wavefronts)
+    float feat[12] = {}, dph[3] = {0.0f, 0.0f, 0.0f}, lg[3] = {0.0f, 0.0f, 0.0f};
+    if constexpr (GM > 0 || COLLECT) {
+        if (p < a.np && !a.layout) {
+            const int c0 = a.cell[p];
+            float v0[3];
+            ld3f(a.v, a.np, p, v0);
+            guess_features<R>(g, a.NR, c0 / g.ny, c0 - (c0 / g.ny) * g.ny, a.off[p], a.off[a.np + p], v0, a.q[p],
+                              a.m[p], (float)a.kappa, (float)a.dt, feat, dph);
+        }
+        if constexpr (GM == 2) mlp_wave_infer(a.mlp, feat, lg, mlp_lds + (threadIdx.x / 64) * kMlpLdsWave);
+    }
     double rsum[2] = {0.0, 0.0}, rmax[2] = {0.0, 0.0};
     int nseg_total = 0;
     if (p < a.np) {
@@ -593,6 +678,12 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
             const float pst[3] = {Pc[0] - qp * A0p[0], Pc[1] - qp * A0p[1], Pc[2] - qp * A0p[2]};
             float gs[3] = {vc[0], vc[1], vc[2]};
             if (a.guess && nsub == 1) ld3f(a.guess, a.np, p, gs);
+            if constexpr (GM > 0) {
+                if (nsub == 1) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) gs[d] = vc[d] + dph[d] + (GM == 2 ? lg[d] : 0.0f);
+                }
+            }
             LocalMapF acc;
             float vacc[3];
             bool conv = false;
@@ -626,6 +717,18 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
             rmax[0] = fmax(rmax[0], (double)its);
             rmax[1] = fmax(rmax[1], scaled);
             if (!conv) break;
+            if constexpr (COLLECT) {
+                if (a.write_out && nsub == 1 && a.train && (p % a.train_stride) == 0) {
+                    const int64_t row = a.train_offset + p / a.train_stride;
+                    if (row < a.train_cap) {
+                        float* o = a.train + 15 * row;
+#pragma unroll
+                        for (int k = 0; k < 12; ++k) o[k] = feat[k];
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) o[12 + d] = vacc[d] - vc[d] - dph[d];
+                    }
+                }
+            }
             const float dux = h * vacc[0] * g.ihx, duy = h * vacc[1] * g.ihy;
             if (a.J_out)
                 nseg_total += deposit_orbit_f<R>(g, qd, split != 0, ci, cj, tx, ty, dux, duy, vacc,
@@ -760,13 +863,29 @@ void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int sp
     mg.nx = g.nx; mg.ny = g.ny; mg.N = g.N; mg.order = g.order;
     mg.hx = (float)g.h[0]; mg.hy = (float)g.h[1];
     mg.ihx = (float)(1.0 / g.h[0]); mg.ihy = (float)(1.0 / g.h[1]);
+    const int gm = (a.layout || (a.guess_mode == 2 && !a.mlp)) ? 0 : a.guess_mode;
+    const bool col = a.train != nullptr && !a.layout;
+    const size_t lds = gm == 2 ? 4 * (size_t)kMlpLdsWave : 0;
+#define UPIC_MIXED_LAUNCH(R_)                                                                                       \
+    do {                                                                                                            \
+        if (col) {                                                                                                  \
+            if (gm == 2) hipLaunchKernelGGL((k_p3_map_mixed<R_, 2, true>), dim3(mblk(a.np)), dim3(256), lds, s, mg, q, split, a); \
+            else if (gm == 1) hipLaunchKernelGGL((k_p3_map_mixed<R_, 1, true>), dim3(mblk(a.np)), dim3(256), 0, s, mg, q, split, a); \
+            else hipLaunchKernelGGL((k_p3_map_mixed<R_, 0, true>), dim3(mblk(a.np)), dim3(256), 0, s, mg, q, split, a); \
+        } else {                                                                                                    \
+            if (gm == 2) hipLaunchKernelGGL((k_p3_map_mixed<R_, 2, false>), dim3(mblk(a.np)), dim3(256), lds, s, mg, q, split, a); \
+            else if (gm == 1) hipLaunchKernelGGL((k_p3_map_mixed<R_, 1, false>), dim3(mblk(a.np)), dim3(256), 0, s, mg, q, split, a); \
+            else hipLaunchKernelGGL((k_p3_map_mixed<R_, 0, false>), dim3(mblk(a.np)), dim3(256), 0, s, mg, q, split, a); \
+        }                                                                                                           \
+    } while (0)
     switch (g.order) {
-        case 1: hipLaunchKernelGGL(k_p3_map_mixed<1>, dim3(mblk(a.np)), dim3(256), 0, s, mg, q, split, a); break;
-        case 2: hipLaunchKernelGGL(k_p3_map_mixed<2>, dim3(mblk(a.np)), dim3(256), 0, s, mg, q, split, a); break;
-        case 3: hipLaunchKernelGGL(k_p3_map_mixed<3>, dim3(mblk(a.np)), dim3(256), 0, s, mg, q, split, a); break;
-        case 4: hipLaunchKernelGGL(k_p3_map_mixed<4>, dim3(mblk(a.np)), dim3(256), 0, s, mg, q, split, a); break;
+        case 1: UPIC_MIXED_LAUNCH(1); break;
+        case 2: UPIC_MIXED_LAUNCH(2); break;
+        case 3: UPIC_MIXED_LAUNCH(3); break;
+        case 4: UPIC_MIXED_LAUNCH(4); break;
         default: break;
     }
+#undef UPIC_MIXED_LAUNCH
 }
 
 void launch_pack_nodes_f32(hipStream_t s, int N, const double* gpm, const double* A_n, const double* A_end, float* NR) {

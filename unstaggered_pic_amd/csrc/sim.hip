@@ -5,6 +5,7 @@
 //   P2 run_simulation step body    (Project2/spectral_pic_part3.cpp:1133-1189)
 // and the per-step diagnostics (P1:1057-1087, P3:3521-3574).
 #include "api_internal.hpp"
+#include "mlp.hpp"
 
 struct upic_sim {
     upic_ctx* c = nullptr;
@@ -59,6 +60,14 @@ struct upic_sim {
     int* itmp = nullptr;     // np sort scratch
     float* NRf = nullptr;   // FP32 node records (8N)
     double outer_atol = 0.0;  // absolute outer tolerance in force for this step
+    // learned initial guess (mixed precision): mode, network, training rows
+    int guess_mode = 0;
+    float* mlp_P = nullptr;     // parameters (kMlpNParam)
+    float* mlp_blob = nullptr;  // inference blob (kBlobN)
+    bool mlp_ready = false;
+    float* train_buf = nullptr;
+    int64_t train_cap = 0, train_n = 0, train_stride = 0;
+    bool train_written = false;
 };
 
 // Mixed precision: an FP32 map resolves each node current only to a fraction
@@ -85,6 +94,7 @@ void sim_free(upic_sim* s) {
                     &s->mo.off, &s->mo.P, &s->mo.v, &s->mo.vb, &s->ftmp, &s->ftmp1, &s->NRf};
     for (float** p : fl) dfree(*p);
     dfree(s->ms.cell); dfree(s->mo.cell); dfree(s->itmp);
+    dfree(s->mlp_P); dfree(s->mlp_blob); dfree(s->train_buf);
 }
 
 // FP64 particle arrays -> mixed state
@@ -372,6 +382,14 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
         a.cell_out = s->mo.cell; a.off_out = s->mo.off; a.P_out = s->mo.P; a.v_out = s->mo.v; a.vbar_out = s->mo.vb;
         a.reason = s->reason;
         a.red = c->dred; a.redmax = c->dred + 2; a.stats = c->dstats;
+        a.guess_mode = s->guess_mode;
+        a.mlp = s->mlp_ready ? s->mlp_blob : nullptr;
+        const bool collect = write_out && !use_layout && s->train_stride > 0 && s->train_n < s->train_cap;
+        a.train = collect ? s->train_buf : nullptr;
+        a.train_stride = s->train_stride > 0 ? s->train_stride : 1;
+        a.train_offset = s->train_n;
+        a.train_cap = s->train_cap;
+        s->train_written = collect;
         timed(c, "p3_map_mixed", [&] { launch_p3_map_mixed(c->stream, c->g, c->qf, c->opt.split, a); });
     } else {
     P3Args a{};
@@ -696,6 +714,7 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
 // nonlinear_step_orbit (P3:6477-6526) with the selective fallback (6328-6454)
 void p3_step(upic_sim* s, upic_step_info* info) {
     upic_ctx* c = s->c;
+    s->train_written = false;
     maybe_sort(s);
     state_spectra(s);
     s->outer_atol = 0.0;
@@ -731,7 +750,102 @@ void p3_step(upic_sim* s, upic_step_info* info) {
             }
         }
     }
+    if (s->train_written) {  // rows of the final map evaluation of this step
+        const int64_t rows = (s->np + s->train_stride - 1) / s->train_stride;
+        s->train_n = std::min(s->train_cap, s->train_n + rows);
+        s->train_written = false;
+    }
     ++s->steps;
+}
+
+// ---------------------------------------------------------------------------
+// Training of the learned guess (no reference counterpart)
+// ---------------------------------------------------------------------------
+uint64_t splitmix(uint64_t& z) {
+    z += 0x9E3779B97F4A7C15ull;
+    uint64_t r = z;
+    r = (r ^ (r >> 30)) * 0xBF58476D1CE4E5B9ull;
+    r = (r ^ (r >> 27)) * 0x94D049BB133111EBull;
+    return r ^ (r >> 31);
+}
+
+// Glorot-uniform weights, zero biases, unit normalisation (set from the data
+// before training).
+std::vector<float> mlp_init_params(uint64_t seed) {
+    std::vector<float> P(kMlpNParam, 0.0f);
+    for (int j = 0; j < 12; ++j) P[kMlpInScale + j] = 1.0f;
+    for (int c = 0; c < 3; ++c) P[kMlpOutScale + c] = 1.0f;
+    uint64_t z = seed;
+    auto fill = [&](int off, int nout, int nin, float gain) {
+        const double a = gain * std::sqrt(6.0 / (double)(nin + nout));
+        for (int i = 0; i < nout * nin; ++i) {
+            const double u = (double)(splitmix(z) >> 11) * (1.0 / 9007199254740992.0);
+            P[off + i] = (float)((2.0 * u - 1.0) * a);
+        }
+    };
+    fill(kMlpW1, kMlpHid, kMlpIn, 1.0f);
+    fill(kMlpW2, kMlpHid, kMlpHid, 1.0f);
+    fill(kMlpW3, kMlpOut, kMlpHid, 0.1f);
+    return P;
+}
+
+void mlp_install(upic_sim* s) {
+    upic_ctx* c = s->c;
+    if (!s->mlp_blob) s->mlp_blob = dalloc<float>(kBlobN);
+    launch_mlp_pack(c->stream, s->mlp_P, s->mlp_blob);
+    HIPCHK(hipGetLastError());
+    c->sync();
+    s->mlp_ready = true;
+    s->guess_mode = UPIC_GUESS_LEARNED;
+}
+
+void train_guess(upic_sim* s, int epochs, int batch, double lr, uint64_t seed, double* loss_per_epoch) {
+    upic_ctx* c = s->c;
+    const int64_t n = s->train_n;
+    if (n < 64) throw Error(UPIC_EINVAL, "at least 64 collected training rows are needed");
+    if (epochs < 1 || batch < 1 || !(lr > 0.0)) throw Error(UPIC_EINVAL, "epochs, batch and lr must be positive");
+    if (!s->mlp_P) s->mlp_P = dalloc<float>(kMlpNParam);
+    const std::vector<float> P0 = mlp_init_params(seed);
+    HIPCHK(hipMemcpy(s->mlp_P, P0.data(), kMlpNParam * sizeof(float), hipMemcpyHostToDevice));
+    double* acc = c->dred + 32;
+    launch_zero(c->stream, 27, acc);
+    launch_mlp_moments(c->stream, s->train_buf, n, acc, s->mlp_P);
+    float* G = dalloc<float>(3 * (size_t)kMlpNParam);
+    float* M = G + kMlpNParam;
+    float* V = M + kMlpNParam;
+    int* idx = dalloc<int>(batch);
+    float* act = dalloc<float>(mlp_grad_scratch_floats(batch));
+    HIPCHK(hipMemsetAsync(M, 0, 2 * kMlpNParam * sizeof(float), c->stream));
+    const int64_t per_epoch = (n + batch - 1) / batch;
+    int t = 0;
+    try {
+        for (int e = 0; e < epochs; ++e) {
+            launch_zero(c->stream, 1, c->dred + 60);
+            for (int64_t b = 0; b < per_epoch; ++b) {
+                ++t;
+                launch_mlp_batch_idx(c->stream, seed, (uint64_t)t, batch, n, idx);
+                HIPCHK(hipMemsetAsync(G, 0, kMlpNParam * sizeof(float), c->stream));
+                launch_mlp_grad(c->stream, s->mlp_P, s->train_buf, idx, batch, G, c->dred + 60, act);
+                launch_mlp_adam(c->stream, s->mlp_P, G, M, V, (float)lr, t);
+            }
+            HIPCHK(hipGetLastError());
+            if (loss_per_epoch) {
+                HIPCHK(hipMemcpyAsync(c->hred + 60, c->dred + 60, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                c->sync();
+                loss_per_epoch[e] = c->hred[60] / (double)per_epoch;
+            }
+        }
+        c->sync();
+    } catch (...) {
+        dfree(G);
+        dfree(idx);
+        dfree(act);
+        throw;
+    }
+    dfree(G);
+    dfree(idx);
+    dfree(act);
+    mlp_install(s);
 }
 
 // ---------------------------------------------------------------------------
@@ -975,6 +1089,7 @@ upic_status upic_sim_create(upic_ctx* c, int64_t np, upic_sim** out) {
             }
             s->fixed_ion = dalloc<double>(N);
         }
+        s->guess_mode = c->p.learned_guess;
         if (c->p.precision == UPIC_PREC_MIXED) {
             s->mixed = true;
             for (MixedState* m : {&s->ms, &s->mo}) {
@@ -1282,6 +1397,96 @@ upic_status upic_sim_diagnostics(upic_sim* s, int mode, upic_diagnostics* out) {
         o.E_mode_abs = std::hypot(modes[2], modes[3]) * invN;
         o.rho_mode = std::hypot(modes[4], modes[5]) * invN;
         o.mean_rho = h[10] / N;
+    });
+}
+
+upic_status upic_sim_collect_guess_data(upic_sim* s, int64_t stride, int64_t capacity) {
+    if (!s || stride < 0 || capacity < 0) return UPIC_EINVAL;
+    return (upic_status)guarded(s->c, [&] {
+        if (!s->mixed) throw Error(UPIC_EINVAL, "the learned guess is implemented for the mixed-precision P3 step");
+        if (stride == 0) { s->train_stride = 0; return; }
+        if (capacity > s->train_cap) {
+            s->c->sync();
+            float* nb = dalloc<float>(15 * (size_t)capacity);
+            if (s->train_n > 0)
+                HIPCHK(hipMemcpy(nb, s->train_buf, 15 * sizeof(float) * (size_t)s->train_n, hipMemcpyDeviceToDevice));
+            dfree(s->train_buf);
+            s->train_buf = nb;
+            s->train_cap = capacity;
+        }
+        s->train_stride = stride;
+    });
+}
+
+upic_status upic_sim_guess_data(upic_sim* s, float* rows, int64_t max_rows, int64_t* n_rows) {
+    if (!s || max_rows < 0) return UPIC_EINVAL;
+    return (upic_status)guarded(s->c, [&] {
+        s->c->sync();
+        if (n_rows) *n_rows = s->train_n;
+        const int64_t k = std::min(max_rows, s->train_n);
+        if (rows && k > 0) HIPCHK(hipMemcpy(rows, s->train_buf, 15 * sizeof(float) * (size_t)k, hipMemcpyDeviceToHost));
+    });
+}
+
+upic_status upic_sim_train_guess(upic_sim* s, int32_t epochs, int32_t batch, double lr, uint64_t seed,
+                                 double* loss_per_epoch) {
+    if (!s) return UPIC_EINVAL;
+    return (upic_status)guarded(s->c, [&] { train_guess(s, epochs, batch, lr, seed, loss_per_epoch); });
+}
+
+upic_status upic_sim_set_guess_mode(upic_sim* s, int32_t mode) {
+    if (!s || mode < 0 || mode > 2) return UPIC_EINVAL;
+    if (mode > 0 && !s->mixed) {
+        s->c->err = "the initial-guess modes are implemented for the mixed-precision P3 step";
+        return UPIC_EINVAL;
+    }
+    if (mode == 2 && !s->mlp_ready) { s->c->err = "no network installed"; return UPIC_EINVAL; }
+    s->guess_mode = mode;
+    return UPIC_OK;
+}
+
+upic_status upic_sim_guess_params(upic_sim* s, float* params) {
+    if (!s || !params) return UPIC_EINVAL;
+    if (!s->mlp_P) { s->c->err = "no network installed"; return UPIC_EINVAL; }
+    return (upic_status)guarded(s->c, [&] {
+        s->c->sync();
+        HIPCHK(hipMemcpy(params, s->mlp_P, kMlpNParam * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+upic_status upic_sim_set_guess_params(upic_sim* s, const float* params) {
+    if (!s || !params) return UPIC_EINVAL;
+    return (upic_status)guarded(s->c, [&] {
+        if (!s->mixed) throw Error(UPIC_EINVAL, "the learned guess is implemented for the mixed-precision P3 step");
+        if (!s->mlp_P) s->mlp_P = dalloc<float>(kMlpNParam);
+        HIPCHK(hipMemcpy(s->mlp_P, params, kMlpNParam * sizeof(float), hipMemcpyHostToDevice));
+        mlp_install(s);
+    });
+}
+
+upic_status upic_mlp_forward(upic_ctx* c, const float* params, const float* X, int64_t n, float* Y) {
+    if (!c || !params || (n > 0 && (!X || !Y)) || n < 0) return UPIC_EINVAL;
+    return (upic_status)guarded(c, [&] {
+        float* blob = reinterpret_cast<float*>(c->workspace((kBlobN + 1) / 2));
+        launch_mlp_pack(c->stream, params, blob);
+        launch_mlp_forward(c->stream, blob, X, n, Y);
+        HIPCHK(hipGetLastError());
+        c->sync();
+    });
+}
+
+upic_status upic_mlp_loss_grad(upic_ctx* c, const float* params, const float* rows, int64_t n, float* grad,
+                               double* loss) {
+    if (!c || !params || !grad || n < 1 || n > (1 << 30)) return UPIC_EINVAL;
+    return (upic_status)guarded(c, [&] {
+        HIPCHK(hipMemsetAsync(grad, 0, kMlpNParam * sizeof(float), c->stream));
+        launch_zero(c->stream, 1, c->dred + 60);
+        float* act = reinterpret_cast<float*>(c->workspace((mlp_grad_scratch_floats((int)n) + 1) / 2));
+        launch_mlp_grad(c->stream, params, rows, nullptr, (int)n, grad, c->dred + 60, act);
+        HIPCHK(hipMemcpyAsync(c->hred + 60, c->dred + 60, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipGetLastError());
+        c->sync();
+        if (loss) *loss = c->hred[60];
     });
 }
 

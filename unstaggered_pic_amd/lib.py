@@ -17,6 +17,8 @@ LIB_PATH = os.environ.get("UPIC_LIBRARY") or os.path.join(HERE, "libupic.so")
 
 dp = C.POINTER(C.c_double)
 vp = C.c_void_p
+MLP_NPARAM = 5214  # UPIC_MLP_NPARAM
+GUESS_PLAIN, GUESS_PREDICTOR, GUESS_LEARNED = 0, 1, 2
 
 
 class UpicError(RuntimeError):
@@ -66,6 +68,13 @@ def lib():
             "upic_memcpy_dtoh": [vp, vp, C.c_size_t],
             "upic_ctx_timing": [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)],
             "upic_load_particles": [C.POINTER(Params), C.POINTER(LoadSpec), C.POINTER(HostState)],
+            "upic_sim_collect_guess_data": [vp, C.c_int64, C.c_int64],
+            "upic_sim_guess_data": [vp, vp, C.c_int64, C.POINTER(C.c_int64)],
+            "upic_sim_train_guess": [vp, C.c_int32, C.c_int32, C.c_double, C.c_uint64, dp],
+            "upic_sim_set_guess_mode": [vp, C.c_int32],
+            "upic_sim_guess_params": [vp, vp], "upic_sim_set_guess_params": [vp, vp],
+            "upic_mlp_forward": [vp, vp, vp, C.c_int64, vp],
+            "upic_mlp_loss_grad": [vp, vp, vp, C.c_int64, vp, dp],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -94,27 +103,28 @@ def device_count():
 
 
 class DeviceArray:
-    """A float64 buffer in HBM, allocated through the library."""
+    """A float64 (or float32) buffer in HBM, allocated through the library."""
 
-    def __init__(self, n):
+    def __init__(self, n, dtype=np.float64):
         self.n = int(n)
+        self.dtype = np.dtype(dtype)
         self.p = vp()
-        rc = lib().upic_device_alloc(8 * max(self.n, 1), C.byref(self.p))
+        rc = lib().upic_device_alloc(self.dtype.itemsize * max(self.n, 1), C.byref(self.p))
         if rc != 0:
             raise UpicError(rc, "device allocation failed")
 
     @classmethod
-    def from_host(cls, a):
-        a = np.ascontiguousarray(a, dtype=np.float64)
-        d = cls(a.size)
+    def from_host(cls, a, dtype=np.float64):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(a.size, dtype)
         if a.size:
-            lib().upic_memcpy_htod(d.p, a.ctypes.data_as(vp), 8 * a.size)
+            lib().upic_memcpy_htod(d.p, a.ctypes.data_as(vp), a.nbytes)
         return d
 
     def numpy(self):
-        out = np.empty(self.n)
+        out = np.empty(self.n, self.dtype)
         if self.n:
-            rc = lib().upic_memcpy_dtoh(out.ctypes.data_as(vp), self.p, 8 * self.n)
+            rc = lib().upic_memcpy_dtoh(out.ctypes.data_as(vp), self.p, self.dtype.itemsize * self.n)
             if rc != 0:
                 raise UpicError(rc, "device copy failed")
         return out
@@ -162,6 +172,26 @@ class Context:
         s, w = np.zeros(n), np.zeros(n)
         self._check(lib().upic_quadrature(self.h, s.ctypes.data_as(dp), w.ctypes.data_as(dp)))
         return s, w
+
+    def mlp_forward(self, params, X):
+        """MFMA inference of the learned-guess network on host arrays:
+        params (5214 floats), X (n x 12) -> Y (n x 3), FP32."""
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, 12)
+        n = X.shape[0]
+        dP, dX = DeviceArray.from_host(params, np.float32), DeviceArray.from_host(X, np.float32)
+        dY = DeviceArray(3 * n, np.float32)
+        self._check(lib().upic_mlp_forward(self.h, dP.p, dX.p, n, dY.p))
+        return dY.numpy().reshape(n, 3)
+
+    def mlp_loss_grad(self, params, rows):
+        """FP32 training loss and gradient on rows (n x 15: features, target)."""
+        rows = np.ascontiguousarray(rows, np.float32).reshape(-1, 15)
+        n = rows.shape[0]
+        dP, dR = DeviceArray.from_host(params, np.float32), DeviceArray.from_host(rows, np.float32)
+        dG = DeviceArray(MLP_NPARAM, np.float32)
+        loss = C.c_double()
+        self._check(lib().upic_mlp_loss_grad(self.h, dP.p, dR.p, n, dG.p, C.byref(loss)))
+        return dG.numpy(), loss.value
 
     def close(self):
         if self.h:
@@ -330,6 +360,35 @@ class Simulation:
         r = np.zeros(self.np, np.uint8)
         self.ctx._check(lib().upic_sim_failure_reasons(self.h, r.ctypes.data_as(C.POINTER(C.c_uint8))))
         return r
+
+    # --- learned initial guess (mixed precision) -------------------------
+    def collect_guess_data(self, stride, capacity):
+        self.ctx._check(lib().upic_sim_collect_guess_data(self.h, int(stride), int(capacity)))
+
+    def guess_data(self, max_rows=None):
+        n = C.c_int64()
+        self.ctx._check(lib().upic_sim_guess_data(self.h, None, 0, C.byref(n)))
+        k = n.value if max_rows is None else min(n.value, int(max_rows))
+        rows = np.zeros((k, 15), np.float32)
+        self.ctx._check(lib().upic_sim_guess_data(self.h, rows.ctypes.data_as(vp), k, C.byref(n)))
+        return rows
+
+    def train_guess(self, epochs=20, batch=4096, lr=1e-3, seed=1):
+        losses = np.zeros(epochs)
+        self.ctx._check(lib().upic_sim_train_guess(self.h, epochs, batch, lr, seed, losses.ctypes.data_as(dp)))
+        return losses
+
+    def set_guess_mode(self, mode):
+        self.ctx._check(lib().upic_sim_set_guess_mode(self.h, int(mode)))
+
+    def guess_params(self):
+        P = np.zeros(MLP_NPARAM, np.float32)
+        self.ctx._check(lib().upic_sim_guess_params(self.h, P.ctypes.data_as(vp)))
+        return P
+
+    def set_guess_params(self, P):
+        P = np.ascontiguousarray(P, np.float32)
+        self.ctx._check(lib().upic_sim_set_guess_params(self.h, P.ctypes.data_as(vp)))
 
 
 def load_particles(params: Params, spec, np_):
