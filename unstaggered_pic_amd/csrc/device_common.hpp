@@ -15,6 +15,9 @@ namespace upic {
 
 constexpr int kMaxQuad = 32;
 
+// slots of the per-block partials of a particle sweep (particles.hpp)
+constexpr int kPartSlots = 9;
+
 // Grid description passed by value to every kernel.
 struct GridDev {
     int nx, ny, nz, N;

@@ -22,6 +22,7 @@ struct P1Args {
     double* J_out;                // current of the next map evaluation (3*N, pre-zeroed)
     double* red;                  // [3] velocity-error sums
     unsigned long long* stats;    // [3] path statistics
+    double* part;                 // per-block partials (kPartSlots per block)
     double *x_out, *v_out, *P_out, *vprev_out, *vbar_out;  // final mode
 };
 
@@ -42,6 +43,7 @@ struct P3Args {
     double* red;                  // [2]: sum of local iterations, failed particles
     double* redmax;               // [2]: max local iterations, max final scaled residual
     unsigned long long* stats;
+    double* part;                 // per-block partials (kPartSlots per block)
 };
 
 // P2 explicit step arguments.
@@ -108,6 +110,7 @@ struct P3MArgs {
     double* red;                   // [2]: sum of local iterations, failed particles
     double* redmax;                // [2]: max local iterations, max final scaled residual
     unsigned long long* stats;
+    double* part;                  // per-block partials (kPartSlots per block)
     // initial guess: 0 = v^n, 1 = physics predictor, 2 = predictor + network
     int guess_mode;
     const float* mlp;              // network inference blob (guess_mode 2)
@@ -140,6 +143,13 @@ void launch_mlp_adam(hipStream_t s, float* P, const float* G, float* M, float* V
 void launch_mlp_pack(hipStream_t s, const float* P, float* blob);
 void launch_mlp_forward(hipStream_t s, const float* blob, const float* X, int64_t n, float* Y);
 void launch_mlp_batch_idx(hipStream_t s, uint64_t seed, uint64_t step, int B, int64_t n, int* idx);
+
+// Combine per-block partials of `nblocks` blocks: sum[k] += sums (k < ns),
+// mx[k] = max(mx[k], maxima) (k < nm), stats += {crossings, segments},
+// stats[2] = max(stats[2], max segments).  Any output may be NULL.
+void launch_finish_partials(hipStream_t s, const double* part, int64_t nblocks, int ns, int nm, double* sum,
+                            double* mx, unsigned long long* stats);
+inline int64_t partial_blocks(int64_t np) { return (np + 255) / 256; }
 
 // field_kernels.hip ---------------------------------------------------------
 struct FFTPlan {

@@ -758,9 +758,7 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
         }
     }
     if (a.J_out) tilef_end(g, T, a.J_out);
-    block_reduce_add<2>(rsum, a.red);
-    block_reduce_max<2>(rmax, a.redmax);
-    path_stats_add(nseg_total, a.stats);
+    block_partials<2, 2>(rsum, rmax, nseg_total, a.part);
 }
 
 // FP32 node records from the FP64 fields
@@ -886,6 +884,7 @@ void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int sp
         default: break;
     }
 #undef UPIC_MIXED_LAUNCH
+    launch_finish_partials(s, a.part, mblk(a.np), 2, 2, a.red, a.redmax, a.stats);
 }
 
 void launch_pack_nodes_f32(hipStream_t s, int N, const double* gpm, const double* A_n, const double* A_end, float* NR) {

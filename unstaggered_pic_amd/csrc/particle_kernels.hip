@@ -118,7 +118,7 @@ __global__ __launch_bounds__(256) void k_p1_deposit_guess(GridDev g, QuadDev q, 
         else nseg = deposit_orbit_2d<R>(g, q, opt, X, D, Vb, a.q[p], T, a.J_out);
     }
     if constexpr (!Z3) tile_end(g, T, a.J_out);
-    path_stats_add(nseg, a.stats);
+    block_partials<0, 0>(nullptr, nullptr, nseg, a.part);
 }
 
 // Map evaluation tail (P1:1481-1499) for one iterate: orbit gather of
@@ -190,10 +190,7 @@ __global__ __launch_bounds__(256) void k_p1_push(GridDev g, QuadDev q, OrbitOpt 
         }
     }
     if constexpr (!Z3 && !FINAL) tile_end(g, T, a.J_out);
-    if constexpr (!FINAL) {
-        block_reduce_add<3>(err, a.red);
-        path_stats_add(nseg, a.stats);
-    }
+    if constexpr (!FINAL) block_partials<3, 0>(err, nullptr, nseg, a.part);
 }
 
 // ---------------------------------------------------------------------------
@@ -407,9 +404,7 @@ __global__ __launch_bounds__(256) void k_p3_map(GridDev g, QuadDev qd, OrbitOpt 
     if constexpr (!Z3) {
         if (a.J_out) tile_end(g, T, a.J_out);
     }
-    block_reduce_add<2>(rsum, a.red);
-    block_reduce_max<2>(rmax, a.redmax);
-    path_stats_add(nseg_total, a.stats);
+    block_partials<2, 2>(rsum, rmax, nseg_total, a.part);
 }
 
 // ---------------------------------------------------------------------------
@@ -550,6 +545,7 @@ void launch_deposit_scalar_point(hipStream_t s, const GridDev& g, const double* 
 void launch_p1_deposit_guess(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const P1Args& a) {
     if (a.np <= 0) return;
     UPIC_DISPATCH(g.order, g.nz > 1, hipLaunchKernelGGL((k_p1_deposit_guess<R, Z>), dim3(nblk(a.np)), dim3(256), 0, s, g, q, o, a));
+    launch_finish_partials(s, a.part, nblk(a.np), 0, 0, nullptr, nullptr, a.stats);
 }
 void launch_p1_push(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const P1Args& a, bool final_) {
     if (a.np <= 0) return;
@@ -557,11 +553,64 @@ void launch_p1_push(hipStream_t s, const GridDev& g, const QuadDev& q, const Orb
         UPIC_DISPATCH(g.order, g.nz > 1, hipLaunchKernelGGL((k_p1_push<R, Z, true>), dim3(nblk(a.np)), dim3(256), 0, s, g, q, o, a));
     } else {
         UPIC_DISPATCH(g.order, g.nz > 1, hipLaunchKernelGGL((k_p1_push<R, Z, false>), dim3(nblk(a.np)), dim3(256), 0, s, g, q, o, a));
+        launch_finish_partials(s, a.part, nblk(a.np), 3, 0, a.red, nullptr, a.stats);
     }
 }
 void launch_p3_map(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const P3Args& a) {
     if (a.np <= 0) return;
     UPIC_DISPATCH(g.order, g.nz > 1, hipLaunchKernelGGL((k_p3_map<R, Z>), dim3(nblk(a.np)), dim3(256), 0, s, g, q, o, a));
+    launch_finish_partials(s, a.part, nblk(a.np), 2, 2, a.red, a.redmax, a.stats);
+}
+
+// Second stage of the sweep reductions: one workgroup folds the per-block
+// partials and updates the accumulators.
+__global__ __launch_bounds__(256) void k_finish_partials(const double* __restrict__ part, int64_t nb, int ns, int nm,
+                                                         double* __restrict__ sum, double* __restrict__ mx,
+                                                         unsigned long long* __restrict__ stats) {
+    __shared__ double sh[kPartSlots][4];
+    double v[kPartSlots];
+#pragma unroll
+    for (int k = 0; k < kPartSlots; ++k) v[k] = 0.0;
+    for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) {
+        const double* p = part + b * kPartSlots;
+#pragma unroll
+        for (int k = 0; k < kPartSlots; ++k) {
+            const bool m = (k == 4 || k == 5 || k == 8);
+            v[k] = m ? fmax(v[k], p[k]) : v[k] + p[k];
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kPartSlots; ++k) {
+        const bool m = (k == 4 || k == 5 || k == 8);
+        double x = v[k];
+        for (int o = 32; o > 0; o >>= 1) {
+            const double y = __shfl_down(x, o, 64);
+            x = m ? fmax(x, y) : x + y;
+        }
+        if (lane == 0) sh[k][wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[kPartSlots];
+#pragma unroll
+        for (int k = 0; k < kPartSlots; ++k) {
+            const bool m = (k == 4 || k == 5 || k == 8);
+            t[k] = sh[k][0];
+            for (int w = 1; w < 4; ++w) t[k] = m ? fmax(t[k], sh[k][w]) : t[k] + sh[k][w];
+        }
+        for (int k = 0; k < ns && sum; ++k) sum[k] += t[k];
+        for (int k = 0; k < nm && mx; ++k) mx[k] = fmax(mx[k], t[4 + k]);
+        if (stats) {
+            stats[0] += (unsigned long long)t[6];
+            stats[1] += (unsigned long long)t[7];
+            stats[2] = max(stats[2], (unsigned long long)t[8]);
+        }
+    }
+}
+void launch_finish_partials(hipStream_t s, const double* part, int64_t nblocks, int ns, int nm, double* sum,
+                            double* mx, unsigned long long* stats) {
+    if (nblocks > 0) hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(256), 0, s, part, nblocks, ns, nm, sum, mx, stats);
 }
 void launch_p2_advance_deposit(hipStream_t s, const P2Args& a) {
     if (a.np <= 0) return;

@@ -272,6 +272,48 @@ __device__ __forceinline__ void block_reduce_max(double v[K], double* __restrict
     }
 }
 
+// Per-block partial results of a particle sweep.  Tens of thousands of
+// blocks adding into the same few global words serialise in L2 (about a
+// million same-address atomics per 16.8M-particle sweep), so each block
+// writes its partials to its own slots and launch_finish_partials combines
+// them.  Slot layout per block (kPartSlots doubles): [0, 4) sums, [4, 6)
+// maxima, [6] particles with more than one path segment, [7] total
+// segments, [8] maximum segments.
+template <int NS, int NM>
+__device__ __forceinline__ void block_partials(const double* sums, const double* maxs, int nseg,
+                                               double* __restrict__ part) {
+    static_assert(NS <= 4 && NM <= 2, "partial slots");
+    __shared__ double sh[kPartSlots][16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double v[kPartSlots];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = k < NS ? sums[k] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) v[4 + k] = k < NM ? maxs[k] : 0.0;
+    v[6] = nseg > 1 ? 1.0 : 0.0;
+    v[7] = (double)nseg;
+    v[8] = (double)nseg;
+#pragma unroll
+    for (int k = 0; k < kPartSlots; ++k) {
+        double x = v[k];
+        const bool mx = (k == 4 || k == 5 || k == 8);
+        for (int o = 32; o > 0; o >>= 1) {
+            const double y = __shfl_down(x, o, 64);
+            x = mx ? fmax(x, y) : x + y;
+        }
+        if (lane == 0) sh[k][wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < kPartSlots) {
+        const int k = threadIdx.x;
+        const bool mx = (k == 4 || k == 5 || k == 8);
+        const int nw = (blockDim.x + 63) >> 6;
+        double x = sh[k][0];
+        for (int i = 1; i < nw; ++i) x = mx ? fmax(x, sh[k][i]) : x + sh[k][i];
+        part[(size_t)blockIdx.x * kPartSlots + k] = x;
+    }
+}
+
 // Path statistics accumulator: [0] crossing particles, [1] total segments,
 // [2] max segments (PathStats, P1:771-776).
 __device__ __forceinline__ void path_stats_add(int nseg, unsigned long long* __restrict__ stats) {

@@ -60,6 +60,7 @@ struct upic_sim {
     int* itmp = nullptr;     // np sort scratch
     float* NRf = nullptr;   // FP32 node records (8N)
     double outer_atol = 0.0;  // absolute outer tolerance in force for this step
+    double* part = nullptr;   // per-block partials of the particle sweeps
     // learned initial guess (mixed precision): mode, network, training rows
     int guess_mode = 0;
     float* mlp_P = nullptr;     // parameters (kMlpNParam)
@@ -94,7 +95,7 @@ void sim_free(upic_sim* s) {
                     &s->mo.off, &s->mo.P, &s->mo.v, &s->mo.vb, &s->ftmp, &s->ftmp1, &s->NRf};
     for (float** p : fl) dfree(*p);
     dfree(s->ms.cell); dfree(s->mo.cell); dfree(s->itmp);
-    dfree(s->mlp_P); dfree(s->mlp_blob); dfree(s->train_buf);
+    dfree(s->mlp_P); dfree(s->mlp_blob); dfree(s->train_buf); dfree(s->part);
 }
 
 // FP64 particle arrays -> mixed state
@@ -268,7 +269,7 @@ void p1_step(upic_sim* s, upic_step_info* info) {
     a.x = s->x; a.v = s->v; a.v_prev = s->v_prev; a.P = s->P; a.q = s->q; a.m = s->m;
     a.A_n = s->A; a.A_end = s->AG; a.gpm = s->AG + 3 * (size_t)N;
     a.NR = s->NR;
-    a.red = c->dred; a.stats = c->dstats;
+    a.red = c->dred; a.stats = c->dstats; a.part = s->part;
     // J_0 from v_guess = 2 v^n - v^{n-1}
     int cur = 0;
     launch_zero(c->stream, 3ll * N, s->Jb[cur]);
@@ -381,7 +382,7 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
         a.J_out = Jmap;
         a.cell_out = s->mo.cell; a.off_out = s->mo.off; a.P_out = s->mo.P; a.v_out = s->mo.v; a.vbar_out = s->mo.vb;
         a.reason = s->reason;
-        a.red = c->dred; a.redmax = c->dred + 2; a.stats = c->dstats;
+        a.red = c->dred; a.redmax = c->dred + 2; a.stats = c->dstats; a.part = s->part;
         a.guess_mode = s->guess_mode;
         a.mlp = s->mlp_ready ? s->mlp_blob : nullptr;
         const bool collect = write_out && !use_layout && s->train_stride > 0 && s->train_n < s->train_cap;
@@ -403,7 +404,7 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     a.J_out = Jmap;
     a.x_out = s->xo; a.P_out = s->Po; a.v_out = s->vo; a.vbar_out = s->vbo;
     a.reason = s->reason;
-    a.red = c->dred; a.redmax = c->dred + 2; a.stats = c->dstats;
+    a.red = c->dred; a.redmax = c->dred + 2; a.stats = c->dstats; a.part = s->part;
     timed(c, "p3_map", [&] { launch_p3_map(c->stream, c->g, c->q, c->opt, a); });
     }
     launch_project_nyquist(c->stream, c->sp, Jmap, 3);
@@ -1062,6 +1063,7 @@ upic_status upic_sim_create(upic_ctx* c, int64_t np, upic_sim** out) {
         s->ptmp = dalloc<double>(n3);
         s->ptmp1 = dalloc<double>(np);
         s->keys = dalloc<int>(np); s->perm = dalloc<int>(np);
+        s->part = dalloc<double>((size_t)(partial_blocks(np) + 1) * kPartSlots);
         s->counts = dalloc<int>(N); s->offsets = dalloc<int>(N);
         if (c->p.code == 1) {
             s->vg[0] = dalloc<double>(n3);
