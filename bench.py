@@ -155,10 +155,11 @@ def run_form(args, prec, learned):
     el = time.perf_counter() - t0
     kname = "p3_map" if prec == 0 else "p3_map_mixed"
     launches, kms = ctx.timing(kname)
+    dep = ctx.timing("deposit_mixed") if prec == 1 else (0, 0.0)
     d1 = sim.diagnostics(1)
     sim.close()
     ctx.close()
-    return {"p": p, "np": np_, "N": N, "elapsed": el, "infos": infos, "launches": launches, "kms": kms,
+    return {"p": p, "np": np_, "N": N, "elapsed": el, "infos": infos, "launches": launches, "kms": kms, "dep": dep,
             "kname": kname, "d0": d0, "d_start": d_start, "d1": d1, "extra": extra}
 
 
@@ -177,12 +178,22 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=15)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel event timing")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device (a known-size kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path", file=sys.stderr)
         sys.exit(2)
     if args.precision:
         args.form = args.precision
+    if args.calibrate_copy_mib > 0:
+        from unstaggered_pic_amd import lib
+        n = args.calibrate_copy_mib * (1 << 20) // 8
+        a, b = lib.DeviceArray(n), lib.DeviceArray(n)
+        lib.lib().upic_memcpy_htod(a.p, np.zeros(n).ctypes.data_as(C.c_void_p), 8 * n)
+        if lib.lib().upic_memcpy_dtod(b.p, a.p, 8 * n) != 0:
+            raise RuntimeError("device copy failed")
+        del a, b
     forms = {"both": [(0, False), (1, True)], "fp64": [(0, False)], "mixed": [(1, False)],
              "learned": [(1, True)]}[args.form]
     runs = []
@@ -248,7 +259,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
                      "kernel": "k_p3_map" if prec == 0 else "k_p3_map_mixed", "avg_launch_ms": avg_ms,
-                     "launches": launches, "bytes_per_launch": bytes_per_launch},
+                     "launches": launches, "bytes_per_launch": bytes_per_launch,
+                     "deposit_kernel_avg_ms": (r["dep"][1] / r["dep"][0]) if r["dep"][0] else None},
         "compute_roofline": {"bound": "fp64_valu" if prec == 0 else "fp32_valu", "achieved": compute_tflops,
                              "peak": peak_c, "unit": "TFLOP/s",
                              "frac": (compute_tflops / peak_c) if compute_tflops else None,

@@ -194,6 +194,7 @@ upic_status upic_device_alloc(size_t bytes, void** ptr);
 upic_status upic_device_free(void* ptr);
 upic_status upic_memcpy_htod(void* dst, const void* src, size_t bytes);
 upic_status upic_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+upic_status upic_memcpy_dtod(void* dst, const void* src, size_t bytes);
 
 /* ---- Group A: kernel-level entry points (device pointers) --------------- */
 
@@ -311,6 +312,9 @@ upic_status upic_sim_guess_data(upic_sim* sim, float* rows, int64_t max_rows, in
  * mean minibatch loss of each epoch to loss_per_epoch when not NULL. */
 upic_status upic_sim_train_guess(upic_sim* sim, int32_t epochs, int32_t batch, double lr, uint64_t seed,
                                  double* loss_per_epoch);
+/* In mode 2 the network runs in the first outer evaluation of a step; the
+ * later evaluations add its stored correction to the predictor evaluated on
+ * their own fields. */
 upic_status upic_sim_set_guess_mode(upic_sim* sim, int32_t mode);
 upic_status upic_sim_guess_params(upic_sim* sim, float* params);          /* UPIC_MLP_NPARAM floats out */
 upic_status upic_sim_set_guess_params(upic_sim* sim, const float* params); /* and in (installs the network) */

@@ -137,6 +137,9 @@ upic_status upic_memcpy_htod(void* dst, const void* src, size_t bytes) {
 upic_status upic_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
     return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? UPIC_OK : UPIC_EDEVICE;
 }
+upic_status upic_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice) == hipSuccess ? UPIC_OK : UPIC_EDEVICE;
+}
 
 // ---- Group A --------------------------------------------------------------
 static void read_stats(upic_ctx* c, upic_path_stats* st, int64_t np) {

@@ -116,8 +116,20 @@ struct P3MArgs {
     const float* mlp;              // network inference blob (guess_mode 2)
     float* train;                  // training rows (15 floats) or NULL
     int64_t train_stride, train_offset, train_cap;
+    // the network's correction of the guess (3*np): written by the first
+    // evaluation of a step, reused by the later ones when warm_in is set
+    float* warm;
+    int warm_in;
+    float* vdep;                   // converged vbar (3*np) for the separate deposit
 };
+// Picard sweep; with a substep layout it also deposits, otherwise it writes
+// vdep and launch_deposit_mixed deposits (reason and vdep as inputs).
 void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& a);
+// cell_end (NULL when the particles are not freshly cell-sorted): end index
+// of each cell's particle range after the counting sort (cell c holds
+// [cell_end[c-1], cell_end[c])); enables the cell-mapped deposit.
+void launch_deposit_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& a,
+                          const int* cell_end);
 // NRf[8*id + ...] = FP32 copy of the FP64 node records
 void launch_pack_nodes_f32(hipStream_t s, int N, const double* gpm, const double* A_n, const double* A_end, float* NR);
 // also replaces q and m by their FP32 values (one charge and mass for all parts)

@@ -17,6 +17,8 @@
 //   path_breakpoints_for_particle     P3:2555-2606
 // with positions relative to the particle's start cell, so that FP32 keeps
 // ~1e-7 of a cell of resolution anywhere on the grid.
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "mlp.hpp"
 #include "particles.hpp"
@@ -635,14 +637,19 @@ __device__ __forceinline__ void guess_features(const MGrid& g, const float4* __r
 }
 
 // GM: initial guess of the inner solve, 0 = v^n (the reference's), 1 =
-// physics predictor, 2 = predictor + network; COLLECT: write (features,
-// converged correction) training rows in the write-out pass.
-template <int R, int GM, bool COLLECT>
+// physics predictor, 2 = predictor + network (its correction stored in
+// a.warm when given), 3 = predictor + the stored correction; COLLECT: write
+// (features, converged correction) training rows in the write-out pass.
+// FUSED: deposit inside the sweep (per-particle substep layouts); otherwise
+// the converged vbar goes to a.vdep and k_deposit_mixed deposits it, which
+// keeps the deposit's registers out of the Picard sweep.
+template <int R, int GM, bool COLLECT, bool FUSED>
 __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ float tile_mem[kTileM * kTileM * 3];
     extern __shared__ unsigned char mlp_lds[];
-    const DepTileF T = tilef_begin(g, tile_mem, a.cell, a.np, (int64_t)blockIdx.x * blockDim.x);
+    DepTileF T{};
+    if constexpr (FUSED) T = tilef_begin(g, tile_mem, a.cell, a.np, (int64_t)blockIdx.x * blockDim.x);
     // features and guess at the start point (all lanes: the network runs on
     // whole wavefronts)
     float feat[12] = {}, dph[3] = {0.0f, 0.0f, 0.0f}, lg[3] = {0.0f, 0.0f, 0.0f};
@@ -654,7 +661,13 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
             guess_features<R>(g, a.NR, c0 / g.ny, c0 - (c0 / g.ny) * g.ny, a.off[p], a.off[a.np + p], v0, a.q[p],
                               a.m[p], (float)a.kappa, (float)a.dt, feat, dph);
         }
-        if constexpr (GM == 2) mlp_wave_infer(a.mlp, feat, lg, mlp_lds + (threadIdx.x / 64) * kMlpLdsWave);
+        if constexpr (GM == 2) {
+            mlp_wave_infer(a.mlp, feat, lg, mlp_lds + (threadIdx.x / 64) * kMlpLdsWave);
+            if (a.warm && p < a.np) st3f(a.warm, a.np, p, lg);
+        }
+        if constexpr (GM == 3) {
+            if (p < a.np) ld3f(a.warm, a.np, p, lg);
+        }
     }
     double rsum[2] = {0.0, 0.0}, rmax[2] = {0.0, 0.0};
     int nseg_total = 0;
@@ -681,7 +694,7 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
             if constexpr (GM > 0) {
                 if (nsub == 1) {
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) gs[d] = vc[d] + dph[d] + (GM == 2 ? lg[d] : 0.0f);
+                    for (int d = 0; d < 3; ++d) gs[d] = vc[d] + dph[d] + (GM >= 2 ? lg[d] : 0.0f);
                 }
             }
             LocalMapF acc;
@@ -730,9 +743,13 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
                 }
             }
             const float dux = h * vacc[0] * g.ihx, duy = h * vacc[1] * g.ihy;
-            if (a.J_out)
-                nseg_total += deposit_orbit_f<R>(g, qd, split != 0, ci, cj, tx, ty, dux, duy, vacc,
-                                                 qp / (float)nsub * a.inv_dV, T, a.J_out);
+            if constexpr (FUSED) {
+                if (a.J_out)
+                    nseg_total += deposit_orbit_f<R>(g, qd, split != 0, ci, cj, tx, ty, dux, duy, vacc,
+                                                     qp / (float)nsub * a.inv_dV, T, a.J_out);
+            } else {
+                st3f(a.vdep, a.np, p, vacc);
+            }
             move_axis(tx, dux, ci, g.nx);
             move_axis(ty, duy, cj, g.ny);
             int cz = 0;
@@ -757,8 +774,157 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
             st3f(a.vbar_out, a.np, p, vm);
         }
     }
-    if (a.J_out) tilef_end(g, T, a.J_out);
+    if constexpr (FUSED) {
+        if (a.J_out) tilef_end(g, T, a.J_out);
+    }
     block_partials<2, 2>(rsum, rmax, nseg_total, a.part);
+}
+
+// Orbit current deposit of the converged single-substep sweep: particle p
+// moves from its start point by dt vbar (vbar from the sweep), FP32 LDS tile
+// flushed into the FP64 mesh current (path_average_deposit_current,
+// P3:2784-2857, with the sweep's q/dV weight).  Particles that failed the
+// inner solve deposit nothing (the step is then rejected anyway).
+template <int R>
+__global__ __launch_bounds__(256) void k_deposit_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ float tile_mem[kTileM * kTileM * 3];
+    const DepTileF T = tilef_begin(g, tile_mem, a.cell, a.np, (int64_t)blockIdx.x * blockDim.x);
+    int nseg = 0;
+    if (p < a.np && a.reason[p] == 0) {
+        const float h = (float)a.dt;
+        const int c0 = a.cell[p];
+        const int ci = c0 / g.ny, cj = c0 - (c0 / g.ny) * g.ny;
+        float vb[3];
+        ld3f(a.vdep, a.np, p, vb);
+        const float dux = h * vb[0] * g.ihx, duy = h * vb[1] * g.ihy;
+        nseg = deposit_orbit_f<R>(g, qd, split != 0, ci, cj, a.off[p], a.off[a.np + p], dux, duy, vb,
+                                  a.q[p] * a.inv_dV, T, a.J_out);
+    }
+    tilef_end(g, T, a.J_out);
+    block_partials<0, 0>(nullptr, nullptr, nseg, a.part);
+}
+
+// Cell-mapped deposit (cell-sorted particles, ny a multiple of 64): a
+// workgroup owns 64 consecutive cells of one grid row; lane l of every wave
+// takes cell c0 + l, and the kCellWaves waves of the kCellSlices workgroups
+// (grid y) of that row segment split each cell's particles.
+// The 64 lanes of an LDS atomic then address 64 different stencil windows
+// instead of one (particles of one cell share their stencil), which removes
+// the 64-way LDS bank serialisation of the particle-order deposit.  The LDS
+// tile spans rows i0-3..i0+4 and columns j0-3..j0+68 of the workgroup's row.
+constexpr int kCellWaves = 4, kCellSlices = 8, kCTR = 8, kCTC = 72;
+
+template <int R>
+__global__ __launch_bounds__(256) void k_deposit_cells_mixed(MGrid g, QuadF qd, int split, P3MArgs a,
+                                                             const int* __restrict__ cell_end) {
+    __shared__ float tile[kCTR * kCTC * 3];
+    const int c0 = blockIdx.x * 64;
+    const int i0 = c0 / g.ny, j0 = c0 - (c0 / g.ny) * g.ny;
+    const int ox = wrapn(i0 - 3, g.nx), oy = wrapn(j0 - 3, g.ny);
+    for (int e = threadIdx.x; e < kCTR * kCTC * 3; e += blockDim.x) tile[e] = 0.0f;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
+    const int c = c0 + lane;
+    const int64_t kb = c == 0 ? 0 : cell_end[c - 1], ke = cell_end[c];
+    double cross = 0.0, segs = 0.0, mseg = 0.0;
+    const int ci = i0, cj = j0 + lane;
+    const int stride = kCellWaves * gridDim.y;
+    for (int64_t p = kb + sub + kCellWaves * blockIdx.y; p < ke; p += stride) {
+        if (a.reason[p] != 0) continue;
+        float vb[3];
+        ld3f(a.vdep, a.np, p, vb);
+        const float h = (float)a.dt;
+        const float dux = h * vb[0] * g.ihx, duy = h * vb[1] * g.ihy;
+        const float qs = a.q[p] * a.inv_dV;
+        // the orbit deposit with the block's tile (window flush as in deposit_orbit_f)
+        float W[R + 1][R + 1];
+        int bx = 0, by = 0;
+        bool have = false;
+        auto flush = [&]() {
+            if (!have) return;
+            const int gx = wrapn(ci + bx, g.nx), gy = wrapn(cj + by, g.ny);
+#pragma unroll
+            for (int aa = 0; aa <= R; ++aa) {
+                const int ia = wrap1n(gx + aa, g.nx);
+                int dx = ia - ox;
+                if (dx < 0) dx += g.nx;
+#pragma unroll
+                for (int bb = 0; bb <= R; ++bb) {
+                    const int ib = wrap1n(gy + bb, g.ny);
+                    int dy = ib - oy;
+                    if (dy < 0) dy += g.ny;
+                    const float w = W[aa][bb] * qs;
+                    if (dx < kCTR && dy < kCTC) {
+                        float* t = tile + 3 * (dx * kCTC + dy);
+                        atomicAdd(t, w * vb[0]);
+                        atomicAdd(t + 1, w * vb[1]);
+                        atomicAdd(t + 2, w * vb[2]);
+                    } else {
+                        const int id = ia * g.ny + ib;
+                        unsafeAtomicAdd(&a.J_out[id], (double)(w * vb[0]));
+                        unsafeAtomicAdd(&a.J_out[g.N + id], (double)(w * vb[1]));
+                        unsafeAtomicAdd(&a.J_out[2 * g.N + id], (double)(w * vb[2]));
+                    }
+                }
+            }
+            have = false;
+        };
+        BreakF bg;
+        const float t0x = a.off[p], t0y = a.off[a.np + p];
+        breakf_init(bg, R, t0x, t0y, split ? dux : 0.0f, split ? duy : 0.0f);
+        float sa = 0.0f;
+        int nseg = 0;
+        for (;;) {
+            const float sb = breakf_next(bg);
+            const float len = sb - sa;
+            if (len > 0.0f) {
+                for (int iq = 0; iq < qd.n; ++iq) {
+                    const float s_ = sa + len * qd.s[iq];
+                    const float w = len * qd.w[iq];
+                    AxF<R> sx, sy;
+                    axis_f<R>(t0x + s_ * dux, g.ihx, sx);
+                    axis_f<R>(t0y + s_ * duy, g.ihy, sy);
+                    if (!(have && sx.fl == bx && sy.fl == by)) {
+                        flush();
+                        bx = sx.fl;
+                        by = sy.fl;
+#pragma unroll
+                        for (int aa = 0; aa <= R; ++aa)
+#pragma unroll
+                            for (int bb = 0; bb <= R; ++bb) W[aa][bb] = 0.0f;
+                        have = true;
+                    }
+#pragma unroll
+                    for (int aa = 0; aa <= R; ++aa) {
+                        const float wa = w * sx.w[aa];
+#pragma unroll
+                        for (int bb = 0; bb <= R; ++bb) W[aa][bb] += wa * sy.w[bb];
+                    }
+                }
+            }
+            ++nseg;
+            sa = sb;
+            if (sb >= 1.0f) break;
+        }
+        flush();
+        cross += nseg > 1 ? 1.0 : 0.0;
+        segs += (double)nseg;
+        mseg = fmax(mseg, (double)nseg);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kCTR * kCTC * 3; e += blockDim.x) {
+        const float v = tile[e];
+        if (v != 0.0f) {
+            const int comp = e % 3, node = e / 3;
+            const int dx = node / kCTC, dy = node - (node / kCTC) * kCTC;
+            if (dx >= g.nx || dy >= g.ny) continue;
+            const int ia = (ox + dx) % g.nx, ib = (oy + dy) % g.ny;
+            unsafeAtomicAdd(&a.J_out[(size_t)comp * g.N + ia * g.ny + ib], (double)v);
+        }
+    }
+    block_partials_stats<0, 0>(nullptr, nullptr, cross, segs, mseg,
+                               a.part + (size_t)blockIdx.y * gridDim.x * kPartSlots);
 }
 
 // FP32 node records from the FP64 fields
@@ -861,19 +1027,25 @@ void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int sp
     mg.nx = g.nx; mg.ny = g.ny; mg.N = g.N; mg.order = g.order;
     mg.hx = (float)g.h[0]; mg.hy = (float)g.h[1];
     mg.ihx = (float)(1.0 / g.h[0]); mg.ihy = (float)(1.0 / g.h[1]);
-    const int gm = (a.layout || (a.guess_mode == 2 && !a.mlp)) ? 0 : a.guess_mode;
+    int gm = (a.layout || (a.guess_mode == 2 && !a.mlp)) ? 0 : a.guess_mode;
+    if (gm == 2 && a.warm_in) gm = 3;  // the network's correction of this step is cached
     const bool col = a.train != nullptr && !a.layout;
     const size_t lds = gm == 2 ? 4 * (size_t)kMlpLdsWave : 0;
 #define UPIC_MIXED_LAUNCH(R_)                                                                                       \
     do {                                                                                                            \
-        if (col) {                                                                                                  \
-            if (gm == 2) hipLaunchKernelGGL((k_p3_map_mixed<R_, 2, true>), dim3(mblk(a.np)), dim3(256), lds, s, mg, q, split, a); \
-            else if (gm == 1) hipLaunchKernelGGL((k_p3_map_mixed<R_, 1, true>), dim3(mblk(a.np)), dim3(256), 0, s, mg, q, split, a); \
-            else hipLaunchKernelGGL((k_p3_map_mixed<R_, 0, true>), dim3(mblk(a.np)), dim3(256), 0, s, mg, q, split, a); \
+        const dim3 gr(mblk(a.np)), bl(256);                                                                         \
+        if (a.layout) {                                                                                             \
+            hipLaunchKernelGGL((k_p3_map_mixed<R_, 0, false, true>), gr, bl, 0, s, mg, q, split, a);               \
+        } else if (col) {                                                                                           \
+            if (gm == 3) hipLaunchKernelGGL((k_p3_map_mixed<R_, 3, true, false>), gr, bl, 0, s, mg, q, split, a);  \
+            else if (gm == 2) hipLaunchKernelGGL((k_p3_map_mixed<R_, 2, true, false>), gr, bl, lds, s, mg, q, split, a); \
+            else if (gm == 1) hipLaunchKernelGGL((k_p3_map_mixed<R_, 1, true, false>), gr, bl, 0, s, mg, q, split, a); \
+            else hipLaunchKernelGGL((k_p3_map_mixed<R_, 0, true, false>), gr, bl, 0, s, mg, q, split, a);          \
         } else {                                                                                                    \
-            if (gm == 2) hipLaunchKernelGGL((k_p3_map_mixed<R_, 2, false>), dim3(mblk(a.np)), dim3(256), lds, s, mg, q, split, a); \
-            else if (gm == 1) hipLaunchKernelGGL((k_p3_map_mixed<R_, 1, false>), dim3(mblk(a.np)), dim3(256), 0, s, mg, q, split, a); \
-            else hipLaunchKernelGGL((k_p3_map_mixed<R_, 0, false>), dim3(mblk(a.np)), dim3(256), 0, s, mg, q, split, a); \
+            if (gm == 3) hipLaunchKernelGGL((k_p3_map_mixed<R_, 3, false, false>), gr, bl, 0, s, mg, q, split, a); \
+            else if (gm == 2) hipLaunchKernelGGL((k_p3_map_mixed<R_, 2, false, false>), gr, bl, lds, s, mg, q, split, a); \
+            else if (gm == 1) hipLaunchKernelGGL((k_p3_map_mixed<R_, 1, false, false>), gr, bl, 0, s, mg, q, split, a); \
+            else hipLaunchKernelGGL((k_p3_map_mixed<R_, 0, false, false>), gr, bl, 0, s, mg, q, split, a);         \
         }                                                                                                           \
     } while (0)
     switch (g.order) {
@@ -884,7 +1056,41 @@ void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int sp
         default: break;
     }
 #undef UPIC_MIXED_LAUNCH
-    launch_finish_partials(s, a.part, mblk(a.np), 2, 2, a.red, a.redmax, a.stats);
+    launch_finish_partials(s, a.part, mblk(a.np), 2, 2, a.red, a.redmax, a.layout ? a.stats : nullptr);
+}
+
+void launch_deposit_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& a,
+                          const int* cell_end) {
+    if (a.np <= 0 || a.layout || !a.J_out) return;
+    MGrid mg;
+    mg.nx = g.nx; mg.ny = g.ny; mg.N = g.N; mg.order = g.order;
+    mg.hx = (float)g.h[0]; mg.hy = (float)g.h[1];
+    mg.ihx = (float)(1.0 / g.h[0]); mg.ihy = (float)(1.0 / g.h[1]);
+    if (cell_end && g.ny % 64 == 0 && g.nx >= kCTR) {
+        const int nb = g.N / 64;
+        // the partial slots of nb * kCellSlices workgroups must fit the
+        // per-particle-block buffer (sized for ceil(np / 256) blocks)
+        const int sl = (int)std::max<int64_t>(1, std::min<int64_t>(kCellSlices, (int64_t)mblk(a.np) / std::max(nb, 1)));
+        const dim3 gr(nb, sl), bl(64 * kCellWaves);
+        switch (g.order) {
+            case 1: hipLaunchKernelGGL(k_deposit_cells_mixed<1>, gr, bl, 0, s, mg, q, split, a, cell_end); break;
+            case 2: hipLaunchKernelGGL(k_deposit_cells_mixed<2>, gr, bl, 0, s, mg, q, split, a, cell_end); break;
+            case 3: hipLaunchKernelGGL(k_deposit_cells_mixed<3>, gr, bl, 0, s, mg, q, split, a, cell_end); break;
+            case 4: hipLaunchKernelGGL(k_deposit_cells_mixed<4>, gr, bl, 0, s, mg, q, split, a, cell_end); break;
+            default: break;
+        }
+        launch_finish_partials(s, a.part, (int64_t)nb * sl, 0, 0, nullptr, nullptr, a.stats);
+        return;
+    }
+    const dim3 gr(mblk(a.np)), bl(256);
+    switch (g.order) {
+        case 1: hipLaunchKernelGGL(k_deposit_mixed<1>, gr, bl, 0, s, mg, q, split, a); break;
+        case 2: hipLaunchKernelGGL(k_deposit_mixed<2>, gr, bl, 0, s, mg, q, split, a); break;
+        case 3: hipLaunchKernelGGL(k_deposit_mixed<3>, gr, bl, 0, s, mg, q, split, a); break;
+        case 4: hipLaunchKernelGGL(k_deposit_mixed<4>, gr, bl, 0, s, mg, q, split, a); break;
+        default: break;
+    }
+    launch_finish_partials(s, a.part, mblk(a.np), 0, 0, nullptr, nullptr, a.stats);
 }
 
 void launch_pack_nodes_f32(hipStream_t s, int N, const double* gpm, const double* A_n, const double* A_end, float* NR) {

@@ -280,8 +280,20 @@ __device__ __forceinline__ void block_reduce_max(double v[K], double* __restrict
 // maxima, [6] particles with more than one path segment, [7] total
 // segments, [8] maximum segments.
 template <int NS, int NM>
+__device__ __forceinline__ void block_partials_stats(const double* sums, const double* maxs, double cross,
+                                                     double segs, double maxseg, double* __restrict__ part);
+
+template <int NS, int NM>
 __device__ __forceinline__ void block_partials(const double* sums, const double* maxs, int nseg,
                                                double* __restrict__ part) {
+    block_partials_stats<NS, NM>(sums, maxs, nseg > 1 ? 1.0 : 0.0, (double)nseg, (double)nseg, part);
+}
+
+// as block_partials, with the thread's path statistics given directly
+// (particles with more than one segment, total segments, maximum segments)
+template <int NS, int NM>
+__device__ __forceinline__ void block_partials_stats(const double* sums, const double* maxs, double cross,
+                                                     double segs, double maxseg, double* __restrict__ part) {
     static_assert(NS <= 4 && NM <= 2, "partial slots");
     __shared__ double sh[kPartSlots][16];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -290,9 +302,9 @@ __device__ __forceinline__ void block_partials(const double* sums, const double*
     for (int k = 0; k < 4; ++k) v[k] = k < NS ? sums[k] : 0.0;
 #pragma unroll
     for (int k = 0; k < 2; ++k) v[4 + k] = k < NM ? maxs[k] : 0.0;
-    v[6] = nseg > 1 ? 1.0 : 0.0;
-    v[7] = (double)nseg;
-    v[8] = (double)nseg;
+    v[6] = cross;
+    v[7] = segs;
+    v[8] = maxseg;
 #pragma unroll
     for (int k = 0; k < kPartSlots; ++k) {
         double x = v[k];

@@ -69,6 +69,12 @@ struct upic_sim {
     float* train_buf = nullptr;
     int64_t train_cap = 0, train_n = 0, train_stride = 0;
     bool train_written = false;
+    // the network's guess correction of the current step (computed by the
+    // first outer evaluation, reused by the later ones)
+    float* warm = nullptr;
+    bool warm_valid = false;
+    float* vdep = nullptr;  // converged vbar of the last sweep (3*np)
+    bool cells_fresh = false;  // particles cell-sorted since their last move (offsets = cell ends)
 };
 
 // Mixed precision: an FP32 map resolves each node current only to a fraction
@@ -95,7 +101,7 @@ void sim_free(upic_sim* s) {
                     &s->mo.off, &s->mo.P, &s->mo.v, &s->mo.vb, &s->ftmp, &s->ftmp1, &s->NRf};
     for (float** p : fl) dfree(*p);
     dfree(s->ms.cell); dfree(s->mo.cell); dfree(s->itmp);
-    dfree(s->mlp_P); dfree(s->mlp_blob); dfree(s->train_buf); dfree(s->part);
+    dfree(s->mlp_P); dfree(s->mlp_blob); dfree(s->train_buf); dfree(s->part); dfree(s->warm); dfree(s->vdep);
 }
 
 // FP64 particle arrays -> mixed state
@@ -251,7 +257,11 @@ void sort_particles(upic_sim* s) {
 
 void maybe_sort(upic_sim* s) {
     const int k = s->c->p.sort_interval;
-    if (k > 0 && (s->steps % k) == 0) sort_particles(s);
+    s->cells_fresh = false;
+    if (k > 0 && (s->steps % k) == 0) {
+        sort_particles(s);
+        s->cells_fresh = true;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -391,7 +401,14 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
         a.train_offset = s->train_n;
         a.train_cap = s->train_cap;
         s->train_written = collect;
+        a.warm = (s->guess_mode == 2 && s->mlp_ready && !use_layout) ? s->warm : nullptr;
+        a.warm_in = (a.warm && s->warm_valid) ? 1 : 0;
+        s->warm_valid = false;  // until this evaluation has completed
+        a.vdep = s->vdep;
         timed(c, "p3_map_mixed", [&] { launch_p3_map_mixed(c->stream, c->g, c->qf, c->opt.split, a); });
+        timed(c, "deposit_mixed", [&] {
+            launch_deposit_mixed(c->stream, c->g, c->qf, c->opt.split, a, s->cells_fresh ? s->offsets : nullptr);
+        });
     } else {
     P3Args a{};
     a.np = s->np; a.dt = p.dt; a.kappa = p.c; a.rtol = p.local_rtol; a.atol = p.local_atol;
@@ -410,6 +427,7 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     launch_project_nyquist(c->stream, c->sp, Jmap, 3);
     HIPCHK(hipGetLastError());
     read_red(c, 4);
+    if (s->mixed && s->guess_mode == 2 && s->mlp_ready && !use_layout) s->warm_valid = true;
     MapOut o;
     o.iter_sum = c->hred[0];
     o.failed = (int64_t)c->hred[1];
@@ -716,6 +734,7 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
 void p3_step(upic_sim* s, upic_step_info* info) {
     upic_ctx* c = s->c;
     s->train_written = false;
+    s->warm_valid = false;
     maybe_sort(s);
     state_spectra(s);
     s->outer_atol = 0.0;
@@ -1106,6 +1125,8 @@ upic_status upic_sim_create(upic_ctx* c, int64_t np, upic_sim** out) {
             s->ms.m = dalloc<float>(np);
             s->ftmp = dalloc<float>(n3);
             s->ftmp1 = dalloc<float>(np);
+            s->warm = dalloc<float>(n3);
+            s->vdep = dalloc<float>(n3);
             s->itmp = dalloc<int>(np);
             s->NRf = dalloc<float>(8 * (size_t)N);
         }
@@ -1391,6 +1412,8 @@ upic_status upic_sim_diagnostics(upic_sim* s, int mode, upic_diagnostics* out) {
         else o.field_energy = (0.5 * p.eps0 * h[0] + 0.5 / p.mu0 * h[9]) * c->g.dV;
         o.kinetic_energy = h[11];
         o.total_energy = o.field_energy + o.kinetic_energy;
+        o.E_l2 = std::sqrt(h[0] / N);
+        o.B_l2 = std::sqrt(h[9] / N);
         o.Bz_y_mode_re = p.code == 3 ? modes[0] * invN : 0.0;
         o.Bz_y_mode_im = p.code == 3 ? modes[1] * invN : 0.0;
         o.Bz_y_mode_abs = p.code == 3 ? std::hypot(modes[0], modes[1]) * invN : 0.0;

@@ -65,7 +65,7 @@ def lib():
             "upic_ctx_set_timing": [vp, C.c_int],
             "upic_device_count": [C.POINTER(C.c_int)], "upic_device_alloc": [C.c_size_t, C.POINTER(vp)],
             "upic_device_free": [vp], "upic_memcpy_htod": [vp, vp, C.c_size_t],
-            "upic_memcpy_dtoh": [vp, vp, C.c_size_t],
+            "upic_memcpy_dtoh": [vp, vp, C.c_size_t], "upic_memcpy_dtod": [vp, vp, C.c_size_t],
             "upic_ctx_timing": [vp, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double)],
             "upic_load_particles": [C.POINTER(Params), C.POINTER(LoadSpec), C.POINTER(HostState)],
             "upic_sim_collect_guess_data": [vp, C.c_int64, C.c_int64],
