@@ -157,10 +157,11 @@ def run_form(args, prec, learned):
     launches, kms = ctx.timing(kname)
     dep = ctx.timing("deposit_mixed") if prec == 1 else (0, 0.0)
     d1 = sim.diagnostics(1)
+    dep_choice = sim.deposit_variant() if prec == 1 else None
     sim.close()
     ctx.close()
     return {"p": p, "np": np_, "N": N, "elapsed": el, "infos": infos, "launches": launches, "kms": kms, "dep": dep,
-            "kname": kname, "d0": d0, "d_start": d_start, "d1": d1, "extra": extra}
+            "kname": kname, "d0": d0, "d_start": d_start, "d1": d1, "extra": extra, "dep_choice": dep_choice}
 
 
 def main():
@@ -260,7 +261,8 @@ def main():
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
                      "kernel": "k_p3_map" if prec == 0 else "k_p3_map_mixed", "avg_launch_ms": avg_ms,
                      "launches": launches, "bytes_per_launch": bytes_per_launch,
-                     "deposit_kernel_avg_ms": (r["dep"][1] / r["dep"][0]) if r["dep"][0] else None},
+                     "deposit_kernel_avg_ms": (r["dep"][1] / r["dep"][0]) if r["dep"][0] else None,
+                     "deposit_variant_sorted_unsorted": r["dep_choice"]},
         "compute_roofline": {"bound": "fp64_valu" if prec == 0 else "fp32_valu", "achieved": compute_tflops,
                              "peak": peak_c, "unit": "TFLOP/s",
                              "frac": (compute_tflops / peak_c) if compute_tflops else None,

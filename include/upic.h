@@ -318,6 +318,20 @@ upic_status upic_sim_train_guess(upic_sim* sim, int32_t epochs, int32_t batch, d
 upic_status upic_sim_set_guess_mode(upic_sim* sim, int32_t mode);
 upic_status upic_sim_guess_params(upic_sim* sim, float* params);          /* UPIC_MLP_NPARAM floats out */
 upic_status upic_sim_set_guess_params(upic_sim* sim, const float* params); /* and in (installs the network) */
+/* ---- Deposit autotuner (mixed precision) --------------------------------- */
+/* No reference counterpart (the reference deposits inside its per-particle
+ * loop, P3:5380-5409).  The mixed-precision current deposit has four
+ * variants: 0 particle order with a wavefront reduction of the common
+ * stencil window, 1 one workgroup per 64 cells, 2 eight workgroups per 64
+ * cells, 3 particle order with per-particle atomics; 1 and 2 need a freshly
+ * cell-sorted particle order and ny % 64 == 0.  By default (variant -1) the
+ * first map evaluations time every usable variant twice, one variant per
+ * evaluation, and then keep the fastest, separately for sorted and unsorted
+ * particle orders.  A variant >= 0 pins it (an unusable pin runs variant 0).
+ * upic_sim_deposit_variant reports the choices (-1 while still tuning). */
+upic_status upic_sim_set_deposit_variant(upic_sim* sim, int32_t variant);
+upic_status upic_sim_deposit_variant(upic_sim* sim, int32_t* sorted_choice, int32_t* unsorted_choice);
+
 /* Standalone network kernels on device buffers: the MFMA inference the push
  * kernel runs (X: n rows of 12 features -> Y: n rows of 3, scaled outputs)
  * and the FP32 training loss/gradient of rows of 15 floats (grad: UPIC_MLP_NPARAM

@@ -36,3 +36,20 @@ def test_load_particles_deterministic():
     g = 1 / np.sqrt(1 - (vv ** 2).sum(0))
     u = vv * g
     assert abs(u[0].std() - 0.05) < 0.01 and abs(u[1].std() - 0.2) < 0.03
+
+
+def test_driver_usage_and_errors(tmp_path):
+    """The deck driver follows the reference's exit codes without a GPU: 2 for
+    usage errors, 1 for a deck it cannot read or an unknown initial
+    condition (raised before any device call)."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "unstaggered_pic_amd", "upic_run")
+    assert subprocess.run([exe], capture_output=True).returncode == 2
+    assert subprocess.run([exe, "p9", "x"], capture_output=True).returncode == 2
+    r = subprocess.run([exe, "p1", str(tmp_path / "missing.txt")], capture_output=True, text=True)
+    assert r.returncode == 1 and "could not open input file" in r.stderr
+    deck = tmp_path / "deck.txt"
+    deck.write_text("test_case = no_such_case\nnx = 8\nny = 1\nnz = 1\nbogus_key = 3\n")
+    r = subprocess.run([exe, "p1", str(deck)], capture_output=True, text=True)
+    assert r.returncode == 1 and "unknown test_case" in r.stderr
+    assert "unknown input key ignored: bogus_key" in r.stderr

@@ -151,3 +151,49 @@ def test_mixed_vs_fp64_gpu_history():
     assert np.max(np.abs(a[:, 0] - b[:, 0]) / np.abs(b[:, 0])) < 1e-6
     assert np.max(np.abs(a[:, 1] - b[:, 1]) / np.abs(b[:, 1])) < 1e-4
     assert np.max(np.abs(a[:, 0] - a[0, 0])) / abs(a[0, 0]) < 1e-6
+
+
+def test_deposit_variants_agree_and_autotune():
+    """64x64 grid (every deposit variant usable), 16 particles per cell,
+    sorted every step: three steps with each deposit variant pinned reach the
+    same fixed point (total energy 1e-6, field energy 1e-4, positions 1e-5 of
+    the box), and the autotuner settles on a usable variant."""
+    lib = gpu()
+    p64 = make_params(code=CODE_P3, nx=64, ny=64, nz=1, Lx=8 * np.pi, Ly=8 * np.pi, Lz=1.0, dt=0.05,
+                      spline_order=3, orbit_quad_order=4, nyquist_projection=1, nonlinear_rtol=1e-11,
+                      nonlinear_atol=1e-13, nonlinear_max_iter=80, anderson_damping=0.5, local_rtol=1e-12,
+                      local_atol=1e-14, local_max_iter=64)
+    p64.sort_interval = 1
+    pm = with_mixed_precision(p64)
+    ppc = 16
+    n = p64.nx * p64.ny * ppc
+    x, v, q, m = lib.load_particles(p64, make_load(LOAD_BIMAXWELL, ppc, seed=8, vth=(0.05, 0.2, 0.2)), n)
+    st = State(n, p64.nx * p64.ny)
+    st.x[:], st.v[:], st.q[:], st.m[:] = x, v, q, m
+    ctx = lib.Context(pm)
+    res = {}
+    for variant in (-1, 0, 1, 2, 3):
+        sim = lib.Simulation(ctx, n)
+        sim.upload(st.copy())
+        sim.init_fields(1.0)
+        sim.set_deposit_variant(variant)
+        rows = [sim.diagnostics(1)]
+        for _ in range(3):
+            assert sim.step().converged
+            rows.append(sim.diagnostics(1))
+        out = State(n, st.N)
+        sim.download(out)
+        res[variant] = (np.array([[r.total_energy, r.field_energy] for r in rows]), out.x.reshape(3, n).copy())
+        if variant == -1:
+            chosen = sim.deposit_variant()
+        sim.close()
+    ctx.close()
+    assert chosen[0] in (0, 1, 2, 3), chosen
+    e0, x0 = res[0]
+    assert np.max(np.abs(e0[:, 0] - e0[0, 0])) / e0[0, 0] < 1e-6
+    for variant in (-1, 1, 2, 3):
+        e, xv = res[variant]
+        assert np.max(np.abs(e[:, 0] - e0[:, 0]) / e0[:, 0]) < 1e-6, variant
+        assert np.max(np.abs(e[:, 1] - e0[:, 1]) / e0[:, 1]) < 1e-4, variant
+        assert np.max(periodic_dist(xv[0], x0[0], p64.Lx)) < 1e-5 * p64.Lx, variant
+        assert np.max(periodic_dist(xv[1], x0[1], p64.Ly)) < 1e-5 * p64.Ly, variant

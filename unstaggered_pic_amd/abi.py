@@ -115,13 +115,16 @@ def make_params(**kw):
 
 # Tolerance floors of the mixed-precision mode: an FP32 orbit map resolves
 # the orbit-averaged velocity to ~1e-7 relative, so the inner fixed-point
-# test needs a relative tolerance of about 1e-6 to be reachable.  The mesh
+# test needs a relative tolerance of about 1e-6 to be reachable; its noise is
+# set by the size of the momentum terms (~0.2 c here), not by the particle's
+# own speed, so a slow particle needs an absolute floor of 1e-7 c (with 1e-9
+# the inner solve of a nearly stopped particle never converges).  The mesh
 # current of an FP32 map carries ~2.5e-7 relative rms noise; an outer
 # tolerance within a few times of that makes the Anderson iteration chase
 # the noise (8-9 map evaluations per step on the 256^2 Weibel deck against 5
 # at 3e-6, with the same energy conservation and field-energy history to
 # 1e-6 of the FP64 run), so the outer floor is 3e-6.
-MIXED_TOLERANCE_FLOORS = dict(local_rtol=1e-6, local_atol=1e-9, nonlinear_rtol=3e-6, nonlinear_atol=1e-9)
+MIXED_TOLERANCE_FLOORS = dict(local_rtol=1e-6, local_atol=1e-7, nonlinear_rtol=3e-6, nonlinear_atol=1e-9)
 
 
 def with_mixed_precision(p):

@@ -48,7 +48,22 @@ def build(verbose=False, defines=(), out=None):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, out)
+    if out == OUT:
+        build_driver()
     return out
+
+
+def build_driver():
+    """The deck-driven driver (csrc/upic_run.cpp), a host C++17 program on
+    the C ABI, next to the library (found through its rpath)."""
+    exe = os.path.join(HERE, "upic_run")
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", os.path.join(CSRC, "upic_run.cpp"),
+           "-L" + HERE, "-lupic", "-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{r.stderr}")
+    os.replace(exe + ".tmp", exe)
+    return exe
 
 
 if __name__ == "__main__":

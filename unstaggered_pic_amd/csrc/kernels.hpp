@@ -128,8 +128,15 @@ void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int sp
 // cell_end (NULL when the particles are not freshly cell-sorted): end index
 // of each cell's particle range after the counting sort (cell c holds
 // [cell_end[c-1], cell_end[c])); enables the cell-mapped deposit.
+// Deposit variants (workgroup tile shapes and thread mappings): 0 = particle
+// order, 8x8 tile, wavefront-reduced; 1 = cell-mapped 8x72 tile (one
+// workgroup per 64 cells); 2 = cell-mapped with 8 workgroups per 64 cells;
+// 3 = particle order, per-lane atomics.  1 and 2 need freshly cell-sorted
+// particles and ny % 64 == 0 (else 0 runs).
+int deposit_variants();
+bool deposit_variant_usable(const GridDev& g, int variant, bool cells_fresh);
 void launch_deposit_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& a,
-                          const int* cell_end);
+                          const int* cell_end, int variant);
 // NRf[8*id + ...] = FP32 copy of the FP64 node records
 void launch_pack_nodes_f32(hipStream_t s, int N, const double* gpm, const double* A_n, const double* A_end, float* NR);
 // also replaces q and m by their FP32 values (one charge and mass for all parts)

@@ -366,6 +366,17 @@ __device__ __forceinline__ void tilef_add(const MGrid& g, const DepTileF& T, dou
     }
 }
 
+// one component of tilef_add
+__device__ __forceinline__ void tilef_add1(const MGrid& g, const DepTileF& T, double* __restrict__ J, int ia, int ib,
+                                           int comp, float v) {
+    int dx = ia - T.ox;
+    if (dx < 0) dx += g.nx;
+    int dy = ib - T.oy;
+    if (dy < 0) dy += g.ny;
+    if (dx < kTileM && dy < kTileM) atomicAdd(T.t + 3 * (dx * kTileM + dy) + comp, v);
+    else unsafeAtomicAdd(&J[(size_t)comp * g.N + ia * g.ny + ib], (double)v);
+}
+
 // Orbit current deposit of one particle (weights per stencil window kept in
 // registers, flushed when the window moves).
 template <int R>
@@ -927,6 +938,130 @@ __global__ __launch_bounds__(256) void k_deposit_cells_mixed(MGrid g, QuadF qd, 
                                a.part + (size_t)blockIdx.y * gridDim.x * kPartSlots);
 }
 
+// Particle-order deposit with a wavefront reduction.  With cell-sorted
+// particles the 64 lanes of a wave almost always share one stencil window
+// (same start cell, no knot crossing), so each lane writes its 16 node
+// weights W and its scaled velocity qs v to LDS and lane j < 48 sums column
+// (node j / 3, component j % 3) over the 64 lanes: one LDS atomic per node
+// and component per wave instead of one per lane, the per-lane float
+// atomics being the bottleneck of the plain tile deposit.  Lanes whose
+// window differs from the wave's leading lane, and the earlier windows of
+// orbits that cross a knot, deposit with per-lane atomics.
+template <int R>
+__global__ __launch_bounds__(256) void k_deposit_wave_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
+    constexpr int NW = (R + 1) * (R + 1);
+    constexpr int ROW = NW + 4;  // W (NW) | qs v (3) | pad
+    __shared__ float tile_mem[kTileM * kTileM * 3];
+    __shared__ float red[4][64 * ROW];
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const DepTileF T = tilef_begin(g, tile_mem, a.cell, a.np, (int64_t)blockIdx.x * blockDim.x);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float W[R + 1][R + 1];
+#pragma unroll
+    for (int i = 0; i <= R; ++i)
+#pragma unroll
+        for (int j = 0; j <= R; ++j) W[i][j] = 0.0f;
+    float sv[3] = {0.0f, 0.0f, 0.0f};
+    int key = -1, gx = 0, gy = 0, nseg = 0;
+    const bool valid = p < a.np && a.reason[p] == 0;
+    if (valid) {
+        const float h = (float)a.dt;
+        const int c0 = a.cell[p];
+        const int ci = c0 / g.ny, cj = c0 - (c0 / g.ny) * g.ny;
+        float vb[3];
+        ld3f(a.vdep, a.np, p, vb);
+        const float dux = h * vb[0] * g.ihx, duy = h * vb[1] * g.ihy;
+        const float qs = a.q[p] * a.inv_dV;
+        sv[0] = qs * vb[0]; sv[1] = qs * vb[1]; sv[2] = qs * vb[2];
+        const float t0x = a.off[p], t0y = a.off[a.np + p];
+        int bx = 0, by = 0;
+        bool have = false;
+        BreakF bg;
+        breakf_init(bg, R, t0x, t0y, split ? dux : 0.0f, split ? duy : 0.0f);
+        float sa = 0.0f;
+        for (;;) {
+            const float sb = breakf_next(bg);
+            const float len = sb - sa;
+            if (len > 0.0f) {
+                for (int iq = 0; iq < qd.n; ++iq) {
+                    const float s_ = sa + len * qd.s[iq];
+                    const float w = len * qd.w[iq];
+                    AxF<R> sx, sy;
+                    axis_f<R>(t0x + s_ * dux, g.ihx, sx);
+                    axis_f<R>(t0y + s_ * duy, g.ihy, sy);
+                    if (have && (sx.fl != bx || sy.fl != by)) {
+                        // the window moved: the finished window goes out with per-lane atomics
+                        const int ox = wrapn(ci + bx, g.nx), oy = wrapn(cj + by, g.ny);
+#pragma unroll
+                        for (int aa = 0; aa <= R; ++aa) {
+                            const int ia = wrap1n(ox + aa, g.nx);
+#pragma unroll
+                            for (int bb = 0; bb <= R; ++bb) {
+                                const int ib = wrap1n(oy + bb, g.ny);
+                                tilef_add(g, T, a.J_out, ia, ib, W[aa][bb] * sv[0], W[aa][bb] * sv[1], W[aa][bb] * sv[2]);
+                                W[aa][bb] = 0.0f;
+                            }
+                        }
+                    }
+                    bx = sx.fl;
+                    by = sy.fl;
+                    have = true;
+#pragma unroll
+                    for (int aa = 0; aa <= R; ++aa) {
+                        const float wa = w * sx.w[aa];
+#pragma unroll
+                        for (int bb = 0; bb <= R; ++bb) W[aa][bb] += wa * sy.w[bb];
+                    }
+                }
+            }
+            ++nseg;
+            sa = sb;
+            if (sb >= 1.0f) break;
+        }
+        gx = wrapn(ci + bx, g.nx);
+        gy = wrapn(cj + by, g.ny);
+        key = gx * g.ny + gy;
+    }
+    // the wave's leading window: that of its first valid lane
+    const unsigned long long vm = __ballot(key >= 0);
+    const int lead = vm ? __ffsll((long long)vm) - 1 : 0;
+    const int lkey = __shfl(key, lead, 64);
+    const bool match = key >= 0 && key == lkey;
+    if (key >= 0 && !match) {
+#pragma unroll
+        for (int aa = 0; aa <= R; ++aa) {
+            const int ia = wrap1n(gx + aa, g.nx);
+#pragma unroll
+            for (int bb = 0; bb <= R; ++bb) {
+                const int ib = wrap1n(gy + bb, g.ny);
+                tilef_add(g, T, a.J_out, ia, ib, W[aa][bb] * sv[0], W[aa][bb] * sv[1], W[aa][bb] * sv[2]);
+            }
+        }
+    }
+    float* rw = red[wv] + lane * ROW;
+#pragma unroll
+    for (int aa = 0; aa <= R; ++aa)
+#pragma unroll
+        for (int bb = 0; bb <= R; ++bb) rw[aa * (R + 1) + bb] = match ? W[aa][bb] : 0.0f;
+    rw[NW] = sv[0];
+    rw[NW + 1] = sv[1];
+    rw[NW + 2] = sv[2];
+    const int lgx = __shfl(gx, lead, 64), lgy = __shfl(gy, lead, 64);
+    wave_sync_lds();
+    // column (node, component) sums over the 64 lanes, 3 (R+1)^2 columns
+    for (int col = lane; vm && col < 3 * NW; col += 64) {
+        const int node = col / 3, comp = col - 3 * (col / 3);
+        float acc = 0.0f;
+        for (int l = 0; l < 64; ++l) acc += red[wv][l * ROW + node] * red[wv][l * ROW + NW + comp];
+        if (acc != 0.0f) {
+            const int ia = wrap1n(lgx + node / (R + 1), g.nx), ib = wrap1n(lgy + node % (R + 1), g.ny);
+            tilef_add1(g, T, a.J_out, ia, ib, comp, acc);
+        }
+    }
+    tilef_end(g, T, a.J_out);
+    block_partials<0, 0>(nullptr, nullptr, nseg, a.part);
+}
+
 // FP32 node records from the FP64 fields
 __global__ void k_pack_nodes_f32(int N, const double* __restrict__ gpm, const double* __restrict__ A0,
                                  const double* __restrict__ A1, float4* __restrict__ NR) {
@@ -1059,18 +1194,38 @@ void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int sp
     launch_finish_partials(s, a.part, mblk(a.np), 2, 2, a.red, a.redmax, a.layout ? a.stats : nullptr);
 }
 
+int deposit_variants() { return 4; }
+bool deposit_variant_usable(const GridDev& g, int variant, bool cells_fresh) {
+    if (variant == 1 || variant == 2) return cells_fresh && g.ny % 64 == 0 && g.nx >= kCTR;
+    return true;
+}
+
 void launch_deposit_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& a,
-                          const int* cell_end) {
+                          const int* cell_end, int variant) {
     if (a.np <= 0 || a.layout || !a.J_out) return;
+    if (!deposit_variant_usable(g, variant, cell_end != nullptr)) variant = 0;
     MGrid mg;
     mg.nx = g.nx; mg.ny = g.ny; mg.N = g.N; mg.order = g.order;
     mg.hx = (float)g.h[0]; mg.hy = (float)g.h[1];
     mg.ihx = (float)(1.0 / g.h[0]); mg.ihy = (float)(1.0 / g.h[1]);
-    if (cell_end && g.ny % 64 == 0 && g.nx >= kCTR) {
+    if (variant == 0) {
+        const dim3 gr(mblk(a.np)), bl(256);
+        switch (g.order) {
+            case 1: hipLaunchKernelGGL(k_deposit_wave_mixed<1>, gr, bl, 0, s, mg, q, split, a); break;
+            case 2: hipLaunchKernelGGL(k_deposit_wave_mixed<2>, gr, bl, 0, s, mg, q, split, a); break;
+            case 3: hipLaunchKernelGGL(k_deposit_wave_mixed<3>, gr, bl, 0, s, mg, q, split, a); break;
+            case 4: hipLaunchKernelGGL(k_deposit_wave_mixed<4>, gr, bl, 0, s, mg, q, split, a); break;
+            default: break;
+        }
+        launch_finish_partials(s, a.part, mblk(a.np), 0, 0, nullptr, nullptr, a.stats);
+        return;
+    }
+    if (variant == 1 || variant == 2) {
         const int nb = g.N / 64;
-        // the partial slots of nb * kCellSlices workgroups must fit the
+        // the partial slots of nb * slices workgroups must fit the
         // per-particle-block buffer (sized for ceil(np / 256) blocks)
-        const int sl = (int)std::max<int64_t>(1, std::min<int64_t>(kCellSlices, (int64_t)mblk(a.np) / std::max(nb, 1)));
+        const int want = variant == 1 ? 1 : kCellSlices;
+        const int sl = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)mblk(a.np) / std::max(nb, 1)));
         const dim3 gr(nb, sl), bl(64 * kCellWaves);
         switch (g.order) {
             case 1: hipLaunchKernelGGL(k_deposit_cells_mixed<1>, gr, bl, 0, s, mg, q, split, a, cell_end); break;

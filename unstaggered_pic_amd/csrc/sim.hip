@@ -75,6 +75,21 @@ struct upic_sim {
     bool warm_valid = false;
     float* vdep = nullptr;  // converged vbar of the last sweep (3*np)
     bool cells_fresh = false;  // particles cell-sorted since their last move (offsets = cell ends)
+    // deposit variant autotuner (mixed precision).  While a particle order
+    // (index 1: freshly cell-sorted, 0: not) has no choice yet, each map
+    // evaluation runs the next usable variant that has fewer than kTuneReps
+    // timings, between two events on the stream (one variant per evaluation,
+    // so tuning adds no work); then the fastest minimum wins.  forced >= 0
+    // pins a variant.
+    struct DepositTuner {
+        static constexpr int kTuneReps = 2;
+        int forced = -1;
+        int chosen[2] = {-1, -1};
+        int count[2][4] = {};
+        float best_ms[2][4] = {};
+        int pending = -1, pending_order = 0;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+    } tuner;
 };
 
 // Mixed precision: an FP32 map resolves each node current only to a fraction
@@ -102,6 +117,45 @@ void sim_free(upic_sim* s) {
     for (float** p : fl) dfree(*p);
     dfree(s->ms.cell); dfree(s->mo.cell); dfree(s->itmp);
     dfree(s->mlp_P); dfree(s->mlp_blob); dfree(s->train_buf); dfree(s->part); dfree(s->warm); dfree(s->vdep);
+    if (s->tuner.e0) (void)hipEventDestroy(s->tuner.e0);
+    if (s->tuner.e1) (void)hipEventDestroy(s->tuner.e1);
+    s->tuner.e0 = s->tuner.e1 = nullptr;
+}
+
+// The deposit variant of this evaluation (see upic_sim::DepositTuner).
+int deposit_pick(upic_sim* s) {
+    auto& t = s->tuner;
+    const int order = s->cells_fresh ? 1 : 0;
+    t.pending = -1;
+    if (t.forced >= 0) return t.forced;
+    if (t.chosen[order] >= 0) return t.chosen[order];
+    const int nv = deposit_variants();
+    for (int rep = 0; rep < upic_sim::DepositTuner::kTuneReps; ++rep)
+        for (int v = 0; v < nv; ++v)
+            if (deposit_variant_usable(s->c->g, v, order == 1) && t.count[order][v] == rep) {
+                t.pending = v;
+                t.pending_order = order;
+                return v;
+            }
+    int best = 0;
+    for (int v = 1; v < nv; ++v)
+        if (t.count[order][v] > 0 && (t.count[order][best] == 0 || t.best_ms[order][v] < t.best_ms[order][best])) best = v;
+    t.chosen[order] = best;
+    return best;
+}
+
+// Reads the timing of the pending tuning launch (the stream has been
+// synchronised since).
+void deposit_record(upic_sim* s) {
+    auto& t = s->tuner;
+    if (t.pending < 0) return;
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, t.e0, t.e1));
+    float& b = t.best_ms[t.pending_order][t.pending];
+    int& n = t.count[t.pending_order][t.pending];
+    b = n == 0 ? ms : std::min(b, ms);
+    n += 1;
+    t.pending = -1;
 }
 
 // FP64 particle arrays -> mixed state
@@ -406,9 +460,19 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
         s->warm_valid = false;  // until this evaluation has completed
         a.vdep = s->vdep;
         timed(c, "p3_map_mixed", [&] { launch_p3_map_mixed(c->stream, c->g, c->qf, c->opt.split, a); });
+        const int variant = deposit_pick(s);
+        if (s->tuner.pending >= 0) {
+            if (!s->tuner.e0) {
+                HIPCHK(hipEventCreate(&s->tuner.e0));
+                HIPCHK(hipEventCreate(&s->tuner.e1));
+            }
+            HIPCHK(hipEventRecord(s->tuner.e0, c->stream));
+        }
         timed(c, "deposit_mixed", [&] {
-            launch_deposit_mixed(c->stream, c->g, c->qf, c->opt.split, a, s->cells_fresh ? s->offsets : nullptr);
+            launch_deposit_mixed(c->stream, c->g, c->qf, c->opt.split, a, s->cells_fresh ? s->offsets : nullptr,
+                                 variant);
         });
+        if (s->tuner.pending >= 0) HIPCHK(hipEventRecord(s->tuner.e1, c->stream));
     } else {
     P3Args a{};
     a.np = s->np; a.dt = p.dt; a.kappa = p.c; a.rtol = p.local_rtol; a.atol = p.local_atol;
@@ -427,6 +491,7 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     launch_project_nyquist(c->stream, c->sp, Jmap, 3);
     HIPCHK(hipGetLastError());
     read_red(c, 4);
+    if (s->mixed) deposit_record(s);
     if (s->mixed && s->guess_mode == 2 && s->mlp_ready && !use_layout) s->warm_valid = true;
     MapOut o;
     o.iter_sum = c->hred[0];
@@ -1467,6 +1532,20 @@ upic_status upic_sim_set_guess_mode(upic_sim* s, int32_t mode) {
     }
     if (mode == 2 && !s->mlp_ready) { s->c->err = "no network installed"; return UPIC_EINVAL; }
     s->guess_mode = mode;
+    return UPIC_OK;
+}
+
+upic_status upic_sim_set_deposit_variant(upic_sim* s, int32_t variant) {
+    if (!s || variant < -1 || variant >= deposit_variants()) return UPIC_EINVAL;
+    s->tuner.forced = variant;
+    return UPIC_OK;
+}
+
+upic_status upic_sim_deposit_variant(upic_sim* s, int32_t* sorted_choice, int32_t* unsorted_choice) {
+    if (!s) return UPIC_EINVAL;
+    const auto& t = s->tuner;
+    if (sorted_choice) *sorted_choice = t.forced >= 0 ? t.forced : t.chosen[1];
+    if (unsorted_choice) *unsorted_choice = t.forced >= 0 ? t.forced : t.chosen[0];
     return UPIC_OK;
 }
 

@@ -1,0 +1,594 @@
+// upic_run.cpp -- deck-driven driver of the hot path: the replacement for the
+// reference codes' main() time loops on top of the C ABI (include/upic.h).
+//
+//   upic_run p1 DECK      (Project1/ec_pic_serial_demo.cpp main, 1776-1892)
+//   upic_run p3 DECK      (Project3/ec_pic_nyquist_projected.cpp main, 8320-9058)
+//
+// Decks are `key = value` lines with `#` comments, parsed with the
+// reference's key names and aliases (P1 read_config 320-367, P3 read_config
+// 1178-1435); keys this driver does not use are reported once and ignored,
+// as P1 does.  Outputs, with the reference's file names and column schemas:
+//   <prefix>_diagnostics.csv    (P1 1811-1842; P3 8444-8460 and 8561-8624)
+//   <prefix>_config_echo.txt    (P1 write_config_echo 1731-1770)
+//   <prefix>_particle_sample.csv (P1 write_particle_sample 1716-1729)
+// Columns whose quantity this build does not compute (P3 linear-theory
+// references, selective-refinement level counts, convergence-rate estimates)
+// are written as nan.  The step-diagnostics files (R_A and work residuals)
+// are not produced.
+//
+// Initial conditions: P1 two_stream and landau_* are the reference's
+// deterministic loads (lattice / quiet start, P1:1151-1304) computed here on
+// the host, with the field tail (rho, J, phi, P, v_prev) on the device
+// (upic_sim_init_fields).  P3 decks use the seeded synthetic loads of
+// upic_load_particles (test_case = bimaxwell | beams | landau); the
+// reference's P3 eigenmode initialisers are not part of this build and are
+// rejected.
+//
+// Exit codes as the reference: 0 ok, 1 error, 2 usage.
+#include "../../include/upic.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <limits>
+#include <map>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+using Real = double;
+constexpr Real kPi = 3.14159265358979323846;
+const Real kNaN = std::numeric_limits<Real>::quiet_NaN();
+
+std::string trim(const std::string& s) {
+    const auto a = s.find_first_not_of(" \t\r\n");
+    if (a == std::string::npos) return "";
+    const auto b = s.find_last_not_of(" \t\r\n");
+    return s.substr(a, b - a + 1);
+}
+std::string lower(std::string s) {
+    for (auto& ch : s) ch = (char)std::tolower((unsigned char)ch);
+    return s;
+}
+bool parse_bool(const std::string& v) {
+    const std::string s = lower(trim(v));
+    return s == "1" || s == "true" || s == "yes" || s == "on";
+}
+
+// key -> value of a deck (later lines win, as in the reference parsers)
+std::map<std::string, std::string> read_deck(const std::string& path) {
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("could not open input file: " + path);
+    std::map<std::string, std::string> kv;
+    std::string line;
+    while (std::getline(in, line)) {
+        const auto sharp = line.find('#');
+        if (sharp != std::string::npos) line = line.substr(0, sharp);
+        const auto eq = line.find('=');
+        if (eq == std::string::npos) continue;
+        const std::string key = trim(line.substr(0, eq));
+        if (!key.empty()) kv[key] = trim(line.substr(eq + 1));
+    }
+    return kv;
+}
+
+struct Deck {
+    std::map<std::string, std::string> kv;
+    std::set<std::string> used;
+    bool has(std::initializer_list<const char*> keys) const {
+        for (const char* k : keys)
+            if (kv.count(k)) return true;
+        return false;
+    }
+    const std::string* find(std::initializer_list<const char*> keys) {
+        const std::string* r = nullptr;
+        for (const char* k : keys) {
+            auto it = kv.find(k);
+            if (it != kv.end()) { used.insert(k); r = &it->second; }
+        }
+        return r;
+    }
+    template <class T>
+    void get(std::initializer_list<const char*> keys, T& out) {
+        if (const std::string* v = find(keys)) {
+            std::istringstream ss(*v);
+            ss >> out;
+        }
+    }
+    void get_bool(std::initializer_list<const char*> keys, bool& out) {
+        if (const std::string* v = find(keys)) out = parse_bool(*v);
+    }
+    void get_str(std::initializer_list<const char*> keys, std::string& out) {
+        if (const std::string* v = find(keys)) out = *v;
+    }
+    void warn_unused() const {
+        for (const auto& kvp : kv)
+            if (!used.count(kvp.first)) std::cerr << "warning: unknown input key ignored: " << kvp.first << "\n";
+    }
+};
+
+void check(upic_status rc, upic_ctx* ctx, const char* what) {
+    if (rc != UPIC_OK) {
+        const char* msg = upic_last_error(ctx);
+        throw std::runtime_error(std::string(what) + ": " + (msg && *msg ? msg : "failed"));
+    }
+}
+
+double wall_since(const std::chrono::steady_clock::time_point& t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// wrap1 (P1:160-164): into [-L/2, L/2)
+Real wrap1(Real x, Real L) {
+    Real y = std::fmod(x + 0.5 * L, L);
+    if (y < 0.0) y += L;
+    return y - 0.5 * L;
+}
+
+std::vector<Real> cell_centered(int n, Real L) {  // cell_centered_grid (P1:1093-1101), centred
+    std::vector<Real> x((size_t)n);
+    const Real dx = L / (Real)n;
+    for (int i = 0; i < n; ++i) x[(size_t)i] = ((Real)i + 0.5) * dx - 0.5 * L;
+    return x;
+}
+
+// Inverse standard-normal CDF by Acklam's rational approximation (the
+// published coefficients; P1:1103-1149 uses the same method).
+Real normal_quantile(Real p) {
+    if (!(p > 0.0 && p < 1.0)) throw std::runtime_error("normal_quantile requires 0<p<1");
+    static const Real a[] = {-3.969683028665376e+01, 2.209460984245205e+02, -2.759285104469687e+02,
+                             1.383577518672690e+02, -3.066479806614716e+01, 2.506628277459239e+00};
+    static const Real b[] = {-5.447609879822406e+01, 1.615858368580409e+02, -1.556989798598866e+02,
+                             6.680131188771972e+01, -1.328068155288572e+01};
+    static const Real c[] = {-7.784894002430293e-03, -3.223964580411365e-01, -2.400758277161838e+00,
+                             -2.549732539343734e+00, 4.374664141464968e+00, 2.938163982698783e+00};
+    static const Real d[] = {7.784695709041462e-03, 3.224671290700398e-01, 2.445134137142996e+00,
+                             3.754408661907416e+00};
+    const Real plow = 0.02425, phigh = 1.0 - plow;
+    if (p < plow) {
+        const Real q = std::sqrt(-2.0 * std::log(p));
+        return (((((c[0] * q + c[1]) * q + c[2]) * q + c[3]) * q + c[4]) * q + c[5]) /
+               ((((d[0] * q + d[1]) * q + d[2]) * q + d[3]) * q + 1.0);
+    }
+    if (p <= phigh) {
+        const Real q = p - 0.5, r = q * q;
+        return (((((a[0] * r + a[1]) * r + a[2]) * r + a[3]) * r + a[4]) * r + a[5]) * q /
+               (((((b[0] * r + b[1]) * r + b[2]) * r + b[3]) * r + b[4]) * r + 1.0);
+    }
+    const Real q = std::sqrt(-2.0 * std::log(1.0 - p));
+    return -(((((c[0] * q + c[1]) * q + c[2]) * q + c[3]) * q + c[4]) * q + c[5]) /
+           ((((d[0] * q + d[1]) * q + d[2]) * q + d[3]) * q + 1.0);
+}
+
+struct Particles {
+    int64_t np = 0;
+    std::vector<Real> x, v, q, m;  // SoA, component-major
+    void resize(int64_t n) {
+        np = n;
+        x.assign(3 * (size_t)n, 0.0);
+        v.assign(3 * (size_t)n, 0.0);
+        q.assign((size_t)n, 0.0);
+        m.assign((size_t)n, 0.0);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// P1
+// ---------------------------------------------------------------------------
+struct P1Config {  // defaults of P1 Config (233-303)
+    int nx = 4, ny = 4, nz = 4;
+    Real Lx = 2.0 * kPi, Ly = 2.0 * kPi, Lz = 2.0 * kPi;
+    int n_steps = 6;
+    Real dt = 0.00625;
+    std::string test_case = "two_stream";
+    int ppc = 4, spline_order = 2, orbit_quad_order = 16;
+    bool split = true, final_sweep = true, run_unit_tests = true;
+    Real picard_tol = 1.0e-11;
+    int picard_max_iter = 16;
+    Real c = 1.0, eps0 = 1.0, mu0 = 1.0, n0 = 1.0, v0 = 0.3, perturbation = 0.02, landau_alpha = -1.0;
+    Real thermal_velocity = 1.0;
+    int perturbation_mode = 1;
+    Real theory_omega = 0.0, theory_gamma = 0.0;
+    unsigned seed = 12345;
+    std::string output_prefix = "orbit_smoke";
+    int precision = UPIC_PREC_FP64, sort_interval = 1;
+};
+
+P1Config read_p1(Deck& d) {
+    P1Config c;
+    d.get({"nx"}, c.nx); d.get({"ny"}, c.ny); d.get({"nz"}, c.nz);
+    d.get({"Lx"}, c.Lx); d.get({"Ly"}, c.Ly); d.get({"Lz"}, c.Lz);
+    d.get({"n_steps"}, c.n_steps); d.get({"dt"}, c.dt);
+    if (const std::string* v = d.find({"test_case", "initial_condition", "example"})) c.test_case = lower(*v);
+    d.get({"particles_per_cell_pair", "particles_per_cell", "velocity_samples_per_cell"}, c.ppc);
+    d.get({"spline_order"}, c.spline_order); d.get({"orbit_quad_order"}, c.orbit_quad_order);
+    d.get_bool({"split_orbit_at_knots"}, c.split);
+    d.get({"picard_tol"}, c.picard_tol); d.get({"picard_max_iter"}, c.picard_max_iter);
+    d.get_bool({"final_consistency_sweep"}, c.final_sweep); d.get_bool({"run_unit_tests"}, c.run_unit_tests);
+    d.get({"c"}, c.c); d.get({"eps0"}, c.eps0); d.get({"mu0"}, c.mu0); d.get({"n0"}, c.n0);
+    d.get({"v0"}, c.v0); d.get({"perturbation"}, c.perturbation);
+    d.get({"landau_alpha", "landau_density_perturbation"}, c.landau_alpha);
+    d.get({"thermal_velocity", "vth"}, c.thermal_velocity);
+    d.get({"perturbation_mode"}, c.perturbation_mode);
+    d.get({"theory_omega", "landau_theory_omega"}, c.theory_omega);
+    d.get({"theory_gamma", "landau_theory_gamma"}, c.theory_gamma);
+    d.get({"seed"}, c.seed);
+    d.get_str({"output_prefix"}, c.output_prefix);
+    d.get({"sort_interval"}, c.sort_interval);
+    d.warn_unused();
+    return c;
+}
+
+bool is_landau(const std::string& tc) {
+    return tc == "landau" || tc == "landau_weak" || tc == "landau_strong" || tc == "weak_landau" ||
+           tc == "strong_landau";
+}
+
+// initialize_two_stream_3d_linear_x (P1:1151-1206) and
+// initialize_landau_damping_1d3d (P1:1208-1291): particle part
+Particles p1_particles(const P1Config& c) {
+    Particles P;
+    const Real L[3] = {c.Lx, c.Ly, c.Lz};
+    const Real kx = 2.0 * kPi * (Real)c.perturbation_mode / c.Lx;
+    const Real volume = c.Lx * c.Ly * c.Lz;
+    const std::string tc = lower(c.test_case);
+    if (tc == "two_stream" || tc == "two-stream" || tc == "twostream") {
+        const int nxp = c.nx * c.ppc;
+        const auto xl = cell_centered(nxp, c.Lx), yl = cell_centered(c.ny, c.Ly), zl = cell_centered(c.nz, c.Lz);
+        const int64_t nbase = (int64_t)nxp * c.ny * c.nz;
+        P.resize(2 * nbase);
+        int64_t b = 0;
+        for (Real y : yl)
+            for (Real z : zl)
+                for (Real x : xl) {
+                    const Real pos[3] = {x, y, z};
+                    const Real seed_v = c.perturbation * std::sin(kx * x);
+                    for (int d = 0; d < 3; ++d) {
+                        P.x[(size_t)d * P.np + b] = wrap1(pos[d], L[d]);
+                        P.x[(size_t)d * P.np + nbase + b] = wrap1(pos[d], L[d]);
+                    }
+                    P.v[b] = c.v0 + seed_v;
+                    P.v[nbase + b] = -c.v0 + seed_v;
+                    ++b;
+                }
+        const Real qm = -c.n0 * volume / (Real)P.np;
+        for (int64_t p = 0; p < P.np; ++p) { P.q[p] = qm; P.m[p] = std::abs(qm); }
+        return P;
+    }
+    if (is_landau(tc)) {
+        const int nv = c.ppc;
+        if (nv < 2) throw std::runtime_error("Landau examples need at least two velocity samples per cell");
+        const Real alpha = c.landau_alpha >= 0.0 ? c.landau_alpha : c.perturbation;
+        if (std::abs(alpha) >= 1.0)
+            throw std::runtime_error("Landau density amplitude must satisfy |alpha| < 1 so all macro weights stay positive");
+        const auto xl = cell_centered(c.nx, c.Lx), yl = cell_centered(c.ny, c.Ly), zl = cell_centered(c.nz, c.Lz);
+        std::vector<Real> vq((size_t)nv);
+        for (int j = 0; j < nv; ++j) vq[(size_t)j] = c.thermal_velocity * normal_quantile(((Real)j + 0.5) / (Real)nv);
+        P.resize((int64_t)c.nx * c.ny * c.nz * nv);
+        const Real q0 = -c.n0 * volume / (Real)P.np;
+        int64_t p = 0;
+        for (Real y : yl)
+            for (Real z : zl)
+                for (Real x : xl) {
+                    const Real wgt = 1.0 + alpha * std::cos(kx * x);
+                    const Real pos[3] = {x, y, z};
+                    for (int j = 0; j < nv; ++j) {
+                        for (int d = 0; d < 3; ++d) P.x[(size_t)d * P.np + p] = wrap1(pos[d], L[d]);
+                        P.v[p] = vq[(size_t)j];
+                        P.q[p] = q0 * wgt;
+                        P.m[p] = std::abs(P.q[p]);
+                        ++p;
+                    }
+                }
+        return P;
+    }
+    throw std::runtime_error("unknown test_case/initial_condition: " + c.test_case);
+}
+
+void write_p1_echo(const P1Config& c, int64_t np) {  // write_config_echo (P1:1731-1770)
+    std::ofstream out(c.output_prefix + "_config_echo.txt");
+    const Real alpha = c.landau_alpha >= 0.0 ? c.landau_alpha : c.perturbation;
+    out << "test_case=" << c.test_case << "\n";
+    out << "nx=" << c.nx << "\nny=" << c.ny << "\nnz=" << c.nz << "\n";
+    out << "Lx=" << std::setprecision(17) << c.Lx << "\n";
+    out << "Ly=" << std::setprecision(17) << c.Ly << "\n";
+    out << "Lz=" << std::setprecision(17) << c.Lz << "\n";
+    out << "n_steps=" << c.n_steps << "\n";
+    out << "dt=" << std::setprecision(17) << c.dt << "\n";
+    out << "particles=" << np << "\n";
+    out << "particles_per_cell_pair_or_velocity_samples=" << c.ppc << "\n";
+    out << "spline_order=" << c.spline_order << "\norbit_quad_order=" << c.orbit_quad_order << "\n";
+    out << "split_orbit_at_knots=" << (c.split ? "true" : "false") << "\n";
+    out << "picard_tol=" << c.picard_tol << "\npicard_max_iter=" << c.picard_max_iter << "\n";
+    out << "final_consistency_sweep=" << (c.final_sweep ? "true" : "false") << "\n";
+    out << "c=" << c.c << "\neps0=" << c.eps0 << "\nmu0=" << c.mu0 << "\nn0=" << c.n0 << "\n";
+    out << "v0=" << c.v0 << "\nperturbation=" << c.perturbation << "\nlandau_alpha=" << alpha << "\n";
+    out << "thermal_velocity=" << c.thermal_velocity << "\nperturbation_mode=" << c.perturbation_mode << "\n";
+    out << "theory_omega=" << c.theory_omega << "\ntheory_gamma=" << c.theory_gamma << "\n";
+    out << "dV=" << (c.Lx / c.nx) * (c.Ly / c.ny) * (c.Lz / c.nz) << "\n";
+}
+
+struct Run {  // device handles and host buffers of one simulation
+    upic_ctx* ctx = nullptr;
+    upic_sim* sim = nullptr;
+    int64_t np = 0;
+    int N = 0;
+    ~Run() {
+        if (sim) upic_sim_destroy(sim);
+        if (ctx) upic_ctx_destroy(ctx);
+    }
+};
+
+void write_particle_sample(const std::string& path, Run& r, size_t max_particles) {  // P1:1716-1729
+    std::vector<Real> x(3 * (size_t)r.np), v(3 * (size_t)r.np), q((size_t)r.np), m((size_t)r.np);
+    upic_host_state hs{};
+    hs.np = r.np;
+    hs.x = x.data(); hs.v = v.data(); hs.q = q.data(); hs.m = m.data();
+    check(upic_sim_download(r.sim, &hs), r.ctx, "download");
+    std::ofstream out(path);
+    out << "particle,x,y,z,vx,vy,vz,q,m\n";
+    const size_t np = (size_t)r.np, nw = std::min(max_particles, np);
+    for (size_t k = 0; k < nw; ++k) {
+        const size_t p = (nw == 1) ? 0 : (k * (np - 1) / (nw - 1));
+        out << p << "," << std::setprecision(17) << x[p] << "," << x[np + p] << "," << x[2 * np + p] << "," << v[p]
+            << "," << v[np + p] << "," << v[2 * np + p] << "," << q[p] << "," << m[p] << "\n";
+    }
+}
+
+int run_p1(const std::string& deck_path) {
+    Deck d;
+    d.kv = read_deck(deck_path);
+    const P1Config c = read_p1(d);
+    upic_params p{};
+    p.code = UPIC_CODE_P1;
+    p.nx = c.nx; p.ny = c.ny; p.nz = c.nz; p.Lx = c.Lx; p.Ly = c.Ly; p.Lz = c.Lz; p.dt = c.dt;
+    p.spline_order = c.spline_order; p.orbit_quad_order = c.orbit_quad_order;
+    p.split_orbit_at_knots = c.split ? 1 : 0;
+    p.c = c.c; p.eps0 = c.eps0; p.mu0 = c.mu0; p.sigma1 = 1.0; p.sigma2 = 1.0;
+    p.picard_tol = c.picard_tol; p.picard_max_iter = c.picard_max_iter;
+    p.final_consistency_sweep = c.final_sweep ? 1 : 0;
+    p.precision = c.precision; p.sort_interval = c.sort_interval;
+    Particles P = p1_particles(c);
+    Run r;
+    r.np = P.np;
+    r.N = c.nx * c.ny * c.nz;
+    check(upic_ctx_create(&p, &r.ctx), nullptr, "context");
+    check(upic_sim_create(r.ctx, P.np, &r.sim), r.ctx, "simulation");
+    upic_host_state hs{};
+    hs.np = P.np;
+    std::vector<Real> zeros(3 * (size_t)P.np, 0.0), fz(3 * (size_t)r.N, 0.0);
+    hs.x = P.x.data(); hs.v = P.v.data(); hs.q = P.q.data(); hs.m = P.m.data();
+    hs.v_prev = zeros.data(); hs.P = zeros.data(); hs.vbar_last = zeros.data();
+    hs.phi = fz.data(); hs.psi = fz.data(); hs.rho = fz.data(); hs.A = fz.data(); hs.U = fz.data(); hs.J = fz.data();
+    check(upic_sim_upload(r.sim, &hs), r.ctx, "upload");
+    check(upic_sim_init_fields(r.sim, c.n0), r.ctx, "initial fields");
+    write_p1_echo(c, P.np);
+
+    upic_diagnostics d0{};
+    check(upic_sim_diagnostics(r.sim, c.perturbation_mode, &d0), r.ctx, "diagnostics");
+    const Real theory_E0 = d0.E_mode_abs;
+    const bool theory = (std::abs(c.theory_gamma) > 0.0 || std::abs(c.theory_omega) > 0.0) && theory_E0 > 0.0;
+    std::ofstream diag(c.output_prefix + "_diagnostics.csv");
+    diag << "step,time,gauss_rms,gauss_max,gauge_rms,field_energy,kinetic_energy,total_energy,"
+         << "rho_mode,E_mode_re,E_mode_im,E_mode_abs,theory_E_abs,E_over_theory,mean_rho,"
+         << "picard_iterations,picard_current_error,picard_velocity_error,"
+         << "crossing_particles,avg_path_segments,max_path_segments\n";
+    auto row = [&](int step, Real t, int iters, Real ce, Real ve, const upic_path_stats& ps) {
+        upic_diagnostics g{};
+        check(upic_sim_diagnostics(r.sim, c.perturbation_mode, &g), r.ctx, "diagnostics");
+        Real th = kNaN, ratio = kNaN;
+        if (theory) {
+            th = theory_E0 * std::exp(c.theory_gamma * t);
+            ratio = g.E_mode_abs / (th + 1.0e-300);
+        }
+        diag << step << "," << std::setprecision(17) << t << "," << g.gauss_rms << "," << g.gauss_max << ","
+             << g.gauge_rms << "," << g.field_energy << "," << g.kinetic_energy << "," << g.total_energy << ","
+             << g.rho_mode << "," << g.E_mode_re << "," << g.E_mode_im << "," << g.E_mode_abs << "," << th << ","
+             << ratio << "," << g.mean_rho << "," << iters << "," << ce << "," << ve << ","
+             << ps.crossing_particles << ","
+             << (ps.n_particles ? (Real)ps.total_segments / (Real)ps.n_particles : 0.0) << "," << ps.max_segments
+             << "\n";
+        return g;
+    };
+    upic_path_stats ps0{};
+    ps0.max_segments = 1;
+    row(0, 0.0, 0, kNaN, kNaN, ps0);
+    for (int n = 0; n < c.n_steps; ++n) {
+        upic_step_info info{};
+        check(upic_step_p1(r.sim, &info), r.ctx, "step");
+        const upic_diagnostics g = row(n + 1, (n + 1) * c.dt, info.iterations, info.current_error,
+                                       info.velocity_error, info.path);
+        if ((n + 1) % std::max(1, c.n_steps / 5) == 0 || n == c.n_steps - 1)
+            std::cout << std::scientific << std::setprecision(3) << "step " << std::setw(5) << (n + 1) << "/"
+                      << c.n_steps << " Gauss=" << g.gauss_rms << " Picard=" << info.iterations << "\n";
+    }
+    write_particle_sample(c.output_prefix + "_particle_sample.csv", r, 20000);
+    upic_diagnostics fd{};
+    check(upic_sim_diagnostics(r.sim, c.perturbation_mode, &fd), r.ctx, "diagnostics");
+    std::cout << std::scientific << std::setprecision(6) << "Final Gauss RMS = " << fd.gauss_rms << "\n"
+              << "Final gauge RMS = " << fd.gauge_rms << "\n"
+              << "Final total energy = " << fd.total_energy << "\n";
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// P3
+// ---------------------------------------------------------------------------
+int run_p3(const std::string& deck_path) {
+    Deck d;
+    d.kv = read_deck(deck_path);
+    upic_params p{};
+    p.code = UPIC_CODE_P3;
+    // defaults of P3 Config (837-990) for the keys used here
+    p.nx = 16; p.ny = 16; p.nz = 1; p.Lx = 2.0 * kPi; p.Ly = 2.0 * kPi; p.Lz = 2.0 * kPi;
+    p.dt = 0.05; p.spline_order = 2; p.orbit_quad_order = 16; p.split_orbit_at_knots = 1;
+    p.nyquist_projection = 1; p.c = 1.0; p.eps0 = 1.0; p.mu0 = 1.0; p.sigma1 = 1.0; p.sigma2 = 1.0;
+    p.nonlinear_solver_picard = 0; p.nonlinear_max_iter = 32; p.nonlinear_rtol = 1e-12; p.nonlinear_atol = 1e-13;
+    p.anderson_memory = 4; p.anderson_start = 2; p.anderson_damping = 0.7; p.anderson_regularization = 1e-12;
+    p.anderson_growth_limit = 1.2; p.local_rtol = 1e-11; p.local_atol = 1e-13; p.local_max_iter = 24;
+    p.local_damping = 1.0; p.selective_particle_fallback = 0; p.selective_particle_max_substeps = 64;
+    p.precision = UPIC_PREC_FP64; p.sort_interval = 1; p.learned_guess = 0;
+    int n_steps = 10, ppc = 8, mode = 1;
+    std::string tc = "bimaxwell", prefix = "ec_pic", solver = "anderson_quasi_newton", prec = "fp64";
+    bool split = true, proj = true, sel = false;
+    Real rho0 = 1.0, v0 = 0.2, noise = 0.0, alpha = 0.01;
+    Real vth[3] = {0.05, 0.2, 0.2};
+    uint64_t seed = 12345;
+    d.get({"nx"}, p.nx); d.get({"ny"}, p.ny); d.get({"nz"}, p.nz);
+    d.get({"Lx"}, p.Lx); d.get({"Ly"}, p.Ly); d.get({"Lz"}, p.Lz);
+    d.get({"n_steps"}, n_steps); d.get({"dt"}, p.dt);
+    if (const std::string* v = d.find({"test_case", "initial_condition", "example"})) tc = lower(*v);
+    d.get({"particles_per_cell_pair", "particles_per_cell", "velocity_samples_per_cell"}, ppc);
+    d.get({"spline_order"}, p.spline_order); d.get({"orbit_quad_order"}, p.orbit_quad_order);
+    d.get_bool({"split_orbit_at_knots"}, split);
+    d.get_str({"nonlinear_solver"}, solver);
+    d.get({"nonlinear_rtol", "nonlinear_tol", "picard_tol"}, p.nonlinear_rtol);
+    d.get({"nonlinear_atol"}, p.nonlinear_atol);
+    d.get_bool({"nyquist_projection"}, proj);
+    d.get({"nonlinear_max_iter", "picard_max_iter"}, p.nonlinear_max_iter);
+    d.get({"anderson_memory"}, p.anderson_memory); d.get({"anderson_start"}, p.anderson_start);
+    d.get({"anderson_damping"}, p.anderson_damping);
+    d.get({"anderson_regularization"}, p.anderson_regularization);
+    d.get({"anderson_growth_limit"}, p.anderson_growth_limit);
+    d.get_bool({"selective_particle_fallback"}, sel);
+    d.get({"selective_particle_max_substeps"}, p.selective_particle_max_substeps);
+    d.get({"particle_local_rtol", "uniform_local_rtol"}, p.local_rtol);
+    d.get({"particle_local_atol", "uniform_local_atol"}, p.local_atol);
+    d.get({"particle_local_max_iter", "uniform_local_max_iter"}, p.local_max_iter);
+    d.get({"particle_local_damping", "uniform_local_damping"}, p.local_damping);
+    d.get({"kappa"}, p.c); d.get({"sigma1"}, p.sigma1); d.get({"sigma2"}, p.sigma2);
+    d.get({"rho0", "n0"}, rho0); d.get({"v0"}, v0); d.get({"perturbation_mode"}, mode);
+    d.get({"perturbation", "landau_alpha"}, alpha);
+    d.get({"vth_x"}, vth[0]); d.get({"vth_y"}, vth[1]); d.get({"vth_z"}, vth[2]);
+    if (const std::string* v = d.find({"vth", "thermal_velocity"})) vth[0] = vth[1] = vth[2] = std::atof(v->c_str());
+    d.get({"position_noise"}, noise); d.get({"seed"}, seed);
+    d.get_str({"output_prefix"}, prefix);
+    d.get_str({"precision"}, prec);
+    d.get({"learned_guess"}, p.learned_guess);
+    d.get({"sort_interval"}, p.sort_interval);
+    d.warn_unused();
+    p.split_orbit_at_knots = split ? 1 : 0;
+    p.nyquist_projection = proj ? 1 : 0;
+    p.selective_particle_fallback = sel ? 1 : 0;
+    p.nonlinear_solver_picard = (lower(solver) == "picard") ? 1 : 0;
+    if (lower(prec) == "mixed") {
+        p.precision = UPIC_PREC_MIXED;
+        // FP32 tolerance floors of the mixed mode
+        p.local_rtol = std::max(p.local_rtol, 1e-6); p.local_atol = std::max(p.local_atol, 1e-7);
+        p.nonlinear_rtol = std::max(p.nonlinear_rtol, 3e-6); p.nonlinear_atol = std::max(p.nonlinear_atol, 1e-9);
+    }
+    const int learned = p.learned_guess;
+    if (learned == UPIC_GUESS_LEARNED) p.learned_guess = UPIC_GUESS_PREDICTOR;  // until the network is trained
+    upic_load_spec ls{};
+    ls.ppc = ppc; ls.seed = seed; ls.n0 = rho0; ls.alpha = alpha; ls.mode = mode;
+    ls.vth[0] = vth[0]; ls.vth[1] = vth[1]; ls.vth[2] = vth[2]; ls.v0 = v0; ls.noise = noise;
+    if (tc == "bimaxwell" || tc == "weibel_bimaxwell" || tc == "maxwellian") ls.kind = UPIC_LOAD_BIMAXWELL;
+    else if (tc == "beams" || tc == "counter_streaming_beams") ls.kind = UPIC_LOAD_BEAMS;
+    else if (tc == "landau") ls.kind = UPIC_LOAD_LANDAU;
+    else throw std::runtime_error("initial condition not available in this build: " + tc +
+                                  " (supported: bimaxwell, beams, landau)");
+    Run r;
+    r.N = p.nx * p.ny * p.nz;
+    r.np = (int64_t)r.N * ppc;
+    Particles P;
+    P.resize(r.np);
+    upic_host_state hs{};
+    hs.np = r.np;
+    hs.x = P.x.data(); hs.v = P.v.data(); hs.q = P.q.data(); hs.m = P.m.data();
+    check(upic_load_particles(&p, &ls, &hs), nullptr, "particle load");
+    check(upic_ctx_create(&p, &r.ctx), nullptr, "context");
+    check(upic_sim_create(r.ctx, r.np, &r.sim), r.ctx, "simulation");
+    std::vector<Real> zeros(3 * (size_t)r.np, 0.0), fz(3 * (size_t)r.N, 0.0);
+    hs.v_prev = zeros.data(); hs.P = zeros.data(); hs.vbar_last = zeros.data();
+    hs.phi = fz.data(); hs.psi = fz.data(); hs.rho = fz.data(); hs.A = fz.data(); hs.U = fz.data(); hs.J = fz.data();
+    check(upic_sim_upload(r.sim, &hs), r.ctx, "upload");
+    check(upic_sim_init_fields(r.sim, rho0), r.ctx, "initial fields");
+    if (learned == UPIC_GUESS_LEARNED) {
+        const int64_t stride = std::max<int64_t>(1, r.np / 16384);
+        check(upic_sim_collect_guess_data(r.sim, stride, ((r.np + stride - 1) / stride) * 20), r.ctx, "collect");
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    std::ofstream diag(prefix + "_diagnostics.csv");
+    diag << "step,time,gauss_rms,gauss_max,divE_rms,sigma1_rho_rms,"
+         << "gauss_normalization_rms,gauss_relative,"
+         << "gauge_rms,gauge_max,psi_over_kappa2_rms,divA_rms,"
+         << "gauge_normalization_rms,gauge_relative,"
+         << "field_energy,kinetic_energy,total_energy,"
+         << "rho_mode,E_mode_re,E_mode_im,E_mode_abs,Bz_y_mode_re,Bz_y_mode_im,Bz_y_mode_abs,"
+         << "theory_growth_rate,theory_mode_abs,mode_over_theory,mean_rho,"
+         << "nonlinear_iterations,nonlinear_converged,nonlinear_rms_residual,"
+         << "nonlinear_max_particle_residual,nonlinear_absolute_rms_residual,"
+         << "anderson_restarts,anderson_steps,picard_fallback_steps,convergence_method,"
+         << "observed_convergence_rate,estimated_map_contraction,backtracked_steps,minimum_update_alpha,"
+         << "local_particle_iterations_mean,local_particle_iterations_max,uniform_substeps_used,"
+         << "selective_fallback_used,direct_attempt_iterations,selective_refinement_restarts,"
+         << "selective_refined_particles,selective_total_particle_substeps,selective_work_ratio,"
+         << "selective_n1,selective_n2,selective_n4,selective_n8,selective_n16,selective_n32,selective_n64,"
+         << "crossing_particles,avg_path_segments,max_path_segments,dt,elapsed_wall_seconds\n";
+    auto row = [&](int step, Real t, const upic_step_info& in, Real step_dt) {
+        upic_diagnostics g{};
+        check(upic_sim_diagnostics(r.sim, mode, &g), r.ctx, "diagnostics");
+        const Real gn = g.divE_rms + g.rho_rms, an = g.psi_over_c2_rms + g.divA_rms;
+        const std::string method = in.iterations == 0 ? "none"
+                                   : (in.anderson_steps > 0 ? "anderson" : (in.picard_fallback_steps > 0 ? "picard" : "direct"));
+        diag << step << "," << std::setprecision(17) << t << "," << g.gauss_rms << "," << g.gauss_max << ","
+             << g.divE_rms << "," << g.rho_rms << "," << gn << "," << (gn > 0.0 ? g.gauss_rms / gn : 0.0) << ","
+             << g.gauge_rms << "," << g.gauge_max << "," << g.psi_over_c2_rms << "," << g.divA_rms << "," << an << ","
+             << (an > 0.0 ? g.gauge_rms / an : 0.0) << "," << g.field_energy << "," << g.kinetic_energy << ","
+             << g.total_energy << "," << g.rho_mode << "," << g.E_mode_re << "," << g.E_mode_im << ","
+             << g.E_mode_abs << "," << g.Bz_y_mode_re << "," << g.Bz_y_mode_im << "," << g.Bz_y_mode_abs << ","
+             << kNaN << "," << kNaN << "," << kNaN << "," << g.mean_rho << "," << in.iterations << ","
+             << in.converged << "," << in.final_rms_residual << "," << in.final_max_particle_residual << ","
+             << in.final_absolute_rms_residual << "," << in.anderson_restarts << "," << in.anderson_steps << ","
+             << in.picard_fallback_steps << "," << method << "," << kNaN << "," << in.estimated_map_contraction
+             << "," << in.backtracked_steps << "," << in.minimum_update_alpha << "," << in.local_iterations_mean
+             << "," << in.local_iterations_max << ",1,0,0,0,0,0,0";
+        for (int k = 0; k < 7; ++k) diag << "," << kNaN;
+        diag << "," << in.path.crossing_particles << ","
+             << (in.path.n_particles ? (Real)in.path.total_segments / (Real)in.path.n_particles : 0.0) << ","
+             << in.path.max_segments << "," << step_dt << "," << wall_since(t0) << "\n";
+        diag.flush();
+        return g;
+    };
+    upic_step_info none{};
+    none.path.max_segments = 1;
+    row(0, 0.0, none, 0.0);
+    for (int n = 0; n < n_steps; ++n) {
+        upic_step_info info{};
+        check(upic_step_p3(r.sim, &info), r.ctx, "step");
+        const upic_diagnostics g = row(n + 1, (n + 1) * p.dt, info, p.dt);
+        if (learned == UPIC_GUESS_LEARNED && n + 1 == 20) {
+            check(upic_sim_collect_guess_data(r.sim, 0, 0), r.ctx, "collect");
+            check(upic_sim_train_guess(r.sim, 15, 4096, 2e-3, 5, nullptr), r.ctx, "training");
+        }
+        if ((n + 1) % std::max(1, n_steps / 5) == 0 || n == n_steps - 1)
+            std::cout << std::scientific << std::setprecision(3) << "step " << std::setw(5) << (n + 1) << "/"
+                      << n_steps << " Gauss=" << g.gauss_rms << " TE=" << g.total_energy
+                      << " outer=" << info.iterations << " inner=" << info.local_iterations_mean << "\n";
+    }
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc != 3 || (std::string(argv[1]) != "p1" && std::string(argv[1]) != "p3")) {
+        std::cerr << "usage: " << argv[0] << " p1|p3 input_deck.txt\n";
+        return 2;
+    }
+    try {
+        return std::string(argv[1]) == "p1" ? run_p1(argv[2]) : run_p3(argv[2]);
+    } catch (const std::exception& e) {
+        std::cerr << "error: " << e.what() << "\n";
+        return 1;
+    }
+}

@@ -72,6 +72,8 @@ def lib():
             "upic_sim_guess_data": [vp, vp, C.c_int64, C.POINTER(C.c_int64)],
             "upic_sim_train_guess": [vp, C.c_int32, C.c_int32, C.c_double, C.c_uint64, dp],
             "upic_sim_set_guess_mode": [vp, C.c_int32],
+            "upic_sim_set_deposit_variant": [vp, C.c_int32],
+            "upic_sim_deposit_variant": [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
             "upic_sim_guess_params": [vp, vp], "upic_sim_set_guess_params": [vp, vp],
             "upic_mlp_forward": [vp, vp, vp, C.c_int64, vp],
             "upic_mlp_loss_grad": [vp, vp, vp, C.c_int64, vp, dp],
@@ -380,6 +382,17 @@ class Simulation:
 
     def set_guess_mode(self, mode):
         self.ctx._check(lib().upic_sim_set_guess_mode(self.h, int(mode)))
+
+    def set_deposit_variant(self, variant):
+        """Pin the mixed-precision deposit variant (0-3) or re-enable the
+        autotuner (-1)."""
+        self.ctx._check(lib().upic_sim_set_deposit_variant(self.h, int(variant)))
+
+    def deposit_variant(self):
+        """(sorted-order choice, unsorted-order choice); -1 while tuning."""
+        a, b = C.c_int32(-1), C.c_int32(-1)
+        self.ctx._check(lib().upic_sim_deposit_variant(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def guess_params(self):
         P = np.zeros(MLP_NPARAM, np.float32)
