@@ -18,6 +18,30 @@ constexpr int kMaxQuad = 32;
 // slots of the per-block partials of a particle sweep (particles.hpp)
 constexpr int kPartSlots = 9;
 
+// Claims slots of per-key counters for the active lanes of a wavefront:
+// lanes with equal keys are grouped by ballot, one atomicAdd per distinct
+// key reserves the group's slots, and each lane receives old value + its
+// rank among the group's lanes (lane order).  Particles stored in cell order
+// put one or two keys in a wavefront, so the counters of a cell see a few
+// atomics instead of one per particle.  Every lane of the wavefront must
+// call it (inactive lanes pass active = false and get 0).
+__device__ __forceinline__ int wave_claim(int* __restrict__ ctr, int key, bool active) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot(active);
+    int slot = 0;
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int lk = __shfl(key, leader, 64);
+        const unsigned long long grp = __ballot(active && key == lk) & pending;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&ctr[lk], __popcll(grp));
+        base = __shfl(base, leader, 64);
+        if ((grp >> lane) & 1ull) slot = base + __popcll(grp & ((1ull << lane) - 1ull));
+        pending &= ~grp;
+    }
+    return slot;
+}
+
 // Grid description passed by value to every kernel.
 struct GridDev {
     int nx, ny, nz, N;

@@ -320,15 +320,33 @@ void launch_spectral_continuity(hipStream_t s, const SpecArgs& sp, double dt, co
 }
 
 // project_nyquist (P3:2044-2061): per line, subtract the alternating mean.
-__global__ void k_project_axis(double* __restrict__ f, int N, int n, int stride, int lines) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= lines) return;
+// A workgroup takes kProjLines lines with kProjThreads threads per line;
+// each thread sums every kProjThreads-th element of its line and the line's
+// partial sums are added in a fixed order.  ROWS (stride 1): a line's
+// threads are adjacent and read adjacent elements; otherwise adjacent
+// threads take adjacent lines, which are adjacent in memory.
+constexpr int kProjLines = 16, kProjThreads = 16;
+template <bool ROWS>
+__global__ __launch_bounds__(256) void k_project_axis(double* __restrict__ f, int N, int n, int stride, int lines) {
+    __shared__ double part[kProjLines][kProjThreads + 1];
+    const int t = threadIdx.x;
+    const int li = ROWS ? t / kProjThreads : t % kProjLines;
+    const int ch = ROWS ? t % kProjThreads : t / kProjLines;
+    const int l = blockIdx.x * kProjLines + li;
     double* F = f + (int64_t)blockIdx.y * N;
-    const int base = (l / stride) * (n * stride) + (l % stride);
     double sum = 0.0;
-    for (int j = 0; j < n; ++j) sum += (j & 1 ? -1.0 : 1.0) * F[base + j * stride];
-    const double mean = sum / n;
-    for (int j = 0; j < n; ++j) F[base + j * stride] -= (j & 1 ? -mean : mean);
+    int base = 0;
+    if (l < lines) {
+        base = (l / stride) * (n * stride) + (l % stride);
+        for (int j = ch; j < n; j += kProjThreads) sum += (j & 1 ? -1.0 : 1.0) * F[base + j * stride];
+    }
+    part[li][ch] = sum;
+    __syncthreads();
+    double tot = 0.0;
+    for (int k = 0; k < kProjThreads; ++k) tot += part[li][k];
+    const double mean = tot / n;
+    if (l < lines)
+        for (int j = ch; j < n; j += kProjThreads) F[base + j * stride] -= (j & 1 ? -mean : mean);
 }
 void launch_project_nyquist(hipStream_t s, const SpecArgs& sp, double* f, int batch) {
     if (!sp.projection) return;
@@ -338,7 +356,11 @@ void launch_project_nyquist(hipStream_t s, const SpecArgs& sp, double* f, int ba
         const int n = dims[ax];
         if (n <= 1 || n % 2) continue;
         const int lines = sp.N / n;
-        hipLaunchKernelGGL(k_project_axis, dim3((lines + 127) / 128, batch), dim3(128), 0, s, f, sp.N, n, strides[ax], lines);
+        const dim3 gr((lines + kProjLines - 1) / kProjLines, batch), bl(kProjLines * kProjThreads);
+        if (strides[ax] == 1)
+            hipLaunchKernelGGL(k_project_axis<true>, gr, bl, 0, s, f, sp.N, n, strides[ax], lines);
+        else
+            hipLaunchKernelGGL(k_project_axis<false>, gr, bl, 0, s, f, sp.N, n, strides[ax], lines);
     }
 }
 
@@ -474,19 +496,22 @@ void launch_kinetic(hipStream_t s, int64_t np, const double* v, const double* m,
 // ---------------------------------------------------------------------------
 __global__ void k_cell_keys(GridDev g, int64_t np, const double* __restrict__ x, int* __restrict__ keys, int* __restrict__ counts) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    int c[3];
-    const int n[3] = {g.nx, g.ny, g.nz};
+    const bool active = p < np;
+    int key = 0;
+    if (active) {
+        int c[3];
+        const int n[3] = {g.nx, g.ny, g.nz};
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const double xw = wrap1(x[d * np + p], g.L[d]);
-        int i = (int)floor((xw + 0.5 * g.L[d]) / g.h[d]);
-        i = i < 0 ? 0 : (i >= n[d] ? n[d] - 1 : i);
-        c[d] = i;
+        for (int d = 0; d < 3; ++d) {
+            const double xw = wrap1(x[d * np + p], g.L[d]);
+            int i = (int)floor((xw + 0.5 * g.L[d]) / g.h[d]);
+            i = i < 0 ? 0 : (i >= n[d] ? n[d] - 1 : i);
+            c[d] = i;
+        }
+        key = (c[0] * g.ny + c[1]) * g.nz + c[2];
+        keys[p] = key;
     }
-    const int key = (c[0] * g.ny + c[1]) * g.nz + c[2];
-    keys[p] = key;
-    atomicAdd(&counts[key], 1);
+    (void)wave_claim(counts, key, active);
 }
 // single-workgroup exclusive scan over ncell counts
 __global__ __launch_bounds__(1024) void k_scan(int ncell, const int* __restrict__ counts, int* __restrict__ offsets) {
@@ -509,9 +534,10 @@ __global__ __launch_bounds__(1024) void k_scan(int ncell, const int* __restrict_
 }
 __global__ void k_scatter_perm(int64_t np, const int* __restrict__ keys, int* __restrict__ cursor, int* __restrict__ perm) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    const int pos = atomicAdd(&cursor[keys[p]], 1);
-    perm[pos] = (int)p;
+    const bool active = p < np;
+    const int key = active ? keys[p] : 0;
+    const int pos = wave_claim(cursor, key, active);
+    if (active) perm[pos] = (int)p;
 }
 __global__ void k_permute(int64_t np, int ncomp, const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ perm) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -144,10 +144,10 @@ void launch_mixed_compress(hipStream_t s, const GridDev& g, int64_t np, const do
                            const double* v, const double* vp, const double* vb, double* q, double* m,
                            const MixedState& ms);
 void launch_mixed_expand(hipStream_t s, const GridDev& g, int64_t np, const MixedState& ms, double* x, double* P,
-                         double* v, double* vp, double* vb);
-void launch_mixed_keys(hipStream_t s, int64_t np, const int* cell, int* keys, int* counts);
-void launch_permute_f32(hipStream_t s, int64_t np, int ncomp, const float* src, float* dst, const int* perm);
-void launch_permute_i32(hipStream_t s, int64_t np, const int* src, int* dst, const int* perm);
+                         double* v, double* vp, double* vb, double* q, double* m);
+void launch_mixed_counts(hipStream_t s, int64_t np, const int* cell, int* counts);
+void launch_mixed_permute(hipStream_t s, int64_t np, const int* perm, const MixedState& src, const MixedState& dst,
+                          const int64_t* id_src, int64_t* id_dst);
 
 // mlp.hip: the learned-guess network (layout in mlp.hpp) -------------------
 // Training rows are 15 floats: 12 features, then the 3 target components.
@@ -168,7 +168,11 @@ void launch_mlp_batch_idx(hipStream_t s, uint64_t seed, uint64_t step, int B, in
 // stats[2] = max(stats[2], max segments).  Any output may be NULL.
 void launch_finish_partials(hipStream_t s, const double* part, int64_t nblocks, int ns, int nm, double* sum,
                             double* mx, unsigned long long* stats);
-inline int64_t partial_blocks(int64_t np) { return (np + 255) / 256; }
+// slots of the per-block partials buffer of a particle sweep over np
+// particles: one per 256-particle workgroup plus kFoldBlocks for the first
+// folding stage
+constexpr int64_t kFoldBlocks = 256;
+inline int64_t partial_blocks(int64_t np) { return (np + 255) / 256 + kFoldBlocks; }
 
 // field_kernels.hip ---------------------------------------------------------
 struct FFTPlan {

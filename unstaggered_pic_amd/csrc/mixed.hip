@@ -1117,7 +1117,8 @@ __global__ void k_mixed_compress(GridDev g, int64_t np, const double* __restrict
 
 // mixed state -> FP64 particle arrays (x in [-L/2, L/2))
 __global__ void k_mixed_expand(GridDev g, int64_t np, MixedState s, double* __restrict__ x, double* __restrict__ P,
-                               double* __restrict__ v, double* __restrict__ vp, double* __restrict__ vb) {
+                               double* __restrict__ v, double* __restrict__ vp, double* __restrict__ vb,
+                               double* __restrict__ q, double* __restrict__ m) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= np) return;
     const int c = s.cell[p];
@@ -1132,24 +1133,39 @@ __global__ void k_mixed_expand(GridDev g, int64_t np, MixedState s, double* __re
         vp[d * np + p] = (double)s.vp[d * np + p];
         vb[d * np + p] = (double)s.vb[d * np + p];
     }
+    // the FP64 charge and mass were rounded to FP32 when the state was
+    // compressed, so the FP32 copies restore them exactly
+    q[p] = (double)s.q[p];
+    m[p] = (double)s.m[p];
 }
 
-// counting-sort keys straight from the cell index
-__global__ void k_mixed_keys(int64_t np, const int* __restrict__ cell, int* __restrict__ keys, int* __restrict__ counts) {
+// cell counts of the counting sort (the cell index is the key)
+__global__ void k_mixed_counts(int64_t np, const int* __restrict__ cell, int* __restrict__ counts) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    const int k = cell[p];
-    keys[p] = k;
-    atomicAdd(&counts[k], 1);
+    const bool active = p < np;
+    (void)wave_claim(counts, active ? cell[p] : 0, active);
 }
 
-template <class T>
-__global__ void k_permute_t(int64_t np, int ncomp, const T* __restrict__ src, T* __restrict__ dst,
-                            const int* __restrict__ perm) {
+// All per-particle arrays of the mixed state through one permutation:
+// dst[i] = src[perm[i]] for the cell index, the 3-vectors (offsets, P, v,
+// v_prev, vbar), q, m and the original particle index.
+__global__ void k_mixed_permute(int64_t np, const int* __restrict__ perm, MixedState src, MixedState dst,
+                                const int64_t* __restrict__ id_src, int64_t* __restrict__ id_dst) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const int64_t j = perm[i];
-    for (int c = 0; c < ncomp; ++c) dst[c * np + i] = src[c * np + j];
+    dst.cell[i] = src.cell[j];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        dst.off[c * np + i] = src.off[c * np + j];
+        dst.P[c * np + i] = src.P[c * np + j];
+        dst.v[c * np + i] = src.v[c * np + j];
+        dst.vp[c * np + i] = src.vp[c * np + j];
+        dst.vb[c * np + i] = src.vb[c * np + j];
+    }
+    dst.q[i] = src.q[j];
+    dst.m[i] = src.m[j];
+    id_dst[i] = id_src[j];
 }
 
 inline unsigned mblk(int64_t n) { return (unsigned)((n + 255) / 256); }
@@ -1260,20 +1276,18 @@ void launch_mixed_compress(hipStream_t s, const GridDev& g, int64_t np, const do
 }
 
 void launch_mixed_expand(hipStream_t s, const GridDev& g, int64_t np, const MixedState& ms, double* x, double* P,
-                         double* v, double* vp, double* vb) {
-    if (np > 0) hipLaunchKernelGGL(k_mixed_expand, dim3(mblk(np)), dim3(256), 0, s, g, np, ms, x, P, v, vp, vb);
+                         double* v, double* vp, double* vb, double* q, double* m) {
+    if (np > 0) hipLaunchKernelGGL(k_mixed_expand, dim3(mblk(np)), dim3(256), 0, s, g, np, ms, x, P, v, vp, vb, q, m);
 }
 
-void launch_mixed_keys(hipStream_t s, int64_t np, const int* cell, int* keys, int* counts) {
-    if (np > 0) hipLaunchKernelGGL(k_mixed_keys, dim3(mblk(np)), dim3(256), 0, s, np, cell, keys, counts);
+void launch_mixed_counts(hipStream_t s, int64_t np, const int* cell, int* counts) {
+    if (np > 0) hipLaunchKernelGGL(k_mixed_counts, dim3(mblk(np)), dim3(256), 0, s, np, cell, counts);
 }
 
-void launch_permute_f32(hipStream_t s, int64_t np, int ncomp, const float* src, float* dst, const int* perm) {
-    if (np > 0) hipLaunchKernelGGL(k_permute_t<float>, dim3(mblk(np)), dim3(256), 0, s, np, ncomp, src, dst, perm);
-}
-
-void launch_permute_i32(hipStream_t s, int64_t np, const int* src, int* dst, const int* perm) {
-    if (np > 0) hipLaunchKernelGGL(k_permute_t<int>, dim3(mblk(np)), dim3(256), 0, s, np, 1, src, dst, perm);
+void launch_mixed_permute(hipStream_t s, int64_t np, const int* perm, const MixedState& src, const MixedState& dst,
+                          const int64_t* id_src, int64_t* id_dst) {
+    if (np > 0)
+        hipLaunchKernelGGL(k_mixed_permute, dim3(mblk(np)), dim3(256), 0, s, np, perm, src, dst, id_src, id_dst);
 }
 
 }  // namespace upic

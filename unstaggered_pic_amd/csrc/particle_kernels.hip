@@ -562,16 +562,16 @@ void launch_p3_map(hipStream_t s, const GridDev& g, const QuadDev& q, const Orbi
     launch_finish_partials(s, a.part, nblk(a.np), 2, 2, a.red, a.redmax, a.stats);
 }
 
-// Second stage of the sweep reductions: one workgroup folds the per-block
-// partials and updates the accumulators.
-__global__ __launch_bounds__(256) void k_finish_partials(const double* __restrict__ part, int64_t nb, int ns, int nm,
-                                                         double* __restrict__ sum, double* __restrict__ mx,
-                                                         unsigned long long* __restrict__ stats) {
+// Folding of the per-block partials of a sweep (slots 4, 5, 8 are maxima,
+// the others sums): the 256 threads of a workgroup fold blocks [b0, b1) in a
+// fixed order; the result is valid in thread 0.
+__device__ __forceinline__ void fold_partials(const double* __restrict__ part, int64_t b0, int64_t b1,
+                                              double t[kPartSlots]) {
     __shared__ double sh[kPartSlots][4];
     double v[kPartSlots];
 #pragma unroll
     for (int k = 0; k < kPartSlots; ++k) v[k] = 0.0;
-    for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) {
+    for (int64_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
         const double* p = part + b * kPartSlots;
 #pragma unroll
         for (int k = 0; k < kPartSlots; ++k) {
@@ -592,13 +592,35 @@ __global__ __launch_bounds__(256) void k_finish_partials(const double* __restric
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double t[kPartSlots];
 #pragma unroll
         for (int k = 0; k < kPartSlots; ++k) {
             const bool m = (k == 4 || k == 5 || k == 8);
             t[k] = sh[k][0];
             for (int w = 1; w < 4; ++w) t[k] = m ? fmax(t[k], sh[k][w]) : t[k] + sh[k][w];
         }
+    }
+}
+
+// First stage for many blocks: workgroup g folds blocks [g per, (g+1) per)
+// into slot g of `out`.
+__global__ __launch_bounds__(256) void k_fold_partials(const double* __restrict__ part, int64_t nb, int64_t per,
+                                                       double* __restrict__ out) {
+    const int64_t b0 = (int64_t)blockIdx.x * per;
+    double t[kPartSlots];
+    fold_partials(part, b0, b0 + per < nb ? b0 + per : nb, t);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < kPartSlots; ++k) out[(int64_t)blockIdx.x * kPartSlots + k] = t[k];
+}
+
+// Last stage: one workgroup folds the (first-stage) partials and updates
+// the accumulators.
+__global__ __launch_bounds__(256) void k_finish_partials(const double* __restrict__ part, int64_t nb, int ns, int nm,
+                                                         double* __restrict__ sum, double* __restrict__ mx,
+                                                         unsigned long long* __restrict__ stats) {
+    double t[kPartSlots];
+    fold_partials(part, 0, nb, t);
+    if (threadIdx.x == 0) {
         for (int k = 0; k < ns && sum; ++k) sum[k] += t[k];
         for (int k = 0; k < nm && mx; ++k) mx[k] = fmax(mx[k], t[4 + k]);
         if (stats) {
@@ -608,9 +630,22 @@ __global__ __launch_bounds__(256) void k_finish_partials(const double* __restric
         }
     }
 }
+// Partials of nblocks workgroups at part[0, nblocks); above kFoldMin blocks a
+// first stage folds them into at most kFoldBlocks slots placed right after
+// them (the buffer holds partial_blocks(np) slots, kFoldBlocks of them spare).
 void launch_finish_partials(hipStream_t s, const double* part, int64_t nblocks, int ns, int nm, double* sum,
                             double* mx, unsigned long long* stats) {
-    if (nblocks > 0) hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(256), 0, s, part, nblocks, ns, nm, sum, mx, stats);
+    if (nblocks <= 0) return;
+    constexpr int64_t kFoldMin = 2048;
+    if (nblocks <= kFoldMin) {
+        hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(256), 0, s, part, nblocks, ns, nm, sum, mx, stats);
+        return;
+    }
+    const int64_t per = (nblocks + kFoldBlocks - 1) / kFoldBlocks;
+    const int64_t g = (nblocks + per - 1) / per;
+    double* out = const_cast<double*>(part) + nblocks * kPartSlots;
+    hipLaunchKernelGGL(k_fold_partials, dim3((unsigned)g), dim3(256), 0, s, part, nblocks, per, out);
+    hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(256), 0, s, out, g, ns, nm, sum, mx, stats);
 }
 void launch_p2_advance_deposit(hipStream_t s, const P2Args& a) {
     if (a.np <= 0) return;

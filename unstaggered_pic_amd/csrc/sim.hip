@@ -57,7 +57,7 @@ struct upic_sim {
     MixedState ms{}, mo{};
     float* ftmp = nullptr;   // 3*np sort scratch
     float* ftmp1 = nullptr;  // np sort scratch
-    int* itmp = nullptr;     // np sort scratch
+    float* ftmp2 = nullptr;  // np sort scratch
     float* NRf = nullptr;   // FP32 node records (8N)
     double outer_atol = 0.0;  // absolute outer tolerance in force for this step
     double* part = nullptr;   // per-block partials of the particle sweeps
@@ -74,6 +74,10 @@ struct upic_sim {
     float* warm = nullptr;
     bool warm_valid = false;
     float* vdep = nullptr;  // converged vbar of the last sweep (3*np)
+    // node current of the previous step (mixed precision): the outer solve
+    // starts from the linear extrapolation 2 J^n - J^{n-1} when valid
+    double* Jprev = nullptr;
+    bool jprev_valid = false;
     bool cells_fresh = false;  // particles cell-sorted since their last move (offsets = cell ends)
     // deposit variant autotuner (mixed precision).  While a particle order
     // (index 1: freshly cell-sorted, 0: not) has no choice yet, each map
@@ -105,7 +109,7 @@ void sim_free(upic_sim* s) {
     double** dbl[] = {&s->x, &s->v, &s->v_prev, &s->P, &s->vbar_last, &s->q, &s->m, &s->phi, &s->psi, &s->rho,
                       &s->A, &s->U, &s->J, &s->S0, &s->S1, &s->zs, &s->AG, &s->NR, &s->Jb[0], &s->Jb[1], &s->gtmp,
                       &s->vg[0], &s->vg[1], &s->xo, &s->vo, &s->Po, &s->vbo, &s->guess, &s->resid, &s->cand,
-                      &s->trial, &s->Jacc, &s->Jtry, &s->Jfield, &s->ptmp, &s->ptmp1, &s->fixed_ion};
+                      &s->trial, &s->Jacc, &s->Jtry, &s->Jfield, &s->ptmp, &s->ptmp1, &s->fixed_ion, &s->Jprev};
     for (double** p : dbl) dfree(*p);
     for (int i = 0; i < 9; ++i) { dfree(s->xh[i]); dfree(s->rh[i]); }
     for (int i = 0; i < 8; ++i) { dfree(s->dX[i]); dfree(s->dR[i]); }
@@ -113,9 +117,9 @@ void sim_free(upic_sim* s) {
     dfree(s->id); dfree(s->idtmp); dfree(s->reason); dfree(s->layout);
     dfree(s->keys); dfree(s->counts); dfree(s->offsets); dfree(s->perm);
     float** fl[] = {&s->ms.off, &s->ms.P, &s->ms.v, &s->ms.vp, &s->ms.vb, &s->ms.q, &s->ms.m,
-                    &s->mo.off, &s->mo.P, &s->mo.v, &s->mo.vb, &s->ftmp, &s->ftmp1, &s->NRf};
+                    &s->mo.off, &s->mo.P, &s->mo.v, &s->mo.vb, &s->ftmp, &s->ftmp1, &s->ftmp2, &s->NRf};
     for (float** p : fl) dfree(*p);
-    dfree(s->ms.cell); dfree(s->mo.cell); dfree(s->itmp);
+    dfree(s->ms.cell); dfree(s->mo.cell);
     dfree(s->mlp_P); dfree(s->mlp_blob); dfree(s->train_buf); dfree(s->part); dfree(s->warm); dfree(s->vdep);
     if (s->tuner.e0) (void)hipEventDestroy(s->tuner.e0);
     if (s->tuner.e1) (void)hipEventDestroy(s->tuner.e1);
@@ -170,7 +174,7 @@ void mixed_compress(upic_sim* s) {
 void fp64_view(upic_sim* s) {
     if (!s->mixed || !s->fp64_stale) return;
     upic_ctx* c = s->c;
-    launch_mixed_expand(c->stream, c->g, s->np, s->ms, s->x, s->P, s->v, s->v_prev, s->vbar_last);
+    launch_mixed_expand(c->stream, c->g, s->np, s->ms, s->x, s->P, s->v, s->v_prev, s->vbar_last, s->q, s->m);
     HIPCHK(hipGetLastError());
     s->fp64_stale = false;
 }
@@ -267,29 +271,25 @@ void sort_particles(upic_sim* s) {
     if (np == 0) return;
     HIPCHK(hipMemsetAsync(s->counts, 0, ncell * sizeof(int), c->stream));
     if (s->mixed) {
-        launch_mixed_keys(c->stream, np, s->ms.cell, s->keys, s->counts);
+        // the map's output arrays (free between steps) and three scratch
+        // arrays receive the permuted state in one pass
+        launch_mixed_counts(c->stream, np, s->ms.cell, s->counts);
         launch_scan_counts(c->stream, ncell, s->counts, s->offsets);
-        launch_scatter_perm(c->stream, np, s->keys, s->offsets, s->perm);
-        float** f3[] = {&s->ms.off, &s->ms.P, &s->ms.v, &s->ms.vp, &s->ms.vb};
-        for (float** a : f3) {
-            launch_permute_f32(c->stream, np, 3, *a, s->ftmp, s->perm);
-            std::swap(*a, s->ftmp);
-        }
-        float** f1[] = {&s->ms.q, &s->ms.m};
-        for (float** a : f1) {
-            launch_permute_f32(c->stream, np, 1, *a, s->ftmp1, s->perm);
-            std::swap(*a, s->ftmp1);
-        }
-        launch_permute_i32(c->stream, np, s->ms.cell, s->itmp, s->perm);
-        std::swap(s->ms.cell, s->itmp);
-        double** arr1[] = {&s->q, &s->m};
-        for (double** a : arr1) {
-            launch_permute(c->stream, np, 1, *a, s->ptmp1, s->perm);
-            std::swap(*a, s->ptmp1);
-        }
-        launch_permute_i64(c->stream, np, s->id, s->idtmp, s->perm);
+        launch_scatter_perm(c->stream, np, s->ms.cell, s->offsets, s->perm);
+        MixedState dst{};
+        dst.cell = s->mo.cell; dst.off = s->mo.off; dst.P = s->mo.P; dst.v = s->mo.v; dst.vb = s->mo.vb;
+        dst.vp = s->ftmp; dst.q = s->ftmp1; dst.m = s->ftmp2;
+        launch_mixed_permute(c->stream, np, s->perm, s->ms, dst, s->id, s->idtmp);
+        std::swap(s->ms.cell, s->mo.cell);
+        std::swap(s->ms.off, s->mo.off);
+        std::swap(s->ms.P, s->mo.P);
+        std::swap(s->ms.v, s->mo.v);
+        std::swap(s->ms.vb, s->mo.vb);
+        std::swap(s->ms.vp, s->ftmp);
+        std::swap(s->ms.q, s->ftmp1);
+        std::swap(s->ms.m, s->ftmp2);
         std::swap(s->id, s->idtmp);
-        s->fp64_stale = true;  // the FP64 view is rebuilt from the permuted state
+        s->fp64_stale = true;  // the FP64 view (q and m included) is rebuilt from the permuted state
         return;
     }
     launch_cell_keys(c->stream, c->g, np, s->x, s->keys, s->counts);
@@ -666,7 +666,18 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
     const int N = c->N();
     const int64_t n3 = 3ll * N;
     const bool picard = p.nonlinear_solver_picard != 0;
-    launch_copy(c->stream, n3, s->J, s->guess);
+    // Mixed precision, uniform layout: the initial macro-current predictor is
+    // extrapolated from the two previous steps (the reference's is J^n,
+    // P3:5930), and every map evaluation writes the particle end state, so
+    // the evaluation that meets the tolerance is the final one (no separate
+    // consistency sweep).  Neither changes the convergence test.
+    const bool fast = s->mixed && !use_layout;
+    if (fast && s->jprev_valid) {
+        LinComb le{2, {2.0, -1.0}, {s->J, s->Jprev}};
+        launch_lincomb(c->stream, n3, le, s->guess);
+    } else {
+        launch_copy(c->stream, n3, s->J, s->guess);
+    }
     if (!finite_vec(s, s->guess)) throw Error(UPIC_ENONLINEAR, "nonfinite initial macro-current predictor");
     History H(s);
     double prev_rms = std::numeric_limits<double>::infinity();
@@ -674,11 +685,13 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
     int restarts = 0, and_steps = 0, pic_steps = 0, backtracked = 0, maps = 0, iters = 0;
     double min_alpha = 1.0, contraction = std::numeric_limits<double>::infinity();
     int64_t local_total = 0;
+    MapOut last{};
     // Jacc: current map of the accepted guess
     auto eval = [&](const double* g, double* Jout, bool wo) {
         ++maps;
-        MapOut o = p3_map(s, g, Jout, wo, use_layout);
+        MapOut o = p3_map(s, g, Jout, wo || fast, use_layout);
         local_total += (int64_t)o.iter_sum;
+        last = o;
         return o;
     };
     const int keep = p.anderson_memory + 1;
@@ -745,7 +758,8 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
     }
     if (!converged)
         throw Error(UPIC_ENONLINEAR, "outer macro-current solve failed after " + std::to_string(iters) + " iterations");
-    const MapOut fo = eval(s->guess, s->Jacc, true);
+    // the last evaluation is that of the converged guess (Jacc)
+    const MapOut fo = fast ? last : eval(s->guess, s->Jacc, true);
     const Norms fn = residual_norms(s, s->Jacc, s->guess);
     const double slack = s->mixed ? kMixedFinalSlack : 1.0;
     const bool ok = !fn.nonfinite && fn.rms_scaled <= slack && fn.max_particle_scaled <= slack;
@@ -782,6 +796,8 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
         std::swap(s->ms.P, s->mo.P);
         std::swap(s->ms.vb, s->mo.vb);
         s->fp64_stale = true;
+        launch_copy(c->stream, n3, s->J, s->Jprev);
+        s->jprev_valid = true;
         commit_fields(s, s->Jacc);
         c->sync();
         return;
@@ -1190,9 +1206,10 @@ upic_status upic_sim_create(upic_ctx* c, int64_t np, upic_sim** out) {
             s->ms.m = dalloc<float>(np);
             s->ftmp = dalloc<float>(n3);
             s->ftmp1 = dalloc<float>(np);
+            s->ftmp2 = dalloc<float>(np);
             s->warm = dalloc<float>(n3);
             s->vdep = dalloc<float>(n3);
-            s->itmp = dalloc<int>(np);
+            s->Jprev = dalloc<double>(3 * (size_t)N);
             s->NRf = dalloc<float>(8 * (size_t)N);
         }
         // identity ids
@@ -1223,6 +1240,7 @@ upic_status upic_sim_upload(upic_sim* s, const upic_host_state* st) {
     if (!s || !st || st->np != s->np) return UPIC_EINVAL;
     upic_ctx* c = s->c;
     return (upic_status)guarded(c, [&] {
+        s->jprev_valid = false;
         fp64_view(s);  // arrays the caller leaves out keep their current values
         const int N = c->N();
         const size_t n3 = 3 * (size_t)s->np * sizeof(double), n1 = (size_t)s->np * sizeof(double);
@@ -1304,6 +1322,7 @@ upic_status upic_sim_init_fields(upic_sim* s, double n0) {
     if (!s) return UPIC_EINVAL;
     upic_ctx* c = s->c;
     return (upic_status)guarded(c, [&] {
+        s->jprev_valid = false;
         const upic_params& p = c->p;
         const int N = c->N();
         const int64_t np = s->np;
@@ -1371,6 +1390,7 @@ upic_status upic_sim_init_fields(upic_sim* s, double n0) {
 
 upic_status upic_sim_device_state(upic_sim* s, upic_host_state* d) {
     if (!s || !d) return UPIC_EINVAL;
+    s->jprev_valid = false;  // the caller may write the node current
     const int rc = guarded(s->c, [&] { fp64_view(s); });
     if (rc != UPIC_OK) return (upic_status)rc;
     std::memset(d, 0, sizeof(*d));
