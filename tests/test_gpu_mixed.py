@@ -153,16 +153,20 @@ def test_mixed_vs_fp64_gpu_history():
     assert np.max(np.abs(a[:, 0] - a[0, 0])) / abs(a[0, 0]) < 1e-6
 
 
-def test_deposit_variants_agree_and_autotune():
-    """64x64 grid (every deposit variant usable), 16 particles per cell,
-    sorted every step: three steps with each deposit variant pinned reach the
-    same fixed point (total energy 1e-6, field energy 1e-4, positions 1e-5 of
-    the box), and the autotuner settles on a usable variant."""
+@pytest.mark.parametrize("split", [1, 0])
+def test_deposit_variants_agree_and_autotune(split):
+    """64x64 grid, 16 particles per cell, sorted every step: three steps with
+    each deposit variant pinned reach the same fixed point (total energy
+    1e-6, field energy 1e-4, positions 1e-5 of the box), and the autotuner
+    settles on a usable variant.  Without knot splitting the sort leaves the
+    particles in plain cell order, which the cell-mapped variants (1, 2) need;
+    with it, crossing orbits are grouped after the others and 1, 2 fall back
+    to variant 0."""
     lib = gpu()
     p64 = make_params(code=CODE_P3, nx=64, ny=64, nz=1, Lx=8 * np.pi, Ly=8 * np.pi, Lz=1.0, dt=0.05,
                       spline_order=3, orbit_quad_order=4, nyquist_projection=1, nonlinear_rtol=1e-11,
                       nonlinear_atol=1e-13, nonlinear_max_iter=80, anderson_damping=0.5, local_rtol=1e-12,
-                      local_atol=1e-14, local_max_iter=64)
+                      local_atol=1e-14, local_max_iter=64, split_orbit_at_knots=split)
     p64.sort_interval = 1
     pm = with_mixed_precision(p64)
     ppc = 16
@@ -188,7 +192,10 @@ def test_deposit_variants_agree_and_autotune():
             chosen = sim.deposit_variant()
         sim.close()
     ctx.close()
-    assert chosen[0] in (0, 1, 2, 3), chosen
+    if split:
+        assert chosen[1] in (0, 3), chosen
+    else:
+        assert chosen[0] in (0, 1, 2, 3), chosen
     e0, x0 = res[0]
     assert np.max(np.abs(e0[:, 0] - e0[0, 0])) / e0[0, 0] < 1e-6
     for variant in (-1, 1, 2, 3):

@@ -145,7 +145,8 @@ void launch_mixed_compress(hipStream_t s, const GridDev& g, int64_t np, const do
                            const MixedState& ms);
 void launch_mixed_expand(hipStream_t s, const GridDev& g, int64_t np, const MixedState& ms, double* x, double* P,
                          double* v, double* vp, double* vb, double* q, double* m);
-void launch_mixed_counts(hipStream_t s, int64_t np, const int* cell, int* counts);
+void launch_mixed_keys(hipStream_t s, const GridDev& g, int64_t np, const MixedState& ms, double dt, int split,
+                       int* keys, int* counts);
 void launch_mixed_permute(hipStream_t s, int64_t np, const int* perm, const MixedState& src, const MixedState& dst,
                           const int64_t* id_src, int64_t* id_dst);
 

@@ -339,9 +339,12 @@ __device__ __forceinline__ int orbit_gg_f(const MGrid& g, const QuadF& q, bool s
     return nseg;
 }
 
-// FP32 LDS deposit tile of kTileM x kTileM nodes x 3 components; origin three
-// cells below the workgroup's first particle cell.
-constexpr int kTileM = 8;
+// FP32 LDS deposit tile of kTileX x kTileY nodes x 3 components; origin
+// three cells below the workgroup's first particle cell.  Long in y: the
+// particles of a workgroup are in cell order (cell = i ny + j), so a
+// workgroup whose particles are spread over several cells (few particles per
+// cell, or the group of knot-crossing orbits) covers a run of cells along y.
+constexpr int kTileX = 8, kTileY = 32;
 struct DepTileF {
     float* t;
     int ox, oy;
@@ -353,8 +356,8 @@ __device__ __forceinline__ void tilef_add(const MGrid& g, const DepTileF& T, dou
     if (dx < 0) dx += g.nx;
     int dy = ib - T.oy;
     if (dy < 0) dy += g.ny;
-    if (dx < kTileM && dy < kTileM) {
-        float* c = T.t + 3 * (dx * kTileM + dy);
+    if (dx < kTileX && dy < kTileY) {
+        float* c = T.t + 3 * (dx * kTileY + dy);
         atomicAdd(c, j0);
         atomicAdd(c + 1, j1);
         atomicAdd(c + 2, j2);
@@ -373,7 +376,7 @@ __device__ __forceinline__ void tilef_add1(const MGrid& g, const DepTileF& T, do
     if (dx < 0) dx += g.nx;
     int dy = ib - T.oy;
     if (dy < 0) dy += g.ny;
-    if (dx < kTileM && dy < kTileM) atomicAdd(T.t + 3 * (dx * kTileM + dy) + comp, v);
+    if (dx < kTileX && dy < kTileY) atomicAdd(T.t + 3 * (dx * kTileY + dy) + comp, v);
     else unsafeAtomicAdd(&J[(size_t)comp * g.N + ia * g.ny + ib], (double)v);
 }
 
@@ -454,18 +457,18 @@ __device__ __forceinline__ DepTileF tilef_begin(const MGrid& g, float* lds, cons
         org[0] = ox;
         org[1] = oy;
     }
-    for (int i = threadIdx.x; i < kTileM * kTileM * 3; i += blockDim.x) lds[i] = 0.0f;
+    for (int i = threadIdx.x; i < kTileX * kTileY * 3; i += blockDim.x) lds[i] = 0.0f;
     __syncthreads();
     return DepTileF{lds, org[0], org[1]};
 }
 
 __device__ __forceinline__ void tilef_end(const MGrid& g, const DepTileF& T, double* __restrict__ J) {
     __syncthreads();
-    for (int i = threadIdx.x; i < kTileM * kTileM * 3; i += blockDim.x) {
+    for (int i = threadIdx.x; i < kTileX * kTileY * 3; i += blockDim.x) {
         const float v = T.t[i];
         if (v != 0.0f) {
             const int comp = i % 3, cell = i / 3;
-            const int dx = cell / kTileM, dy = cell % kTileM;
+            const int dx = cell / kTileY, dy = cell % kTileY;
             if (dx >= g.nx || dy >= g.ny) continue;
             const int ia = (T.ox + dx) % g.nx, ib = (T.oy + dy) % g.ny;
             unsafeAtomicAdd(&J[(size_t)comp * g.N + ia * g.ny + ib], (double)v);
@@ -657,7 +660,7 @@ __device__ __forceinline__ void guess_features(const MGrid& g, const float4* __r
 template <int R, int GM, bool COLLECT, bool FUSED>
 __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    __shared__ float tile_mem[kTileM * kTileM * 3];
+    __shared__ float tile_mem[kTileX * kTileY * 3];
     extern __shared__ unsigned char mlp_lds[];
     DepTileF T{};
     if constexpr (FUSED) T = tilef_begin(g, tile_mem, a.cell, a.np, (int64_t)blockIdx.x * blockDim.x);
@@ -799,7 +802,7 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
 template <int R>
 __global__ __launch_bounds__(256) void k_deposit_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    __shared__ float tile_mem[kTileM * kTileM * 3];
+    __shared__ float tile_mem[kTileX * kTileY * 3];
     const DepTileF T = tilef_begin(g, tile_mem, a.cell, a.np, (int64_t)blockIdx.x * blockDim.x);
     int nseg = 0;
     if (p < a.np && a.reason[p] == 0) {
@@ -951,7 +954,7 @@ template <int R>
 __global__ __launch_bounds__(256) void k_deposit_wave_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
     constexpr int NW = (R + 1) * (R + 1);
     constexpr int ROW = NW + 4;  // W (NW) | qs v (3) | pad
-    __shared__ float tile_mem[kTileM * kTileM * 3];
+    __shared__ float tile_mem[kTileX * kTileY * 3];
     __shared__ float red[4][64 * ROW];
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const DepTileF T = tilef_begin(g, tile_mem, a.cell, a.np, (int64_t)blockIdx.x * blockDim.x);
@@ -1139,11 +1142,26 @@ __global__ void k_mixed_expand(GridDev g, int64_t np, MixedState s, double* __re
     m[p] = (double)s.m[p];
 }
 
-// cell counts of the counting sort (the cell index is the key)
-__global__ void k_mixed_counts(int64_t np, const int* __restrict__ cell, int* __restrict__ counts) {
+// Counting-sort keys: the cell index, plus N when `split` and the orbit
+// t -> t + dt vbar / h over the stored vbar (the last step's) crosses a knot
+// of the degree-R splines (integer t - (R - 1)/2) along x or y.
+__global__ void k_mixed_keys(MGrid g, int64_t np, const int* __restrict__ cell, const float* __restrict__ off,
+                             const float* __restrict__ vb, float dt, int split, int* __restrict__ keys,
+                             int* __restrict__ counts) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = p < np;
-    (void)wave_claim(counts, active ? cell[p] : 0, active);
+    int key = 0;
+    if (active) {
+        key = cell[p];
+        if (split) {
+            const float koff = 0.5f * (float)(g.order - 1);
+            const float ax = off[p] - koff, ay = off[np + p] - koff;
+            const float bx = ax + dt * vb[p] * g.ihx, by = ay + dt * vb[np + p] * g.ihy;
+            if (floorf(ax) != floorf(bx) || floorf(ay) != floorf(by)) key += g.N;
+        }
+        keys[p] = key;
+    }
+    (void)wave_claim(counts, key, active);
 }
 
 // All per-particle arrays of the mixed state through one permutation:
@@ -1280,8 +1298,15 @@ void launch_mixed_expand(hipStream_t s, const GridDev& g, int64_t np, const Mixe
     if (np > 0) hipLaunchKernelGGL(k_mixed_expand, dim3(mblk(np)), dim3(256), 0, s, g, np, ms, x, P, v, vp, vb, q, m);
 }
 
-void launch_mixed_counts(hipStream_t s, int64_t np, const int* cell, int* counts) {
-    if (np > 0) hipLaunchKernelGGL(k_mixed_counts, dim3(mblk(np)), dim3(256), 0, s, np, cell, counts);
+void launch_mixed_keys(hipStream_t s, const GridDev& g, int64_t np, const MixedState& ms, double dt, int split,
+                       int* keys, int* counts) {
+    if (np <= 0) return;
+    MGrid mg;
+    mg.nx = g.nx; mg.ny = g.ny; mg.N = g.N; mg.order = g.order;
+    mg.hx = (float)g.h[0]; mg.hy = (float)g.h[1];
+    mg.ihx = (float)(1.0 / g.h[0]); mg.ihy = (float)(1.0 / g.h[1]);
+    hipLaunchKernelGGL(k_mixed_keys, dim3(mblk(np)), dim3(256), 0, s, mg, np, ms.cell, ms.off, ms.vb, (float)dt, split,
+                       keys, counts);
 }
 
 void launch_mixed_permute(hipStream_t s, int64_t np, const int* perm, const MixedState& src, const MixedState& dst,

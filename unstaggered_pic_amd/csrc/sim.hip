@@ -264,18 +264,26 @@ void fill_path(upic_ctx* c, int64_t np, upic_path_stats& ps) {
 }
 
 // Counting sort of the particles by the cell that contains them.
-void sort_particles(upic_sim* s) {
+// Returns whether offsets[] holds the end of each cell's range afterwards.
+bool sort_particles(upic_sim* s) {
     upic_ctx* c = s->c;
     const int64_t np = s->np;
     const int ncell = c->N();
-    if (np == 0) return;
-    HIPCHK(hipMemsetAsync(s->counts, 0, ncell * sizeof(int), c->stream));
+    if (np == 0) return false;
     if (s->mixed) {
+        // Key = cell, plus ncell for a particle whose orbit over the last
+        // step's vbar would cross a spline knot: crossing orbits take two or
+        // more Gauss-segment passes, so grouping them keeps the wavefronts of
+        // the map and deposit sweeps from running the second pass for a few
+        // lanes each.  Both halves stay in cell order.
+        const bool split = c->opt.split != 0;
+        const int nkeys = split ? 2 * ncell : ncell;
+        HIPCHK(hipMemsetAsync(s->counts, 0, nkeys * sizeof(int), c->stream));
+        launch_mixed_keys(c->stream, c->g, np, s->ms, c->p.dt, split ? 1 : 0, s->keys, s->counts);
+        launch_scan_counts(c->stream, nkeys, s->counts, s->offsets);
+        launch_scatter_perm(c->stream, np, s->keys, s->offsets, s->perm);
         // the map's output arrays (free between steps) and three scratch
         // arrays receive the permuted state in one pass
-        launch_mixed_counts(c->stream, np, s->ms.cell, s->counts);
-        launch_scan_counts(c->stream, ncell, s->counts, s->offsets);
-        launch_scatter_perm(c->stream, np, s->ms.cell, s->offsets, s->perm);
         MixedState dst{};
         dst.cell = s->mo.cell; dst.off = s->mo.off; dst.P = s->mo.P; dst.v = s->mo.v; dst.vb = s->mo.vb;
         dst.vp = s->ftmp; dst.q = s->ftmp1; dst.m = s->ftmp2;
@@ -290,8 +298,9 @@ void sort_particles(upic_sim* s) {
         std::swap(s->ms.m, s->ftmp2);
         std::swap(s->id, s->idtmp);
         s->fp64_stale = true;  // the FP64 view (q and m included) is rebuilt from the permuted state
-        return;
+        return !split;
     }
+    HIPCHK(hipMemsetAsync(s->counts, 0, ncell * sizeof(int), c->stream));
     launch_cell_keys(c->stream, c->g, np, s->x, s->keys, s->counts);
     launch_scan_counts(c->stream, ncell, s->counts, s->offsets);
     launch_scatter_perm(c->stream, np, s->keys, s->offsets, s->perm);
@@ -307,15 +316,13 @@ void sort_particles(upic_sim* s) {
     }
     launch_permute_i64(c->stream, np, s->id, s->idtmp, s->perm);
     std::swap(s->id, s->idtmp);
+    return true;
 }
 
 void maybe_sort(upic_sim* s) {
     const int k = s->c->p.sort_interval;
     s->cells_fresh = false;
-    if (k > 0 && (s->steps % k) == 0) {
-        sort_particles(s);
-        s->cells_fresh = true;
-    }
+    if (k > 0 && (s->steps % k) == 0) s->cells_fresh = sort_particles(s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1164,7 +1171,10 @@ upic_status upic_sim_create(upic_ctx* c, int64_t np, upic_sim** out) {
         s->ptmp1 = dalloc<double>(np);
         s->keys = dalloc<int>(np); s->perm = dalloc<int>(np);
         s->part = dalloc<double>((size_t)(partial_blocks(np) + 1) * kPartSlots);
-        s->counts = dalloc<int>(N); s->offsets = dalloc<int>(N);
+        // two key ranges per cell in the mixed sort (orbits predicted to stay in
+        // their knot interval first, crossing orbits after)
+        const int nkeys = c->p.precision == UPIC_PREC_MIXED ? 2 * N : N;
+        s->counts = dalloc<int>(nkeys); s->offsets = dalloc<int>(nkeys);
         if (c->p.code == 1) {
             s->vg[0] = dalloc<double>(n3);
             s->vg[1] = dalloc<double>(n3);
