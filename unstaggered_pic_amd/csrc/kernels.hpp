@@ -121,6 +121,11 @@ struct P3MArgs {
     float* warm;
     int warm_in;
     float* vdep;                   // converged vbar (3*np) for the separate deposit
+    // deferred second Picard iterations (uniform layout; NULL = none): queue
+    // length, particle indices (np) and queued guesses (3*np)
+    int* retry_n;
+    int* retry_idx;
+    float* retry_gs;
 };
 // Picard sweep; with a substep layout it also deposits, otherwise it writes
 // vdep and launch_deposit_mixed deposits (reason and vdep as inputs).

@@ -657,6 +657,138 @@ __device__ __forceinline__ void guess_features(const MGrid& g, const float4* __r
 // FUSED: deposit inside the sweep (per-particle substep layouts); otherwise
 // the converged vbar goes to a.vdep and k_deposit_mixed deposits it, which
 // keeps the deposit's registers out of the Picard sweep.
+// One particle of the sweep: the substep loop of
+// evaluate_selective_current_map_at_layout (P3:5290-5431) in cell-relative
+// FP32.  gin: the initial guess of the (single-substep) inner solve; it0: the
+// Picard iterations already spent on it (a deferred retry continues at 1).
+// With defer set, a particle that the first Picard iteration leaves
+// unconverged is queued (index and updated guess) for k_p3_map_retry and
+// nothing else is written for it here: a handful of such lanes would
+// otherwise keep their whole wavefront iterating.
+template <int R, bool COLLECT, bool FUSED>
+__device__ __forceinline__ void map_particle(const MGrid& g, const QuadF& qd, int split, const P3MArgs& a, int64_t p,
+                                             const float gin[3], int it0, bool defer, const float feat[12],
+                                             const float dph[3], const DepTileF& T, double rsum[2], double rmax[2],
+                                             int& nseg_total) {
+    const int nsub = a.layout ? (int)a.layout[p] : 1;
+    const float h = (float)(a.dt / (double)nsub);
+    const int c0 = a.cell[p];
+    int ci = c0 / g.ny, cj = c0 - (c0 / g.ny) * g.ny;
+    float tx = a.off[p], ty = a.off[a.np + p], tz = a.off[2 * a.np + p];
+    float Pc[3], vc[3];
+    ld3f(a.P, a.np, p, Pc);
+    ld3f(a.v, a.np, p, vc);
+    float vmac[3] = {0.0f, 0.0f, 0.0f};
+    const float qp = a.q[p], mp = a.m[p];
+    const float kappa = (float)a.kappa;
+    const float rtol = (float)a.rtol, atol = (float)a.atol, damp = (float)a.damping;
+    const float inv_nsub = 1.0f / (float)nsub;
+    int reason = 0;
+    for (int s = 0; s < nsub && reason == 0; ++s) {
+        const float th0 = (float)s * inv_nsub, th1 = (float)(s + 1) * inv_nsub;
+        float A0p[3];
+        gather_interp_f<R>(g, a.NR, ci, cj, tx, ty, th0, A0p);
+        const float pst[3] = {Pc[0] - qp * A0p[0], Pc[1] - qp * A0p[1], Pc[2] - qp * A0p[2]};
+        float gs[3] = {vc[0], vc[1], vc[2]};
+        if (nsub == 1) { gs[0] = gin[0]; gs[1] = gin[1]; gs[2] = gin[2]; }
+        LocalMapF acc;
+        float vacc[3];
+        bool conv = false;
+        int its = nsub == 1 ? it0 : 0;
+        float scaled = 0.0f;
+        reason = 5;  // maximum_iterations
+        for (int it = its; it < a.max_iter; ++it) {
+            LocalMapF mpd;
+            local_map_f<R>(g, qd, split != 0, a.NR, gs, ci, cj, tx, ty, Pc, A0p, pst, qp, mp, kappa, h, th0, th1, mpd);
+            ++its;
+            if (!finite3f(mpd.vbar)) { reason = 1; break; }
+            // convergence test of solve_local_particle_substep (P3:4951-4968)
+            const float nm = sqrtf(mpd.vbar[0] * mpd.vbar[0] + mpd.vbar[1] * mpd.vbar[1] + mpd.vbar[2] * mpd.vbar[2]);
+            if (nm >= kappa) { reason = 2; break; }
+            const float res[3] = {mpd.vbar[0] - gs[0], mpd.vbar[1] - gs[1], mpd.vbar[2] - gs[2]};
+            const float ng = sqrtf(gs[0] * gs[0] + gs[1] * gs[1] + gs[2] * gs[2]);
+            const float nr = sqrtf(res[0] * res[0] + res[1] * res[1] + res[2] * res[2]);
+            const float tol = atol + rtol * (0.5f * (nm + ng));
+            scaled = nr / tol;
+            if (nr <= tol) {
+                vacc[0] = gs[0]; vacc[1] = gs[1]; vacc[2] = gs[2];
+                acc = mpd;
+                conv = true;
+                reason = 0;
+                break;
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) gs[d] = gs[d] + damp * res[d];
+            if (!finite3f(gs)) { reason = 3; break; }
+            if (gs[0] * gs[0] + gs[1] * gs[1] + gs[2] * gs[2] >= kappa * kappa) { reason = 4; break; }
+            if (defer && nsub == 1) {
+                // one queue reservation for the lanes of the wavefront deferring here
+                const unsigned long long grp = __ballot(1);
+                const int lane = threadIdx.x & 63, leader = __ffsll((long long)grp) - 1;
+                int base = 0;
+                if (lane == leader) base = atomicAdd(a.retry_n, __popcll(grp));
+                base = __shfl(base, leader, 64);
+                const int k = base + __popcll(grp & ((1ull << lane) - 1ull));
+                a.retry_idx[k] = (int)p;
+                st3f(a.retry_gs, a.np, k, gs);
+                return;
+            }
+        }
+        rsum[0] += (double)its;
+        rmax[0] = fmax(rmax[0], (double)its);
+        rmax[1] = fmax(rmax[1], (double)scaled);
+        if (!conv) break;
+        if constexpr (COLLECT) {
+            if (a.write_out && nsub == 1 && a.train && (p % a.train_stride) == 0) {
+                const int64_t row = a.train_offset + p / a.train_stride;
+                if (row < a.train_cap) {
+                    float* o = a.train + 15 * row;
+#pragma unroll
+                    for (int k = 0; k < 12; ++k) o[k] = feat[k];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) o[12 + d] = vacc[d] - vc[d] - dph[d];
+                }
+            }
+        }
+        const float dux = h * vacc[0] * g.ihx, duy = h * vacc[1] * g.ihy;
+        if constexpr (FUSED) {
+            if (a.J_out)
+                nseg_total += deposit_orbit_f<R>(g, qd, split != 0, ci, cj, tx, ty, dux, duy, vacc,
+                                                 qp * inv_nsub * a.inv_dV, T, a.J_out);
+        } else {
+            st3f(a.vdep, a.np, p, vacc);
+        }
+        move_axis(tx, dux, ci, g.nx);
+        move_axis(ty, duy, cj, g.ny);
+        int cz = 0;
+        move_axis(tz, h * vacc[2] * a.inv_Lz, cz, 1);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            Pc[d] = acc.P_end[d];
+            vc[d] = acc.v_end[d];
+            vmac[d] += vacc[d] * inv_nsub;
+        }
+    }
+    if (a.reason) a.reason[p] = (uint8_t)reason;
+    if (reason != 0) rsum[1] += 1.0;
+    if (reason == 0 && a.write_out) {
+        a.cell_out[p] = ci * g.ny + cj;
+        a.off_out[p] = tx;
+        a.off_out[a.np + p] = ty;
+        a.off_out[2 * a.np + p] = tz;
+        st3f(a.P_out, a.np, p, Pc);
+        st3f(a.v_out, a.np, p, vc);
+        st3f(a.vbar_out, a.np, p, vmac);
+    }
+}
+
+// GM: initial guess of the inner solve, 0 = v^n (the reference's), 1 =
+// physics predictor, 2 = predictor + network (its correction stored in
+// a.warm when given), 3 = predictor + the stored correction; COLLECT: write
+// (features, converged correction) training rows in the write-out pass.
+// FUSED: deposit inside the sweep (per-particle substep layouts); otherwise
+// the converged vbar goes to a.vdep and the deposit kernel deposits it,
+// which keeps the deposit's registers out of the Picard sweep.
 template <int R, int GM, bool COLLECT, bool FUSED>
 __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -686,110 +818,44 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
     double rsum[2] = {0.0, 0.0}, rmax[2] = {0.0, 0.0};
     int nseg_total = 0;
     if (p < a.np) {
-        const int nsub = a.layout ? (int)a.layout[p] : 1;
-        const float h = (float)(a.dt / (double)nsub);
-        const int c0 = a.cell[p];
-        int ci = c0 / g.ny, cj = c0 - (c0 / g.ny) * g.ny;
-        float tx = a.off[p], ty = a.off[a.np + p], tz = a.off[2 * a.np + p];
-        float Pc[3], vc[3];
-        ld3f(a.P, a.np, p, Pc);
-        ld3f(a.v, a.np, p, vc);
-        double vmac[3] = {0.0, 0.0, 0.0};
-        const float qp = a.q[p], mp = a.m[p];
-        const float kappa = (float)a.kappa;
-        int reason = 0;
-        for (int s = 0; s < nsub && reason == 0; ++s) {
-            const float th0 = (float)s / (float)nsub, th1 = (float)(s + 1) / (float)nsub;
-            float A0p[3];
-            gather_interp_f<R>(g, a.NR, ci, cj, tx, ty, th0, A0p);
-            const float pst[3] = {Pc[0] - qp * A0p[0], Pc[1] - qp * A0p[1], Pc[2] - qp * A0p[2]};
-            float gs[3] = {vc[0], vc[1], vc[2]};
-            if (a.guess && nsub == 1) ld3f(a.guess, a.np, p, gs);
-            if constexpr (GM > 0) {
-                if (nsub == 1) {
+        float gin[3];
+        ld3f(a.v, a.np, p, gin);
+        if (a.guess) ld3f(a.guess, a.np, p, gin);
+        if constexpr (GM > 0) {
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) gs[d] = vc[d] + dph[d] + (GM >= 2 ? lg[d] : 0.0f);
-                }
-            }
-            LocalMapF acc;
-            float vacc[3];
-            bool conv = false;
-            int its = 0;
-            double scaled = 0.0;
-            reason = 5;  // maximum_iterations
-            for (int it = 0; it < a.max_iter; ++it) {
-                LocalMapF mpd;
-                local_map_f<R>(g, qd, split != 0, a.NR, gs, ci, cj, tx, ty, Pc, A0p, pst, qp, mp, kappa, h, th0, th1, mpd);
-                ++its;
-                if (!finite3f(mpd.vbar)) { reason = 1; break; }
-                const double nm = norm3f(mpd.vbar);
-                if (nm >= a.kappa) { reason = 2; break; }
-                const float res[3] = {mpd.vbar[0] - gs[0], mpd.vbar[1] - gs[1], mpd.vbar[2] - gs[2]};
-                const double scale = 0.5 * (nm + norm3f(gs));
-                scaled = norm3f(res) / (a.atol + a.rtol * scale);
-                if (scaled <= 1.0) {
-                    vacc[0] = gs[0]; vacc[1] = gs[1]; vacc[2] = gs[2];
-                    acc = mpd;
-                    conv = true;
-                    reason = 0;
-                    break;
-                }
-                const float damp = (float)a.damping;
-#pragma unroll
-                for (int d = 0; d < 3; ++d) gs[d] = gs[d] + damp * res[d];
-                if (!finite3f(gs)) { reason = 3; break; }
-                if (norm3f(gs) >= a.kappa) { reason = 4; break; }
-            }
-            rsum[0] += (double)its;
-            rmax[0] = fmax(rmax[0], (double)its);
-            rmax[1] = fmax(rmax[1], scaled);
-            if (!conv) break;
-            if constexpr (COLLECT) {
-                if (a.write_out && nsub == 1 && a.train && (p % a.train_stride) == 0) {
-                    const int64_t row = a.train_offset + p / a.train_stride;
-                    if (row < a.train_cap) {
-                        float* o = a.train + 15 * row;
-#pragma unroll
-                        for (int k = 0; k < 12; ++k) o[k] = feat[k];
-#pragma unroll
-                        for (int d = 0; d < 3; ++d) o[12 + d] = vacc[d] - vc[d] - dph[d];
-                    }
-                }
-            }
-            const float dux = h * vacc[0] * g.ihx, duy = h * vacc[1] * g.ihy;
-            if constexpr (FUSED) {
-                if (a.J_out)
-                    nseg_total += deposit_orbit_f<R>(g, qd, split != 0, ci, cj, tx, ty, dux, duy, vacc,
-                                                     qp / (float)nsub * a.inv_dV, T, a.J_out);
-            } else {
-                st3f(a.vdep, a.np, p, vacc);
-            }
-            move_axis(tx, dux, ci, g.nx);
-            move_axis(ty, duy, cj, g.ny);
-            int cz = 0;
-            move_axis(tz, h * vacc[2] * a.inv_Lz, cz, 1);
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                Pc[d] = acc.P_end[d];
-                vc[d] = acc.v_end[d];
-                vmac[d] += (double)vacc[d] / (double)nsub;
-            }
+            for (int d = 0; d < 3; ++d) gin[d] += dph[d] + (GM >= 2 ? lg[d] : 0.0f);
         }
-        if (a.reason) a.reason[p] = (uint8_t)reason;
-        if (reason != 0) rsum[1] = 1.0;
-        if (reason == 0 && a.write_out) {
-            a.cell_out[p] = ci * g.ny + cj;
-            a.off_out[p] = tx;
-            a.off_out[a.np + p] = ty;
-            a.off_out[2 * a.np + p] = tz;
-            st3f(a.P_out, a.np, p, Pc);
-            st3f(a.v_out, a.np, p, vc);
-            const float vm[3] = {(float)vmac[0], (float)vmac[1], (float)vmac[2]};
-            st3f(a.vbar_out, a.np, p, vm);
-        }
+        map_particle<R, COLLECT, FUSED>(g, qd, split, a, p, gin, 0, a.retry_n != nullptr && !a.layout, feat, dph, T,
+                                        rsum, rmax, nseg_total);
     }
     if constexpr (FUSED) {
         if (a.J_out) tilef_end(g, T, a.J_out);
+    }
+    block_partials<2, 2>(rsum, rmax, nseg_total, a.part);
+}
+
+// The deferred particles of a sweep (uniform layout), packed into full
+// wavefronts: queue entries [0, *retry_n), grid-stride.  Picks the Picard
+// iteration up at its second step from the queued guess.
+template <int R, bool COLLECT>
+__global__ __launch_bounds__(256) void k_p3_map_retry(MGrid g, QuadF qd, int split, P3MArgs a) {
+    const int n = *a.retry_n;
+    double rsum[2] = {0.0, 0.0}, rmax[2] = {0.0, 0.0};
+    int nseg_total = 0;
+    const DepTileF T{};
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int64_t p = a.retry_idx[k];
+        float gin[3];
+        ld3f(a.retry_gs, a.np, k, gin);
+        float feat[12] = {}, dph[3] = {0.0f, 0.0f, 0.0f};
+        if constexpr (COLLECT) {
+            const int c0 = a.cell[p];
+            float v0[3];
+            ld3f(a.v, a.np, p, v0);
+            guess_features<R>(g, a.NR, c0 / g.ny, c0 - (c0 / g.ny) * g.ny, a.off[p], a.off[a.np + p], v0, a.q[p],
+                              a.m[p], (float)a.kappa, (float)a.dt, feat, dph);
+        }
+        map_particle<R, COLLECT, false>(g, qd, split, a, p, gin, 1, false, feat, dph, T, rsum, rmax, nseg_total);
     }
     block_partials<2, 2>(rsum, rmax, nseg_total, a.part);
 }
@@ -1188,10 +1254,15 @@ __global__ void k_mixed_permute(int64_t np, const int* __restrict__ perm, MixedS
 
 inline unsigned mblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
+__global__ void k_zero_int(int* __restrict__ x) {
+    if (threadIdx.x == 0) *x = 0;
+}
+
 }  // namespace
 
 void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& a) {
     if (a.np <= 0) return;
+    if (a.retry_n && !a.layout) hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, a.retry_n);
     MGrid mg;
     mg.nx = g.nx; mg.ny = g.ny; mg.N = g.N; mg.order = g.order;
     mg.hx = (float)g.h[0]; mg.hy = (float)g.h[1];
@@ -1226,6 +1297,25 @@ void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int sp
     }
 #undef UPIC_MIXED_LAUNCH
     launch_finish_partials(s, a.part, mblk(a.np), 2, 2, a.red, a.redmax, a.layout ? a.stats : nullptr);
+    if (a.retry_n && !a.layout) {
+        // the deferred particles; their partials reuse the buffer after the
+        // fold above
+        const unsigned nb = std::min<unsigned>(mblk(a.np), 1024u);
+#define UPIC_RETRY_LAUNCH(R_)                                                                            \
+    do {                                                                                                 \
+        if (col) hipLaunchKernelGGL((k_p3_map_retry<R_, true>), dim3(nb), dim3(256), 0, s, mg, q, split, a);  \
+        else hipLaunchKernelGGL((k_p3_map_retry<R_, false>), dim3(nb), dim3(256), 0, s, mg, q, split, a);    \
+    } while (0)
+        switch (g.order) {
+            case 1: UPIC_RETRY_LAUNCH(1); break;
+            case 2: UPIC_RETRY_LAUNCH(2); break;
+            case 3: UPIC_RETRY_LAUNCH(3); break;
+            case 4: UPIC_RETRY_LAUNCH(4); break;
+            default: break;
+        }
+#undef UPIC_RETRY_LAUNCH
+        launch_finish_partials(s, a.part, nb, 2, 2, a.red, a.redmax, nullptr);
+    }
 }
 
 int deposit_variants() { return 4; }

@@ -78,6 +78,10 @@ struct upic_sim {
     // starts from the linear extrapolation 2 J^n - J^{n-1} when valid
     double* Jprev = nullptr;
     bool jprev_valid = false;
+    // deferred second Picard iterations of the sweep (count, indices, guesses)
+    int* retry_n = nullptr;
+    int* retry_idx = nullptr;
+    float* retry_gs = nullptr;
     bool cells_fresh = false;  // particles cell-sorted since their last move (offsets = cell ends)
     // deposit variant autotuner (mixed precision).  While a particle order
     // (index 1: freshly cell-sorted, 0: not) has no choice yet, each map
@@ -120,6 +124,7 @@ void sim_free(upic_sim* s) {
                     &s->mo.off, &s->mo.P, &s->mo.v, &s->mo.vb, &s->ftmp, &s->ftmp1, &s->ftmp2, &s->NRf};
     for (float** p : fl) dfree(*p);
     dfree(s->ms.cell); dfree(s->mo.cell);
+    dfree(s->retry_n); dfree(s->retry_idx); dfree(s->retry_gs);
     dfree(s->mlp_P); dfree(s->mlp_blob); dfree(s->train_buf); dfree(s->part); dfree(s->warm); dfree(s->vdep);
     if (s->tuner.e0) (void)hipEventDestroy(s->tuner.e0);
     if (s->tuner.e1) (void)hipEventDestroy(s->tuner.e1);
@@ -466,6 +471,7 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
         a.warm_in = (a.warm && s->warm_valid) ? 1 : 0;
         s->warm_valid = false;  // until this evaluation has completed
         a.vdep = s->vdep;
+        a.retry_n = s->retry_n; a.retry_idx = s->retry_idx; a.retry_gs = s->retry_gs;
         timed(c, "p3_map_mixed", [&] { launch_p3_map_mixed(c->stream, c->g, c->qf, c->opt.split, a); });
         const int variant = deposit_pick(s);
         if (s->tuner.pending >= 0) {
@@ -1220,6 +1226,9 @@ upic_status upic_sim_create(upic_ctx* c, int64_t np, upic_sim** out) {
             s->warm = dalloc<float>(n3);
             s->vdep = dalloc<float>(n3);
             s->Jprev = dalloc<double>(3 * (size_t)N);
+            s->retry_n = dalloc<int>(1);
+            s->retry_idx = dalloc<int>(np);
+            s->retry_gs = dalloc<float>(n3);
             s->NRf = dalloc<float>(8 * (size_t)N);
         }
         // identity ids
