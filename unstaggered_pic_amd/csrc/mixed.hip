@@ -1091,12 +1091,43 @@ __global__ __launch_bounds__(256) void k_deposit_wave_mixed(MGrid g, QuadF qd, i
         gy = wrapn(cj + by, g.ny);
         key = gx * g.ny + gy;
     }
-    // the wave's leading window: that of its first valid lane
-    const unsigned long long vm = __ballot(key >= 0);
-    const int lead = vm ? __ffsll((long long)vm) - 1 : 0;
-    const int lkey = __shfl(key, lead, 64);
-    const bool match = key >= 0 && key == lkey;
-    if (key >= 0 && !match) {
+    // Lanes whose final windows agree are reduced together: each round takes
+    // the window of the first pending lane, sums W (masked to that window's
+    // lanes) times q v / dV over the wavefront through LDS, one lane per
+    // (node, component), and adds the sums to the tile.  A wavefront of
+    // cell-ordered particles holds one or two windows; lanes still pending
+    // after kRounds rounds use per-lane atomics.
+    constexpr int kRounds = 2;
+    float* rw = red[wv] + lane * ROW;
+    rw[NW] = sv[0];
+    rw[NW + 1] = sv[1];
+    rw[NW + 2] = sv[2];
+    unsigned long long pending = __ballot(key >= 0);
+    for (int round = 0; round < kRounds && pending; ++round) {
+        const int lead = __ffsll((long long)pending) - 1;
+        const int lkey = __shfl(key, lead, 64);
+        const int lgx = __shfl(gx, lead, 64), lgy = __shfl(gy, lead, 64);
+        const unsigned long long grp = __ballot(key == lkey) & pending;
+        const bool in = (grp >> lane) & 1ull;
+#pragma unroll
+        for (int aa = 0; aa <= R; ++aa)
+#pragma unroll
+            for (int bb = 0; bb <= R; ++bb) rw[aa * (R + 1) + bb] = in ? W[aa][bb] : 0.0f;
+        wave_sync_lds();
+        // column (node, component) sums over the 64 lanes, 3 (R+1)^2 columns
+        for (int col = lane; col < 3 * NW; col += 64) {
+            const int node = col / 3, comp = col - 3 * (col / 3);
+            float acc = 0.0f;
+            for (int l = 0; l < 64; ++l) acc += red[wv][l * ROW + node] * red[wv][l * ROW + NW + comp];
+            if (acc != 0.0f) {
+                const int ia = wrap1n(lgx + node / (R + 1), g.nx), ib = wrap1n(lgy + node % (R + 1), g.ny);
+                tilef_add1(g, T, a.J_out, ia, ib, comp, acc);
+            }
+        }
+        wave_sync_lds();
+        pending &= ~grp;
+    }
+    if ((pending >> lane) & 1ull) {
 #pragma unroll
         for (int aa = 0; aa <= R; ++aa) {
             const int ia = wrap1n(gx + aa, g.nx);
@@ -1105,26 +1136,6 @@ __global__ __launch_bounds__(256) void k_deposit_wave_mixed(MGrid g, QuadF qd, i
                 const int ib = wrap1n(gy + bb, g.ny);
                 tilef_add(g, T, a.J_out, ia, ib, W[aa][bb] * sv[0], W[aa][bb] * sv[1], W[aa][bb] * sv[2]);
             }
-        }
-    }
-    float* rw = red[wv] + lane * ROW;
-#pragma unroll
-    for (int aa = 0; aa <= R; ++aa)
-#pragma unroll
-        for (int bb = 0; bb <= R; ++bb) rw[aa * (R + 1) + bb] = match ? W[aa][bb] : 0.0f;
-    rw[NW] = sv[0];
-    rw[NW + 1] = sv[1];
-    rw[NW + 2] = sv[2];
-    const int lgx = __shfl(gx, lead, 64), lgy = __shfl(gy, lead, 64);
-    wave_sync_lds();
-    // column (node, component) sums over the 64 lanes, 3 (R+1)^2 columns
-    for (int col = lane; vm && col < 3 * NW; col += 64) {
-        const int node = col / 3, comp = col - 3 * (col / 3);
-        float acc = 0.0f;
-        for (int l = 0; l < 64; ++l) acc += red[wv][l * ROW + node] * red[wv][l * ROW + NW + comp];
-        if (acc != 0.0f) {
-            const int ia = wrap1n(lgx + node / (R + 1), g.nx), ib = wrap1n(lgy + node % (R + 1), g.ny);
-            tilef_add1(g, T, a.J_out, ia, ib, comp, acc);
         }
     }
     tilef_end(g, T, a.J_out);
