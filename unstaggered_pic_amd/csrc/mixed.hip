@@ -259,23 +259,31 @@ __device__ __forceinline__ int orbit_gg_f(const MGrid& g, const QuadF& q, bool s
 #pragma unroll
                     for (int a = 0; a <= R; ++a) {
                         const int ia = wrap1n(ox + a, g.nx);
+                        // x factors of the node weights of this stencil row:
+                        // u = wq wx, du = wq dwx and their theta-weighted forms
+                        float u[4], du[4], tu[4], tdu[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            u[k] = wq[k] * wx[k][a];
+                            du[k] = wq[k] * dwx[k][a];
+                            tu[k] = th[k] * u[k];
+                            tdu[k] = th[k] * du[k];
+                        }
 #pragma unroll
                         for (int b = 0; b <= R; ++b) {
                             const int ib = wrap1n(oy + b, g.ny);
-                            float W = 0.0f, Gx0 = 0.0f, Gx1 = 0.0f, Gy0 = 0.0f, Gy1 = 0.0f;
+                            // W = sum_k u wy, S = sum_k (du wy, u dwy), G1 = theta-weighted S
+                            float W = u[0] * wy[0][b], Sx = du[0] * wy[0][b], Gx1 = tdu[0] * wy[0][b];
+                            float Sy = u[0] * dwy[0][b], Gy1 = tu[0] * dwy[0][b];
 #pragma unroll
-                            for (int k = 0; k < 4; ++k) {
-                                const float ws = wq[k] * wy[k][b];
-                                const float sx_ = dwx[k][a] * ws;
-                                const float sy_ = wx[k][a] * wq[k] * dwy[k][b];
-                                W += wx[k][a] * ws;
-                                Gx1 += th[k] * sx_;
-                                Gx0 += sx_;
-                                Gy1 += th[k] * sy_;
-                                Gy0 += sy_;
+                            for (int k = 1; k < 4; ++k) {
+                                W += u[k] * wy[k][b];
+                                Sx += du[k] * wy[k][b];
+                                Gx1 += tdu[k] * wy[k][b];
+                                Sy += u[k] * dwy[k][b];
+                                Gy1 += tu[k] * dwy[k][b];
                             }
-                            Gx0 -= Gx1;
-                            Gy0 -= Gy1;
+                            const float Gx0 = Sx - Gx1, Gy0 = Sy - Gy1;
                             float4 r0, r1;
                             rec_f(NR, ia * g.ny + ib, r0, r1);
                             g0 += W * r0.x;
@@ -591,7 +599,10 @@ __device__ __forceinline__ void move_axis(float& t, float du, int& c, int n) {
 //   delta = (h/2) q/(m gamma) (F - v (v.F)/kappa^2),  F = E + v x B,
 // and the network (mlp.hpp) adds a learned correction from the 12 features
 // (v, E, B, (v . grad) E).
-template <int R>
+// FULL = false computes only the predictor (E and B at the start point; f
+// is left untouched), for sweeps that neither run the network nor collect
+// training rows.
+template <int R, bool FULL = true>
 __device__ __forceinline__ void guess_features(const MGrid& g, const float4* __restrict__ NR, int ci, int cj, float tx,
                                                float ty, const float v[3], float q, float m, float kappa, float h,
                                                float f[12], float dphys[3]) {
@@ -612,32 +623,40 @@ __device__ __forceinline__ void guess_features(const MGrid& g, const float4* __r
             const float W = sx.w[a] * sy.w[b], Wx = sx.dw[a] * sy.w[b], Wy = sx.w[a] * sy.dw[b];
             gp0 += W * r0.x;
             gp1 += W * r0.y;
-            dg00 += Wx * r0.x; dg01 += Wy * r0.x;
-            dg10 += Wx * r0.y; dg11 += Wy * r0.y;
             const float da[3] = {r1.y - r0.z, r1.z - r0.w, r1.w - r1.x};
             const float am[3] = {0.5f * (r0.z + r1.y), 0.5f * (r0.w + r1.z), 0.5f * (r1.x + r1.w)};
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                dA[c] += W * da[c];
-                ddA[c][0] += Wx * da[c];
-                ddA[c][1] += Wy * da[c];
-                dAm[c][0] += Wx * am[c];
-                dAm[c][1] += Wy * am[c];
+            for (int c = 0; c < 3; ++c) dA[c] += W * da[c];
+            // B = curl A_mid needs d/dy of A_mid x, z and d/dx of A_mid y, z
+            dAm[0][1] += Wy * am[0];
+            dAm[1][0] += Wx * am[1];
+            dAm[2][0] += Wx * am[2];
+            dAm[2][1] += Wy * am[2];
+            if constexpr (FULL) {
+                dg00 += Wx * r0.x; dg01 += Wy * r0.x;
+                dg10 += Wx * r0.y; dg11 += Wy * r0.y;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    ddA[c][0] += Wx * da[c];
+                    ddA[c][1] += Wy * da[c];
+                }
             }
         }
     }
     const float ih = 1.0f / h;
     const float E[3] = {-gp0 - dA[0] * ih, -gp1 - dA[1] * ih, -dA[2] * ih};
     const float Bm[3] = {dAm[2][1], -dAm[2][0], dAm[1][0] - dAm[0][1]};
-    const float vgE[3] = {-(v[0] * dg00 + v[1] * dg01) - (v[0] * ddA[0][0] + v[1] * ddA[0][1]) * ih,
-                          -(v[0] * dg10 + v[1] * dg11) - (v[0] * ddA[1][0] + v[1] * ddA[1][1]) * ih,
-                          -(v[0] * ddA[2][0] + v[1] * ddA[2][1]) * ih};
+    if constexpr (FULL) {
+        const float vgE[3] = {-(v[0] * dg00 + v[1] * dg01) - (v[0] * ddA[0][0] + v[1] * ddA[0][1]) * ih,
+                              -(v[0] * dg10 + v[1] * dg11) - (v[0] * ddA[1][0] + v[1] * ddA[1][1]) * ih,
+                              -(v[0] * ddA[2][0] + v[1] * ddA[2][1]) * ih};
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        f[c] = v[c];
-        f[3 + c] = E[c];
-        f[6 + c] = Bm[c];
-        f[9 + c] = vgE[c];
+        for (int c = 0; c < 3; ++c) {
+            f[c] = v[c];
+            f[3 + c] = E[c];
+            f[6 + c] = Bm[c];
+            f[9 + c] = vgE[c];
+        }
     }
     const float ik2 = 1.0f / (kappa * kappa);
     const float v2 = (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * ik2;
@@ -804,8 +823,9 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
             const int c0 = a.cell[p];
             float v0[3];
             ld3f(a.v, a.np, p, v0);
-            guess_features<R>(g, a.NR, c0 / g.ny, c0 - (c0 / g.ny) * g.ny, a.off[p], a.off[a.np + p], v0, a.q[p],
-                              a.m[p], (float)a.kappa, (float)a.dt, feat, dph);
+            guess_features<R, GM == 2 || COLLECT>(g, a.NR, c0 / g.ny, c0 - (c0 / g.ny) * g.ny, a.off[p],
+                                                   a.off[a.np + p], v0, a.q[p], a.m[p], (float)a.kappa,
+                                                   (float)a.dt, feat, dph);
         }
         if constexpr (GM == 2) {
             mlp_wave_infer(a.mlp, feat, lg, mlp_lds + (threadIdx.x / 64) * kMlpLdsWave);
