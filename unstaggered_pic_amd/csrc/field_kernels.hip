@@ -513,24 +513,59 @@ __global__ void k_cell_keys(GridDev g, int64_t np, const double* __restrict__ x,
     }
     (void)wave_claim(counts, key, active);
 }
-// single-workgroup exclusive scan over ncell counts
-__global__ __launch_bounds__(1024) void k_scan(int ncell, const int* __restrict__ counts, int* __restrict__ offsets) {
-    __shared__ int part[1024];
+// Exclusive scan of the cell counts in three passes: workgroups of 1024
+// threads scan chunks of kScanChunk counts (4 per thread) and record chunk
+// totals, one workgroup scans the totals, and the totals are added back.
+constexpr int kScanChunk = 4096;
+__device__ __forceinline__ int block_exclusive_scan_1024(int v, int* sh, int& total) {
     const int t = threadIdx.x;
-    const int per = (ncell + 1023) / 1024;
-    const int b = t * per, e = min(ncell, b + per);
-    int s = 0;
-    for (int i = b; i < e; ++i) s += counts[i];
-    part[t] = s;
+    sh[t] = v;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {
-        const int v = (t >= o) ? part[t - o] : 0;
+        const int x = (t >= o) ? sh[t - o] : 0;
         __syncthreads();
-        part[t] += v;
+        sh[t] += x;
         __syncthreads();
     }
-    int run = (t == 0) ? 0 : part[t - 1];
-    for (int i = b; i < e; ++i) { offsets[i] = run; run += counts[i]; }
+    total = sh[1023];
+    const int r = sh[t] - v;
+    __syncthreads();
+    return r;
+}
+__global__ __launch_bounds__(1024) void k_scan_local(int ncell, const int* __restrict__ counts,
+                                                     int* __restrict__ offsets, int* __restrict__ totals) {
+    __shared__ int sh[1024];
+    const int base = blockIdx.x * kScanChunk + 4 * threadIdx.x;
+    int c[4], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        c[k] = base + k < ncell ? counts[base + k] : 0;
+        sum += c[k];
+    }
+    int total;
+    int run = block_exclusive_scan_1024(sum, sh, total);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < ncell) offsets[base + k] = run;
+        run += c[k];
+    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = total;
+}
+__global__ __launch_bounds__(1024) void k_scan_totals(int nb, int* __restrict__ totals) {
+    __shared__ int sh[1024];
+    int carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += 1024) {
+        const int i = b0 + (int)threadIdx.x;
+        const int v = i < nb ? totals[i] : 0;
+        int total;
+        const int r = block_exclusive_scan_1024(v, sh, total);
+        if (i < nb) totals[i] = carry + r;
+        carry += total;
+    }
+}
+__global__ void k_scan_add(int ncell, int* __restrict__ offsets, const int* __restrict__ totals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ncell) offsets[i] += totals[i / kScanChunk];
 }
 __global__ void k_scatter_perm(int64_t np, const int* __restrict__ keys, int* __restrict__ cursor, int* __restrict__ perm) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -552,8 +587,13 @@ __global__ void k_permute_i64(int64_t np, const int64_t* __restrict__ src, int64
 void launch_cell_keys(hipStream_t s, const GridDev& g, int64_t np, const double* x, int* keys, int* counts) {
     if (np > 0) hipLaunchKernelGGL(k_cell_keys, dim3(gblk(np)), dim3(256), 0, s, g, np, x, keys, counts);
 }
-void launch_scan_counts(hipStream_t s, int ncell, int* counts, int* offsets) {
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, ncell, counts, offsets);
+int scan_scratch_ints(int ncell) { return (ncell + kScanChunk - 1) / kScanChunk + 1; }
+void launch_scan_counts(hipStream_t s, int ncell, int* counts, int* offsets, int* scratch) {
+    if (ncell <= 0) return;
+    const int nb = (ncell + kScanChunk - 1) / kScanChunk;
+    hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(1024), 0, s, ncell, counts, offsets, scratch);
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(1024), 0, s, nb, scratch);
+    hipLaunchKernelGGL(k_scan_add, dim3((ncell + 255) / 256), dim3(256), 0, s, ncell, offsets, scratch);
 }
 void launch_scatter_perm(hipStream_t s, int64_t np, const int* keys, int* cursor, int* perm) {
     if (np > 0) hipLaunchKernelGGL(k_scatter_perm, dim3(gblk(np)), dim3(256), 0, s, np, keys, cursor, perm);

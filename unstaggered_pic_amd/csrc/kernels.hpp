@@ -261,7 +261,9 @@ void launch_kinetic(hipStream_t s, int64_t np, const double* v, const double* m,
 // Cell sort of particles (counting sort by cell of x): keys, histogram, scan,
 // permutation, and the gather of SoA arrays through it.
 void launch_cell_keys(hipStream_t s, const GridDev& g, int64_t np, const double* x, int* keys, int* counts);
-void launch_scan_counts(hipStream_t s, int ncell, int* counts, int* offsets);
+// exclusive scan of ncell counts; scratch holds scan_scratch_ints(ncell) ints
+int scan_scratch_ints(int ncell);
+void launch_scan_counts(hipStream_t s, int ncell, int* counts, int* offsets, int* scratch);
 void launch_scatter_perm(hipStream_t s, int64_t np, const int* keys, int* cursor, int* perm);
 void launch_permute(hipStream_t s, int64_t np, int ncomp, const double* src, double* dst, const int* perm);
 void launch_permute_i64(hipStream_t s, int64_t np, const int64_t* src, int64_t* dst, const int* perm);

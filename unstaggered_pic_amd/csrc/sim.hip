@@ -38,7 +38,7 @@ struct upic_sim {
     double* dX[8] = {};
     double* dR[8] = {};
     // cell sort
-    int *keys = nullptr, *counts = nullptr, *offsets = nullptr, *perm = nullptr;
+    int *keys = nullptr, *counts = nullptr, *offsets = nullptr, *perm = nullptr, *scan_tmp = nullptr;
     double* ptmp = nullptr;   // 3*np
     double* ptmp1 = nullptr;  // np
     int64_t* idtmp = nullptr;
@@ -119,7 +119,7 @@ void sim_free(upic_sim* s) {
     for (int i = 0; i < 8; ++i) { dfree(s->dX[i]); dfree(s->dR[i]); }
     for (int i = 0; i < 5; ++i) { dfree(s->phi_hist[i]); dfree(s->A_hist[i]); dfree(s->rho_hist[i]); }
     dfree(s->id); dfree(s->idtmp); dfree(s->reason); dfree(s->layout);
-    dfree(s->keys); dfree(s->counts); dfree(s->offsets); dfree(s->perm);
+    dfree(s->keys); dfree(s->counts); dfree(s->offsets); dfree(s->perm); dfree(s->scan_tmp);
     float** fl[] = {&s->ms.off, &s->ms.P, &s->ms.v, &s->ms.vp, &s->ms.vb, &s->ms.q, &s->ms.m,
                     &s->mo.off, &s->mo.P, &s->mo.v, &s->mo.vb, &s->ftmp, &s->ftmp1, &s->ftmp2, &s->NRf};
     for (float** p : fl) dfree(*p);
@@ -285,7 +285,7 @@ bool sort_particles(upic_sim* s) {
         const int nkeys = split ? 2 * ncell : ncell;
         HIPCHK(hipMemsetAsync(s->counts, 0, nkeys * sizeof(int), c->stream));
         launch_mixed_keys(c->stream, c->g, np, s->ms, c->p.dt, split ? 1 : 0, s->keys, s->counts);
-        launch_scan_counts(c->stream, nkeys, s->counts, s->offsets);
+        launch_scan_counts(c->stream, nkeys, s->counts, s->offsets, s->scan_tmp);
         launch_scatter_perm(c->stream, np, s->keys, s->offsets, s->perm);
         // the map's output arrays (free between steps) and three scratch
         // arrays receive the permuted state in one pass
@@ -307,7 +307,7 @@ bool sort_particles(upic_sim* s) {
     }
     HIPCHK(hipMemsetAsync(s->counts, 0, ncell * sizeof(int), c->stream));
     launch_cell_keys(c->stream, c->g, np, s->x, s->keys, s->counts);
-    launch_scan_counts(c->stream, ncell, s->counts, s->offsets);
+    launch_scan_counts(c->stream, ncell, s->counts, s->offsets, s->scan_tmp);
     launch_scatter_perm(c->stream, np, s->keys, s->offsets, s->perm);
     double** arr3[] = {&s->x, &s->v, &s->v_prev, &s->P, &s->vbar_last};
     for (double** a : arr3) {
@@ -1181,6 +1181,7 @@ upic_status upic_sim_create(upic_ctx* c, int64_t np, upic_sim** out) {
         // their knot interval first, crossing orbits after)
         const int nkeys = c->p.precision == UPIC_PREC_MIXED ? 2 * N : N;
         s->counts = dalloc<int>(nkeys); s->offsets = dalloc<int>(nkeys);
+        s->scan_tmp = dalloc<int>(scan_scratch_ints(nkeys));
         if (c->p.code == 1) {
             s->vg[0] = dalloc<double>(n3);
             s->vg[1] = dalloc<double>(n3);
