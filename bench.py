@@ -109,6 +109,17 @@ def cpu_baseline(nx, dt_budget=30.0):
                       f"serial FP64, {el:.1f} s"}
 
 
+def measured_traffic(args, form):
+    """HBM bytes per launch of the dominant kernel as measured by rocprofv3
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated) for this workload and form,
+    kept in profiles/; None for other workloads."""
+    path = os.path.join(ROOT, "profiles", f"traffic_weibel{args.nx}_ppc{args.ppc}.json")
+    if args.nx != 256 or args.ppc != 256 or not os.path.exists(path):
+        return None
+    d = json.load(open(path))["forms"].get(form)
+    return d["bytes_per_launch"] if d else None
+
+
 def run_form(args, prec, learned):
     """Load, initialise and time one form of the step; returns a dict."""
     from unstaggered_pic_amd import lib
@@ -212,14 +223,16 @@ def main():
     # algorithmic HBM bytes of one map launch.  FP64: reads x, P, v (72 B),
     # q, m (16 B) per particle, writes x, P, v, vbar_macro (96 B) once per
     # step, reads A_n, A_end, grad phi_mid (9 N doubles), writes J (3 N).
-    # Mixed: reads cell (4 B), offsets, P, v (3 x 12 B), q, m (8 B); writes
-    # cell, offsets, P, v, vbar_macro (52 B) once per step; reads the FP32
-    # node records (32 N B), writes the FP64 current (24 N B).
+    # Mixed: reads cell (4 B), offsets, P, v (3 x 12 B), q, m (8 B) and, with
+    # the learned guess, the cached network correction (12 B, written instead
+    # by the first sweep of a step); every sweep writes the end state cell,
+    # offsets, P, v, vbar_macro (52 B) and the converged vbar for the deposit
+    # (12 B); reads the FP32 node records (32 N B).
     maps = sum(i.map_evaluations for i in infos)
     if prec == 0:
         bytes_per_launch = np_ * 88.0 + (np_ * 96.0 * args.steps) / max(maps, 1) + 12.0 * 8 * N
     else:
-        bytes_per_launch = np_ * 48.0 + (np_ * 52.0 * args.steps) / max(maps, 1) + (32.0 + 24.0) * N
+        bytes_per_launch = np_ * (48.0 + (12.0 if learned else 0.0) + 64.0) + 32.0 * N
     avg_ms = kms / max(launches, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if launches else None
     # operations of one inner iteration at r = 3, nq = 4 (counted from the
@@ -258,7 +271,8 @@ def main():
                    "form": form_name[(prec, learned)],
                    "solver": "P3 nested solve (Anderson outer, per-particle Picard inner)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": measured_traffic(args, form_name[(prec, learned)]),
                      "kernel": "k_p3_map" if prec == 0 else "k_p3_map_mixed", "avg_launch_ms": avg_ms,
                      "launches": launches, "bytes_per_launch": bytes_per_launch,
                      "deposit_kernel_avg_ms": (r["dep"][1] / r["dep"][0]) if r["dep"][0] else None,

@@ -2,8 +2,9 @@
 FETCH_SIZE / WRITE_SIZE, --output-format csv) into the markdown tables kept
 with the sources.
 
-  python -m unstaggered_pic_amd.profile_summary --trace DIR [--fetch DIR]
-         [--write DIR] [--copy-mib 1024] [--kernel SUBSTRING] --command "..."
+  python -m unstaggered_pic_amd.profile_summary --trace DIR [--after NAME]
+         [--steps K] [--fetch DIR] [--write DIR] [--copy-mib 1024]
+         [--kernel SUBSTRING[,SUBSTRING]] [--json-out FILE] --command "..."
 
 HBM traffic per dispatch = FETCH_SIZE x (copy bytes / FETCH_SIZE of the
 calibration copy) + WRITE_SIZE x (copy bytes / WRITE_SIZE of the copy): the
@@ -38,9 +39,28 @@ def counters(dirname):
     return agg
 
 
+def trace_stats(dirname, after):
+    """Per-kernel (calls, total ns) over the dispatches that start after the
+    last dispatch whose name contains `after`, from the kernel trace."""
+    rows = list(csv.DictReader(open(_one(dirname, "*kernel_trace.csv"))))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    marks = [i for i, r in enumerate(rows) if after in r["Kernel_Name"]]
+    tail = rows[marks[-1] + 1:] if marks else rows
+    agg = collections.defaultdict(lambda: [0, 0])
+    for r in tail:
+        agg[r["Kernel_Name"]][0] += 1
+        agg[r["Kernel_Name"]][1] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    span = int(tail[-1]["End_Timestamp"]) - int(tail[0]["Start_Timestamp"]) if tail else 0
+    return agg, span
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--trace", required=True)
+    ap.add_argument("--after", default=None,
+                    help="summarise only the dispatches after the last one whose name contains this")
+    ap.add_argument("--steps", type=int, default=0, help="time steps in the summarised region (per-step totals)")
+    ap.add_argument("--json-out", default=None, help="write the per-dispatch HBM traffic of the kernels as JSON")
     ap.add_argument("--fetch")
     ap.add_argument("--write")
     ap.add_argument("--copy-mib", type=int, default=1024)
@@ -48,13 +68,27 @@ def main():
     ap.add_argument("--command", default="")
     ap.add_argument("--top", type=int, default=12)
     a = ap.parse_args()
-    rows = list(csv.DictReader(open(_one(a.trace, "*kernel_stats.csv"))))
-    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
-    print("| kernel | calls | total ms | average us | % |")
-    print("|---|---|---|---|---|")
-    for r in rows[:a.top]:
-        print(f"| `{short(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
-              f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+    if a.after:
+        agg, span = trace_stats(a.trace, a.after)
+        busy = sum(v[1] for v in agg.values())
+        print(f"Dispatches after the last `{a.after}`: GPU busy {busy / 1e6:.2f} ms over a span of {span / 1e6:.2f} ms"
+              + (f" ({a.steps} steps: {busy / 1e6 / a.steps:.2f} ms busy per step)" if a.steps else "") + ".")
+        print()
+        print("| kernel | calls | total ms | average us | % of busy |" + (" ms per step |" if a.steps else ""))
+        print("|---|---|---|---|---|" + ("---|" if a.steps else ""))
+        for k, (n, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:a.top]:
+            line = f"| `{short(k)}` | {n} | {t / 1e6:.2f} | {t / n / 1e3:.1f} | {100.0 * t / busy:.1f} |"
+            if a.steps:
+                line += f" {t / 1e6 / a.steps:.3f} |"
+            print(line)
+    else:
+        rows = list(csv.DictReader(open(_one(a.trace, "*kernel_stats.csv"))))
+        rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+        print("| kernel | calls | total ms | average us | % |")
+        print("|---|---|---|---|---|")
+        for r in rows[:a.top]:
+            print(f"| `{short(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
+                  f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
     if a.fetch and a.write:
         copy = a.copy_mib * 1024.0  # KiB
         f, w = counters(a.fetch), counters(a.write)
@@ -67,13 +101,19 @@ def main():
         print()
         print("| kernel | dispatches | fetch MB/dispatch | write MB/dispatch | HBM MB/dispatch |")
         print("|---|---|---|---|---|")
+        out = {"copy_factors": {"fetch": sf, "write": sw}, "kernels": {}}
         for k in sorted(f):
-            if a.kernel not in k or k not in w:
+            if not any(x in k for x in a.kernel.split(",")) or k not in w:
                 continue
             fm = sum(f[k]) / len(f[k]) * sf * 1024 / 1e6
             wm = sum(w[k]) / len(w[k]) * sw * 1024 / 1e6
             print(f"| `{short(k)}` | {len(f[k])} | "
                   f"{fm:.1f} | {wm:.1f} | {fm + wm:.1f} |")
+            out["kernels"][short(k)] = {"dispatches": len(f[k]), "fetch_bytes": fm * 1e6, "write_bytes": wm * 1e6}
+        if a.json_out:
+            import json
+            with open(a.json_out, "w") as fh:
+                json.dump(out, fh, indent=1)
     if a.command:
         print()
         print("Command: `" + a.command + "`")
