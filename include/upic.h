@@ -183,6 +183,12 @@ const char* upic_last_error(const upic_ctx* ctx);
  * and milliseconds since the last upic_ctx_set_timing call. */
 upic_status upic_ctx_set_timing(upic_ctx* ctx, int enable);
 upic_status upic_ctx_timing(upic_ctx* ctx, const char* name, int64_t* launches, double* total_ms);
+/* No reference counterpart: replay the field solve of every map evaluation
+ * (forward FFTs of the current, the k-space solve, inverse FFTs, node
+ * record packing -- about a dozen kernels on fixed buffers) from a hipGraph
+ * captured at its first use per simulation and source buffer.  Off by
+ * default; results are identical to the eager launches. */
+upic_status upic_ctx_set_graphs(upic_ctx* ctx, int enable);
 /* Gauss-Legendre nodes/weights on [0,1] used by the context (host arrays of
  * orbit_quad_order entries). */
 upic_status upic_quadrature(const upic_ctx* ctx, double* s, double* w);
@@ -267,6 +273,16 @@ upic_status upic_step_p1(upic_sim* sim, upic_step_info* info);
 upic_status upic_step_p3(upic_sim* sim, upic_step_info* info);
 /* P2 one iteration of run_simulation's step loop (P2:1133-1189). */
 upic_status upic_step_p2(upic_sim* sim, upic_step_info* info);
+/* No reference counterpart (the reference runs one simulation per process):
+ * one step of each of n independent simulations -- any codes, decks and
+ * sizes, each on its own context (its own stream) -- advanced concurrently
+ * by min(n, threads) host threads (threads <= 0: 16), so that small members
+ * fill the GPU together.  infos (n entries) and status (n entries) may be
+ * NULL.  Returns UPIC_OK when every member stepped, else the first member
+ * status that is not UPIC_OK (each member's error text is on its context);
+ * UPIC_EINVAL when two members share a context. */
+upic_status upic_ensemble_step(upic_sim* const* sims, int32_t n, int32_t threads, upic_step_info* infos,
+                               upic_status* status);
 /* compute_diagnostics (P1:1057-1087, P3:3521-3574, P2:898-962) on the current
  * state; mode = perturbation_mode. */
 upic_status upic_sim_diagnostics(upic_sim* sim, int mode, upic_diagnostics* out);

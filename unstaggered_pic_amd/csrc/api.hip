@@ -97,6 +97,12 @@ upic_status upic_ctx_set_timing(upic_ctx* c, int enable) {
     return UPIC_OK;
 }
 
+upic_status upic_ctx_set_graphs(upic_ctx* c, int enable) {
+    if (!c) return UPIC_EINVAL;
+    c->graphs = enable != 0;
+    return UPIC_OK;
+}
+
 upic_status upic_ctx_timing(upic_ctx* c, const char* name, int64_t* launches, double* total_ms) {
     if (!c || !name || !launches || !total_ms) return UPIC_EINVAL;
     auto it = c->timers.find(name);

@@ -150,6 +150,8 @@ struct upic_ctx {
     struct KTimer { int64_t launches = 0; double ms = 0.0; };
     bool timing = false;
     std::map<std::string, KTimer> timers;
+    // replay the field solve of a map evaluation from a captured hipGraph
+    bool graphs = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     int N() const { return g.N; }

@@ -4,6 +4,8 @@
 //                                   5878-6215, 5235-5431)
 //   P2 run_simulation step body    (Project2/spectral_pic_part3.cpp:1133-1189)
 // and the per-step diagnostics (P1:1057-1087, P3:3521-3574).
+#include <atomic>
+#include <thread>
 #include "api_internal.hpp"
 #include "mlp.hpp"
 
@@ -98,6 +100,8 @@ struct upic_sim {
         int pending = -1, pending_order = 0;
         hipEvent_t e0 = nullptr, e1 = nullptr;
     } tuner;
+    // captured field solves, one per source-current buffer
+    std::vector<std::pair<const double*, hipGraphExec_t>> fs_graphs;
 };
 
 // Mixed precision: an FP32 map resolves each node current only to a fraction
@@ -126,6 +130,8 @@ void sim_free(upic_sim* s) {
     dfree(s->ms.cell); dfree(s->mo.cell);
     dfree(s->retry_n); dfree(s->retry_idx); dfree(s->retry_gs);
     dfree(s->mlp_P); dfree(s->mlp_blob); dfree(s->train_buf); dfree(s->part); dfree(s->warm); dfree(s->vdep);
+    for (auto& gx : s->fs_graphs) (void)hipGraphExecDestroy(gx.second);
+    s->fs_graphs.clear();
     if (s->tuner.e0) (void)hipEventDestroy(s->tuner.e0);
     if (s->tuner.e1) (void)hipEventDestroy(s->tuner.e1);
     s->tuner.e0 = s->tuner.e1 = nullptr;
@@ -205,7 +211,7 @@ void state_spectra(upic_sim* s) {
 // Field solve of one map evaluation from a real current (3N): continuity,
 // CN potentials, grad phi_mid.  Leaves the end-of-step spectra in S1 and the
 // real A_end and grad phi_mid in AG.
-void field_solve(upic_sim* s, const double* Jreal) {
+void field_solve_body(upic_sim* s, const double* Jreal) {
     upic_ctx* c = s->c;
     const int N = c->N();
     to_spectra(c, Jreal, Zf(s->S1, 0, N), 3);
@@ -234,6 +240,29 @@ void field_solve(upic_sim* s, const double* Jreal) {
     from_spectra(c, s->zs, s->AG, 6);
     if (s->mixed) launch_pack_nodes_f32(c->stream, N, s->AG + 3 * (size_t)N, s->A, s->AG, s->NRf);
     else if (c->g.nz == 1) launch_pack_nodes(c->stream, N, s->AG + 3 * (size_t)N, s->A, s->AG, s->NR);
+}
+
+// The field solve of a map evaluation: a fixed sequence of about a dozen
+// small kernels on fixed buffers.  With graphs on, it is captured once per
+// source buffer and replayed with one launch (it holds no memset nodes).
+void field_solve(upic_sim* s, const double* Jreal) {
+    upic_ctx* c = s->c;
+    if (!c->graphs) { field_solve_body(s, Jreal); return; }
+    for (auto& gx : s->fs_graphs)
+        if (gx.first == Jreal) {
+            HIPCHK(hipGraphLaunch(gx.second, c->stream));
+            return;
+        }
+    hipGraph_t gph = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    field_solve_body(s, Jreal);
+    HIPCHK(hipStreamEndCapture(c->stream, &gph));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, gph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gph);
+    HIPCHK(e);
+    s->fs_graphs.emplace_back(Jreal, ex);
+    HIPCHK(hipGraphLaunch(ex, c->stream));
 }
 
 // End-of-step real mesh state from the last field solve.
@@ -1336,6 +1365,42 @@ upic_status upic_step_p2(upic_sim* s, upic_step_info* info) {
     if (!s) return UPIC_EINVAL;
     if (s->c->p.code != 2) { s->c->err = "upic_step_p2 needs a P2 context"; return UPIC_EINVAL; }
     return (upic_status)guarded(s->c, [&] { p2_step(s, info); });
+}
+
+upic_status upic_ensemble_step(upic_sim* const* sims, int32_t n, int32_t threads, upic_step_info* infos,
+                               upic_status* status) {
+    if (n < 0 || (n > 0 && !sims)) return UPIC_EINVAL;
+    for (int32_t i = 0; i < n; ++i) {
+        if (!sims[i]) return UPIC_EINVAL;
+        for (int32_t j = 0; j < i; ++j)
+            if (sims[j]->c == sims[i]->c) {
+                sims[i]->c->err = "ensemble members need distinct contexts (each context has one stream)";
+                return UPIC_EINVAL;
+            }
+    }
+    std::atomic<int32_t> next{0};
+    std::vector<upic_status> rc((size_t)n, UPIC_OK);
+    auto work = [&] {
+        for (;;) {
+            const int32_t i = next.fetch_add(1);
+            if (i >= n) break;
+            upic_sim* m = sims[i];
+            upic_step_info* inf = infos ? infos + i : nullptr;
+            const int code = m->c->p.code;
+            rc[(size_t)i] = code == 1 ? upic_step_p1(m, inf) : code == 2 ? upic_step_p2(m, inf) : upic_step_p3(m, inf);
+        }
+    };
+    const int32_t nt = std::max<int32_t>(1, std::min<int32_t>(n, threads > 0 ? threads : 16));
+    std::vector<std::thread> pool;
+    for (int32_t t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    upic_status first = UPIC_OK;
+    for (int32_t i = 0; i < n; ++i) {
+        if (status) status[i] = rc[(size_t)i];
+        if (first == UPIC_OK && rc[(size_t)i] != UPIC_OK) first = rc[(size_t)i];
+    }
+    return first;
 }
 
 upic_status upic_sim_init_fields(upic_sim* s, double n0) {

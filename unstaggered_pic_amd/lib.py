@@ -73,6 +73,8 @@ def lib():
             "upic_sim_train_guess": [vp, C.c_int32, C.c_int32, C.c_double, C.c_uint64, dp],
             "upic_sim_set_guess_mode": [vp, C.c_int32],
             "upic_sim_set_deposit_variant": [vp, C.c_int32],
+            "upic_ctx_set_graphs": [vp, C.c_int],
+            "upic_ensemble_step": [vp, C.c_int32, C.c_int32, vp, vp],
             "upic_sim_deposit_variant": [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
             "upic_sim_guess_params": [vp, vp], "upic_sim_set_guess_params": [vp, vp],
             "upic_mlp_forward": [vp, vp, vp, C.c_int64, vp],
@@ -163,6 +165,10 @@ class Context:
 
     def set_timing(self, on=True):
         self._check(lib().upic_ctx_set_timing(self.h, int(on)))
+
+    def set_graphs(self, on=True):
+        """Replay each map evaluation's field solve from a captured hipGraph."""
+        self._check(lib().upic_ctx_set_graphs(self.h, int(on)))
 
     def timing(self, name):
         n, ms = C.c_int64(), C.c_double()
@@ -415,3 +421,20 @@ def load_particles(params: Params, spec, np_):
     if rc != 0:
         raise UpicError(rc, "upic_load_particles rejected its arguments")
     return x, v, q, m
+
+
+def ensemble_step(sims, threads=0):
+    """One step of every simulation in `sims` (each on its own Context),
+    advanced concurrently; returns the list of StepInfo.  Raises with the
+    first failing member's error."""
+    n = len(sims)
+    arr = (C.c_void_p * n)(*[m.h for m in sims])
+    infos = (StepInfo * n)()
+    status = (C.c_int * n)()
+    rc = lib().upic_ensemble_step(arr, n, int(threads), infos, status)
+    if rc != 0:
+        for m, st in zip(sims, status):
+            if st != 0:
+                m.ctx._check(st)
+        raise RuntimeError(f"upic_ensemble_step failed ({rc})")
+    return list(infos)
