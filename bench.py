@@ -182,7 +182,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nx", type=int, default=256)
     ap.add_argument("--ppc", type=int, default=256)
-    ap.add_argument("--form", choices=["both", "fp64", "mixed", "learned"], default="both",
+    ap.add_argument("--members", type=int, default=16, help="ensemble members (--form ensemble)")
+    ap.add_argument("--form", choices=["both", "fp64", "mixed", "learned", "ensemble"], default="both",
                     help="both: plain FP64 and mixed + learned guess; value is the last")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default=None, help="alias of --form fp64|mixed")
     ap.add_argument("--collect-steps", type=int, default=40)
@@ -206,6 +207,10 @@ def main():
         if lib.lib().upic_memcpy_dtod(b.p, a.p, 8 * n) != 0:
             raise RuntimeError("device copy failed")
         del a, b
+    if args.form == "ensemble":
+        from unstaggered_pic_amd import ensemble_bench
+        ensemble_bench.run(args)
+        return
     forms = {"both": [(0, False), (1, True)], "fp64": [(0, False)], "mixed": [(1, False)],
              "learned": [(1, True)]}[args.form]
     runs = []

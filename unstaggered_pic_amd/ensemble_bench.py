@@ -1,0 +1,122 @@
+"""Batched-ensemble benchmark: independent P3 runs of uneven sizes, drawn
+from a seed (grid side in {64, 128, 256}, particles per cell in {16, 64,
+256}, thermal speed in [0.02, 0.2] c, perturbation amplitude in [1e-3,
+1e-2]; Landau loads with quadratic splines at dt = 0.1 and Weibel loads with
+cubic splines at dt = 0.05), mixed precision with the physics-predictor
+guess, each on its own context with graph-replayed field solves.
+
+The same members are timed twice from the same state: stepped one after the
+other (one member on the GPU at a time), and advanced together by
+upic_ensemble_step.  Prints one JSON line: aggregate particle-steps/s of
+both, their ratio, and the largest relative total-energy change of any
+member over the timed steps.
+
+  python bench.py --form ensemble [--members 16] [--steps 5] [--warmup 2]
+"""
+import json
+import time
+
+import numpy as np
+
+
+def member_specs(n, seed=2024):
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(n):
+        specs.append({"kind": "landau" if i % 2 == 0 else "weibel", "side": int(rng.choice([64, 128, 256])),
+                      "ppc": int(rng.choice([16, 64, 256])), "vth": float(rng.uniform(0.02, 0.2)),
+                      "amp": float(rng.uniform(1e-3, 1e-2)), "seed": int(rng.integers(1, 2 ** 31))})
+    return specs
+
+
+def build(lib, spec):
+    from unstaggered_pic_amd.abi import (CODE_P3, LOAD_BIMAXWELL, LOAD_LANDAU, make_load, make_params,
+                                         with_mixed_precision)
+    s = spec["side"]
+    if spec["kind"] == "landau":
+        L = 4.0 * np.pi * s / 64.0
+        p = make_params(code=CODE_P3, nx=s, ny=s, nz=1, Lx=L, Ly=L, Lz=1.0, dt=0.1, spline_order=2,
+                        orbit_quad_order=3, nyquist_projection=1, nonlinear_rtol=1e-11, nonlinear_atol=1e-13,
+                        nonlinear_max_iter=80, anderson_damping=0.5, local_rtol=1e-12, local_atol=1e-14,
+                        local_max_iter=64, sort_interval=1)
+        load = make_load(LOAD_LANDAU, spec["ppc"], seed=spec["seed"], alpha=spec["amp"],
+                         vth=(spec["vth"],) * 3)
+    else:
+        L = 20.0 * np.pi * s / 256.0
+        p = make_params(code=CODE_P3, nx=s, ny=s, nz=1, Lx=L, Ly=L, Lz=1.0, dt=0.05, spline_order=3,
+                        orbit_quad_order=4, nyquist_projection=1, nonlinear_rtol=1e-11, nonlinear_atol=1e-13,
+                        nonlinear_max_iter=80, anderson_damping=0.5, local_rtol=1e-12, local_atol=1e-14,
+                        local_max_iter=64, sort_interval=1)
+        load = make_load(LOAD_BIMAXWELL, spec["ppc"], seed=spec["seed"],
+                         vth=(0.25 * spec["vth"], spec["vth"], spec["vth"]))
+    p = with_mixed_precision(p)
+    p.learned_guess = 1
+    n = s * s * spec["ppc"]
+    x, v, q, m = lib.load_particles(p, load, n)
+    return p, n, (x, v, q, m)
+
+
+class _Host:
+    def __init__(self, n, N):
+        self.x, self.v = np.zeros(3 * n), np.zeros(3 * n)
+        self.v_prev, self.P, self.vbar_last = np.zeros(3 * n), np.zeros(3 * n), np.zeros(3 * n)
+        self.q, self.m = np.zeros(n), np.zeros(n)
+        self.phi, self.psi, self.rho = np.zeros(N), np.zeros(N), np.zeros(N)
+        self.A, self.U, self.J = np.zeros(3 * N), np.zeros(3 * N), np.zeros(3 * N)
+
+
+def run(args):
+    from unstaggered_pic_amd import lib
+    specs = member_specs(args.members)
+    loads = [build(lib, sp) for sp in specs]
+    total = sum(n for _, n, _ in loads)
+
+    def fresh():
+        ms = []
+        for p, n, (x, v, q, m) in loads:
+            ctx = lib.Context(p)
+            ctx.set_graphs(True)
+            sim = lib.Simulation(ctx, n)
+            h = _Host(n, p.nx * p.ny)
+            h.x[:], h.v[:], h.q[:], h.m[:] = x, v, q, m
+            sim.upload(h)
+            sim.init_fields(1.0)
+            ms.append((ctx, sim))
+        return ms
+
+    out = {}
+    for mode in ("sequential", "ensemble"):
+        ms = fresh()
+        sims = [s for _, s in ms]
+        for _ in range(args.warmup):
+            lib.ensemble_step(sims, threads=16) if mode == "ensemble" else [s.step() for s in sims]
+        e0 = [s.diagnostics(1).total_energy for s in sims]
+        for c, _ in ms:
+            c.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            if mode == "ensemble":
+                lib.ensemble_step(sims, threads=16)
+            else:
+                for s in sims:
+                    s.step()
+        for c, _ in ms:
+            c.sync()
+        el = time.perf_counter() - t0
+        drift = max(abs(s.diagnostics(1).total_energy - a) / abs(a) for s, a in zip(sims, e0))
+        out[mode] = {"value": total * args.steps / el, "ms_per_step": 1e3 * el / args.steps, "max_rel_energy_change": drift}
+        for c, s in ms:
+            s.close()
+            c.close()
+    line = {
+        "metric": "aggregate particle-steps/s of a batched ensemble of independent P3 runs on 1 MI355X",
+        "value": out["ensemble"]["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": out["ensemble"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64",
+        "data": "synthetic (seeded Landau and bi-Maxwellian loads)",
+        "config": {"workload": f"ensemble_{args.members}_members_p3_mixed_predictor", "particles": total,
+                   "members": specs},
+        "sequential": out["sequential"], "ensemble_vs_sequential": out["ensemble"]["value"] / out["sequential"]["value"],
+        "max_rel_energy_change": out["ensemble"]["max_rel_energy_change"],
+    }
+    print(json.dumps(line))
