@@ -131,7 +131,10 @@ def run_form(args, prec, learned):
     sim = lib.Simulation(ctx, np_)
     h = HostArrays(np_, N)
     h.x[:], h.v[:], h.q[:], h.m[:] = x, v, q, m
-    sim.upload(h)
+    t0 = time.perf_counter()
+    sim.upload(h)  # host arrays -> HBM (pageable host memory)
+    ctx.sync()
+    t_upload = time.perf_counter() - t0
     del h, x, v, q, m
     sim.init_fields(1.0)
     d0 = sim.diagnostics(1)
@@ -172,7 +175,8 @@ def run_form(args, prec, learned):
     sim.close()
     ctx.close()
     return {"p": p, "np": np_, "N": N, "elapsed": el, "infos": infos, "launches": launches, "kms": kms, "dep": dep,
-            "kname": kname, "d0": d0, "d_start": d_start, "d1": d1, "extra": extra, "dep_choice": dep_choice}
+            "kname": kname, "d0": d0, "d_start": d_start, "d1": d1, "extra": extra, "dep_choice": dep_choice,
+            "upload_seconds": t_upload}
 
 
 def main():
@@ -289,6 +293,8 @@ def main():
         "physics": phys(r),
     }
     out["physics"]["avg_path_segments"] = float(segs)
+    # the state upload over PCIe (not part of `value`: inputs resident in HBM)
+    out["host_upload_seconds"] = r["upload_seconds"]
     if r["extra"]:
         out["learned_guess"] = r["extra"]
     if len(runs) > 1:
