@@ -41,13 +41,28 @@ FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector, vendor figure
 FP32_PEAK_TFLOPS = 157.3       # MI355X FP32 vector, vendor figure
 
 
-def workload_params(nx=256, ppc=256, precision=0):
-    """The Weibel deck; precision 1 selects the mixed mode with its FP32
-    tolerance floors (inner relative tolerance 1e-6, outer 3e-6)."""
+WORKLOADS = {
+    # name: (grid side, particles per cell) defaults
+    "weibel": (256, 256),
+    "beams": (1024, 64),
+    "landau": (64, 100),
+}
+
+
+def workload_params(nx=256, ppc=256, precision=0, workload="weibel"):
+    """The P3 decks: Weibel (the headline: [0, 20 pi)^2, cubic, dt 0.05),
+    counter-streaming beams ([0, 40 pi)^2, quadratic, dt 0.1) and Landau
+    damping ([0, 4 pi)^2, quadratic, dt 0.1); precision 1 selects the mixed
+    mode with its FP32 tolerance floors (inner relative 1e-6, outer 3e-6)."""
     from unstaggered_pic_amd.abi import CODE_P3, make_params, with_mixed_precision
-    L = 20.0 * np.pi
-    p = make_params(code=CODE_P3, nx=nx, ny=nx, nz=1, Lx=L, Ly=L, Lz=1.0, dt=0.05, spline_order=3,
-                       orbit_quad_order=4, split_orbit_at_knots=1, nyquist_projection=1, c=1.0, sigma1=1.0,
+    if workload == "weibel":
+        L, order, nq, dt = 20.0 * np.pi, 3, 4, 0.05
+    elif workload == "beams":
+        L, order, nq, dt = 40.0 * np.pi, 2, 3, 0.1
+    else:
+        L, order, nq, dt = 4.0 * np.pi, 2, 3, 0.1
+    p = make_params(code=CODE_P3, nx=nx, ny=nx, nz=1, Lx=L, Ly=L, Lz=1.0, dt=dt, spline_order=order,
+                       orbit_quad_order=nq, split_orbit_at_knots=1, nyquist_projection=1, c=1.0, sigma1=1.0,
                        sigma2=1.0, nonlinear_solver_picard=0, nonlinear_max_iter=80, nonlinear_rtol=1e-11,
                        nonlinear_atol=1e-13, anderson_memory=4, anderson_start=2, anderson_damping=0.5,
                        anderson_regularization=1e-12, anderson_growth_limit=1.2, local_rtol=1e-12,
@@ -56,9 +71,21 @@ def workload_params(nx=256, ppc=256, precision=0):
     return with_mixed_precision(p) if precision == 1 else p
 
 
-def workload_load(ppc, seed=12345):
-    from unstaggered_pic_amd.abi import LOAD_BIMAXWELL, make_load
+def workload_load(ppc, seed=12345, workload="weibel"):
+    from unstaggered_pic_amd.abi import LOAD_BEAMS, LOAD_BIMAXWELL, LOAD_LANDAU, make_load
+    if workload == "beams":
+        return make_load(LOAD_BEAMS, ppc, seed=seed, n0=1.0, v0=0.2, noise=1e-3, vth=(0.01, 0.01, 0.01))
+    if workload == "landau":
+        return make_load(LOAD_LANDAU, ppc, seed=seed, n0=1.0, alpha=0.01, mode=1, vth=(0.05, 0.05, 0.05))
     return make_load(LOAD_BIMAXWELL, ppc, seed=seed, n0=1.0, vth=(0.05, 0.2, 0.2))
+
+
+def workload_name(args):
+    if args.workload == "weibel":
+        return f"weibel_p3_{args.nx}x{args.nx}_ppc{args.ppc}_cubic_dt0.05"
+    if args.workload == "beams":
+        return f"beams_p3_{args.nx}x{args.nx}_ppc{args.ppc}_quadratic_dt0.1"
+    return f"landau_p3_{args.nx}x{args.nx}_ppc{args.ppc}_quadratic_dt0.1"
 
 
 class HostArrays:
@@ -70,7 +97,7 @@ class HostArrays:
         self.A, self.U, self.J = np.zeros(3 * N), np.zeros(3 * N), np.zeros(3 * N)
 
 
-def cpu_baseline(nx, dt_budget=30.0):
+def cpu_baseline(nx, dt_budget=30.0, workload="weibel"):
     """The reference's own FP64 step (compiled from its sources into
     oracle/_ref/libref_p3.so) -- or, where that is absent, the oracle port --
     timed on the host for one step of the same deck at 1 particle per cell."""
@@ -78,9 +105,9 @@ def cpu_baseline(nx, dt_budget=30.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_bind import Oracle, State, ptr
     ppc = 1
-    p = workload_params(nx=nx, ppc=ppc)
+    p = workload_params(nx=nx, ppc=ppc, workload=workload)
     np_ = nx * nx * ppc
-    x, v, q, m = lib.load_particles(p, workload_load(ppc), np_)
+    x, v, q, m = lib.load_particles(p, workload_load(ppc, workload=workload), np_)
     orc = Oracle()
     st = orc.init_fields(p, x, v, q, m, 1.0)
     refso = os.path.join(ROOT, "oracle", "_ref", "libref_p3.so")
@@ -114,7 +141,7 @@ def measured_traffic(args, form):
     (FETCH_SIZE + WRITE_SIZE, copy-calibrated) for this workload and form,
     kept in profiles/; None for other workloads."""
     path = os.path.join(ROOT, "profiles", f"traffic_weibel{args.nx}_ppc{args.ppc}.json")
-    if args.nx != 256 or args.ppc != 256 or not os.path.exists(path):
+    if args.workload != "weibel" or args.nx != 256 or args.ppc != 256 or not os.path.exists(path):
         return None
     d = json.load(open(path))["forms"].get(form)
     return d["bytes_per_launch"] if d else None
@@ -123,10 +150,10 @@ def measured_traffic(args, form):
 def run_form(args, prec, learned):
     """Load, initialise and time one form of the step; returns a dict."""
     from unstaggered_pic_amd import lib
-    p = workload_params(args.nx, args.ppc, prec)
+    p = workload_params(args.nx, args.ppc, prec, args.workload)
     N = p.nx * p.ny
     np_ = N * args.ppc
-    x, v, q, m = lib.load_particles(p, workload_load(args.ppc), np_)
+    x, v, q, m = lib.load_particles(p, workload_load(args.ppc, workload=args.workload), np_)
     ctx = lib.Context(p)
     sim = lib.Simulation(ctx, np_)
     h = HostArrays(np_, N)
@@ -184,8 +211,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--nx", type=int, default=256)
-    ap.add_argument("--ppc", type=int, default=256)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="weibel",
+                    help="P3 deck; sets the grid side and particles per cell unless --nx/--ppc are given")
+    ap.add_argument("--nx", type=int, default=None)
+    ap.add_argument("--ppc", type=int, default=None)
     ap.add_argument("--members", type=int, default=16, help="ensemble members (--form ensemble)")
     ap.add_argument("--form", choices=["both", "fp64", "mixed", "learned", "ensemble"], default="both",
                     help="both: plain FP64 and mixed + learned guess; value is the last")
@@ -203,6 +232,8 @@ def main():
         sys.exit(2)
     if args.precision:
         args.form = args.precision
+    args.nx = args.nx or WORKLOADS[args.workload][0]
+    args.ppc = args.ppc or WORKLOADS[args.workload][1]
     if args.calibrate_copy_mib > 0:
         from unstaggered_pic_amd import lib
         n = args.calibrate_copy_mib * (1 << 20) // 8
@@ -275,8 +306,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if prec == 0 else "f32/f64",
         "data": "synthetic (seeded bi-Maxwellian load, splitmix64 seed 12345)",
-        "config": {"workload": f"weibel_p3_{args.nx}x{args.nx}_ppc{args.ppc}_cubic_dt0.05", "particles": np_,
-                   "grid": [args.nx, args.nx], "spline_order": 3, "orbit_quad_order": 4,
+        "config": {"workload": workload_name(args), "particles": np_,
+                   "grid": [args.nx, args.nx], "spline_order": r["p"].spline_order,
+                   "orbit_quad_order": r["p"].orbit_quad_order,
                    "form": form_name[(prec, learned)],
                    "solver": "P3 nested solve (Anderson outer, per-particle Picard inner)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -304,7 +336,7 @@ def main():
                              "avg_launch_ms": r0["kms"] / max(r0["launches"], 1), "physics": phys(r0)}
         out["speedup_vs_plain_fp64"] = value / v0
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.nx)
+        out["cpu_baseline"] = cpu_baseline(args.nx, workload=args.workload)
     print(json.dumps(out))
 
 

@@ -138,6 +138,8 @@ void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int sp
 // workgroup per 64 cells); 2 = cell-mapped with 8 workgroups per 64 cells;
 // 3 = particle order, per-lane atomics.  1 and 2 need freshly cell-sorted
 // particles and ny % 64 == 0 (else 0 runs).
+// out[0] += sum over particles of |q| |v| (FP64 sum of the FP32 state)
+void launch_mixed_gross(hipStream_t s, int64_t np, const MixedState& ms, double* part, double* out);
 int deposit_variants();
 bool deposit_variant_usable(const GridDev& g, int variant, bool cells_fresh);
 void launch_deposit_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& a,

@@ -1289,6 +1289,19 @@ __global__ void k_zero_int(int* __restrict__ x) {
     if (threadIdx.x == 0) *x = 0;
 }
 
+// Sum over particles of |q| |v| (the gross current, times the cell volume
+// and the number of nodes, of a uniform plasma).
+__global__ __launch_bounds__(256) void k_mixed_gross(int64_t np, const float* __restrict__ q,
+                                                     const float* __restrict__ v, double* __restrict__ part) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double sums[1] = {0.0};
+    if (p < np) {
+        const double a = v[p], b = v[np + p], c = v[2 * np + p];
+        sums[0] = fabs((double)q[p]) * sqrt(a * a + b * b + c * c);
+    }
+    block_partials<1, 0>(sums, nullptr, 0, part);
+}
+
 }  // namespace
 
 void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& a) {
@@ -1347,6 +1360,12 @@ void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int sp
 #undef UPIC_RETRY_LAUNCH
         launch_finish_partials(s, a.part, nb, 2, 2, a.red, a.redmax, nullptr);
     }
+}
+
+void launch_mixed_gross(hipStream_t s, int64_t np, const MixedState& ms, double* part, double* out) {
+    if (np <= 0) return;
+    hipLaunchKernelGGL(k_mixed_gross, dim3(mblk(np)), dim3(256), 0, s, np, ms.q, ms.v, part);
+    launch_finish_partials(s, part, mblk(np), 1, 0, out, nullptr, nullptr);
 }
 
 int deposit_variants() { return 4; }

@@ -106,10 +106,17 @@ struct upic_sim {
 
 // Mixed precision: an FP32 map resolves each node current only to a fraction
 // of the current's overall scale, so the outer absolute tolerance is floored
-// at nonlinear_rtol times the step-start rms node current, and the final
+// at nonlinear_rtol times the step-start rms node current (and at
+// kMixedGrossFloor times the gross current below), and the final
 // consistency evaluation (which re-runs the map with a different FP32 atomic
 // summation order) is accepted within twice the tolerance.
 constexpr double kMixedFinalSlack = 2.0;
+// The FP32 map's rounding noise on a node current scales with the gross
+// current (sum of |q v| over the particles near the node), not with the
+// net current, which nearly cancels for counter-streaming or cold plasmas;
+// the outer absolute tolerance is also floored at this fraction of the
+// mean gross current density (about 2 FP32 ulps).
+constexpr double kMixedGrossFloor = 1.0e-7;
 
 namespace {
 
@@ -863,11 +870,14 @@ void p3_step(upic_sim* s, upic_step_info* info) {
     s->outer_atol = 0.0;
     if (s->mixed) {
         const int N = c->N();
-        launch_zero(c->stream, 1, c->dred + 20);
+        launch_zero(c->stream, 2, c->dred + 20);
         launch_sum_squares(c->stream, 3ll * N, s->J, c->dred + 20);
-        HIPCHK(hipMemcpyAsync(c->hred + 20, c->dred + 20, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        launch_mixed_gross(c->stream, s->np, s->ms, s->part, c->dred + 21);
+        HIPCHK(hipMemcpyAsync(c->hred + 20, c->dred + 20, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         c->sync();
-        s->outer_atol = std::max(c->p.nonlinear_atol, c->p.nonlinear_rtol * std::sqrt(c->hred[20] / (double)N));
+        const double gross = c->hred[21] / ((double)N * c->g.dV);
+        s->outer_atol = std::max({c->p.nonlinear_atol, c->p.nonlinear_rtol * std::sqrt(c->hred[20] / (double)N),
+                                  kMixedGrossFloor * gross});
     }
     try {
         p3_step_layout(s, false, info);
