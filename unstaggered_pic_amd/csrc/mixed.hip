@@ -605,13 +605,14 @@ __device__ __forceinline__ void move_axis(float& t, float du, int& c, int n) {
 template <int R, bool FULL = true>
 __device__ __forceinline__ void guess_features(const MGrid& g, const float4* __restrict__ NR, int ci, int cj, float tx,
                                                float ty, const float v[3], float q, float m, float kappa, float h,
-                                               float f[12], float dphys[3]) {
+                                               float f[12], float dphys[3], float A0[3]) {
     AxF<R> sx, sy;
     axis_f<R>(tx, g.ihx, sx);
     axis_f<R>(ty, g.ihy, sy);
     const int bx = wrapn(ci + sx.fl, g.nx), by = wrapn(cj + sy.fl, g.ny);
     float gp0 = 0.0f, gp1 = 0.0f, dg00 = 0.0f, dg01 = 0.0f, dg10 = 0.0f, dg11 = 0.0f;
     float dA[3] = {0.0f, 0.0f, 0.0f}, ddA[3][2] = {}, dAm[3][2] = {};
+    float a00 = 0.0f, a01 = 0.0f, a02 = 0.0f;
 #pragma unroll
     for (int a = 0; a <= R; ++a) {
         const int ia = wrap1n(bx + a, g.nx);
@@ -627,6 +628,10 @@ __device__ __forceinline__ void guess_features(const MGrid& g, const float4* __r
             const float am[3] = {0.5f * (r0.z + r1.y), 0.5f * (r0.w + r1.z), 0.5f * (r1.x + r1.w)};
 #pragma unroll
             for (int c = 0; c < 3; ++c) dA[c] += W * da[c];
+            // A^n at the start point (the push's theta = 0 gather)
+            a00 += W * r0.z;
+            a01 += W * r0.w;
+            a02 += W * r1.x;
             // B = curl A_mid needs d/dy of A_mid x, z and d/dx of A_mid y, z
             dAm[0][1] += Wy * am[0];
             dAm[1][0] += Wx * am[1];
@@ -643,6 +648,7 @@ __device__ __forceinline__ void guess_features(const MGrid& g, const float4* __r
             }
         }
     }
+    A0[0] = a00; A0[1] = a01; A0[2] = a02;
     const float ih = 1.0f / h;
     const float E[3] = {-gp0 - dA[0] * ih, -gp1 - dA[1] * ih, -dA[2] * ih};
     const float Bm[3] = {dAm[2][1], -dAm[2][0], dAm[1][0] - dAm[0][1]};
@@ -687,8 +693,8 @@ __device__ __forceinline__ void guess_features(const MGrid& g, const float4* __r
 template <int R, bool COLLECT, bool FUSED>
 __device__ __forceinline__ void map_particle(const MGrid& g, const QuadF& qd, int split, const P3MArgs& a, int64_t p,
                                              const float gin[3], int it0, bool defer, const float feat[12],
-                                             const float dph[3], const DepTileF& T, double rsum[2], double rmax[2],
-                                             int& nseg_total) {
+                                             const float dph[3], const float* A0start, const DepTileF& T,
+                                             double rsum[2], double rmax[2], int& nseg_total) {
     const int nsub = a.layout ? (int)a.layout[p] : 1;
     const float h = (float)(a.dt / (double)nsub);
     const int c0 = a.cell[p];
@@ -706,7 +712,11 @@ __device__ __forceinline__ void map_particle(const MGrid& g, const QuadF& qd, in
     for (int s = 0; s < nsub && reason == 0; ++s) {
         const float th0 = (float)s * inv_nsub, th1 = (float)(s + 1) * inv_nsub;
         float A0p[3];
-        gather_interp_f<R>(g, a.NR, ci, cj, tx, ty, th0, A0p);
+        if (A0start && nsub == 1) {
+            A0p[0] = A0start[0]; A0p[1] = A0start[1]; A0p[2] = A0start[2];
+        } else {
+            gather_interp_f<R>(g, a.NR, ci, cj, tx, ty, th0, A0p);
+        }
         const float pst[3] = {Pc[0] - qp * A0p[0], Pc[1] - qp * A0p[1], Pc[2] - qp * A0p[2]};
         float gs[3] = {vc[0], vc[1], vc[2]};
         if (nsub == 1) { gs[0] = gin[0]; gs[1] = gin[1]; gs[2] = gin[2]; }
@@ -817,7 +827,7 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
     if constexpr (FUSED) T = tilef_begin(g, tile_mem, a.cell, a.np, (int64_t)blockIdx.x * blockDim.x);
     // features and guess at the start point (all lanes: the network runs on
     // whole wavefronts)
-    float feat[12] = {}, dph[3] = {0.0f, 0.0f, 0.0f}, lg[3] = {0.0f, 0.0f, 0.0f};
+    float feat[12] = {}, dph[3] = {0.0f, 0.0f, 0.0f}, lg[3] = {0.0f, 0.0f, 0.0f}, a0s[3] = {0.0f, 0.0f, 0.0f};
     if constexpr (GM > 0 || COLLECT) {
         if (p < a.np && !a.layout) {
             const int c0 = a.cell[p];
@@ -825,7 +835,7 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
             ld3f(a.v, a.np, p, v0);
             guess_features<R, GM == 2 || COLLECT>(g, a.NR, c0 / g.ny, c0 - (c0 / g.ny) * g.ny, a.off[p],
                                                    a.off[a.np + p], v0, a.q[p], a.m[p], (float)a.kappa,
-                                                   (float)a.dt, feat, dph);
+                                                   (float)a.dt, feat, dph, a0s);
         }
         if constexpr (GM == 2) {
             mlp_wave_infer(a.mlp, feat, lg, mlp_lds + (threadIdx.x / 64) * kMlpLdsWave);
@@ -845,8 +855,9 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
 #pragma unroll
             for (int d = 0; d < 3; ++d) gin[d] += dph[d] + (GM >= 2 ? lg[d] : 0.0f);
         }
-        map_particle<R, COLLECT, FUSED>(g, qd, split, a, p, gin, 0, a.retry_n != nullptr && !a.layout, feat, dph, T,
-                                        rsum, rmax, nseg_total);
+        map_particle<R, COLLECT, FUSED>(g, qd, split, a, p, gin, 0, a.retry_n != nullptr && !a.layout, feat, dph,
+                                        (GM > 0 || COLLECT) && !a.layout ? a0s : nullptr, T, rsum, rmax,
+                                        nseg_total);
     }
     if constexpr (FUSED) {
         if (a.J_out) tilef_end(g, T, a.J_out);
@@ -872,10 +883,12 @@ __global__ __launch_bounds__(256) void k_p3_map_retry(MGrid g, QuadF qd, int spl
             const int c0 = a.cell[p];
             float v0[3];
             ld3f(a.v, a.np, p, v0);
+            float a0s[3];
             guess_features<R>(g, a.NR, c0 / g.ny, c0 - (c0 / g.ny) * g.ny, a.off[p], a.off[a.np + p], v0, a.q[p],
-                              a.m[p], (float)a.kappa, (float)a.dt, feat, dph);
+                              a.m[p], (float)a.kappa, (float)a.dt, feat, dph, a0s);
         }
-        map_particle<R, COLLECT, false>(g, qd, split, a, p, gin, 1, false, feat, dph, T, rsum, rmax, nseg_total);
+        map_particle<R, COLLECT, false>(g, qd, split, a, p, gin, 1, false, feat, dph, nullptr, T, rsum, rmax,
+                                        nseg_total);
     }
     block_partials<2, 2>(rsum, rmax, nseg_total, a.part);
 }
