@@ -465,6 +465,44 @@ struct MapOut {
     upic_path_stats ps{};
 };
 
+struct Norms {
+    double rms_relative, max_particle_relative, rms_absolute, rms_scaled, max_particle_scaled;
+    bool nonfinite;
+};
+// nonlinear_residual_norms (P3:5576-5610): the kernel and its copy to the
+// host are queued by residual_norms_launch; residual_norms_finish reads them
+// after the next synchronisation of the stream.
+double norms_floor(const upic_params& p) { return 64.0 * std::numeric_limits<double>::epsilon() * p.c; }
+double norms_atol(const upic_sim* s) { return s->outer_atol > 0.0 ? s->outer_atol : s->c->p.nonlinear_atol; }
+void residual_norms_launch(upic_sim* s, const double* map, const double* guess) {
+    upic_ctx* c = s->c;
+    launch_zero(c->stream, 8, c->dred + 8);
+    launch_residual_norms(c->stream, c->N(), map, guess, norms_floor(c->p), c->p.nonlinear_rtol, norms_atol(s),
+                          c->dred + 8);
+    HIPCHK(hipMemcpyAsync(c->hred + 8, c->dred + 8, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+}
+Norms residual_norms_finish(upic_sim* s) {
+    upic_ctx* c = s->c;
+    const upic_params& p = c->p;
+    const double* h = c->hred + 8;
+    const double cnt = 3.0 * (double)c->N();
+    const double floor_ = norms_floor(p), atol = norms_atol(s);
+    Norms n;
+    n.rms_absolute = std::sqrt(h[0] / cnt);
+    const double vrms = 0.5 * (std::sqrt(h[1] / cnt) + std::sqrt(h[2] / cnt));
+    n.rms_relative = n.rms_absolute / (vrms + floor_);
+    n.rms_scaled = n.rms_absolute / (atol + p.nonlinear_rtol * vrms);
+    n.max_particle_relative = h[3];
+    n.max_particle_scaled = h[4];
+    n.nonfinite = h[5] != 0.0;
+    return n;
+}
+Norms residual_norms(upic_sim* s, const double* map, const double* guess) {
+    residual_norms_launch(s, map, guess);
+    s->c->sync();
+    return residual_norms_finish(s);
+}
+
 struct RefineNeeded : Error {
     std::vector<int64_t> failed;  // stored-order particle indices
     RefineNeeded(const std::string& m, std::vector<int64_t> f) : Error(UPIC_EREFINE, m), failed(std::move(f)) {}
@@ -472,7 +510,8 @@ struct RefineNeeded : Error {
 
 // evaluate_selective_current_map_at_layout (P3:5235-5431): guess (3N) ->
 // J_map (3N).  write_out stores the particle end state in xo/Po/vo/vbo.
-MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bool use_layout) {
+MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bool use_layout,
+              Norms* norms = nullptr) {
     upic_ctx* c = s->c;
     const upic_params& p = c->p;
     const int N = c->N();
@@ -539,7 +578,10 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     }
     launch_project_nyquist(c->stream, c->sp, Jmap, 3);
     HIPCHK(hipGetLastError());
+    // the residual of this map against its guess, read in the same synchronisation
+    if (norms) residual_norms_launch(s, Jmap, guess);
     read_red(c, 4);
+    if (norms) *norms = residual_norms_finish(s);
     if (s->mixed) deposit_record(s);
     if (s->mixed && s->guess_mode == 2 && s->mlp_ready && !use_layout) s->warm_valid = true;
     MapOut o;
@@ -559,33 +601,6 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     return o;
 }
 
-struct Norms {
-    double rms_relative, max_particle_relative, rms_absolute, rms_scaled, max_particle_scaled;
-    bool nonfinite;
-};
-// nonlinear_residual_norms (P3:5576-5610)
-Norms residual_norms(upic_sim* s, const double* map, const double* guess) {
-    upic_ctx* c = s->c;
-    const upic_params& p = c->p;
-    const int N = c->N();
-    const double floor_ = 64.0 * std::numeric_limits<double>::epsilon() * p.c;
-    launch_zero(c->stream, 8, c->dred + 8);
-    const double atol = s->outer_atol > 0.0 ? s->outer_atol : p.nonlinear_atol;
-    launch_residual_norms(c->stream, N, map, guess, floor_, p.nonlinear_rtol, atol, c->dred + 8);
-    HIPCHK(hipMemcpyAsync(c->hred + 8, c->dred + 8, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    c->sync();
-    const double* h = c->hred + 8;
-    const double cnt = 3.0 * (double)N;
-    Norms n;
-    n.rms_absolute = std::sqrt(h[0] / cnt);
-    const double vrms = 0.5 * (std::sqrt(h[1] / cnt) + std::sqrt(h[2] / cnt));
-    n.rms_relative = n.rms_absolute / (vrms + floor_);
-    n.rms_scaled = n.rms_absolute / (atol + p.nonlinear_rtol * vrms);
-    n.max_particle_relative = h[3];
-    n.max_particle_scaled = h[4];
-    n.nonfinite = h[5] != 0.0;
-    return n;
-}
 bool finite_vec(upic_sim* s, const double* v) {
     // the nonfinite flag of the residual kernel on (v, v)
     return !residual_norms(s, v, v).nonfinite;
@@ -735,24 +750,35 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
     double min_alpha = 1.0, contraction = std::numeric_limits<double>::infinity();
     int64_t local_total = 0;
     MapOut last{};
-    // Jacc: current map of the accepted guess
-    auto eval = [&](const double* g, double* Jout, bool wo) {
+    // Jacc: current map of the accepted guess.  An evaluation also returns
+    // the residual norms of its map against its guess (same synchronisation).
+    auto eval = [&](const double* g, double* Jout, bool wo, Norms& n) {
         ++maps;
-        MapOut o = p3_map(s, g, Jout, wo || fast, use_layout);
+        MapOut o = p3_map(s, g, Jout, wo || fast, use_layout, &n);
         local_total += (int64_t)o.iter_sum;
         last = o;
         return o;
     };
+    Norms prefetched{};  // norms of (Jacc, guess) from the accepted trial
+    Norms conv_norms{};
     const int keep = p.anderson_memory + 1;
     for (int it = 0; it < p.nonlinear_max_iter; ++it) {
-        if (have_prefetched) have_prefetched = false;
-        else eval(s->guess, s->Jacc, false);
+        Norms nm{};
+        if (have_prefetched) {
+            have_prefetched = false;
+            nm = prefetched;
+        } else {
+            eval(s->guess, s->Jacc, false, nm);
+        }
         LinComb lr{2, {1.0, -1.0}, {s->Jacc, s->guess}};
         launch_lincomb(c->stream, n3, lr, s->resid);
-        const Norms nm = residual_norms(s, s->Jacc, s->guess);
         if (nm.nonfinite) throw Error(UPIC_ENONLINEAR, "uniform particle map returned a nonfinite macro current");
         iters = it + 1;
-        if (nm.rms_scaled <= 1.0 && nm.max_particle_scaled <= 1.0) { converged = true; break; }
+        if (nm.rms_scaled <= 1.0 && nm.max_particle_scaled <= 1.0) {
+            converged = true;
+            conv_norms = nm;
+            break;
+        }
         if (!picard && std::isfinite(prev_rms) && nm.rms_relative > p.anderson_growth_limit * prev_rms) {
             H.clear();
             ++restarts;
@@ -774,9 +800,13 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
                 // guess + alpha (full - guess)
                 LinComb lt{3, {1.0, alpha, -alpha}, {s->guess, full, s->guess}};
                 launch_lincomb(c->stream, n3, lt, s->trial);
-                if (!finite_vec(s, s->trial)) continue;
-                eval(s->trial, s->Jtry, false);
-                const Norms tn = residual_norms(s, s->Jtry, s->trial);
+                // full (a checked Anderson candidate, or guess + damping x
+                // residual of finite vectors) and guess are finite, so the
+                // alpha = 1 trial is too; shorter steps are checked
+                if (bt > 0 && !finite_vec(s, s->trial)) continue;
+                Norms tn{};
+                eval(s->trial, s->Jtry, false, tn);
+                prefetched = tn;
                 const bool dec = tn.rms_scaled <= (1.0 - 1.0e-4 * alpha) * nm.rms_scaled;
                 const bool safe = tn.max_particle_scaled <= p.anderson_growth_limit * nm.max_particle_scaled;
                 if (dec && safe) { alpha_out = alpha; return true; }
@@ -808,8 +838,9 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
     if (!converged)
         throw Error(UPIC_ENONLINEAR, "outer macro-current solve failed after " + std::to_string(iters) + " iterations");
     // the last evaluation is that of the converged guess (Jacc)
-    const MapOut fo = fast ? last : eval(s->guess, s->Jacc, true);
-    const Norms fn = residual_norms(s, s->Jacc, s->guess);
+    Norms fn{};
+    const MapOut fo = fast ? last : eval(s->guess, s->Jacc, true, fn);
+    if (fast) fn = conv_norms;
     const double slack = s->mixed ? kMixedFinalSlack : 1.0;
     const bool ok = !fn.nonfinite && fn.rms_scaled <= slack && fn.max_particle_scaled <= slack;
     if (info) {
