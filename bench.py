@@ -155,6 +155,7 @@ def run_form(args, prec, learned):
     np_ = N * args.ppc
     x, v, q, m = lib.load_particles(p, workload_load(args.ppc, workload=args.workload), np_)
     ctx = lib.Context(p)
+    ctx.set_graphs(args.graphs)
     sim = lib.Simulation(ctx, np_)
     h = HostArrays(np_, N)
     h.x[:], h.v[:], h.q[:], h.m[:] = x, v, q, m
@@ -223,6 +224,7 @@ def main():
     ap.add_argument("--rows-per-step", type=int, default=16384)
     ap.add_argument("--train-epochs", type=int, default=15)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graphs", type=int, default=1, help="replay the field solves from hipGraphs (1) or not (0)")
     ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel event timing")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (a known-size kernel for HBM counters)")

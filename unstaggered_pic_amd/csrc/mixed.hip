@@ -219,11 +219,22 @@ __device__ __forceinline__ void gather_interp_f(const MGrid& g, const float4* NR
 // path t(s) = t0 + s du (cell units), theta(s) = th0 + (th1 - th0) s.
 // Gauss points in chunks of four share one node loop while their stencil
 // base agrees (the quadrature folded into node weights, as in the FP64 path).
-template <int R>
+// FOLD (single substep, th1 = 1): the node loop of the last segment also
+// gathers A^{n+1} at the end point t0 + du when the end point's stencil is
+// the chunk's (A1, a1_ok set); otherwise the caller gathers it separately.
+template <int R, bool FOLD = false>
 __device__ __forceinline__ int orbit_gg_f(const MGrid& g, const QuadF& q, bool split, const float4* __restrict__ NR,
                                           int ci, int cj, float t0x, float t0y, float dux, float duy, float th0,
-                                          float th1, float gat[2], float D[3][2]) {
+                                          float th1, float gat[2], float D[3][2], float* A1 = nullptr,
+                                          bool* a1_ok = nullptr) {
     float g0 = 0.0f, g1 = 0.0f, d00 = 0.0f, d01 = 0.0f, d10 = 0.0f, d11 = 0.0f, d20 = 0.0f, d21 = 0.0f;
+    float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
+    bool got = false;
+    AxF<R> ex, ey;
+    if constexpr (FOLD) {
+        axis_f<R>(t0x + dux, g.ihx, ex);
+        axis_f<R>(t0y + duy, g.ihy, ey);
+    }
     BreakF bg;
     breakf_init(bg, R, t0x, t0y, split ? dux : 0.0f, split ? duy : 0.0f);
     float sa = 0.0f;
@@ -256,9 +267,15 @@ __device__ __forceinline__ int orbit_gg_f(const MGrid& g, const QuadF& q, bool s
                                   by[2] == by[0] && by[3] == by[0];
                 if (same) {
                     const int ox = wrapn(ci + bx[0], g.nx), oy = wrapn(cj + by[0], g.ny);
+                    bool take = false;
+                    if constexpr (FOLD) {
+                        take = sb >= 1.0f && q0 == 0 && ex.fl == bx[0] && ey.fl == by[0];
+                        got = got || take;
+                    }
 #pragma unroll
                     for (int a = 0; a <= R; ++a) {
                         const int ia = wrap1n(ox + a, g.nx);
+                        const float xe = FOLD && take ? ex.w[a] : 0.0f;
                         // x factors of the node weights of this stencil row:
                         // u = wq wx, du = wq dwx and their theta-weighted forms
                         float u[4], du[4], tu[4], tdu[4];
@@ -294,6 +311,12 @@ __device__ __forceinline__ int orbit_gg_f(const MGrid& g, const QuadF& q, bool s
                             d11 += r0.w * Gy0 + r1.z * Gy1;
                             d20 += r1.x * Gx0 + r1.w * Gx1;
                             d21 += r1.x * Gy0 + r1.w * Gy1;
+                            if constexpr (FOLD) {
+                                const float we = xe * ey.w[b];
+                                e0 += we * r1.y;
+                                e1 += we * r1.z;
+                                e2 += we * r1.w;
+                            }
                         }
                         UPIC_MIXED_FENCE();
                     }
@@ -341,6 +364,10 @@ __device__ __forceinline__ int orbit_gg_f(const MGrid& g, const QuadF& q, bool s
         if (sb >= 1.0f) break;
     }
     gat[0] = g0; gat[1] = g1;
+    if constexpr (FOLD) {
+        A1[0] = e0; A1[1] = e1; A1[2] = e2;
+        *a1_ok = got;
+    }
     D[0][0] = d00; D[0][1] = d01;
     D[1][0] = d10; D[1][1] = d11;
     D[2][0] = d20; D[2][1] = d21;
@@ -524,16 +551,23 @@ struct LocalMapF {
 };
 
 // evaluate_local_particle_substep (P3:4843-4884) in cell-relative FP32.
-template <int R>
+template <int R, bool FOLD = false>
 __device__ __forceinline__ void local_map_f(const MGrid& g, const QuadF& qd, bool split, const float4* NR,
                                             const float gs[3], int ci, int cj, float tx, float ty, const float P0[3],
                                             const float A0p[3], const float pst[3], float q, float m, float kappa,
                                             float h, float th0, float th1, LocalMapF& out) {
     const float dux = h * gs[0] * g.ihx, duy = h * gs[1] * g.ihy;
     float A1p[3];
-    gather_interp_f<R>(g, NR, ci, cj, tx + dux, ty + duy, th1, A1p);
     float gpb[2], D[3][2];
-    orbit_gg_f<R>(g, qd, split, NR, ci, cj, tx, ty, dux, duy, th0, th1, gpb, D);
+    if constexpr (FOLD) {
+        // single substep: th1 = 1, A at the end point is A^{n+1}
+        bool ok;
+        orbit_gg_f<R, true>(g, qd, split, NR, ci, cj, tx, ty, dux, duy, th0, th1, gpb, D, A1p, &ok);
+        if (!ok) gather_interp_f<R>(g, NR, ci, cj, tx + dux, ty + duy, th1, A1p);
+    } else {
+        gather_interp_f<R>(g, NR, ci, cj, tx + dux, ty + duy, th1, A1p);
+        orbit_gg_f<R>(g, qd, split, NR, ci, cj, tx, ty, dux, duy, th0, th1, gpb, D);
+    }
     const float Aep[3] = {0.5f * (A0p[0] + A1p[0]), 0.5f * (A0p[1] + A1p[1]), 0.5f * (A0p[2] + A1p[2])};
     float Dv[3];
 #pragma unroll
@@ -690,7 +724,7 @@ __device__ __forceinline__ void guess_features(const MGrid& g, const float4* __r
 // unconverged is queued (index and updated guess) for k_p3_map_retry and
 // nothing else is written for it here: a handful of such lanes would
 // otherwise keep their whole wavefront iterating.
-template <int R, bool COLLECT, bool FUSED>
+template <int R, bool COLLECT, bool FUSED, bool FOLD = false>
 __device__ __forceinline__ void map_particle(const MGrid& g, const QuadF& qd, int split, const P3MArgs& a, int64_t p,
                                              const float gin[3], int it0, bool defer, const float feat[12],
                                              const float dph[3], const float* A0start, const DepTileF& T,
@@ -728,7 +762,12 @@ __device__ __forceinline__ void map_particle(const MGrid& g, const QuadF& qd, in
         reason = 5;  // maximum_iterations
         for (int it = its; it < a.max_iter; ++it) {
             LocalMapF mpd;
-            local_map_f<R>(g, qd, split != 0, a.NR, gs, ci, cj, tx, ty, Pc, A0p, pst, qp, mp, kappa, h, th0, th1, mpd);
+            if (FOLD && nsub == 1)
+                local_map_f<R, true>(g, qd, split != 0, a.NR, gs, ci, cj, tx, ty, Pc, A0p, pst, qp, mp, kappa, h, th0,
+                                     th1, mpd);
+            else
+                local_map_f<R>(g, qd, split != 0, a.NR, gs, ci, cj, tx, ty, Pc, A0p, pst, qp, mp, kappa, h, th0, th1,
+                               mpd);
             ++its;
             if (!finite3f(mpd.vbar)) { reason = 1; break; }
             // convergence test of solve_local_particle_substep (P3:4951-4968)
@@ -855,7 +894,8 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
 #pragma unroll
             for (int d = 0; d < 3; ++d) gin[d] += dph[d] + (GM >= 2 ? lg[d] : 0.0f);
         }
-        map_particle<R, COLLECT, FUSED>(g, qd, split, a, p, gin, 0, a.retry_n != nullptr && !a.layout, feat, dph,
+        map_particle<R, COLLECT, FUSED, !FUSED && GM != 2>(g, qd, split, a, p, gin, 0,
+                                                           a.retry_n != nullptr && !a.layout, feat, dph,
                                         (GM > 0 || COLLECT) && !a.layout ? a0s : nullptr, T, rsum, rmax,
                                         nseg_total);
     }
@@ -887,7 +927,7 @@ __global__ __launch_bounds__(256) void k_p3_map_retry(MGrid g, QuadF qd, int spl
             guess_features<R>(g, a.NR, c0 / g.ny, c0 - (c0 / g.ny) * g.ny, a.off[p], a.off[a.np + p], v0, a.q[p],
                               a.m[p], (float)a.kappa, (float)a.dt, feat, dph, a0s);
         }
-        map_particle<R, COLLECT, false>(g, qd, split, a, p, gin, 1, false, feat, dph, nullptr, T, rsum, rmax,
+        map_particle<R, COLLECT, false, true>(g, qd, split, a, p, gin, 1, false, feat, dph, nullptr, T, rsum, rmax,
                                         nseg_total);
     }
     block_partials<2, 2>(rsum, rmax, nseg_total, a.part);
