@@ -86,7 +86,7 @@ def test_mixed_state_roundtrip(orc):
     ctx.close()
 
 
-@pytest.mark.parametrize("order,nq,sort", [(3, 4, 1), (2, 3, 0)])
+@pytest.mark.parametrize("order,nq,sort", [(3, 4, 1), (2, 3, 0), (1, 2, 1), (4, 5, 1)])
 def test_mixed_p3_steps_vs_oracle(orc, order, nq, sort):
     """Five P3 steps in the mixed mode against the FP64 oracle run on the
     same inputs with the FP64 tolerances."""
