@@ -4,7 +4,8 @@ with the sources.
 
   python -m unstaggered_pic_amd.profile_summary --trace DIR [--after NAME]
          [--steps K] [--fetch DIR] [--write DIR] [--copy-mib 1024]
-         [--kernel SUBSTRING[,SUBSTRING]] [--json-out FILE] --command "..."
+         [--kernel SUBSTRING[,SUBSTRING]] [--json-out FILE] [--counters DIR,DIR]
+         --command "..."
 
 HBM traffic per dispatch = FETCH_SIZE x (copy bytes / FETCH_SIZE of the
 calibration copy) + WRITE_SIZE x (copy bytes / WRITE_SIZE of the copy): the
@@ -39,6 +40,15 @@ def counters(dirname):
     return agg
 
 
+def counters_by_name(dirname):
+    """{kernel: {counter: (mean over the last 4 dispatches, dispatches)}}."""
+    rows = list(csv.DictReader(open(_one(dirname, "*counter_collection.csv"))))
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in rows:
+        agg[r["Kernel_Name"]][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    return {k: {c: (sum(v[-4:]) / len(v[-4:]), len(v)) for c, v in d.items()} for k, d in agg.items()}
+
+
 def trace_stats(dirname, after):
     """Per-kernel (calls, total ns) over the dispatches that start after the
     last dispatch whose name contains `after`, from the kernel trace."""
@@ -61,6 +71,8 @@ def main():
                     help="summarise only the dispatches after the last one whose name contains this")
     ap.add_argument("--steps", type=int, default=0, help="time steps in the summarised region (per-step totals)")
     ap.add_argument("--json-out", default=None, help="write the per-dispatch HBM traffic of the kernels as JSON")
+    ap.add_argument("--counters", default=None,
+                    help="comma-separated --pmc output directories of the SQ/GRBM counter passes")
     ap.add_argument("--fetch")
     ap.add_argument("--write")
     ap.add_argument("--copy-mib", type=int, default=1024)
@@ -114,6 +126,28 @@ def main():
             import json
             with open(a.json_out, "w") as fh:
                 json.dump(out, fh, indent=1)
+    if a.counters:
+        res = collections.defaultdict(dict)
+        for d in a.counters.split(","):
+            for k, vals in counters_by_name(d).items():
+                if any(x in k for x in a.kernel.split(",")):
+                    res[k].update(vals)
+        print()
+        print("| kernel | dispatches | VALU / wavefront | VMEM loads / wavefront | LDS / wavefront | "
+              "VALU busy | GPU-active ms |")
+        print("|---|---|---|---|---|---|---|")
+        for k in sorted(res):
+            d = res[k]
+            if "SQ_WAVES" not in d or "GRBM_GUI_ACTIVE" not in d:
+                continue
+            w = d["SQ_WAVES"][0]
+            # GRBM_GUI_ACTIVE sums the 8 XCDs; a wave64 VALU op occupies a SIMD
+            # (1024 per GPU) for 4 cycles; 2.4 GHz
+            cyc = d["GRBM_GUI_ACTIVE"][0] / 8.0
+            busy = d["SQ_INSTS_VALU"][0] * 4.0 / 1024.0 / cyc
+            print(f"| `{short(k)}` | {d['SQ_WAVES'][1]} | {d['SQ_INSTS_VALU'][0] / w:.0f} | "
+                  f"{d['SQ_INSTS_VMEM_RD'][0] / w:.0f} | {d['SQ_INSTS_LDS'][0] / w:.0f} | {busy:.2f} | "
+                  f"{cyc / 2.4e6:.3f} |")
     if a.command:
         print()
         print("Command: `" + a.command + "`")
