@@ -178,7 +178,8 @@ const char* upic_last_error(const upic_ctx* ctx);
 /* Kernel timing with HIP events on the context stream (no reference
  * counterpart; the reference only times whole steps, P3:145-194).  When
  * enabled, each launch of a timed kernel family ("p1_push", "p1_push_final",
- * "p1_deposit_guess", "p3_map", "p2_deposit", "p2_push") is bracketed by
+ * "p1_deposit_guess", "p3_map", "p3_map_mixed", "deposit_mixed", "p2_deposit",
+ * "p2_push") is bracketed by
  * events and synchronised; upic_ctx_timing returns the accumulated launches
  * and milliseconds since the last upic_ctx_set_timing call. */
 upic_status upic_ctx_set_timing(upic_ctx* ctx, int enable);

@@ -8,8 +8,9 @@ states its error bound against the FP64 result:
     positions to 2^-23 of a cell);
   * P3 steps vs the oracle: total energy within 1e-6 relative, field energy
     within 1e-4 relative, particle state within 1e-5 of the box / 1e-4 of the
-    velocity scale; Gauss's law and the energy balance of the mixed run hold
-    as in FP64 (Gauss to roundoff, total-energy drift within 1e-6).
+    velocity scale; Gauss's law, the Lorenz gauge and the energy balance of
+    the mixed run hold as in FP64 (Gauss and gauge residuals at roundoff,
+    total-energy drift within 1e-6).
 """
 import numpy as np
 import pytest
@@ -107,6 +108,7 @@ def test_mixed_p3_steps_vs_oracle(orc, order, nq, sort):
         assert abs(fe - orows[k, 2]) / abs(orows[k, 2]) < 1e-4, k
         assert abs(te - te0) / abs(te0) < 1e-6, k
         assert rows[k].gauss_rms < 1e-12 * max(1.0, rows[k].rho_rms), k
+        assert rows[k].gauge_rms < 1e-12 * max(1e-12, rows[k].divA_rms + rows[k].psi_over_c2_rms), k
     assert all(i.converged for i in infos)
     h = p64.Lx / p64.nx
     xa, xb = out.x.reshape(3, n), so.x.reshape(3, n)

@@ -11,10 +11,10 @@ line every 100 steps and one JSON summary at the end:
     energy);
   * the magnetic energy of the mixed form against the FP64 form, relative
     to the FP64 maximum (the field starts at B = 0);
-  * the magnetic-energy growth rate of each form, fitted on the interval
-    where the magnetic energy rises from 3x its thermal-noise plateau (the
-    median over the first quarter of the time to the maximum) to 1/5 of its
-    maximum, and their relative difference.
+  * the magnetic-energy growth rate of each form: half the slope of
+    log(W_B - plateau) on the interval where W_B rises from 3x its
+    thermal-noise plateau (the median over the first quarter of the time to
+    the maximum) to 1/5 of its maximum, and their relative difference.
 The histories go to <out>_fp64.csv and <out>_mixed.csv.
 
 Usage: python -m unstaggered_pic_amd.validate --steps 2000 --out DIR/weibel
@@ -86,7 +86,8 @@ def growth_rate(t, wb):
     imax = int(np.argmax(wb))
     if imax < 8:
         return None, None
-    lo = 3.0 * float(np.median(wb[1:imax // 4]))  # above the thermal-noise plateau
+    plateau = float(np.median(wb[1:imax // 4]))  # thermal-noise level of W_B
+    lo = 3.0 * plateau
     if not np.any(wb > lo):
         return None, None
     i0 = int(np.argmax(wb > lo))
@@ -94,7 +95,8 @@ def growth_rate(t, wb):
     i1 = i0 + int(np.argmax(wb[i0:] > hi)) if np.any(wb[i0:] > hi) else len(wb) - 1
     if i1 - i0 < 5:
         return None, None
-    sl = np.polyfit(t[i0:i1], np.log(wb[i0:i1]), 1)[0]
+    # the unstable mode's energy grows on top of the noise plateau
+    sl = np.polyfit(t[i0:i1], np.log(wb[i0:i1] - plateau), 1)[0]
     return 0.5 * sl, (float(t[i0]), float(t[i1]))
 
 
