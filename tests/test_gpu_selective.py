@@ -1,0 +1,96 @@
+"""Selective particle refinement of the P3 step (nonlinear_step_orbit's
+fallback, P3:6328-6454): particles whose inner Picard solve fails over the
+full dt are re-run with 2, 4, ... substeps while the macro dt stays.
+
+The inner iteration cap is set low (local_max_iter = 3 with the plain v^n
+start, which needs about 3 iterations), so some particles fail at full dt:
+  * without the fallback the step reports UPIC_ENONLINEAR;
+  * FP64 with the fallback: two steps agree with the oracle (which restates
+    the same fallback) to 1e-12 in the energies and 1e-9 in the particle
+    state, so the refined layout is the reference's;
+  * mixed precision with the fallback (the FUSED sweep with per-particle
+    substeps): the steps converge and conserve energy to 1e-6.
+"""
+import numpy as np
+import pytest
+
+from oracle_bind import Oracle, State
+from unstaggered_pic_amd.abi import CODE_P3, LOAD_BIMAXWELL, make_load, make_params, with_mixed_precision
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle()
+
+
+def deck(fallback):
+    return make_params(code=CODE_P3, nx=16, ny=16, nz=1, Lx=2 * np.pi, Ly=2 * np.pi, Lz=1.0, dt=0.05,
+                       spline_order=3, orbit_quad_order=4, nyquist_projection=1, nonlinear_rtol=1e-11,
+                       nonlinear_atol=1e-13, nonlinear_max_iter=80, anderson_damping=0.5, local_rtol=1e-12,
+                       local_atol=1e-14, local_max_iter=3, selective_particle_fallback=fallback,
+                       selective_particle_max_substeps=64)
+
+
+def initial(orc, p, seed=21):
+    from unstaggered_pic_amd import lib
+    n = p.nx * p.ny * 8
+    x, v, q, m = lib.load_particles(p, make_load(LOAD_BIMAXWELL, 8, seed=seed, vth=(0.1, 0.3, 0.3)), n)
+    return orc.init_fields(p, x, v, q, m, 1.0)
+
+
+def gpu_run(p, st, steps):
+    from unstaggered_pic_amd import lib
+    ctx = lib.Context(p)
+    sim = lib.Simulation(ctx, st.np)
+    sim.upload(st.copy())
+    rows = [sim.diagnostics(1)]
+    for _ in range(steps):
+        sim.step()
+        rows.append(sim.diagnostics(1))
+    out = State(st.np, st.N)
+    sim.download(out)
+    sim.close()
+    ctx.close()
+    return out, rows
+
+
+def test_without_fallback_the_step_fails(orc):
+    from unstaggered_pic_amd import lib
+    p = deck(0)
+    st = initial(orc, p)
+    with pytest.raises(lib.UpicError) as e:
+        gpu_run(p, st, 1)
+    assert "UPIC_ENONLINEAR" in str(e.value)
+
+
+def test_fp64_fallback_matches_oracle(orc):
+    p = deck(1)
+    st = initial(orc, p)
+    steps = 2
+    out, rows = gpu_run(p, st, steps)
+    so, orows = orc.run(p, st, steps)
+    for k in range(steps + 1):
+        assert abs(rows[k].total_energy - orows[k, 1]) / abs(orows[k, 1]) < 1e-12, k
+        assert abs(rows[k].field_energy - orows[k, 2]) / abs(orows[k, 2]) < 1e-12, k
+    L = p.Lx
+    d = np.abs(out.x - so.x)
+    d = np.minimum(d, L - d)
+    assert np.max(d) < 1e-9 * L
+    assert np.max(np.abs(out.v - so.v)) < 1e-9 * np.max(np.abs(so.v))
+
+
+def test_mixed_fallback_converges_and_conserves(orc):
+    from unstaggered_pic_amd import lib
+    p0 = with_mixed_precision(deck(0))
+    p0.local_max_iter = 2  # the FP32 tolerance floors need fewer iterations
+    st = initial(orc, p0)
+    with pytest.raises(lib.UpicError):
+        gpu_run(p0, st, 1)
+    p = with_mixed_precision(deck(1))
+    p.local_max_iter = 2
+    steps = 3
+    _, rows = gpu_run(p, st, steps)
+    te = np.array([r.total_energy for r in rows])
+    assert np.max(np.abs(te - te[0])) / te[0] < 1e-6

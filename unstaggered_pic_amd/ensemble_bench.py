@@ -3,7 +3,8 @@ from a seed (grid side in {64, 128, 256}, particles per cell in {16, 64,
 256}, thermal speed in [0.02, 0.2] c, perturbation amplitude in [1e-3,
 1e-2]; Landau loads with quadratic splines at dt = 0.1 and Weibel loads with
 cubic splines at dt = 0.05), mixed precision with the physics-predictor
-guess, each on its own context with graph-replayed field solves.
+guess and the selective particle refinement on, each on its own context
+with graph-replayed field solves.
 
 The same members are timed twice from the same state: stepped one after the
 other (one member on the GPU at a time), and advanced together by
@@ -14,6 +15,7 @@ member over the timed steps.
   python bench.py --form ensemble [--members 16] [--steps 5] [--warmup 2]
 """
 import json
+import sys
 import time
 
 import numpy as np
@@ -38,7 +40,7 @@ def build(lib, spec):
         p = make_params(code=CODE_P3, nx=s, ny=s, nz=1, Lx=L, Ly=L, Lz=1.0, dt=0.1, spline_order=2,
                         orbit_quad_order=3, nyquist_projection=1, nonlinear_rtol=1e-11, nonlinear_atol=1e-13,
                         nonlinear_max_iter=80, anderson_damping=0.5, local_rtol=1e-12, local_atol=1e-14,
-                        local_max_iter=64, sort_interval=1)
+                        local_max_iter=64, sort_interval=1, selective_particle_fallback=1)
         load = make_load(LOAD_LANDAU, spec["ppc"], seed=spec["seed"], alpha=spec["amp"],
                          vth=(spec["vth"],) * 3)
     else:
@@ -46,7 +48,7 @@ def build(lib, spec):
         p = make_params(code=CODE_P3, nx=s, ny=s, nz=1, Lx=L, Ly=L, Lz=1.0, dt=0.05, spline_order=3,
                         orbit_quad_order=4, nyquist_projection=1, nonlinear_rtol=1e-11, nonlinear_atol=1e-13,
                         nonlinear_max_iter=80, anderson_damping=0.5, local_rtol=1e-12, local_atol=1e-14,
-                        local_max_iter=64, sort_interval=1)
+                        local_max_iter=64, sort_interval=1, selective_particle_fallback=1)
         load = make_load(LOAD_BIMAXWELL, spec["ppc"], seed=spec["seed"],
                          vth=(0.25 * spec["vth"], spec["vth"], spec["vth"]))
     p = with_mixed_precision(p)
@@ -68,7 +70,11 @@ class _Host:
 def run(args):
     from unstaggered_pic_amd import lib
     specs = member_specs(args.members)
-    loads = [build(lib, sp) for sp in specs]
+    loads = []
+    for i, sp in enumerate(specs):
+        loads.append(build(lib, sp))
+        if (i + 1) % 8 == 0 or i + 1 == len(specs):
+            print(f"# loaded {i + 1}/{len(specs)} members", file=sys.stderr, flush=True)
     total = sum(n for _, n, _ in loads)
 
     def fresh():
@@ -87,6 +93,7 @@ def run(args):
     out = {}
     for mode in ("sequential", "ensemble"):
         ms = fresh()
+        print(f"# {mode}: {len(ms)} members on the GPU", file=sys.stderr, flush=True)
         sims = [s for _, s in ms]
         for _ in range(args.warmup):
             lib.ensemble_step(sims, threads=16) if mode == "ensemble" else [s.step() for s in sims]
@@ -104,6 +111,7 @@ def run(args):
             c.sync()
         el = time.perf_counter() - t0
         drift = max(abs(s.diagnostics(1).total_energy - a) / abs(a) for s, a in zip(sims, e0))
+        print(f"# {mode}: {1e3 * el / args.steps:.1f} ms per ensemble step", file=sys.stderr, flush=True)
         out[mode] = {"value": total * args.steps / el, "ms_per_step": 1e3 * el / args.steps, "max_rel_energy_change": drift}
         for c, s in ms:
             s.close()
