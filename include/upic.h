@@ -364,7 +364,8 @@ upic_status upic_mlp_loss_grad(upic_ctx* ctx, const float* params, const float* 
  * and deterministic for a seed. */
 typedef enum upic_load_kind {
     UPIC_LOAD_LANDAU = 1,   /* density 1 + alpha cos(k x) by inverse CDF, y uniform,
-                               Maxwellian velocities of width vth[d]            */
+                               Maxwellian velocities of width vth[d] (P3:
+                               Maxwellian momenta u = gamma v / c, below c)     */
     UPIC_LOAD_BIMAXWELL = 2,/* uniform density, Gaussian momenta u = gamma v / c of
                                widths vth[d] (stays sub-luminal)                 */
     UPIC_LOAD_BEAMS = 3     /* cell lattice + noise*dx jitter, two beams at +-v0 in x

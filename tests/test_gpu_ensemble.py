@@ -141,3 +141,18 @@ def test_graph_field_solve_matches_eager(k):
         sim.close()
         ctx.close()
     compare(res[1], res[0], fp64)
+
+
+def test_landau_load_stays_below_c_for_p3():
+    """The ensemble draws hot Landau members (thermal width up to 0.2 c); for
+    the relativistic P3 step the Landau load draws momenta, so every speed is
+    below c even in the far tail (a 1e6-particle draw at width 0.3 c)."""
+    lib = gpu()
+    p = p3_deck(64, 64, False)
+    n = p.nx * p.ny * 256
+    _, v, _, _ = lib.load_particles(p, make_load(LOAD_LANDAU, 256, seed=3, alpha=0.01, vth=(0.3, 0.3, 0.3)), n)
+    speed = np.sqrt(np.sum(v.reshape(3, n) ** 2, axis=0))
+    assert np.max(speed) < 1.0
+    # momenta u = gamma v are Maxwellian of width 0.3
+    u = v.reshape(3, n) / np.sqrt(1.0 - speed ** 2)
+    assert abs(np.std(u[0]) - 0.3) < 0.01

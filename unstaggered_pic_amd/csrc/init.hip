@@ -84,7 +84,9 @@ extern "C" upic_status upic_load_particles(const upic_params* p, const upic_load
                     x[1] = (iy + rng.uniform()) * h[1];
                     x[2] = (p->nz > 1) ? (iz + rng.uniform()) * h[2] : (iz + 0.5) * h[2];
                 }
-                if (sp->kind == UPIC_LOAD_BIMAXWELL) {
+                // P3 is relativistic: thermal spreads are drawn in momentum
+                // (u = gamma v) so that no speed reaches c
+                if (sp->kind == UPIC_LOAD_BIMAXWELL || (sp->kind == UPIC_LOAD_LANDAU && p->code == 3)) {
                     const double u[3] = {sp->vth[0] * rng.normal(), sp->vth[1] * rng.normal(), sp->vth[2] * rng.normal()};
                     const double g = std::sqrt(1.0 + (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) / (kappa * kappa));
                     for (int d = 0; d < 3; ++d) v[d] = u[d] / g;
