@@ -294,6 +294,9 @@ def main():
                 "outer_iterations": [i.iterations for i in rr["infos"]],
                 "map_evaluations": [i.map_evaluations for i in rr["infos"]],
                 "mean_inner_iterations": float(np.mean([i.local_iterations_mean for i in rr["infos"]])),
+                # summed over the step's sweeps (the guess acts in the first)
+                "inner_iterations_per_particle_step": float(np.mean([i.local_iterations_total for i in rr["infos"]]))
+                / rr["np"],
                 # constraints on the final state: Gauss's law and the Lorenz gauge residuals (FP64 diagnostics)
                 "gauss_rms": d1.gauss_rms, "rho_rms": d1.rho_rms, "gauge_rms": d1.gauge_rms,
                 "divA_rms": d1.divA_rms, "psi_over_c2_rms": d1.psi_over_c2_rms}

@@ -329,9 +329,10 @@ upic_status upic_sim_guess_data(upic_sim* sim, float* rows, int64_t max_rows, in
  * mean minibatch loss of each epoch to loss_per_epoch when not NULL. */
 upic_status upic_sim_train_guess(upic_sim* sim, int32_t epochs, int32_t batch, double lr, uint64_t seed,
                                  double* loss_per_epoch);
-/* In mode 2 the network runs in the first outer evaluation of a step; the
- * later evaluations add its stored correction to the predictor evaluated on
- * their own fields. */
+/* Modes 1 and 2 apply to the first outer evaluation of a step; the later
+ * evaluations (whose fields differ from it by the outer update only) start
+ * each particle from the orbit-averaged velocity it converged to in the
+ * previous evaluation.  Mode 0 keeps the reference's v^n throughout. */
 upic_status upic_sim_set_guess_mode(upic_sim* sim, int32_t mode);
 upic_status upic_sim_guess_params(upic_sim* sim, float* params);          /* UPIC_MLP_NPARAM floats out */
 upic_status upic_sim_set_guess_params(upic_sim* sim, const float* params); /* and in (installs the network) */

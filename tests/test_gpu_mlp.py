@@ -10,8 +10,9 @@ Bounds:
     accumulation order and the device exp), and vs the exact FP64 network
     within the BF16 quantisation error (5e-2 of the output scale);
   * the step with a learned or predicted guess converges to the same fixed
-    point as the plain start (energies within 1e-6 relative) and takes no more
-    inner iterations.
+    point as the plain start (energies within 1e-6 relative); the learned
+    guess takes fewer inner iterations per particle and step than the
+    predictor alone, which takes fewer than the plain start.
 """
 import numpy as np
 import pytest
@@ -109,7 +110,8 @@ def test_learned_guess_same_fixed_point_fewer_iterations():
     """Collect rows from 12 mixed steps, train on the device, then run 4 steps
     from the same state with the plain start v^n, the physics predictor and
     predictor + network: the three runs agree to the solver tolerance and the
-    guessed starts need no more inner iterations than the plain one."""
+    inner iterations per particle and step fall from plain to predictor to
+    learned."""
     lib = gpu()
     p, n, ppc = weibel_small()
     x, v, q, m = lib.load_particles(p, make_load(LOAD_BIMAXWELL, ppc, seed=12345, vth=(0.05, 0.2, 0.2)), n)
@@ -140,7 +142,10 @@ def test_learned_guess_same_fixed_point_fewer_iterations():
         its, te = [], []
         for _ in range(4):
             info = s2.step()
-            its.append(info.local_iterations_mean)
+            # inner iterations per particle summed over the step's sweeps: the
+            # guess acts in the first sweep, the later ones start from the
+            # previous sweep's converged vbar in modes 1 and 2
+            its.append(info.local_iterations_total / n)
             te.append(s2.diagnostics(1).total_energy)
         results[mode] = (np.array(its), np.array(te))
         s2.close()
@@ -149,6 +154,7 @@ def test_learned_guess_same_fixed_point_fewer_iterations():
     for mode in (1, 2):
         itm, tem = results[mode]
         assert np.max(np.abs(tem - te0) / np.abs(te0)) < 1e-6, mode
-        assert np.mean(itm) <= np.mean(it0) + 1e-9, (mode, itm, it0)
-    print("mean inner iterations plain/predictor/learned:",
+    # the network's gain over the predictor alone, and the predictor's over v^n
+    assert np.mean(results[2][0]) < np.mean(results[1][0]) < np.mean(it0)
+    print("inner iterations per particle per step plain/predictor/learned:",
           [float(np.mean(results[k][0])) for k in (0, 1, 2)])

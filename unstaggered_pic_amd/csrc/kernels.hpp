@@ -111,15 +111,12 @@ struct P3MArgs {
     double* redmax;                // [2]: max local iterations, max final scaled residual
     unsigned long long* stats;
     double* part;                  // per-block partials (kPartSlots per block)
-    // initial guess: 0 = v^n, 1 = physics predictor, 2 = predictor + network
+    // initial guess: 0 = v^n (or `guess` when given), 1 = physics predictor,
+    // 2 = predictor + network
     int guess_mode;
     const float* mlp;              // network inference blob (guess_mode 2)
     float* train;                  // training rows (15 floats) or NULL
     int64_t train_stride, train_offset, train_cap;
-    // the network's correction of the guess (3*np): written by the first
-    // evaluation of a step, reused by the later ones when warm_in is set
-    float* warm;
-    int warm_in;
     float* vdep;                   // converged vbar (3*np) for the separate deposit
     // deferred second Picard iterations (uniform layout; NULL = none): queue
     // length, particle indices (np) and queued guesses (3*np)

@@ -709,13 +709,6 @@ __device__ __forceinline__ void guess_features(const MGrid& g, const float4* __r
     for (int c = 0; c < 3; ++c) dphys[c] = coef * (F[c] - v[c] * vF);
 }
 
-// GM: initial guess of the inner solve, 0 = v^n (the reference's), 1 =
-// physics predictor, 2 = predictor + network (its correction stored in
-// a.warm when given), 3 = predictor + the stored correction; COLLECT: write
-// (features, converged correction) training rows in the write-out pass.
-// FUSED: deposit inside the sweep (per-particle substep layouts); otherwise
-// the converged vbar goes to a.vdep and k_deposit_mixed deposits it, which
-// keeps the deposit's registers out of the Picard sweep.
 // One particle of the sweep: the substep loop of
 // evaluate_selective_current_map_at_layout (P3:5290-5431) in cell-relative
 // FP32.  gin: the initial guess of the (single-substep) inner solve; it0: the
@@ -850,10 +843,10 @@ __device__ __forceinline__ void map_particle(const MGrid& g, const QuadF& qd, in
     }
 }
 
-// GM: initial guess of the inner solve, 0 = v^n (the reference's), 1 =
-// physics predictor, 2 = predictor + network (its correction stored in
-// a.warm when given), 3 = predictor + the stored correction; COLLECT: write
-// (features, converged correction) training rows in the write-out pass.
+// GM: initial guess of the inner solve, 0 = v^n (the reference's) or
+// a.guess when given, 1 = physics predictor, 2 = predictor + network;
+// COLLECT: write (features, converged correction) training rows in the
+// write-out pass.
 // FUSED: deposit inside the sweep (per-particle substep layouts); otherwise
 // the converged vbar goes to a.vdep and the deposit kernel deposits it,
 // which keeps the deposit's registers out of the Picard sweep.
@@ -876,13 +869,7 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
                                                    a.off[a.np + p], v0, a.q[p], a.m[p], (float)a.kappa,
                                                    (float)a.dt, feat, dph, a0s);
         }
-        if constexpr (GM == 2) {
-            mlp_wave_infer(a.mlp, feat, lg, mlp_lds + (threadIdx.x / 64) * kMlpLdsWave);
-            if (a.warm && p < a.np) st3f(a.warm, a.np, p, lg);
-        }
-        if constexpr (GM == 3) {
-            if (p < a.np) ld3f(a.warm, a.np, p, lg);
-        }
+        if constexpr (GM == 2) mlp_wave_infer(a.mlp, feat, lg, mlp_lds + (threadIdx.x / 64) * kMlpLdsWave);
     }
     double rsum[2] = {0.0, 0.0}, rmax[2] = {0.0, 0.0};
     int nseg_total = 0;
@@ -892,7 +879,7 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
         if (a.guess) ld3f(a.guess, a.np, p, gin);
         if constexpr (GM > 0) {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) gin[d] += dph[d] + (GM >= 2 ? lg[d] : 0.0f);
+            for (int d = 0; d < 3; ++d) gin[d] += dph[d] + (GM == 2 ? lg[d] : 0.0f);
         }
         map_particle<R, COLLECT, FUSED, !FUSED && GM != 2>(g, qd, split, a, p, gin, 0,
                                                            a.retry_n != nullptr && !a.layout, feat, dph,
@@ -1365,7 +1352,6 @@ void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int sp
     mg.hx = (float)g.h[0]; mg.hy = (float)g.h[1];
     mg.ihx = (float)(1.0 / g.h[0]); mg.ihy = (float)(1.0 / g.h[1]);
     int gm = (a.layout || (a.guess_mode == 2 && !a.mlp)) ? 0 : a.guess_mode;
-    if (gm == 2 && a.warm_in) gm = 3;  // the network's correction of this step is cached
     const bool col = a.train != nullptr && !a.layout;
     const size_t lds = gm == 2 ? 4 * (size_t)kMlpLdsWave : 0;
 #define UPIC_MIXED_LAUNCH(R_)                                                                                       \
@@ -1374,13 +1360,11 @@ void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int sp
         if (a.layout) {                                                                                             \
             hipLaunchKernelGGL((k_p3_map_mixed<R_, 0, false, true>), gr, bl, 0, s, mg, q, split, a);               \
         } else if (col) {                                                                                           \
-            if (gm == 3) hipLaunchKernelGGL((k_p3_map_mixed<R_, 3, true, false>), gr, bl, 0, s, mg, q, split, a);  \
-            else if (gm == 2) hipLaunchKernelGGL((k_p3_map_mixed<R_, 2, true, false>), gr, bl, lds, s, mg, q, split, a); \
+            if (gm == 2) hipLaunchKernelGGL((k_p3_map_mixed<R_, 2, true, false>), gr, bl, lds, s, mg, q, split, a); \
             else if (gm == 1) hipLaunchKernelGGL((k_p3_map_mixed<R_, 1, true, false>), gr, bl, 0, s, mg, q, split, a); \
             else hipLaunchKernelGGL((k_p3_map_mixed<R_, 0, true, false>), gr, bl, 0, s, mg, q, split, a);          \
         } else {                                                                                                    \
-            if (gm == 3) hipLaunchKernelGGL((k_p3_map_mixed<R_, 3, false, false>), gr, bl, 0, s, mg, q, split, a); \
-            else if (gm == 2) hipLaunchKernelGGL((k_p3_map_mixed<R_, 2, false, false>), gr, bl, lds, s, mg, q, split, a); \
+            if (gm == 2) hipLaunchKernelGGL((k_p3_map_mixed<R_, 2, false, false>), gr, bl, lds, s, mg, q, split, a); \
             else if (gm == 1) hipLaunchKernelGGL((k_p3_map_mixed<R_, 1, false, false>), gr, bl, 0, s, mg, q, split, a); \
             else hipLaunchKernelGGL((k_p3_map_mixed<R_, 0, false, false>), gr, bl, 0, s, mg, q, split, a);         \
         }                                                                                                           \

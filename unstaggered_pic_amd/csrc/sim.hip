@@ -71,11 +71,10 @@ struct upic_sim {
     float* train_buf = nullptr;
     int64_t train_cap = 0, train_n = 0, train_stride = 0;
     bool train_written = false;
-    // the network's guess correction of the current step (computed by the
-    // first outer evaluation, reused by the later ones)
-    float* warm = nullptr;
-    bool warm_valid = false;
     float* vdep = nullptr;  // converged vbar of the last sweep (3*np)
+    // set after a uniform-layout sweep of the current step: the next sweep's
+    // inner solves start from vdep (guess modes 1 and 2)
+    bool vdep_valid = false;
     // node current of the previous step (mixed precision): the outer solve
     // starts from the linear extrapolation 2 J^n - J^{n-1} when valid
     double* Jprev = nullptr;
@@ -136,7 +135,7 @@ void sim_free(upic_sim* s) {
     for (float** p : fl) dfree(*p);
     dfree(s->ms.cell); dfree(s->mo.cell);
     dfree(s->retry_n); dfree(s->retry_idx); dfree(s->retry_gs);
-    dfree(s->mlp_P); dfree(s->mlp_blob); dfree(s->train_buf); dfree(s->part); dfree(s->warm); dfree(s->vdep);
+    dfree(s->mlp_P); dfree(s->mlp_blob); dfree(s->train_buf); dfree(s->part); dfree(s->vdep);
     for (auto& gx : s->fs_graphs) (void)hipGraphExecDestroy(gx.second);
     s->fs_graphs.clear();
     if (s->tuner.e0) (void)hipEventDestroy(s->tuner.e0);
@@ -542,9 +541,15 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
         a.train_offset = s->train_n;
         a.train_cap = s->train_cap;
         s->train_written = collect;
-        a.warm = (s->guess_mode == 2 && s->mlp_ready && !use_layout) ? s->warm : nullptr;
-        a.warm_in = (a.warm && s->warm_valid) ? 1 : 0;
-        s->warm_valid = false;  // until this evaluation has completed
+        // The step's first sweep starts from the predictor (+ network); the
+        // later ones, whose fields differ by the outer update only, from the
+        // converged vbar of the previous sweep (no counterpart in the
+        // reference, whose start is always v^n, P3:5318)
+        if (s->guess_mode >= 1 && s->vdep_valid && !use_layout) {
+            a.guess_mode = 0;
+            a.guess = s->vdep;
+        }
+        s->vdep_valid = false;  // until this evaluation has completed
         a.vdep = s->vdep;
         a.retry_n = s->retry_n; a.retry_idx = s->retry_idx; a.retry_gs = s->retry_gs;
         timed(c, "p3_map_mixed", [&] { launch_p3_map_mixed(c->stream, c->g, c->qf, c->opt.split, a); });
@@ -583,7 +588,7 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     read_red(c, 4);
     if (norms) *norms = residual_norms_finish(s);
     if (s->mixed) deposit_record(s);
-    if (s->mixed && s->guess_mode == 2 && s->mlp_ready && !use_layout) s->warm_valid = true;
+    if (s->mixed && !use_layout) s->vdep_valid = true;
     MapOut o;
     o.iter_sum = c->hred[0];
     o.failed = (int64_t)c->hred[1];
@@ -895,7 +900,7 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
 void p3_step(upic_sim* s, upic_step_info* info) {
     upic_ctx* c = s->c;
     s->train_written = false;
-    s->warm_valid = false;
+    s->vdep_valid = false;
     maybe_sort(s);
     state_spectra(s);
     s->outer_atol = 0.0;
@@ -1294,7 +1299,6 @@ upic_status upic_sim_create(upic_ctx* c, int64_t np, upic_sim** out) {
             s->ftmp = dalloc<float>(n3);
             s->ftmp1 = dalloc<float>(np);
             s->ftmp2 = dalloc<float>(np);
-            s->warm = dalloc<float>(n3);
             s->vdep = dalloc<float>(n3);
             s->Jprev = dalloc<double>(3 * (size_t)N);
             s->retry_n = dalloc<int>(1);
