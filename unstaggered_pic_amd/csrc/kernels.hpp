@@ -123,6 +123,7 @@ struct P3MArgs {
     int* retry_n;
     int* retry_idx;
     float* retry_gs;
+    float* retry_old;              // the queued particles' starting guesses (3*np) or NULL
 };
 // Picard sweep; with a substep layout it also deposits, otherwise it writes
 // vdep and launch_deposit_mixed deposits (reason and vdep as inputs).
@@ -137,6 +138,12 @@ void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int sp
 // particles and ny % 64 == 0 (else 0 runs).
 // out[0] += sum over particles of |q| |v| (FP64 sum of the FP32 state)
 void launch_mixed_gross(hipStream_t s, int64_t np, const MixedState& ms, double* part, double* out);
+// Incremental deposit into a.J_out after a sweep whose particles all
+// started from the previous sweep's vbar (a.retry_old set): only the queued
+// particles move their current.  out[0..1] += segment and crossing count
+// changes, out[2] = max(out[2], largest new segment count).
+void launch_deposit_delta_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& a,
+                                double* out);
 int deposit_variants();
 bool deposit_variant_usable(const GridDev& g, int variant, bool cells_fresh);
 void launch_deposit_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& a,

@@ -791,6 +791,7 @@ __device__ __forceinline__ void map_particle(const MGrid& g, const QuadF& qd, in
                 base = __shfl(base, leader, 64);
                 const int k = base + __popcll(grp & ((1ull << lane) - 1ull));
                 a.retry_idx[k] = (int)p;
+                if (a.retry_old) st3f(a.retry_old, a.np, k, gin);
                 st3f(a.retry_gs, a.np, k, gs);
                 return;
             }
@@ -943,6 +944,42 @@ __global__ __launch_bounds__(256) void k_deposit_mixed(MGrid g, QuadF qd, int sp
     }
     tilef_end(g, T, a.J_out);
     block_partials<0, 0>(nullptr, nullptr, nseg, a.part);
+}
+
+// Incremental deposit of a sweep that started every particle from the
+// previous sweep's vbar: a particle that converged at its first Picard
+// iteration accepted that vbar unchanged, so its current is the one already
+// in the kept deposit; only the deferred particles (queue entries [0,
+// *retry_n)) changed, and each moves its current from the starting vbar
+// (retry_old) to the converged one (vdep).  sums: [0] segment count change,
+// [1] crossing count change; maxs: [0] largest new segment count.
+template <int R>
+__global__ __launch_bounds__(256) void k_deposit_delta_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
+    const int n = *a.retry_n;
+    const int64_t k0 = (int64_t)blockIdx.x * blockDim.x;
+    __shared__ float tile_mem[kTileX * kTileY * 3];
+    const DepTileF T = tilef_begin(g, tile_mem, a.cell, a.np, k0 < n ? (int64_t)a.retry_idx[k0] : a.np);
+    double sums[2] = {0.0, 0.0}, maxs[1] = {0.0};
+    for (int64_t k = k0 + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = a.retry_idx[k];
+        if (a.reason[p] != 0) continue;
+        const float h = (float)a.dt;
+        const int c0 = a.cell[p];
+        const int ci = c0 / g.ny, cj = c0 - (c0 / g.ny) * g.ny;
+        const float qs = a.q[p] * a.inv_dV;
+        float vo[3], vn[3];
+        ld3f(a.retry_old, a.np, k, vo);
+        ld3f(a.vdep, a.np, p, vn);
+        const int so = deposit_orbit_f<R>(g, qd, split != 0, ci, cj, a.off[p], a.off[a.np + p], h * vo[0] * g.ihx,
+                                          h * vo[1] * g.ihy, vo, -qs, T, a.J_out);
+        const int sn = deposit_orbit_f<R>(g, qd, split != 0, ci, cj, a.off[p], a.off[a.np + p], h * vn[0] * g.ihx,
+                                          h * vn[1] * g.ihy, vn, qs, T, a.J_out);
+        sums[0] += (double)(sn - so);
+        sums[1] += (double)((sn > 1 ? 1 : 0) - (so > 1 ? 1 : 0));
+        maxs[0] = fmax(maxs[0], (double)sn);
+    }
+    tilef_end(g, T, a.J_out);
+    block_partials<2, 1>(sums, maxs, 0, a.part);
 }
 
 // Cell-mapped deposit (cell-sorted particles, ny a multiple of 64): a
@@ -1403,6 +1440,24 @@ void launch_mixed_gross(hipStream_t s, int64_t np, const MixedState& ms, double*
     if (np <= 0) return;
     hipLaunchKernelGGL(k_mixed_gross, dim3(mblk(np)), dim3(256), 0, s, np, ms.q, ms.v, part);
     launch_finish_partials(s, part, mblk(np), 1, 0, out, nullptr, nullptr);
+}
+
+void launch_deposit_delta_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& a,
+                                double* out) {
+    if (a.np <= 0 || !a.retry_n || !a.retry_old) return;
+    MGrid mg;
+    mg.nx = g.nx; mg.ny = g.ny; mg.N = g.N; mg.order = g.order;
+    mg.hx = (float)g.h[0]; mg.hy = (float)g.h[1];
+    mg.ihx = (float)(1.0 / g.h[0]); mg.ihy = (float)(1.0 / g.h[1]);
+    const unsigned nb = std::min<unsigned>(mblk(a.np), 256u);
+    switch (g.order) {
+        case 1: hipLaunchKernelGGL(k_deposit_delta_mixed<1>, dim3(nb), dim3(256), 0, s, mg, q, split, a); break;
+        case 2: hipLaunchKernelGGL(k_deposit_delta_mixed<2>, dim3(nb), dim3(256), 0, s, mg, q, split, a); break;
+        case 3: hipLaunchKernelGGL(k_deposit_delta_mixed<3>, dim3(nb), dim3(256), 0, s, mg, q, split, a); break;
+        case 4: hipLaunchKernelGGL(k_deposit_delta_mixed<4>, dim3(nb), dim3(256), 0, s, mg, q, split, a); break;
+        default: break;
+    }
+    launch_finish_partials(s, a.part, nb, 2, 1, out, out + 2, nullptr);
 }
 
 int deposit_variants() { return 4; }

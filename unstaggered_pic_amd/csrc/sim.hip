@@ -79,10 +79,17 @@ struct upic_sim {
     // starts from the linear extrapolation 2 J^n - J^{n-1} when valid
     double* Jprev = nullptr;
     bool jprev_valid = false;
-    // deferred second Picard iterations of the sweep (count, indices, guesses)
+    // deferred second Picard iterations of the sweep (count, indices, guesses,
+    // starting guesses)
     int* retry_n = nullptr;
     int* retry_idx = nullptr;
     float* retry_gs = nullptr;
+    float* retry_old = nullptr;
+    // the current deposited by the last uniform-layout sweep of this step
+    // (before the Nyquist projection) and its path statistics
+    double* Jdep = nullptr;
+    bool jdep_valid = false;
+    unsigned long long jdep_stats[3] = {0, 0, 0};
     bool cells_fresh = false;  // particles cell-sorted since their last move (offsets = cell ends)
     // deposit variant autotuner (mixed precision).  While a particle order
     // (index 1: freshly cell-sorted, 0: not) has no choice yet, each map
@@ -134,7 +141,7 @@ void sim_free(upic_sim* s) {
                     &s->mo.off, &s->mo.P, &s->mo.v, &s->mo.vb, &s->ftmp, &s->ftmp1, &s->ftmp2, &s->NRf};
     for (float** p : fl) dfree(*p);
     dfree(s->ms.cell); dfree(s->mo.cell);
-    dfree(s->retry_n); dfree(s->retry_idx); dfree(s->retry_gs);
+    dfree(s->retry_n); dfree(s->retry_idx); dfree(s->retry_gs); dfree(s->retry_old); dfree(s->Jdep);
     dfree(s->mlp_P); dfree(s->mlp_blob); dfree(s->train_buf); dfree(s->part); dfree(s->vdep);
     for (auto& gx : s->fs_graphs) (void)hipGraphExecDestroy(gx.second);
     s->fs_graphs.clear();
@@ -519,6 +526,7 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     field_solve(s, s->Jfield);
     launch_zero(c->stream, 3ll * N, Jmap);
     zero_red(c);
+    bool incremental = false;
     if (s->mixed) {
         P3MArgs a{};
         a.np = s->np; a.dt = p.dt; a.kappa = p.c; a.rtol = p.local_rtol; a.atol = p.local_atol;
@@ -552,20 +560,40 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
         s->vdep_valid = false;  // until this evaluation has completed
         a.vdep = s->vdep;
         a.retry_n = s->retry_n; a.retry_idx = s->retry_idx; a.retry_gs = s->retry_gs;
+        // A sweep that started every particle from the previous sweep's vbar
+        // moves the kept deposit (Jdep) by the particles it changed only.
+        incremental = a.guess != nullptr && s->jdep_valid;
+        a.retry_old = incremental ? s->retry_old : nullptr;
+        s->jdep_valid = false;  // until this evaluation has completed
         timed(c, "p3_map_mixed", [&] { launch_p3_map_mixed(c->stream, c->g, c->qf, c->opt.split, a); });
-        const int variant = deposit_pick(s);
-        if (s->tuner.pending >= 0) {
-            if (!s->tuner.e0) {
-                HIPCHK(hipEventCreate(&s->tuner.e0));
-                HIPCHK(hipEventCreate(&s->tuner.e1));
+        if (!use_layout) {
+            P3MArgs ad = a;
+            ad.J_out = s->Jdep;
+            if (incremental) {
+                launch_zero(c->stream, 3, c->dred + 40);
+                timed(c, "deposit_mixed", [&] {
+                    launch_deposit_delta_mixed(c->stream, c->g, c->qf, c->opt.split, ad, c->dred + 40);
+                });
+                HIPCHK(hipMemcpyAsync(c->hred + 40, c->dred + 40, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                                      c->stream));
+            } else {
+                launch_zero(c->stream, 3ll * N, s->Jdep);
+                const int variant = deposit_pick(s);
+                if (s->tuner.pending >= 0) {
+                    if (!s->tuner.e0) {
+                        HIPCHK(hipEventCreate(&s->tuner.e0));
+                        HIPCHK(hipEventCreate(&s->tuner.e1));
+                    }
+                    HIPCHK(hipEventRecord(s->tuner.e0, c->stream));
+                }
+                timed(c, "deposit_mixed", [&] {
+                    launch_deposit_mixed(c->stream, c->g, c->qf, c->opt.split, ad,
+                                         s->cells_fresh ? s->offsets : nullptr, variant);
+                });
+                if (s->tuner.pending >= 0) HIPCHK(hipEventRecord(s->tuner.e1, c->stream));
             }
-            HIPCHK(hipEventRecord(s->tuner.e0, c->stream));
+            launch_copy(c->stream, 3ll * N, s->Jdep, Jmap);
         }
-        timed(c, "deposit_mixed", [&] {
-            launch_deposit_mixed(c->stream, c->g, c->qf, c->opt.split, a, s->cells_fresh ? s->offsets : nullptr,
-                                 variant);
-        });
-        if (s->tuner.pending >= 0) HIPCHK(hipEventRecord(s->tuner.e1, c->stream));
     } else {
     P3Args a{};
     a.np = s->np; a.dt = p.dt; a.kappa = p.c; a.rtol = p.local_rtol; a.atol = p.local_atol;
@@ -587,6 +615,15 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     if (norms) residual_norms_launch(s, Jmap, guess);
     read_red(c, 4);
     if (norms) *norms = residual_norms_finish(s);
+    if (s->mixed && !use_layout) {
+        if (incremental) {  // path statistics of the kept deposit, moved by the changed particles
+            c->hstats[0] = (unsigned long long)((double)s->jdep_stats[0] + c->hred[41]);
+            c->hstats[1] = (unsigned long long)((double)s->jdep_stats[1] + c->hred[40]);
+            c->hstats[2] = std::max<unsigned long long>(s->jdep_stats[2], (unsigned long long)c->hred[42]);
+        }
+        for (int k = 0; k < 3; ++k) s->jdep_stats[k] = c->hstats[k];
+        s->jdep_valid = true;
+    }
     if (s->mixed) deposit_record(s);
     if (s->mixed && !use_layout) s->vdep_valid = true;
     MapOut o;
@@ -901,6 +938,7 @@ void p3_step(upic_sim* s, upic_step_info* info) {
     upic_ctx* c = s->c;
     s->train_written = false;
     s->vdep_valid = false;
+    s->jdep_valid = false;
     maybe_sort(s);
     state_spectra(s);
     s->outer_atol = 0.0;
@@ -1304,6 +1342,8 @@ upic_status upic_sim_create(upic_ctx* c, int64_t np, upic_sim** out) {
             s->retry_n = dalloc<int>(1);
             s->retry_idx = dalloc<int>(np);
             s->retry_gs = dalloc<float>(n3);
+            s->retry_old = dalloc<float>(n3);
+            s->Jdep = dalloc<double>(3 * (size_t)N);
             s->NRf = dalloc<float>(8 * (size_t)N);
         }
         // identity ids
