@@ -951,18 +951,23 @@ __global__ __launch_bounds__(256) void k_deposit_mixed(MGrid g, QuadF qd, int sp
 // iteration accepted that vbar unchanged, so its current is the one already
 // in the kept deposit; only the deferred particles (queue entries [0,
 // *retry_n)) changed, and each moves its current from the starting vbar
-// (retry_old) to the converged one (vdep).  sums: [0] segment count change,
+// (retry_old) to the converged one (vdep).  The partial slots of all
+// workgroups are written (neutral ones past the queue).  sums: [0] segment count change,
 // [1] crossing count change; maxs: [0] largest new segment count.
 template <int R>
 __global__ __launch_bounds__(256) void k_deposit_delta_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
     const int n = *a.retry_n;
     const int64_t k0 = (int64_t)blockIdx.x * blockDim.x;
+    if (k0 >= n) {  // no queue entries for this workgroup: neutral partials
+        if (threadIdx.x < kPartSlots) a.part[(int64_t)blockIdx.x * kPartSlots + threadIdx.x] = 0.0;
+        return;
+    }
     __shared__ float tile_mem[kTileX * kTileY * 3];
-    const DepTileF T = tilef_begin(g, tile_mem, a.cell, a.np, k0 < n ? (int64_t)a.retry_idx[k0] : a.np);
+    const DepTileF T = tilef_begin(g, tile_mem, a.cell, a.np, (int64_t)a.retry_idx[k0]);
     double sums[2] = {0.0, 0.0}, maxs[1] = {0.0};
-    for (int64_t k = k0 + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = k0 + threadIdx.x;
+    if (k < n && a.reason[a.retry_idx[k]] == 0) {
         const int64_t p = a.retry_idx[k];
-        if (a.reason[p] != 0) continue;
         const float h = (float)a.dt;
         const int c0 = a.cell[p];
         const int ci = c0 / g.ny, cj = c0 - (c0 / g.ny) * g.ny;
@@ -974,9 +979,9 @@ __global__ __launch_bounds__(256) void k_deposit_delta_mixed(MGrid g, QuadF qd, 
                                           h * vo[1] * g.ihy, vo, -qs, T, a.J_out);
         const int sn = deposit_orbit_f<R>(g, qd, split != 0, ci, cj, a.off[p], a.off[a.np + p], h * vn[0] * g.ihx,
                                           h * vn[1] * g.ihy, vn, qs, T, a.J_out);
-        sums[0] += (double)(sn - so);
-        sums[1] += (double)((sn > 1 ? 1 : 0) - (so > 1 ? 1 : 0));
-        maxs[0] = fmax(maxs[0], (double)sn);
+        sums[0] = (double)(sn - so);
+        sums[1] = (double)((sn > 1 ? 1 : 0) - (so > 1 ? 1 : 0));
+        maxs[0] = (double)sn;
     }
     tilef_end(g, T, a.J_out);
     block_partials<2, 1>(sums, maxs, 0, a.part);
@@ -1449,7 +1454,9 @@ void launch_deposit_delta_mixed(hipStream_t s, const GridDev& g, const QuadF& q,
     mg.nx = g.nx; mg.ny = g.ny; mg.N = g.N; mg.order = g.order;
     mg.hx = (float)g.h[0]; mg.hy = (float)g.h[1];
     mg.ihx = (float)(1.0 / g.h[0]); mg.ihy = (float)(1.0 / g.h[1]);
-    const unsigned nb = std::min<unsigned>(mblk(a.np), 256u);
+    // one workgroup per 256 possible queue entries; those past the queue
+    // length exit at once
+    const unsigned nb = mblk(a.np);
     switch (g.order) {
         case 1: hipLaunchKernelGGL(k_deposit_delta_mixed<1>, dim3(nb), dim3(256), 0, s, mg, q, split, a); break;
         case 2: hipLaunchKernelGGL(k_deposit_delta_mixed<2>, dim3(nb), dim3(256), 0, s, mg, q, split, a); break;
