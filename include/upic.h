@@ -294,6 +294,12 @@ upic_status upic_sim_diagnostics(upic_sim* sim, int mode, upic_diagnostics* out)
  * history velocity v_prev by the Taylor start-up (P1:1296-1303,
  * P3:3423-3441). */
 upic_status upic_sim_init_fields(upic_sim* sim, double n0);
+/* upic_sim_init_fields with the mesh vector potential A and U (host arrays of
+ * 3N, NULL = zero; P3 only), as the Weibel initialiser sets them
+ * (P3:4081-4104): A and U are Nyquist-projected, E = -grad phi - U and
+ * B = curl A drive the history velocity, and P = m gamma v + q A(x)
+ * (project_initial_state, P3:3398-3405). */
+upic_status upic_sim_init_fields_AU(upic_sim* sim, double n0, const double* A, const double* U);
 /* Device pointers of the resident state (for zero-copy callers).  With
  * UPIC_PREC_MIXED the particle pointers are an FP64 view rebuilt from the
  * FP32 state by this call; writes to them reach the simulation only through
@@ -385,6 +391,44 @@ typedef struct upic_load_spec {
 /* Fills st->x, v, q, m (host arrays of st->np = nx*ny*nz*ppc particles) with
  * q = -n0 Lx Ly Lz / np and m = |q|. */
 upic_status upic_load_particles(const upic_params* params, const upic_load_spec* spec, upic_host_state* st);
+
+/* ---- The reference's P3 initial conditions ------------------------------ */
+typedef enum upic_p3_case_kind {
+    UPIC_P3_TWO_STREAM = 1, /* initialize_two_stream_3d_linear_x (P3:3689-3851)     */
+    UPIC_P3_WEIBEL = 2,     /* initialize_weibel_filamentation_3d (P3:3961-4108)    */
+    UPIC_P3_LANDAU = 3      /* initialize_landau_damping_1d3d (P3:3853-3937)        */
+} upic_p3_case_kind;
+/* The deck keys the initialisers read (Config, P3:872-987; parser
+ * P3:1201-1334), with the reference's defaults in brackets. */
+typedef struct upic_p3_case {
+    int32_t kind;                          /* upic_p3_case_kind                     */
+    int32_t particles_per_cell_pair;       /* [4]                                   */
+    int32_t perturbation_mode;             /* [1]                                   */
+    int32_t tsi_transverse_mode;           /* [1]                                   */
+    int32_t two_stream_eigenmode;          /* [1]                                   */
+    int32_t two_stream_finite_grid_theory; /* [1]                                   */
+    int32_t weibel_eigenmode;              /* [1]                                   */
+    int32_t weibel_finite_grid_theory;     /* [1]                                   */
+    double n0, v0, perturbation;
+    double two_stream_density_alpha;       /* [-1]: < 0 = velocity seed only        */
+    double weibel_B0;                      /* [-1]: < 0 = perturbation              */
+    double landau_alpha;                   /* [-1]: < 0 = perturbation              */
+    double thermal_velocity;               /* [1]                                   */
+    double tsi_transverse_seed;            /* [0]                                   */
+} upic_p3_case;
+/* Fills upic_p3_case with the reference's defaults for a kind. */
+void upic_p3_case_defaults(int32_t kind, upic_p3_case* cs);
+/* The particle part of the initialiser: st->x, v, q, m for st->np particles
+ * (two streams of nx*ny*nz*ppc each for two-stream and Weibel, nx*ny*nz*ppc
+ * quiet-start samples for Landau), and, when st->A / st->U are not NULL, the
+ * mesh A and U (3N each; nonzero for the Weibel eigenmode only).  With
+ * st->x == NULL only st->np is set.  growth_rate (may be NULL) receives the
+ * cold linear-theory rate (cold_two_stream_growth_rate P3:4160-4195,
+ * cold_weibel_growth_rate 4197-4226; 0 for Landau).  The input checks and
+ * their messages are the reference's (UPIC_EINVAL, upic_last_error(NULL)).
+ * The field tail is upic_sim_init_fields_AU. */
+upic_status upic_p3_initial_state(const upic_params* params, const upic_p3_case* cs, upic_host_state* st,
+                                  double* growth_rate);
 
 #ifdef __cplusplus
 }

@@ -121,3 +121,51 @@ def test_driver_p3_small_weibel(tmp_path, prec, tol):
     assert np.max(np.abs(te - te[0])) / te[0] < tol
     assert all(r["nonlinear_converged"] == "1" for r in rows[1:])
     assert all(float(r["gauss_rms"]) < 1e-12 for r in rows)
+
+
+# The reference's cold relativistic Weibel eigenmode deck (its initialiser,
+# P3:3961-4108, and the linear-theory columns of its CSV, P3:8427-8433 and
+# 8576-8594) on a small 2-D grid.
+P3_WEIBEL_DECK = """test_case = cold_relativistic_weibel
+nx = 8
+ny = 32
+nz = 1
+Lx = 6.283185307179586
+Ly = 6.283185307179586
+Lz = 1.0
+n_steps = 20
+dt = 0.1
+particles_per_cell_pair = 4
+spline_order = 2
+orbit_quad_order = 6
+v0 = 0.9
+weibel_B0 = 1.0e-4
+nonlinear_rtol = 1.0e-11
+nonlinear_atol = 1.0e-13
+nonlinear_max_iter = 80
+anderson_damping = 0.5
+particle_local_rtol = 1.0e-12
+particle_local_atol = 1.0e-14
+particle_local_max_iter = 64
+output_prefix = weibel_eig
+"""
+
+
+def test_driver_p3_weibel_eigenmode_follows_theory(tmp_path):
+    from unstaggered_pic_amd import lib as U
+    from unstaggered_pic_amd.abi import P3_WEIBEL, make_params
+    out = run_driver(tmp_path, "p3", P3_WEIBEL_DECK)
+    rows = list(csv.DictReader(open(tmp_path / "weibel_eig_diagnostics.csv")))
+    assert len(rows) == 21 and len(rows[0]) == 62
+    p = make_params(code=3, nx=8, ny=32, nz=1, Lx=6.283185307179586, Ly=6.283185307179586, Lz=1.0,
+                    spline_order=2)
+    rate = U.p3_initial_state(p, U.p3_case(P3_WEIBEL, particles_per_cell_pair=4, v0=0.9,
+                                           weibel_B0=1e-4))["growth_rate"]
+    assert all(float(r["theory_growth_rate"]) == pytest.approx(rate, rel=1e-15) for r in rows)
+    ratio = np.array([float(r["mode_over_theory"]) for r in rows])
+    print("weibel |B_z(k_y)| / theory:", " ".join(f"{x:.5f}" for x in ratio), flush=True)
+    assert ratio[0] == pytest.approx(1.0, rel=1e-14)
+    assert np.max(np.abs(ratio - 1.0)) < 0.005
+    te = np.array([float(r["total_energy"]) for r in rows])
+    assert np.max(np.abs(te - te[0])) / te[0] < 1e-12
+    assert "linear theory growth/damping rate" in out.stdout

@@ -1,0 +1,273 @@
+"""The reference's P3 initial conditions: upic_p3_initial_state (host C++)
+and the field tail upic_sim_init_fields_AU.
+
+Pinned three ways: bit for bit against the reference's own
+initialize_state (P3:4125-4141, compiled from its sources in oracle/_ref;
+those tests skip where it is absent), against the cold-fluid dispersion
+relations the growth rates solve (P3:4160-4226), and against the identities
+the eigenmode loads are built to satisfy (the two-stream seed deposits the
+grid charge alpha n0 cos(k x), P3:3723-3729; the Weibel streams cancel in
+charge, P3:4064-4066; the Landau samples are normal quantiles).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+from scipy.stats import norm
+
+from oracle_bind import Ref, State
+from unstaggered_pic_amd import lib as U
+from unstaggered_pic_amd.abi import HostState, P3_LANDAU, P3_TWO_STREAM, P3_WEIBEL, make_params
+
+TWO_PI = 2.0 * math.pi
+TSI_L = 25.132741228718345
+REF_KIND = {P3_LANDAU: 0, P3_TWO_STREAM: 1, P3_WEIBEL: 2}
+
+# (kind, ppc, alpha, vth, v0, perturbation, mode, grid): alpha is
+# two_stream_density_alpha / weibel_B0 / landau_alpha
+CASES = [
+    (P3_TWO_STREAM, 2, 0.005, 1.0, 0.9, 0.0, 1, (16, 4, 1)),   # growing eigenmode
+    (P3_TWO_STREAM, 3, 0.02, 1.0, 0.5, 0.0, 2, (16, 4, 1)),
+    (P3_TWO_STREAM, 2, -1.0, 1.0, 0.5, 0.03, 1, (8, 4, 2)),    # legacy velocity seed, 3-D grid
+    (P3_WEIBEL, 2, 1e-4, 1.0, 0.9, 0.0, 1, (4, 16, 1)),
+    (P3_WEIBEL, 3, 1e-3, 1.0, 0.6, 0.0, 2, (8, 16, 1)),
+    (P3_LANDAU, 8, 0.05, 0.1, 0.0, 0.0, 1, (16, 2, 1)),
+]
+
+
+def case_params(kind, grid, order=2, proj=1):
+    L = TSI_L if kind == P3_TWO_STREAM else TWO_PI
+    nx, ny, nz = grid
+    return make_params(code=3, nx=nx, ny=ny, nz=nz, Lx=L, Ly=L, Lz=L if nz > 1 else 1.0, dt=0.1,
+                       spline_order=order, orbit_quad_order=8, nyquist_projection=proj)
+
+
+def case_keys(kind, ppc, alpha, vth, v0, pert, mode):
+    keys = dict(particles_per_cell_pair=ppc, v0=v0, perturbation=pert, perturbation_mode=mode, n0=1.0,
+                thermal_velocity=vth)
+    keys[{P3_TWO_STREAM: "two_stream_density_alpha", P3_WEIBEL: "weibel_B0", P3_LANDAU: "landau_alpha"}[kind]] = alpha
+    return keys
+
+
+def ref_state(ref, p, kind, ppc, alpha, vth, v0, pert, mode):
+    args = (C.byref(p), REF_KIND[kind], ppc, C.c_double(alpha), C.c_double(vth), C.c_double(v0), C.c_double(pert),
+            mode, C.c_double(1.0))
+    h = HostState()
+    assert ref.p3.ref_p3_init(*args, C.byref(h)) == 0, ref.p3.ref_p3_last_error()
+    st = State(h.np, p.nx * p.ny * p.nz)
+    hh = st.host()
+    assert ref.p3.ref_p3_init(*args, C.byref(hh)) == 0
+    return st
+
+
+@pytest.fixture(scope="module")
+def ref():
+    r = Ref()
+    if r.p3 is None:
+        pytest.skip("reference harness (oracle/_ref) not built here")
+    r.p3.ref_p3_last_error.restype = C.c_char_p
+    return r
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"kind{c[0]}_ppc{c[1]}_mode{c[6]}")
+def test_particles_match_reference(ref, case):
+    """x, v, q, m bit for bit; A and U to rounding (the reference's are
+    Nyquist-projected, P3:3398-3405)."""
+    kind, ppc, alpha, vth, v0, pert, mode, grid = case
+    p = case_params(kind, grid)
+    got = U.p3_initial_state(p, U.p3_case(kind, **case_keys(kind, ppc, alpha, vth, v0, pert, mode)))
+    want = ref_state(ref, p, kind, ppc, alpha, vth, v0, pert, mode)
+    assert got["np"] == want.np
+    for f in ("x", "v", "q", "m"):
+        assert np.array_equal(got[f], getattr(want, f)), f
+    for f in ("A", "U"):
+        scale = max(1e-300, np.abs(getattr(want, f)).max())
+        assert np.abs(got[f] - getattr(want, f)).max() <= 1e-14 * scale, f
+
+
+def test_two_stream_rate_solves_dispersion_relation():
+    """1 = a/(w - k v0)^2 + a/(w + k v0)^2 at w = i Gamma, a = wp^2 |S|^2 /
+    (2 gamma0^3), and the eigenmode seed deposits alpha n0 cos(k x):
+    particle density amplitude -alpha/S (weights of mean 1 + nc cos)."""
+    for v0, order, mode in ((0.9, 2, 1), (0.5, 3, 2), (0.7, 1, 1)):
+        p = case_params(P3_TWO_STREAM, (16, 2, 1), order=order)
+        alpha = 0.002
+        s = U.p3_initial_state(p, U.p3_case(P3_TWO_STREAM, particles_per_cell_pair=2, v0=v0,
+                                            two_stream_density_alpha=alpha, perturbation_mode=mode))
+        G = s["growth_rate"]
+        assert G > 0
+        k = TWO_PI * mode / p.Lx
+        S = np.sinc(k * (p.Lx / p.nx) / (2 * np.pi)) ** (order + 1)
+        g0 = 1.0 / math.sqrt(1.0 - v0 * v0)
+        a = S * S / (2.0 * g0 ** 3)
+        w = 1j * G
+        assert abs(1.0 - a / (w - k * v0) ** 2 - a / (w + k * v0) ** 2) < 1e-12
+        # stream weights: mean 1 + nc cos(k x) with nc = -alpha / S
+        n = s["np"] // 2
+        x = s["x"][:n]
+        wsum = (s["q"][:n] + s["q"][n:2 * n]) / (2.0 * s["q"].mean())
+        nc = 2.0 * np.mean((wsum - 1.0) * np.cos(k * x))
+        assert abs(nc + alpha / S) < 1e-12 * alpha / S
+
+
+def test_weibel_rate_solves_dispersion_relation():
+    """w^4 - (kappa^2 k^2 + wp^2/gamma0^3) w^2 - k^2 wp^2 v0^2 / gamma0 = 0
+    at w = i Gamma (wp^2 with |S(k)|^2), and the two streams cancel in
+    charge at every lattice point."""
+    for v0, order, mode, kappa in ((0.9, 2, 1, 1.0), (0.6, 3, 2, 1.0), (0.3, 1, 1, 2.0)):
+        p = case_params(P3_WEIBEL, (4, 16, 1), order=order)
+        p.c = kappa
+        s = U.p3_initial_state(p, U.p3_case(P3_WEIBEL, particles_per_cell_pair=2, v0=v0, weibel_B0=1e-4,
+                                            perturbation_mode=mode))
+        G = s["growth_rate"]
+        k = TWO_PI * mode / p.Ly
+        S = np.sinc(k * (p.Ly / p.ny) / (2 * np.pi)) ** (order + 1)
+        g0 = 1.0 / math.sqrt(1.0 - (v0 / kappa) ** 2)
+        wp2 = S * S
+        w2 = -G * G
+        res = w2 * w2 - (kappa ** 2 * k * k + wp2 / g0 ** 3) * w2 - k * k * wp2 * v0 * v0 / g0
+        assert abs(res) < 1e-13 * (kappa ** 2 * k * k) ** 2
+        n = s["np"] // 2
+        assert np.allclose(s["q"][:n] + s["q"][n:], 2.0 * s["q"].mean(), rtol=1e-15, atol=0)
+        # A_x = (B0/k) cos(k y) at the nodes, U_x = Gamma A_x
+        y = -0.5 * p.Ly + np.arange(p.ny) * (p.Ly / p.ny)
+        Ax = np.tile((1e-4 / k) * np.cos(k * y), p.nx)
+        assert np.allclose(s["A"][:p.nx * p.ny], Ax, rtol=1e-15, atol=0)
+        assert np.allclose(s["U"][:p.nx * p.ny], G * Ax, rtol=1e-15, atol=0)
+        assert not s["A"][p.nx * p.ny:].any() and not s["U"][p.nx * p.ny:].any()
+
+
+def test_landau_quiet_start_is_normal_quantiles():
+    """Acklam's approximation: |relative error| < 1.15e-9 against the exact
+    quantile; every cell centre carries the same nv samples, weights
+    1 + alpha cos(k x)."""
+    nv, vth, alpha = 64, 0.1, 0.05
+    p = case_params(P3_LANDAU, (8, 2, 1))
+    s = U.p3_initial_state(p, U.p3_case(P3_LANDAU, particles_per_cell_pair=nv, thermal_velocity=vth,
+                                        landau_alpha=alpha))
+    exact = vth * norm.ppf((np.arange(nv) + 0.5) / nv)
+    v = s["v"][:s["np"]].reshape(-1, nv)
+    assert np.all(np.abs(v - exact) <= 1.2e-9 * np.abs(exact))
+    x = s["x"][:s["np"]]
+    q = s["q"]
+    q0 = -p.Lx * p.Ly * p.Lz / s["np"]
+    assert np.allclose(q, q0 * (1.0 + alpha * np.cos(TWO_PI / p.Lx * x)), rtol=1e-15, atol=0)
+
+
+def test_input_checks_raise_reference_messages():
+    p = case_params(P3_WEIBEL, (4, 16, 1))
+    with pytest.raises(U.UpicError, match="requires \\|v0\\| < kappa"):
+        U.p3_initial_state(p, U.p3_case(P3_WEIBEL, v0=1.0, weibel_B0=1e-4))
+    with pytest.raises(U.UpicError, match="density perturbation is too large"):
+        U.p3_initial_state(p, U.p3_case(P3_WEIBEL, v0=0.9, weibel_B0=0.5))
+    pt = case_params(P3_TWO_STREAM, (16, 2, 1))
+    with pytest.raises(U.UpicError, match="should be < 0.25"):
+        U.p3_initial_state(pt, U.p3_case(P3_TWO_STREAM, two_stream_density_alpha=0.3))
+    pl = case_params(P3_LANDAU, (8, 2, 1))
+    with pytest.raises(U.UpicError, match="at least two velocity samples"):
+        U.p3_initial_state(pl, U.p3_case(P3_LANDAU, particles_per_cell_pair=1))
+    with pytest.raises(U.UpicError, match="\\|v\\| < kappa"):
+        U.p3_initial_state(pl, U.p3_case(P3_LANDAU, particles_per_cell_pair=8, thermal_velocity=1.0))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"kind{c[0]}_ppc{c[1]}_mode{c[6]}")
+def test_initial_state_on_device_matches_reference(ref, case):
+    """The whole initial state (rho, phi, J, A, U, P, v_prev, vbar_last)
+    from upic_p3_initial_state + upic_sim_init_fields_AU against the
+    reference's initialize_state, FP64 to 1e-12 of each field's scale."""
+    kind, ppc, alpha, vth, v0, pert, mode, grid = case
+    p = case_params(kind, grid)
+    s = U.p3_initial_state(p, U.p3_case(kind, **case_keys(kind, ppc, alpha, vth, v0, pert, mode)))
+    want = ref_state(ref, p, kind, ppc, alpha, vth, v0, pert, mode)
+    N = p.nx * p.ny * p.nz
+    st = State(s["np"], N)
+    for f in ("x", "v", "q", "m"):
+        getattr(st, f)[:] = s[f]
+    print(f"init case kind={kind} ppc={ppc} grid={grid}", flush=True)
+    ctx = U.Context(p)
+    sim = U.Simulation(ctx, s["np"])
+    sim.upload(st)
+    sim.init_fields(1.0, A=s["A"] if kind == P3_WEIBEL else None, U=s["U"] if kind == P3_WEIBEL else None)
+    got = sim.download(State(s["np"], N))
+    for f in ("rho", "phi", "J", "A", "U", "P", "v_prev", "vbar_last", "psi"):
+        a, b = getattr(got, f), getattr(want, f)
+        # charge, potential and current of the neutral loads are rounding
+        # noise: their scale is n0 (= 1)
+        scale = max(np.abs(b).max(), 1.0 if f in ("rho", "phi", "psi", "J") else 1e-300)
+        assert np.abs(a - b).max() <= 1e-12 * scale + 1e-300, (f, np.abs(a - b).max(), scale)
+    if kind == P3_TWO_STREAM and alpha > 0:
+        # the eigenmode's grid charge: alpha n0 cos(k x) (P3:3723-3729)
+        xg = -0.5 * p.Lx + np.arange(p.nx) * (p.Lx / p.nx)
+        rho = got.rho.reshape(p.nx, p.ny * p.nz)
+        k = TWO_PI * mode / p.Lx
+        assert np.allclose(rho, (alpha * np.cos(k * xg))[:, None], rtol=0, atol=0.05 * alpha)
+
+
+def device_init(p, kind, **keys):
+    s = U.p3_initial_state(p, U.p3_case(kind, **keys))
+    N = p.nx * p.ny * p.nz
+    st = State(s["np"], N)
+    for f in ("x", "v", "q", "m"):
+        getattr(st, f)[:] = s[f]
+    ctx = U.Context(p)
+    sim = U.Simulation(ctx, s["np"])
+    sim.upload(st)
+    weibel = kind == P3_WEIBEL
+    sim.init_fields(keys.get("n0", 1.0), A=s["A"] if weibel else None, U=s["U"] if weibel else None)
+    return ctx, sim, s
+
+
+@pytest.mark.gpu
+def test_instability_initialization_and_theory():
+    """test_instability_initialization_and_theory (P3:7306-7351): both
+    instability loads are neutral (Gauss residual < 5e-10), the cold rates
+    are positive, the Weibel seed carries B_z(k_y) > 1e-6 and J > 1e-8."""
+    L = 2.0 * math.pi
+    pts = make_params(code=3, nx=4, ny=4, nz=4, Lx=4 * L, Ly=L, Lz=L, spline_order=1, nyquist_projection=1)
+    print("instability init: two-stream 4^3", flush=True)
+    _, sim, s = device_init(pts, P3_TWO_STREAM, particles_per_cell_pair=2, v0=0.9, perturbation=0.0,
+                            two_stream_density_alpha=0.01)
+    dts = sim.diagnostics(1)
+    pwb = make_params(code=3, nx=4, ny=4, nz=4, Lx=L, Ly=L, Lz=L, spline_order=1, nyquist_projection=1)
+    print("instability init: weibel 4^3", flush=True)
+    _, simw, sw = device_init(pwb, P3_WEIBEL, particles_per_cell_pair=1, v0=0.9, weibel_B0=1e-4)
+    dwb = simw.diagnostics(1)
+    st = simw.download(State(sw["np"], 64))
+    J_rms = math.sqrt(sum(np.mean(st.J[c * 64:(c + 1) * 64] ** 2) for c in range(3)) / 3.0)
+    assert s["growth_rate"] > 0 and dts.gauss_rms < 5e-10
+    assert sw["growth_rate"] > 0 and dwb.Bz_y_mode_abs > 1e-6 and J_rms > 1e-8 and dwb.gauss_rms < 5e-10
+
+
+@pytest.mark.gpu
+def test_two_stream_eigenmode_growth():
+    """test_two_stream_eigenmode_growth (P3:7371-7451): 80 FP64 steps of the
+    growing cold relativistic two-stream eigenmode on 16 cells; the fitted
+    growth of |E_x(k)| matches the finite-grid cold-fluid rate to 5e-3, the
+    mode follows E0 exp(Gamma t) to 5e-3 and the energy drifts < 5e-10."""
+    p = make_params(code=3, nx=16, ny=1, nz=1, Lx=8.0 * math.pi, Ly=2.0 * math.pi, Lz=2.0 * math.pi, dt=0.05,
+                    spline_order=1, orbit_quad_order=6, nonlinear_rtol=1e-10, nonlinear_max_iter=20,
+                    nyquist_projection=1)
+    print("two-stream eigenmode growth: 16 cells, 80 steps", flush=True)
+    _, sim, s = device_init(p, P3_TWO_STREAM, particles_per_cell_pair=4, v0=0.9, perturbation=0.0,
+                            two_stream_density_alpha=1e-3)
+    G = s["growth_rate"]
+    d0 = sim.diagnostics(1)
+    E0, W0 = d0.E_mode_abs, d0.total_energy
+    assert E0 > 0 and G > 0
+    ts, ys = [0.0], [math.log(E0)]
+    worst, drift = 0.0, 0.0
+    for n in range(80):
+        sim.step()
+        t = (n + 1) * p.dt
+        d = sim.diagnostics(1)
+        ts.append(t)
+        ys.append(math.log(max(d.E_mode_abs, 1e-300)))
+        worst = max(worst, abs(d.E_mode_abs / (E0 * math.exp(G * t)) - 1.0))
+        drift = max(drift, abs((d.total_energy - W0) / W0))
+    fit = np.polyfit(ts, ys, 1)[0]
+    print(f"gamma_fit={fit:.6e} gamma_ref={G:.6e} mode_err={worst:.3e} drift={drift:.3e}", flush=True)
+    assert abs(fit - G) / G < 5e-3
+    assert worst < 5e-3
+    assert drift < 5e-10

@@ -84,6 +84,19 @@ class LoadSpec(C.Structure):
                 ("v0", C.c_double), ("noise", C.c_double)]
 
 
+P3_TWO_STREAM, P3_WEIBEL, P3_LANDAU = 1, 2, 3  # upic_p3_case_kind
+
+
+class P3Case(C.Structure):
+    """upic_p3_case: the deck keys of the reference's P3 initialisers."""
+    _fields_ = [("kind", C.c_int32), ("particles_per_cell_pair", C.c_int32), ("perturbation_mode", C.c_int32),
+                ("tsi_transverse_mode", C.c_int32), ("two_stream_eigenmode", C.c_int32),
+                ("two_stream_finite_grid_theory", C.c_int32), ("weibel_eigenmode", C.c_int32),
+                ("weibel_finite_grid_theory", C.c_int32), ("n0", C.c_double), ("v0", C.c_double),
+                ("perturbation", C.c_double), ("two_stream_density_alpha", C.c_double), ("weibel_B0", C.c_double),
+                ("landau_alpha", C.c_double), ("thermal_velocity", C.c_double), ("tsi_transverse_seed", C.c_double)]
+
+
 def make_load(kind, ppc, seed=12345, n0=1.0, alpha=0.0, mode=1, vth=(0.0, 0.0, 0.0), v0=0.0, noise=0.0):
     s = LoadSpec()
     s.kind, s.ppc, s.seed, s.n0, s.alpha, s.mode = kind, ppc, seed, n0, alpha, mode

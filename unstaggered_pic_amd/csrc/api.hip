@@ -2,10 +2,8 @@
 // entry points.  The per-step drivers are in sim.hip.
 #include "api_internal.hpp"
 
-namespace {
+// message of the last failed call that has no context (upic_last_error(NULL))
 thread_local std::string g_create_err;
-
-}  // namespace
 
 extern "C" {
 

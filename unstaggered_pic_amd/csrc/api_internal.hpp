@@ -19,6 +19,9 @@
 
 using namespace upic;
 
+// message of the last failed call that has no context (upic_last_error(NULL))
+extern thread_local std::string g_create_err;
+
 namespace upic_host {
 
 struct Error : std::runtime_error {
@@ -180,6 +183,14 @@ inline void validate(const upic_params& p) {
     if (p.code != 2 && (p.spline_order < 1 || p.spline_order > 4)) throw Error(UPIC_EINVAL, "spline_order must be 1, 2, 3, or 4");
     if (p.code != 2 && (p.orbit_quad_order < 1 || p.orbit_quad_order > kMaxQuad))
         throw Error(UPIC_EINVAL, "orbit_quad_order must be in [1, 32]");
+    {
+        // the particle kernels wrap a stencil's node indices by one period,
+        // which needs at least as many nodes as the spline degree on every
+        // axis they span
+        const int order = p.code == 2 ? 2 : p.spline_order;
+        if (p.nx < order || p.ny < order || (p.nz > 1 && p.nz < order))
+            throw Error(UPIC_EINVAL, "every grid axis needs at least spline_order nodes (nz may be 1)");
+    }
     if (!(p.dt > 0.0)) throw Error(UPIC_EINVAL, "time stepping requires dt>0");
     if (!(p.Lx > 0.0) || !(p.Ly > 0.0) || !(p.Lz > 0.0)) throw Error(UPIC_EINVAL, "domain lengths must be positive");
     if (p.code == 3) {  // P3:1349-1433

@@ -1488,8 +1488,7 @@ upic_status upic_ensemble_step(upic_sim* const* sims, int32_t n, int32_t threads
     return first;
 }
 
-upic_status upic_sim_init_fields(upic_sim* s, double n0) {
-    if (!s) return UPIC_EINVAL;
+static upic_status init_fields(upic_sim* s, double n0, const double* A_host, const double* U_host) {
     upic_ctx* c = s->c;
     return (upic_status)guarded(c, [&] {
         s->jprev_valid = false;
@@ -1542,20 +1541,53 @@ upic_status upic_sim_init_fields(upic_sim* s, double n0) {
         launch_copy(c->stream, 2ll * N, Zf(s->zs, 1, N), Zf(s->zs, 2, N));
         from_spectra(c, Zf(s->zs, 2, N), s->phi, 1);
         launch_zero(c->stream, N, s->psi);
-        launch_zero(c->stream, 3ll * N, s->A);
-        launch_zero(c->stream, 3ll * N, s->U);
-        // E = -grad phi (U = 0), B = 0
+        const size_t fbytes = 3 * (size_t)N * sizeof(double);
+        if (A_host) HIPCHK(hipMemcpyAsync(s->A, A_host, fbytes, hipMemcpyHostToDevice, c->stream));
+        else launch_zero(c->stream, 3ll * N, s->A);
+        if (U_host) HIPCHK(hipMemcpyAsync(s->U, U_host, fbytes, hipMemcpyHostToDevice, c->stream));
+        else launch_zero(c->stream, 3ll * N, s->U);
+        // project_initial_state (P3:3398-3405): A and U on the projected space
+        if (A_host) launch_project_nyquist(c->stream, c->sp, s->A, 3);
+        if (U_host) launch_project_nyquist(c->stream, c->sp, s->U, 3);
+        // E = -grad phi - U (compute_E, P3:3338-3346)
         launch_spectral_grad(c->stream, c->sp, Zf(s->zs, 1, N), Zf(s->zs, 3, N));
         from_spectra(c, Zf(s->zs, 3, N), s->gtmp, 3);
-        launch_axpby(c->stream, 3ll * N, 0.0, nullptr, -1.0, s->gtmp);
-        launch_zero(c->stream, 6ll * N, s->gtmp + 3ll * N);  // B = 0 | A = 0
+        if (U_host) {
+            LinComb e{2, {-1.0, -1.0}, {s->gtmp, s->U}};
+            launch_lincomb(c->stream, 3ll * N, e, s->gtmp);
+        } else {
+            launch_axpby(c->stream, 3ll * N, 0.0, nullptr, -1.0, s->gtmp);
+        }
+        // B = curl A
+        if (A_host) {
+            to_spectra(c, s->A, Zf(s->zs, 6, N), 3);
+            launch_spectral_curl(c->stream, c->sp, Zf(s->zs, 6, N), Zf(s->zs, 9, N));
+            from_spectra(c, Zf(s->zs, 9, N), s->gtmp + 3ll * N, 3);
+        } else {
+            launch_zero(c->stream, 3ll * N, s->gtmp + 3ll * N);
+        }
+        // P = m gamma v + q A(x) and the history velocity (P3:3423-3441)
         launch_init_momenta(c->stream, c->g, np, p.code, p.c, p.dt, s->x, s->v, s->q, s->m, s->gtmp,
-                            s->gtmp + 3ll * N, s->gtmp + 6ll * N, s->P, s->v_prev, s->vbar_last);
+                            s->gtmp + 3ll * N, s->A, s->P, s->v_prev, s->vbar_last);
         HIPCHK(hipGetLastError());
         if (s->mixed) mixed_compress(s);
         c->sync();
         s->steps = 0;
     });
+}
+
+upic_status upic_sim_init_fields(upic_sim* s, double n0) {
+    if (!s) return UPIC_EINVAL;
+    return init_fields(s, n0, nullptr, nullptr);
+}
+
+upic_status upic_sim_init_fields_AU(upic_sim* s, double n0, const double* A, const double* U) {
+    if (!s) return UPIC_EINVAL;
+    if ((A || U) && s->c->p.code != 3) {
+        s->c->err = "upic_sim_init_fields_AU: a vector potential at t = 0 is a P3 initial condition";
+        return UPIC_EINVAL;
+    }
+    return init_fields(s, n0, A, U);
 }
 
 upic_status upic_sim_device_state(upic_sim* s, upic_host_state* d) {

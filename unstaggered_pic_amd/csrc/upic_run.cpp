@@ -11,18 +11,18 @@
 //   <prefix>_diagnostics.csv    (P1 1811-1842; P3 8444-8460 and 8561-8624)
 //   <prefix>_config_echo.txt    (P1 write_config_echo 1731-1770)
 //   <prefix>_particle_sample.csv (P1 write_particle_sample 1716-1729)
-// Columns whose quantity this build does not compute (P3 linear-theory
-// references, selective-refinement level counts, convergence-rate estimates)
-// are written as nan.  The step-diagnostics files (R_A and work residuals)
+// Columns whose quantity this build does not compute (selective-refinement
+// level counts, convergence-rate estimates) are written as nan.  The step-diagnostics files (R_A and work residuals)
 // are not produced.
 //
 // Initial conditions: P1 two_stream and landau_* are the reference's
 // deterministic loads (lattice / quiet start, P1:1151-1304) computed here on
 // the host, with the field tail (rho, J, phi, P, v_prev) on the device
-// (upic_sim_init_fields).  P3 decks use the seeded synthetic loads of
-// upic_load_particles (test_case = bimaxwell | beams | landau); the
-// reference's P3 eigenmode initialisers are not part of this build and are
-// rejected.
+// (upic_sim_init_fields).  P3 decks: the reference's two-stream, Weibel
+// and Landau initial conditions (P3:3689-4141, upic_p3_initial_state +
+// upic_sim_init_fields_AU) with its linear-theory growth-rate columns, or
+// the seeded synthetic loads of upic_load_particles (test_case = bimaxwell |
+// beams | landau_random).
 //
 // Exit codes as the reference: 0 ok, 1 error, 2 usage.
 #include "../../include/upic.h"
@@ -466,11 +466,56 @@ int run_p3(const std::string& deck_path) {
     d.get({"particle_local_max_iter", "uniform_local_max_iter"}, p.local_max_iter);
     d.get({"particle_local_damping", "uniform_local_damping"}, p.local_damping);
     d.get({"kappa"}, p.c); d.get({"sigma1"}, p.sigma1); d.get({"sigma2"}, p.sigma2);
-    d.get({"rho0", "n0"}, rho0); d.get({"v0"}, v0); d.get({"perturbation_mode"}, mode);
-    d.get({"perturbation", "landau_alpha"}, alpha);
-    d.get({"vth_x"}, vth[0]); d.get({"vth_y"}, vth[1]); d.get({"vth_z"}, vth[2]);
-    if (const std::string* v = d.find({"vth", "thermal_velocity"})) vth[0] = vth[1] = vth[2] = std::atof(v->c_str());
-    d.get({"position_noise"}, noise); d.get({"seed"}, seed);
+    // the reference's initial conditions (is_two_stream_case / is_weibel_case
+    // / is_landau_case, P3:1042-1105)
+    int ref_kind = 0;
+    if (tc == "two_stream" || tc == "two-stream" || tc == "twostream" || tc == "relativistic_two_stream" ||
+        tc == "relativistic-two-stream" || tc == "cold_relativistic_two_stream" ||
+        tc == "cold-relativistic-two-stream" || tc == "cold_two_stream" || tc == "cold-two-stream")
+        ref_kind = UPIC_P3_TWO_STREAM;
+    else if (tc == "weibel" || tc == "relativistic_weibel" || tc == "relativistic-weibel" ||
+             tc == "cold_relativistic_weibel" || tc == "cold-relativistic-weibel" || tc == "filamentation" ||
+             tc == "cold_filamentation" || tc == "cold-filamentation")
+        ref_kind = UPIC_P3_WEIBEL;
+    else if (is_landau(tc))
+        ref_kind = UPIC_P3_LANDAU;
+    upic_p3_case cs{};
+    Real theory_gamma = 0.0, theory_omega = 0.0;
+    d.get({"theory_gamma", "landau_theory_gamma"}, theory_gamma);
+    d.get({"theory_omega", "landau_theory_omega"}, theory_omega);
+    if (ref_kind) {
+        upic_p3_case_defaults(ref_kind, &cs);
+        cs.particles_per_cell_pair = d.has({"particles_per_cell_pair", "particles_per_cell", "velocity_samples_per_cell"})
+                                         ? ppc : cs.particles_per_cell_pair;
+        d.get({"n0", "rho0", "background_density"}, cs.n0); d.get({"v0"}, cs.v0);
+        d.get({"perturbation"}, cs.perturbation); d.get({"perturbation_mode"}, cs.perturbation_mode);
+        d.get({"two_stream_density_alpha", "density_perturbation", "alpha_x"}, cs.two_stream_density_alpha);
+        d.get({"weibel_B0", "magnetic_perturbation", "initial_B0"}, cs.weibel_B0);
+        d.get({"landau_alpha", "landau_density_perturbation"}, cs.landau_alpha);
+        d.get({"thermal_velocity", "vth"}, cs.thermal_velocity);
+        d.get({"tsi_transverse_seed"}, cs.tsi_transverse_seed); d.get({"tsi_transverse_mode"}, cs.tsi_transverse_mode);
+        bool b = true;
+        if (d.has({"finite_grid_theory"})) {
+            d.get_bool({"finite_grid_theory"}, b);
+            cs.two_stream_finite_grid_theory = cs.weibel_finite_grid_theory = b ? 1 : 0;
+        }
+        b = cs.two_stream_eigenmode != 0;
+        d.get_bool({"two_stream_eigenmode", "initialize_two_stream_eigenmode"}, b); cs.two_stream_eigenmode = b;
+        b = cs.two_stream_finite_grid_theory != 0;
+        d.get_bool({"two_stream_finite_grid_theory"}, b); cs.two_stream_finite_grid_theory = b;
+        b = cs.weibel_eigenmode != 0;
+        d.get_bool({"weibel_eigenmode", "initialize_weibel_eigenmode"}, b); cs.weibel_eigenmode = b;
+        b = cs.weibel_finite_grid_theory != 0;
+        d.get_bool({"weibel_finite_grid_theory"}, b); cs.weibel_finite_grid_theory = b;
+        rho0 = cs.n0;
+        mode = cs.perturbation_mode;
+    } else {
+        d.get({"rho0", "n0"}, rho0); d.get({"v0"}, v0); d.get({"perturbation_mode"}, mode);
+        d.get({"perturbation", "landau_alpha"}, alpha);
+        d.get({"vth_x"}, vth[0]); d.get({"vth_y"}, vth[1]); d.get({"vth_z"}, vth[2]);
+        if (const std::string* v = d.find({"vth", "thermal_velocity"})) vth[0] = vth[1] = vth[2] = std::atof(v->c_str());
+        d.get({"position_noise"}, noise); d.get({"seed"}, seed);
+    }
     d.get_str({"output_prefix"}, prefix);
     d.get_str({"precision"}, prec);
     d.get({"learned_guess"}, p.learned_guess);
@@ -488,30 +533,55 @@ int run_p3(const std::string& deck_path) {
     }
     const int learned = p.learned_guess;
     if (learned == UPIC_GUESS_LEARNED) p.learned_guess = UPIC_GUESS_PREDICTOR;  // until the network is trained
-    upic_load_spec ls{};
-    ls.ppc = ppc; ls.seed = seed; ls.n0 = rho0; ls.alpha = alpha; ls.mode = mode;
-    ls.vth[0] = vth[0]; ls.vth[1] = vth[1]; ls.vth[2] = vth[2]; ls.v0 = v0; ls.noise = noise;
-    if (tc == "bimaxwell" || tc == "weibel_bimaxwell" || tc == "maxwellian") ls.kind = UPIC_LOAD_BIMAXWELL;
-    else if (tc == "beams" || tc == "counter_streaming_beams") ls.kind = UPIC_LOAD_BEAMS;
-    else if (tc == "landau") ls.kind = UPIC_LOAD_LANDAU;
-    else throw std::runtime_error("initial condition not available in this build: " + tc +
-                                  " (supported: bimaxwell, beams, landau)");
     Run r;
     r.N = p.nx * p.ny * p.nz;
-    r.np = (int64_t)r.N * ppc;
     Particles P;
-    P.resize(r.np);
     upic_host_state hs{};
-    hs.np = r.np;
-    hs.x = P.x.data(); hs.v = P.v.data(); hs.q = P.q.data(); hs.m = P.m.data();
-    check(upic_load_particles(&p, &ls, &hs), nullptr, "particle load");
+    std::vector<Real> A0(3 * (size_t)r.N, 0.0), U0(3 * (size_t)r.N, 0.0);
+    double growth = 0.0;
+    if (ref_kind) {
+        check(upic_p3_initial_state(&p, &cs, &hs, nullptr), nullptr, "initial condition");
+        r.np = hs.np;
+        P.resize(r.np);
+        hs.x = P.x.data(); hs.v = P.v.data(); hs.q = P.q.data(); hs.m = P.m.data();
+        hs.A = A0.data(); hs.U = U0.data();
+        check(upic_p3_initial_state(&p, &cs, &hs, &growth), nullptr, "initial condition");
+    } else {
+        upic_load_spec ls{};
+        ls.ppc = ppc; ls.seed = seed; ls.n0 = rho0; ls.alpha = alpha; ls.mode = mode;
+        ls.vth[0] = vth[0]; ls.vth[1] = vth[1]; ls.vth[2] = vth[2]; ls.v0 = v0; ls.noise = noise;
+        if (tc == "bimaxwell" || tc == "weibel_bimaxwell" || tc == "maxwellian") ls.kind = UPIC_LOAD_BIMAXWELL;
+        else if (tc == "beams" || tc == "counter_streaming_beams") ls.kind = UPIC_LOAD_BEAMS;
+        else if (tc == "landau_random") ls.kind = UPIC_LOAD_LANDAU;
+        else throw std::runtime_error("unknown test_case/initial_condition: " + tc +
+                                      " (two_stream, weibel, landau or the seeded bimaxwell, beams, landau_random)");
+        r.np = (int64_t)r.N * ppc;
+        P.resize(r.np);
+        hs.np = r.np;
+        hs.x = P.x.data(); hs.v = P.v.data(); hs.q = P.q.data(); hs.m = P.m.data();
+        check(upic_load_particles(&p, &ls, &hs), nullptr, "particle load");
+    }
     check(upic_ctx_create(&p, &r.ctx), nullptr, "context");
     check(upic_sim_create(r.ctx, r.np, &r.sim), r.ctx, "simulation");
     std::vector<Real> zeros(3 * (size_t)r.np, 0.0), fz(3 * (size_t)r.N, 0.0);
     hs.v_prev = zeros.data(); hs.P = zeros.data(); hs.vbar_last = zeros.data();
     hs.phi = fz.data(); hs.psi = fz.data(); hs.rho = fz.data(); hs.A = fz.data(); hs.U = fz.data(); hs.J = fz.data();
     check(upic_sim_upload(r.sim, &hs), r.ctx, "upload");
-    check(upic_sim_init_fields(r.sim, rho0), r.ctx, "initial fields");
+    if (ref_kind == UPIC_P3_WEIBEL)
+        check(upic_sim_init_fields_AU(r.sim, rho0, A0.data(), U0.data()), r.ctx, "initial fields");
+    else
+        check(upic_sim_init_fields(r.sim, rho0), r.ctx, "initial fields");
+    // linear-theory reference (P3:8427-8433): the selected growth rate times
+    // the initial |E_x(k_x)| (|B_z(k_y)| for Weibel)
+    const Real theory_growth = std::abs(theory_gamma) > 0.0 ? theory_gamma : growth;
+    upic_diagnostics d0{};
+    check(upic_sim_diagnostics(r.sim, mode, &d0), r.ctx, "diagnostics");
+    const bool use_B = ref_kind == UPIC_P3_WEIBEL;
+    const Real theory_amp0 = use_B ? d0.Bz_y_mode_abs : d0.E_mode_abs;
+    const bool has_theory = (std::abs(theory_growth) > 0.0 || std::abs(theory_omega) > 0.0) && theory_amp0 > 0.0;
+    if (std::abs(theory_growth) > 0.0)
+        std::cout << std::scientific << std::setprecision(6) << "linear theory growth/damping rate = " << theory_growth
+                  << (use_B ? " using |B_z(k_y)|" : " using |E_x(k_x)|") << "\n";
     if (learned == UPIC_GUESS_LEARNED) {
         const int64_t stride = std::max<int64_t>(1, r.np / 16384);
         check(upic_sim_collect_guess_data(r.sim, stride, ((r.np + stride - 1) / stride) * 20), r.ctx, "collect");
@@ -540,13 +610,18 @@ int run_p3(const std::string& deck_path) {
         const Real gn = g.divE_rms + g.rho_rms, an = g.psi_over_c2_rms + g.divA_rms;
         const std::string method = in.iterations == 0 ? "none"
                                    : (in.anderson_steps > 0 ? "anderson" : (in.picard_fallback_steps > 0 ? "picard" : "direct"));
+        Real th_abs = kNaN, th_ratio = kNaN;
+        if (has_theory) {
+            th_abs = theory_amp0 * std::exp(theory_growth * t);
+            th_ratio = (use_B ? g.Bz_y_mode_abs : g.E_mode_abs) / (th_abs + 1.0e-300);
+        }
         diag << step << "," << std::setprecision(17) << t << "," << g.gauss_rms << "," << g.gauss_max << ","
              << g.divE_rms << "," << g.rho_rms << "," << gn << "," << (gn > 0.0 ? g.gauss_rms / gn : 0.0) << ","
              << g.gauge_rms << "," << g.gauge_max << "," << g.psi_over_c2_rms << "," << g.divA_rms << "," << an << ","
              << (an > 0.0 ? g.gauge_rms / an : 0.0) << "," << g.field_energy << "," << g.kinetic_energy << ","
              << g.total_energy << "," << g.rho_mode << "," << g.E_mode_re << "," << g.E_mode_im << ","
              << g.E_mode_abs << "," << g.Bz_y_mode_re << "," << g.Bz_y_mode_im << "," << g.Bz_y_mode_abs << ","
-             << kNaN << "," << kNaN << "," << kNaN << "," << g.mean_rho << "," << in.iterations << ","
+             << theory_growth << "," << th_abs << "," << th_ratio << "," << g.mean_rho << "," << in.iterations << ","
              << in.converged << "," << in.final_rms_residual << "," << in.final_max_particle_residual << ","
              << in.final_absolute_rms_residual << "," << in.anderson_restarts << "," << in.anderson_steps << ","
              << in.picard_fallback_steps << "," << method << "," << kNaN << "," << in.estimated_map_contraction

@@ -9,7 +9,7 @@ import os
 
 import numpy as np
 
-from .abi import Diagnostics, HostState, LoadSpec, Params, PathStats, StepInfo, STATUS_NAMES
+from .abi import Diagnostics, HostState, LoadSpec, P3Case, Params, PathStats, StepInfo, STATUS_NAMES
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # UPIC_LIBRARY selects a tuning variant built with build.py -o
@@ -62,6 +62,8 @@ def lib():
             "upic_sim_device_state": [vp, C.POINTER(HostState)],
             "upic_sim_failure_reasons": [vp, C.POINTER(C.c_uint8)],
             "upic_sim_init_fields": [vp, C.c_double],
+            "upic_sim_init_fields_AU": [vp, C.c_double, vp, vp],
+            "upic_p3_initial_state": [C.POINTER(Params), C.POINTER(P3Case), C.POINTER(HostState), dp],
             "upic_ctx_set_timing": [vp, C.c_int],
             "upic_device_count": [C.POINTER(C.c_int)], "upic_device_alloc": [C.c_size_t, C.POINTER(vp)],
             "upic_device_free": [vp], "upic_memcpy_htod": [vp, vp, C.c_size_t],
@@ -84,6 +86,8 @@ def lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = C.c_int
+        L.upic_p3_case_defaults.argtypes = [C.c_int32, C.POINTER(P3Case)]
+        L.upic_p3_case_defaults.restype = None
         _lib = L
     return _lib
 
@@ -361,8 +365,16 @@ class Simulation:
         self.ctx._check(lib().upic_sim_diagnostics(self.h, mode, C.byref(d)))
         return d
 
-    def init_fields(self, n0=1.0):
-        self.ctx._check(lib().upic_sim_init_fields(self.h, n0))
+    def init_fields(self, n0=1.0, A=None, U=None):
+        """Initial mesh state from the uploaded particles; A and U (3N
+        float64 host arrays, P3 only) set the vector potential at t = 0."""
+        if A is None and U is None:
+            self.ctx._check(lib().upic_sim_init_fields(self.h, n0))
+            return
+        A = None if A is None else np.ascontiguousarray(A, np.float64)
+        U = None if U is None else np.ascontiguousarray(U, np.float64)
+        self.ctx._check(lib().upic_sim_init_fields_AU(self.h, n0, None if A is None else A.ctypes.data_as(vp),
+                                                      None if U is None else U.ctypes.data_as(vp)))
 
     def failure_reasons(self):
         r = np.zeros(self.np, np.uint8)
@@ -421,6 +433,40 @@ def load_particles(params: Params, spec, np_):
     if rc != 0:
         raise UpicError(rc, "upic_load_particles rejected its arguments")
     return x, v, q, m
+
+
+def p3_case(kind, **keys):
+    """upic_p3_case with the reference's defaults, overridden by `keys`."""
+    cs = P3Case()
+    lib().upic_p3_case_defaults(int(kind), C.byref(cs))
+    for k, v in keys.items():
+        if not hasattr(cs, k):
+            raise KeyError(k)
+        setattr(cs, k, v)
+    return cs
+
+
+def p3_initial_state(params: Params, case):
+    """The reference's P3 initial condition (host C++): returns a dict with
+    x, v, q, m (particles), A, U (mesh, 3N) and the linear-theory growth
+    rate."""
+    h = HostState()
+    rc = lib().upic_p3_initial_state(C.byref(params), C.byref(case), C.byref(h), None)
+    if rc != 0:
+        raise UpicError(rc, lib().upic_last_error(None).decode())
+    n = h.np
+    N = params.nx * params.ny * params.nz
+    out = {"x": np.zeros(3 * n), "v": np.zeros(3 * n), "q": np.zeros(n), "m": np.zeros(n),
+           "A": np.zeros(3 * N), "U": np.zeros(3 * N)}
+    for k, a in out.items():
+        setattr(h, k, a.ctypes.data_as(dp))
+    rate = C.c_double()
+    rc = lib().upic_p3_initial_state(C.byref(params), C.byref(case), C.byref(h), C.byref(rate))
+    if rc != 0:
+        raise UpicError(rc, lib().upic_last_error(None).decode())
+    out["np"] = n
+    out["growth_rate"] = rate.value
+    return out
 
 
 def ensemble_step(sims, threads=0):
