@@ -300,6 +300,29 @@ upic_status upic_sim_init_fields(upic_sim* sim, double n0);
  * B = curl A drive the history velocity, and P = m gamma v + q A(x)
  * (project_initial_state, P3:3398-3405). */
 upic_status upic_sim_init_fields_AU(upic_sim* sim, double n0, const double* A, const double* U);
+/* ---- Step diagnostics (compute_step_diagnostics, P1:1384-1439,
+ * P3:4501-4570): the orbit residual R_A of the vector-potential chain rule
+ * (compute_RA_orbit_residual, P3:4242-4296) and the energy/work identities
+ * of the last step, from the step's start state (kept on the device while
+ * upic_sim_keep_step_start is enabled, after the step's cell sort) and the
+ * current state.  vbar is (v^n + v^{n+1})/2 for P1 and the accepted
+ * vbar_last for P3.  Fields in the order of the reference's
+ * <prefix>_step_diagnostics.csv columns (P1:1818-1819). */
+typedef struct upic_step_diagnostics {
+    double A_chain_abs_rms, A_chain_rel_rms, A_chain_max_norm, A_chain_mean_norm, A_chain_work_defect;
+    double delta_K, delta_W, delta_total_energy, particle_work, grid_work;
+    double deposit_gather_work_residual, particle_energy_residual, field_energy_residual;
+    int64_t crossing_particles, total_crossings;
+    double avg_path_segments;
+    int32_t max_path_segments, pad;
+} upic_step_diagnostics;
+/* Keep (enable != 0) or stop keeping each step's start state (P1, P3;
+ * 6 np + 7 N doubles of HBM). */
+upic_status upic_sim_keep_step_start(upic_sim* sim, int enable);
+/* The step diagnostics of the last step; UPIC_EINVAL without a kept start
+ * or after a step that needed the selective per-particle refinement (the
+ * reference's subcycled variant, P3:4363-4490, is not implemented). */
+upic_status upic_sim_step_diagnostics(upic_sim* sim, upic_step_diagnostics* out);
 /* Device pointers of the resident state (for zero-copy callers).  With
  * UPIC_PREC_MIXED the particle pointers are an FP64 view rebuilt from the
  * FP32 state by this call; writes to them reach the simulation only through

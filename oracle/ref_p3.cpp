@@ -145,6 +145,25 @@ int ref_p3_step(const upic_params* p, const upic_host_state* in, upic_host_state
         return 0;
     } catch (const std::exception& e) { ref_err = e.what(); return 1; }
 }
+// compute_step_diagnostics (P3:4501-4570) of s0 -> s1; out17 in the order
+// of upic_step_diagnostics (13 doubles, crossing_particles, total_crossings,
+// avg_path_segments, max_path_segments)
+int ref_p3_step_diagnostics(const upic_params* p, const upic_host_state* s0, const upic_host_state* s1,
+                            double* out17) {
+    try {
+        const Config c = p3_cfg(p); const Grid g = make_grid(c); const Quadrature qd = gauss_legendre_01(c.orbit_quad_order);
+        const State a = p3_load(s0, g.N, c.dt), b = p3_load(s1, g.N, c.dt);
+        const StepDiagnostics d = compute_step_diagnostics(a, b, g, c.dt, c.spline_order, qd, c.sigma1, c.sigma2,
+                                                           c.kappa, c.split_orbit_at_knots);
+        const double v[17] = {d.A_chain_abs_rms, d.A_chain_rel_rms, d.A_chain_max_norm, d.A_chain_mean_norm,
+                              d.A_chain_work_defect, d.delta_K, d.delta_W, d.delta_total_energy, d.particle_work,
+                              d.grid_work, d.deposit_gather_work_residual, d.particle_energy_residual,
+                              d.field_energy_residual, (double)d.crossing_particles, (double)d.total_crossings,
+                              d.average_path_segments, (double)d.max_path_segments};
+        std::memcpy(out17, v, sizeof(v));
+        return 0;
+    } catch (const std::exception& e) { ref_err = e.what(); return 1; }
+}
 int ref_p3_diagnostics(const upic_params* p, const upic_host_state* s, int mode, double* out10) {
     try {
         const Config c = p3_cfg(p); const Grid g = make_grid(c);

@@ -83,6 +83,23 @@ def test_driver_p1_landau_weak_matches_reference_csv(tmp_path):
     assert echo == ref_echo
     sample = list(csv.DictReader(open(tmp_path / "landau_weak_particle_sample.csv")))
     assert len(sample) == 1024 and list(sample[0].keys()) == ["particle", "x", "y", "z", "vx", "vy", "vz", "q", "m"]
+    # <prefix>_step_diagnostics.csv (compute_step_diagnostics, P1:1384-1439):
+    # energy changes and works as the reference's own file, path statistics
+    # exact; the chain-rule defect and the work residuals are rounding noise
+    # in both (P1's E and J are exactly conjugate)
+    got = list(csv.DictReader(open(tmp_path / "landau_weak_step_diagnostics.csv")))
+    ref = list(csv.DictReader(open(os.path.join(HERE, "golden", "landau_weak_step_diagnostics.csv"))))
+    assert len(got) == len(ref) == 300
+    assert list(got[0].keys()) == list(ref[0].keys())
+    for k, (a, b) in enumerate(zip(got, ref)):
+        for col in ("delta_K", "delta_W", "particle_work", "grid_work"):
+            assert abs(float(a[col]) - float(b[col])) <= 1e-9 * abs(float(b[col])) + 1e-14, (k, col)
+        for col in ("crossing_particles", "total_crossings", "max_path_segments", "avg_path_segments"):
+            assert float(a[col]) == float(b[col]), (k, col)
+        assert float(a["A_chain_abs_rms"]) < 1e-15 and float(a["A_chain_rel_rms"]) < 1e-12, k
+        for col in ("delta_total_energy", "deposit_gather_work_residual", "particle_energy_residual",
+                    "field_energy_residual"):
+            assert abs(float(a[col])) < 1e-12, (k, col)
 
 
 P3_DECK = """test_case = bimaxwell

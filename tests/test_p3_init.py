@@ -271,3 +271,56 @@ def test_two_stream_eigenmode_growth():
     assert abs(fit - G) / G < 5e-3
     assert worst < 5e-3
     assert drift < 5e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_step_diagnostics_match_reference(ref, precision):
+    """upic_sim_step_diagnostics against the reference's
+    compute_step_diagnostics (P3:4501-4570) evaluated on the same start and
+    end states (the FP64 view of a mixed-precision state): R_A norms, the
+    energy changes, particle and grid works, path statistics."""
+    from unstaggered_pic_amd.abi import PREC_MIXED
+    L = 2.0 * math.pi
+    p = make_params(code=3, nx=8, ny=16, nz=1, Lx=L, Ly=L, Lz=1.0, dt=0.1, spline_order=2, orbit_quad_order=6,
+                    nyquist_projection=1, nonlinear_rtol=1e-11, nonlinear_atol=1e-13, nonlinear_max_iter=80,
+                    anderson_damping=0.5, local_rtol=1e-12, local_atol=1e-14, local_max_iter=64)
+    if precision == "mixed":
+        p.precision = PREC_MIXED
+        p.local_rtol, p.local_atol, p.nonlinear_rtol, p.nonlinear_atol = 1e-6, 1e-7, 3e-6, 1e-9
+    print(f"step diagnostics: weibel 8x16 {precision}", flush=True)
+    ctx, sim, s = device_init(p, P3_WEIBEL, particles_per_cell_pair=4, v0=0.9, weibel_B0=1e-3)
+    sim.keep_step_start(True)
+    N = p.nx * p.ny
+    for _ in range(2):
+        s0 = sim.download(State(s["np"], N))
+        sim.step()
+        d = sim.step_diagnostics()
+        s1 = sim.download(State(s["np"], N))
+        want = np.zeros(17)
+        assert ref.p3.ref_p3_step_diagnostics(C.byref(p), C.byref(s0.host()), C.byref(s1.host()),
+                                              want.ctypes.data_as(C.POINTER(C.c_double))) == 0
+        scaleK = max(abs(want[5]), abs(want[6]))
+        # energy changes are differences of the total energies: rounding of E
+        E = sim.diagnostics(1).total_energy
+        for name, k in (("delta_K", 5), ("delta_W", 6)):
+            assert abs(getattr(d, name) - want[k]) <= 1e-9 * scaleK + 1e-14 * E, (name, getattr(d, name), want[k])
+        for name, k in (("particle_work", 8), ("grid_work", 9)):
+            assert abs(getattr(d, name) - want[k]) <= 1e-9 * scaleK, (name, getattr(d, name), want[k])
+        # the chain-rule defect: to 1e-6 where it is resolved, else rounding
+        # noise in both
+        print(f"R_A abs rms {d.A_chain_abs_rms:.3e} (reference {want[0]:.3e})", flush=True)
+        for got_v, want_v in ((d.A_chain_abs_rms, want[0]), (d.A_chain_max_norm, want[2]),
+                              (d.A_chain_mean_norm, want[3])):
+            if want_v > 1e-14:
+                assert abs(got_v - want_v) <= 1e-6 * want_v
+            else:
+                assert got_v <= 1e-14
+        assert abs(d.A_chain_work_defect - want[4]) <= 1e-6 * abs(want[4]) + 1e-10 * scaleK
+        assert (d.crossing_particles, d.total_crossings, d.max_path_segments) == (want[13], want[14], want[16])
+        assert d.avg_path_segments == pytest.approx(want[15], rel=1e-15)
+        # the scheme's exact energy identity: delta_K = particle work, delta_W = -grid work
+        # (mixed: kinetic energies of FP32 velocities, ~1e-7 relative per step)
+        tol = 1e-13 * E if precision == "fp64" else 1e-6 * E
+        assert abs(d.particle_energy_residual) <= tol
+        assert abs(d.field_energy_residual) <= tol

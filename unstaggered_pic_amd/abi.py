@@ -84,6 +84,16 @@ class LoadSpec(C.Structure):
                 ("v0", C.c_double), ("noise", C.c_double)]
 
 
+class StepDiagnostics(C.Structure):
+    """upic_step_diagnostics (the reference's step-diagnostics CSV columns)."""
+    _fields_ = [(n, C.c_double) for n in (
+        "A_chain_abs_rms", "A_chain_rel_rms", "A_chain_max_norm", "A_chain_mean_norm", "A_chain_work_defect",
+        "delta_K", "delta_W", "delta_total_energy", "particle_work", "grid_work", "deposit_gather_work_residual",
+        "particle_energy_residual", "field_energy_residual")] + [
+        ("crossing_particles", C.c_int64), ("total_crossings", C.c_int64), ("avg_path_segments", C.c_double),
+        ("max_path_segments", C.c_int32), ("pad", C.c_int32)]
+
+
 P3_TWO_STREAM, P3_WEIBEL, P3_LANDAU = 1, 2, 3  # upic_p3_case_kind
 
 

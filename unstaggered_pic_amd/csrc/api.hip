@@ -160,6 +160,7 @@ upic_status upic_deposit_current_orbit(upic_ctx* c, const double* x0, const doub
                                        const double* q, int64_t np, double* J, upic_path_stats* stats) {
     if (!c || np < 0) return UPIC_EINVAL;
     return (upic_status)guarded(c, [&] {
+        require_stencil_grid(c->p);
         launch_zero(c->stream, 3 * (int64_t)c->N(), J);
         HIPCHK(hipMemsetAsync(c->dstats, 0, 3 * sizeof(unsigned long long), c->stream));
         launch_deposit_orbit(c->stream, c->g, c->q, c->opt, x0, dx, vbar, q, np, J, c->dstats);
@@ -172,6 +173,7 @@ upic_status upic_deposit_current_orbit(upic_ctx* c, const double* x0, const doub
 upic_status upic_gather_orbit(upic_ctx* c, const double* x0, const double* dx, const double* f3, int64_t np, double* out) {
     if (!c || np < 0) return UPIC_EINVAL;
     return (upic_status)guarded(c, [&] {
+        require_stencil_grid(c->p);
         launch_gather_orbit(c->stream, c->g, c->q, c->opt, x0, dx, f3, np, out);
         HIPCHK(hipGetLastError());
     });
@@ -181,6 +183,7 @@ upic_status upic_orbit_grad_A(upic_ctx* c, const double* x0, const double* dx, c
                               double theta0, double theta1, int64_t np, double* D) {
     if (!c || np < 0) return UPIC_EINVAL;
     return (upic_status)guarded(c, [&] {
+        require_stencil_grid(c->p);
         const int p1_theta = (c->p.code == 1 && theta0 == 0.0 && theta1 == 1.0) ? 1 : 0;
         launch_grad_A(c->stream, c->g, c->q, c->opt, x0, dx, A_n, A_np1, theta0, theta1, p1_theta, np, D);
         HIPCHK(hipGetLastError());
@@ -190,6 +193,7 @@ upic_status upic_orbit_grad_A(upic_ctx* c, const double* x0, const double* dx, c
 upic_status upic_gather_point(upic_ctx* c, const double* pos, const double* f3, int64_t np, double* out) {
     if (!c || np < 0) return UPIC_EINVAL;
     return (upic_status)guarded(c, [&] {
+        require_stencil_grid(c->p);
         launch_gather_point(c->stream, c->g, pos, f3, np, out);
         HIPCHK(hipGetLastError());
     });
@@ -199,6 +203,7 @@ upic_status upic_deposit_scalar_point(upic_ctx* c, const double* pos, const doub
                                       int divide_by_dV, double* mesh) {
     if (!c || np < 0) return UPIC_EINVAL;
     return (upic_status)guarded(c, [&] {
+        require_stencil_grid(c->p);
         launch_zero(c->stream, c->N(), mesh);
         launch_deposit_scalar_point(c->stream, c->g, pos, values, np, divide_by_dV ? 1.0 / c->g.dV : 1.0, mesh);
         launch_project_nyquist(c->stream, c->sp, mesh, 1);

@@ -174,6 +174,15 @@ struct upic_ctx {
 
 namespace upic_host {
 
+// The particle kernels wrap a stencil's node indices by one period, which
+// needs at least as many nodes as the spline degree on every axis they span
+// (field-only operations have no such limit).
+inline void require_stencil_grid(const upic_params& p) {
+    const int order = p.code == 2 ? 2 : p.spline_order;
+    if (p.nx < order || p.ny < order || (p.nz > 1 && p.nz < order))
+        throw Error(UPIC_EINVAL, "particle operations need at least spline_order grid nodes on every axis (nz may be 1)");
+}
+
 inline void validate(const upic_params& p) {
     if (p.code < 1 || p.code > 3) throw Error(UPIC_EINVAL, "code must be 1, 2 or 3");
     if (!is_pow2(p.nx) || !is_pow2(p.ny) || !is_pow2(p.nz))
@@ -183,14 +192,6 @@ inline void validate(const upic_params& p) {
     if (p.code != 2 && (p.spline_order < 1 || p.spline_order > 4)) throw Error(UPIC_EINVAL, "spline_order must be 1, 2, 3, or 4");
     if (p.code != 2 && (p.orbit_quad_order < 1 || p.orbit_quad_order > kMaxQuad))
         throw Error(UPIC_EINVAL, "orbit_quad_order must be in [1, 32]");
-    {
-        // the particle kernels wrap a stencil's node indices by one period,
-        // which needs at least as many nodes as the spline degree on every
-        // axis they span
-        const int order = p.code == 2 ? 2 : p.spline_order;
-        if (p.nx < order || p.ny < order || (p.nz > 1 && p.nz < order))
-            throw Error(UPIC_EINVAL, "every grid axis needs at least spline_order nodes (nz may be 1)");
-    }
     if (!(p.dt > 0.0)) throw Error(UPIC_EINVAL, "time stepping requires dt>0");
     if (!(p.Lx > 0.0) || !(p.Ly > 0.0) || !(p.Lz > 0.0)) throw Error(UPIC_EINVAL, "domain lengths must be positive");
     if (p.code == 3) {  // P3:1349-1433

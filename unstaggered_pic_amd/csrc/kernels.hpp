@@ -263,6 +263,18 @@ void launch_max_abs(hipStream_t s, int64_t n, const double* f, double* out);
 // P3 relativistic), P2 max speed and mean gamma.
 void launch_kinetic(hipStream_t s, int64_t np, const double* v, const double* m, double c, int relativistic,
                     double* out3);
+// Step diagnostics (compute_step_diagnostics, P1:1384-1439, P3:4501-4570):
+// vbar = (v0 + v1)/2 (midpoint, P1) or vbar_last (P3), dx = dt vbar,
+// x1 = wrap(x0 + dx); then the orbit residual R_A and the particle sums,
+// accumulated into sums6 = {sum|R|^2, sum|A_delta|^2, sum|rhs|^2, sum|R|,
+// sum -q vbar.R, sum q vbar.Ebar} and max1 = {max|R|}.  part and part2 hold
+// partial_blocks(np) * kPartSlots doubles each.
+void launch_step_diag_orbit(hipStream_t s, int64_t np, int midpoint, double dt, const double L[3], const double* x0,
+                            const double* v0, const double* v1, const double* vbar_last, double* vbar, double* dx,
+                            double* x1);
+void launch_step_diag_particles(hipStream_t s, int64_t np, double dt, const double* q, const double* vbar,
+                                const double* Ubar, const double* D, const double* A0p, const double* A1p,
+                                const double* Ebar, double* part, double* part2, double* sums6, double* max1);
 
 // Cell sort of particles (counting sort by cell of x): keys, histogram, scan,
 // permutation, and the gather of SoA arrays through it.

@@ -647,6 +647,84 @@ void launch_finish_partials(hipStream_t s, const double* part, int64_t nblocks, 
     hipLaunchKernelGGL(k_fold_partials, dim3((unsigned)g), dim3(256), 0, s, part, nblocks, per, out);
     hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(256), 0, s, out, g, ns, nm, sum, mx, stats);
 }
+// Step diagnostics (compute_step_diagnostics, P1:1384-1439 / P3:4501-4570)
+__global__ __launch_bounds__(256) void k_step_diag_orbit(int64_t np, int midpoint, double dt, double Lx, double Ly,
+                                                         double Lz, const double* __restrict__ x0,
+                                                         const double* __restrict__ v0, const double* __restrict__ v1,
+                                                         const double* __restrict__ vbl, double* __restrict__ vbar,
+                                                         double* __restrict__ dx, double* __restrict__ x1) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const double L[3] = {Lx, Ly, Lz};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int64_t i = d * np + p;
+        const double vb = midpoint ? 0.5 * (v1[i] + v0[i]) : vbl[i];
+        const double dd = dt * vb;
+        vbar[i] = vb;
+        dx[i] = dd;
+        x1[i] = wrap1(x0[i] + dd, L[d]);
+    }
+}
+
+// R = (A^{n+1}(x1) - A^n(x0)) - dt (Ubar + D_A vbar) per particle
+// (compute_RA_orbit_residual, P3:4242-4296) and the particle sums:
+// part: sum |R|^2, sum |A_delta|^2, sum |rhs|^2, sum |R| and max |R|;
+// part2: sum -q vbar.R (work defect), sum q vbar.Ebar (particle work / dt)
+__global__ __launch_bounds__(256) void k_step_diag_particles(int64_t np, double dt, const double* __restrict__ q,
+                                                             const double* __restrict__ vbar,
+                                                             const double* __restrict__ Ubar,
+                                                             const double* __restrict__ D,
+                                                             const double* __restrict__ A0p,
+                                                             const double* __restrict__ A1p,
+                                                             const double* __restrict__ Ebar,
+                                                             double* __restrict__ part, double* __restrict__ part2) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s4[4] = {0.0, 0.0, 0.0, 0.0}, mx[1] = {0.0}, w2[2] = {0.0, 0.0};
+    if (p < np) {
+        double vb[3], R[3], Ad[3], rhs[3], Eb[3];
+        ld3(vbar, np, p, vb);
+        ld3(Ebar, np, p, Eb);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            Ad[a] = A1p[a * np + p] - A0p[a * np + p];
+            const double DAv = D[(3 * a) * np + p] * vb[0] + D[(3 * a + 1) * np + p] * vb[1] +
+                               D[(3 * a + 2) * np + p] * vb[2];
+            rhs[a] = dt * (Ubar[a * np + p] + DAv);
+            R[a] = Ad[a] - rhs[a];
+        }
+        const double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+        const double nr = sqrt(r2);
+        s4[0] = r2;
+        s4[1] = Ad[0] * Ad[0] + Ad[1] * Ad[1] + Ad[2] * Ad[2];
+        s4[2] = rhs[0] * rhs[0] + rhs[1] * rhs[1] + rhs[2] * rhs[2];
+        s4[3] = nr;
+        mx[0] = nr;
+        const double qp = q[p];
+        w2[0] = -qp * (vb[0] * R[0] + vb[1] * R[1] + vb[2] * R[2]);
+        w2[1] = qp * (vb[0] * Eb[0] + vb[1] * Eb[1] + vb[2] * Eb[2]);
+    }
+    block_partials<4, 1>(s4, mx, 0, part);
+    block_partials<2, 0>(w2, nullptr, 0, part2);
+}
+
+void launch_step_diag_orbit(hipStream_t s, int64_t np, int midpoint, double dt, const double L[3], const double* x0,
+                            const double* v0, const double* v1, const double* vbar_last, double* vbar, double* dx,
+                            double* x1) {
+    if (np <= 0) return;
+    hipLaunchKernelGGL(k_step_diag_orbit, dim3(nblk(np)), dim3(256), 0, s, np, midpoint, dt, L[0], L[1], L[2], x0, v0,
+                       v1, vbar_last, vbar, dx, x1);
+}
+void launch_step_diag_particles(hipStream_t s, int64_t np, double dt, const double* q, const double* vbar,
+                                const double* Ubar, const double* D, const double* A0p, const double* A1p,
+                                const double* Ebar, double* part, double* part2, double* sums6, double* max1) {
+    if (np <= 0) return;
+    hipLaunchKernelGGL(k_step_diag_particles, dim3(nblk(np)), dim3(256), 0, s, np, dt, q, vbar, Ubar, D, A0p, A1p,
+                       Ebar, part, part2);
+    launch_finish_partials(s, part, nblk(np), 4, 1, sums6, max1, nullptr);
+    launch_finish_partials(s, part2, nblk(np), 2, 0, sums6 + 4, nullptr, nullptr);
+}
+
 void launch_p2_advance_deposit(hipStream_t s, const P2Args& a) {
     if (a.np <= 0) return;
     hipLaunchKernelGGL(k_p2_advance_deposit, dim3(nblk(a.np)), dim3(256), 0, s, a);

@@ -90,6 +90,11 @@ struct upic_sim {
     double* Jdep = nullptr;
     bool jdep_valid = false;
     unsigned long long jdep_stats[3] = {0, 0, 0};
+    // state at the start of the last step, for the step diagnostics
+    // (upic_sim_keep_step_start): x | v (3np each) | phi (N) | A | U (3N each)
+    bool keep_start = false, start_valid = false, start_refined = false;
+    double* s0 = nullptr;
+    double K0 = 0.0, W0 = 0.0;
     bool cells_fresh = false;  // particles cell-sorted since their last move (offsets = cell ends)
     // deposit variant autotuner (mixed precision).  While a particle order
     // (index 1: freshly cell-sorted, 0: not) has no choice yet, each map
@@ -141,7 +146,7 @@ void sim_free(upic_sim* s) {
                     &s->mo.off, &s->mo.P, &s->mo.v, &s->mo.vb, &s->ftmp, &s->ftmp1, &s->ftmp2, &s->NRf};
     for (float** p : fl) dfree(*p);
     dfree(s->ms.cell); dfree(s->mo.cell);
-    dfree(s->retry_n); dfree(s->retry_idx); dfree(s->retry_gs); dfree(s->retry_old); dfree(s->Jdep);
+    dfree(s->retry_n); dfree(s->retry_idx); dfree(s->retry_gs); dfree(s->retry_old); dfree(s->Jdep); dfree(s->s0);
     dfree(s->mlp_P); dfree(s->mlp_blob); dfree(s->train_buf); dfree(s->part); dfree(s->vdep);
     for (auto& gx : s->fs_graphs) (void)hipGraphExecDestroy(gx.second);
     s->fs_graphs.clear();
@@ -191,7 +196,8 @@ void mixed_compress(upic_sim* s) {
     upic_ctx* c = s->c;
     launch_mixed_compress(c->stream, c->g, s->np, s->x, s->P, s->v, s->v_prev, s->vbar_last, s->q, s->m, s->ms);
     HIPCHK(hipGetLastError());
-    s->fp64_stale = false;
+    // the FP64 view shows the state the simulation evolves: the rounded one
+    s->fp64_stale = true;
 }
 
 // Bring the FP64 particle arrays up to date with the mixed state.
@@ -375,12 +381,39 @@ void maybe_sort(upic_sim* s) {
 // ---------------------------------------------------------------------------
 // P1 step (picard_step_orbit, P1:1509-1552)
 // ---------------------------------------------------------------------------
+// Keep the step's starting state (after the cell sort, so that particle
+// order matches the end state) for upic_sim_step_diagnostics.
+void keep_step_start(upic_sim* s) {
+    s->start_valid = false;
+    if (!s->keep_start) return;
+    upic_ctx* c = s->c;
+    const int N = c->N();
+    const int64_t np = s->np;
+    upic_diagnostics d{};
+    const upic_status rc = upic_sim_diagnostics(s, 1, &d);  // energies; FP64 view of a mixed state
+    if (rc != UPIC_OK) throw Error(rc, c->err);
+    if (!s->s0) s->s0 = dalloc<double>(6 * (size_t)np + 7 * (size_t)N);
+    double* b = s->s0;
+    launch_copy(c->stream, 3 * np, s->x, b);
+    launch_copy(c->stream, 3 * np, s->v, b + 3 * np);
+    double* m = b + 6 * np;
+    launch_copy(c->stream, N, s->phi, m);
+    launch_copy(c->stream, 3ll * N, s->A, m + N);
+    launch_copy(c->stream, 3ll * N, s->U, m + 4ll * N);
+    HIPCHK(hipGetLastError());
+    s->K0 = d.kinetic_energy;
+    s->W0 = d.field_energy;
+    s->start_valid = true;
+    s->start_refined = false;
+}
+
 void p1_step(upic_sim* s, upic_step_info* info) {
     upic_ctx* c = s->c;
     const upic_params& p = c->p;
     const int N = c->N();
     const int64_t np = s->np;
     maybe_sort(s);
+    keep_step_start(s);
     state_spectra(s);
     P1Args a{};
     a.np = np; a.dt = p.dt;
@@ -940,6 +973,7 @@ void p3_step(upic_sim* s, upic_step_info* info) {
     s->vdep_valid = false;
     s->jdep_valid = false;
     maybe_sort(s);
+    keep_step_start(s);
     state_spectra(s);
     s->outer_atol = 0.0;
     if (s->mixed) {
@@ -958,6 +992,7 @@ void p3_step(upic_sim* s, upic_step_info* info) {
     } catch (const RefineNeeded& e) {
         if (!c->p.selective_particle_fallback)
             throw Error(UPIC_ENONLINEAR, std::string("inner particle solve failed over full dt; time and dt unchanged: ") + e.what());
+        s->start_refined = true;
         std::vector<int8_t> lay((size_t)s->np, 1);
         std::vector<int64_t> failed = e.failed;
         for (;;) {
@@ -1267,6 +1302,7 @@ upic_status upic_sim_create(upic_ctx* c, int64_t np, upic_sim** out) {
     upic_sim* s = new upic_sim();
     const int rc = guarded(c, [&] {
         if (np > (int64_t)1 << 31) throw Error(UPIC_EINVAL, "at most 2^31 particles per simulation");
+        require_stencil_grid(c->p);
         s->c = c;
         s->np = np;
         const int N = c->N();
@@ -1397,6 +1433,7 @@ upic_status upic_sim_upload(upic_sim* s, const upic_host_state* st) {
         for (int64_t i = 0; i < s->np; ++i) ids[i] = i;
         HIPCHK(hipMemcpy(s->id, ids.data(), s->np * sizeof(int64_t), hipMemcpyHostToDevice));
         s->steps = 0;
+        s->start_valid = false;
         if (s->mixed) mixed_compress(s);
     });
 }
@@ -1517,6 +1554,7 @@ static upic_status init_fields(upic_sim* s, double n0, const double* A_host, con
             HIPCHK(hipGetLastError());
             c->sync();
             s->steps = 0;
+            s->start_valid = false;
             return;
         }
         fp64_view(s);
@@ -1573,6 +1611,7 @@ static upic_status init_fields(upic_sim* s, double n0, const double* A_host, con
         if (s->mixed) mixed_compress(s);
         c->sync();
         s->steps = 0;
+        s->start_valid = false;
     });
 }
 
@@ -1588,6 +1627,110 @@ upic_status upic_sim_init_fields_AU(upic_sim* s, double n0, const double* A, con
         return UPIC_EINVAL;
     }
     return init_fields(s, n0, A, U);
+}
+
+upic_status upic_sim_keep_step_start(upic_sim* s, int enable) {
+    if (!s) return UPIC_EINVAL;
+    if (s->c->p.code == 2) {
+        s->c->err = "step diagnostics are defined for P1 and P3";
+        return UPIC_EINVAL;
+    }
+    s->keep_start = enable != 0;
+    s->start_valid = false;
+    return UPIC_OK;
+}
+
+upic_status upic_sim_step_diagnostics(upic_sim* s, upic_step_diagnostics* out) {
+    if (!s || !out) return UPIC_EINVAL;
+    upic_ctx* c = s->c;
+    if (!s->start_valid) {
+        c->err = "no step start kept: enable upic_sim_keep_step_start before the step";
+        return UPIC_EINVAL;
+    }
+    if (s->start_refined) {
+        c->err = "step diagnostics of a selectively refined step (per-particle substep paths) are not implemented";
+        return UPIC_EINVAL;
+    }
+    upic_diagnostics d1{};
+    const upic_status rc = upic_sim_diagnostics(s, 1, &d1);  // end-state energies, FP64 view
+    if (rc != UPIC_OK) return rc;
+    return (upic_status)guarded(c, [&] {
+        const upic_params& p = c->p;
+        const int N = c->N();
+        const int64_t np = s->np;
+        const double dt = p.dt;
+        const double* x0 = s->s0;
+        const double* v0 = s->s0 + 3 * np;
+        const double* phi0 = s->s0 + 6 * np;
+        const double* A0 = phi0 + N;
+        const double* U0 = phi0 + 4ll * N;
+        const int64_t pb = (partial_blocks(np) + 1) * kPartSlots;
+        double* w = c->workspace(30 * (size_t)np + 6 * (size_t)N + 2 * (size_t)pb);
+        double *vbar = w, *dx = w + 3 * np, *x1 = w + 6 * np, *Ubar = w + 9 * np, *D = w + 12 * np;
+        double *A0p = w + 21 * np, *A1p = w + 24 * np, *Ebar = w + 27 * np;
+        double *Umid = w + 30 * np, *Emid = Umid + 3ll * N, *part = Emid + 3ll * N, *part2 = part + pb;
+        double* Jbar = s->gtmp + 6ll * N;
+        double* red = c->dred + 32;  // the diagnostics' accumulators, free again
+        const double L[3] = {p.Lx, p.Ly, p.Lz};
+        // orbit of the step: vbar (P1 midpoint, P3 the accepted vbar_last)
+        launch_step_diag_orbit(c->stream, np, p.code == 1 ? 1 : 0, dt, L, x0, v0, s->v, s->vbar_last, vbar, dx, x1);
+        {
+            LinComb um{2, {0.5, 0.5}, {U0, s->U}};
+            launch_lincomb(c->stream, 3ll * N, um, Umid);
+        }
+        // E_mid = -(grad phi0 + grad phi1)/2 - U_mid
+        to_spectra(c, phi0, Zf(s->zs, 0, N), 1);
+        launch_spectral_grad(c->stream, c->sp, Zf(s->zs, 0, N), Zf(s->zs, 1, N));
+        from_spectra(c, Zf(s->zs, 1, N), s->gtmp, 3);
+        to_spectra(c, s->phi, Zf(s->zs, 0, N), 1);
+        launch_spectral_grad(c->stream, c->sp, Zf(s->zs, 0, N), Zf(s->zs, 1, N));
+        from_spectra(c, Zf(s->zs, 1, N), s->gtmp + 3ll * N, 3);
+        {
+            LinComb em{3, {-0.5, -0.5, -1.0}, {s->gtmp, s->gtmp + 3ll * N, Umid}};
+            launch_lincomb(c->stream, 3ll * N, em, Emid);
+        }
+        // orbit residual R_A (compute_RA_orbit_residual)
+        launch_gather_orbit(c->stream, c->g, c->q, c->opt, x0, dx, Umid, np, Ubar);
+        launch_grad_A(c->stream, c->g, c->q, c->opt, x0, dx, A0, s->A, 0.0, 1.0, p.code == 1 ? 1 : 0, np, D);
+        launch_gather_point(c->stream, c->g, x0, A0, np, A0p);
+        launch_gather_point(c->stream, c->g, x1, s->A, np, A1p);
+        // works: particles against Ebar, mesh against the orbit current
+        launch_gather_orbit(c->stream, c->g, c->q, c->opt, x0, dx, Emid, np, Ebar);
+        launch_zero(c->stream, 3ll * N, Jbar);
+        HIPCHK(hipMemsetAsync(c->dstats, 0, 3 * sizeof(unsigned long long), c->stream));
+        launch_deposit_orbit(c->stream, c->g, c->q, c->opt, x0, dx, vbar, s->q, np, Jbar, c->dstats);
+        launch_zero(c->stream, 8, red);
+        launch_step_diag_particles(c->stream, np, dt, s->q, vbar, Ubar, D, A0p, A1p, Ebar, part, part2, red, red + 6);
+        const double* xs[1] = {Jbar};
+        const double* ys[1] = {Emid};
+        launch_dot_many(c->stream, 3ll * N, 1, xs, ys, red + 7);
+        HIPCHK(hipMemcpyAsync(c->hred + 32, red, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(c->hstats, c->dstats, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipGetLastError());
+        c->sync();
+        const double* h = c->hred + 32;
+        const double n3 = 3.0 * (double)std::max<int64_t>(1, np);
+        upic_step_diagnostics& o = *out;
+        std::memset(&o, 0, sizeof(o));
+        o.A_chain_abs_rms = std::sqrt(h[0] / n3);
+        o.A_chain_rel_rms = o.A_chain_abs_rms / (0.5 * (std::sqrt(h[1] / n3) + std::sqrt(h[2] / n3)) + 1.0e-300);
+        o.A_chain_max_norm = h[6];
+        o.A_chain_mean_norm = h[3] / (double)std::max<int64_t>(1, np);
+        o.A_chain_work_defect = h[4];
+        o.particle_work = dt * h[5];
+        o.grid_work = dt * h[7] * c->g.dV;
+        o.delta_K = d1.kinetic_energy - s->K0;
+        o.delta_W = d1.field_energy - s->W0;
+        o.delta_total_energy = o.delta_K + o.delta_W;
+        o.deposit_gather_work_residual = o.particle_work - o.grid_work;
+        o.particle_energy_residual = o.delta_K - o.particle_work;
+        o.field_energy_residual = o.delta_W + o.grid_work;
+        const int64_t cross = (int64_t)c->hstats[0], segs = (int64_t)c->hstats[1];
+        o.crossing_particles = cross;
+        o.total_crossings = segs >= np ? segs - np : 0;
+        o.avg_path_segments = np ? (double)segs / (double)np : 1.0;
+        o.max_path_segments = std::max<int32_t>(1, (int32_t)c->hstats[2]);
+    });
 }
 
 upic_status upic_sim_device_state(upic_sim* s, upic_host_state* d) {

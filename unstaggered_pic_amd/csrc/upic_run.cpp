@@ -12,8 +12,9 @@
 //   <prefix>_config_echo.txt    (P1 write_config_echo 1731-1770)
 //   <prefix>_particle_sample.csv (P1 write_particle_sample 1716-1729)
 // Columns whose quantity this build does not compute (selective-refinement
-// level counts, convergence-rate estimates) are written as nan.  The step-diagnostics files (R_A and work residuals)
-// are not produced.
+// level counts, convergence-rate estimates) are written as nan.  P1 also
+// writes <prefix>_step_diagnostics.csv (R_A and the work residuals of every
+// step, P1:1818-1867) from upic_sim_step_diagnostics.
 //
 // Initial conditions: P1 two_stream and landau_* are the reference's
 // deterministic loads (lattice / quiet start, P1:1151-1304) computed here on
@@ -399,14 +400,29 @@ int run_p1(const std::string& deck_path) {
              << "\n";
         return g;
     };
+    // <prefix>_step_diagnostics.csv (P1:1818-1819, 1859-1867)
+    std::ofstream stepcsv(c.output_prefix + "_step_diagnostics.csv");
+    stepcsv << "step,time,A_chain_abs_rms,A_chain_rel_rms,A_chain_max_norm,A_chain_mean_norm,A_chain_work_defect,"
+            << "delta_K,delta_W,delta_total_energy,particle_work,grid_work,deposit_gather_work_residual,"
+            << "particle_energy_residual,field_energy_residual,crossing_particles,total_crossings,"
+            << "avg_path_segments,max_path_segments\n";
+    check(upic_sim_keep_step_start(r.sim, 1), r.ctx, "step diagnostics");
     upic_path_stats ps0{};
     ps0.max_segments = 1;
     row(0, 0.0, 0, kNaN, kNaN, ps0);
     for (int n = 0; n < c.n_steps; ++n) {
         upic_step_info info{};
         check(upic_step_p1(r.sim, &info), r.ctx, "step");
+        upic_step_diagnostics sd{};
+        check(upic_sim_step_diagnostics(r.sim, &sd), r.ctx, "step diagnostics");
         const upic_diagnostics g = row(n + 1, (n + 1) * c.dt, info.iterations, info.current_error,
                                        info.velocity_error, info.path);
+        stepcsv << (n + 1) << "," << std::setprecision(17) << (n + 1) * c.dt << "," << sd.A_chain_abs_rms << ","
+                << sd.A_chain_rel_rms << "," << sd.A_chain_max_norm << "," << sd.A_chain_mean_norm << ","
+                << sd.A_chain_work_defect << "," << sd.delta_K << "," << sd.delta_W << "," << sd.delta_total_energy
+                << "," << sd.particle_work << "," << sd.grid_work << "," << sd.deposit_gather_work_residual << ","
+                << sd.particle_energy_residual << "," << sd.field_energy_residual << "," << sd.crossing_particles
+                << "," << sd.total_crossings << "," << sd.avg_path_segments << "," << sd.max_path_segments << "\n";
         if ((n + 1) % std::max(1, c.n_steps / 5) == 0 || n == c.n_steps - 1)
             std::cout << std::scientific << std::setprecision(3) << "step " << std::setw(5) << (n + 1) << "/"
                       << c.n_steps << " Gauss=" << g.gauss_rms << " Picard=" << info.iterations << "\n";

@@ -9,7 +9,8 @@ import os
 
 import numpy as np
 
-from .abi import Diagnostics, HostState, LoadSpec, P3Case, Params, PathStats, StepInfo, STATUS_NAMES
+from .abi import (Diagnostics, HostState, LoadSpec, P3Case, Params, PathStats, StepDiagnostics, StepInfo,
+                  STATUS_NAMES)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # UPIC_LIBRARY selects a tuning variant built with build.py -o
@@ -63,6 +64,8 @@ def lib():
             "upic_sim_failure_reasons": [vp, C.POINTER(C.c_uint8)],
             "upic_sim_init_fields": [vp, C.c_double],
             "upic_sim_init_fields_AU": [vp, C.c_double, vp, vp],
+            "upic_sim_keep_step_start": [vp, C.c_int],
+            "upic_sim_step_diagnostics": [vp, C.POINTER(StepDiagnostics)],
             "upic_p3_initial_state": [C.POINTER(Params), C.POINTER(P3Case), C.POINTER(HostState), dp],
             "upic_ctx_set_timing": [vp, C.c_int],
             "upic_device_count": [C.POINTER(C.c_int)], "upic_device_alloc": [C.c_size_t, C.POINTER(vp)],
@@ -375,6 +378,16 @@ class Simulation:
         U = None if U is None else np.ascontiguousarray(U, np.float64)
         self.ctx._check(lib().upic_sim_init_fields_AU(self.h, n0, None if A is None else A.ctypes.data_as(vp),
                                                       None if U is None else U.ctypes.data_as(vp)))
+
+    def keep_step_start(self, enable=True):
+        """Keep each step's start state on the device (step diagnostics)."""
+        self.ctx._check(lib().upic_sim_keep_step_start(self.h, 1 if enable else 0))
+
+    def step_diagnostics(self):
+        """compute_step_diagnostics of the last step (needs keep_step_start)."""
+        d = StepDiagnostics()
+        self.ctx._check(lib().upic_sim_step_diagnostics(self.h, C.byref(d)))
+        return d
 
     def failure_reasons(self):
         r = np.zeros(self.np, np.uint8)
