@@ -186,3 +186,69 @@ def test_driver_p3_weibel_eigenmode_follows_theory(tmp_path):
     te = np.array([float(r["total_energy"]) for r in rows])
     assert np.max(np.abs(te - te[0])) / te[0] < 1e-12
     assert "linear theory growth/damping rate" in out.stdout
+
+
+# The reference's P2 example decks (Project2/examples/*.txt; their key
+# values): the driver's runs against Project2/docs/validation_summary.json.
+P2_DECK = """# {name}
+example = {example}
+method = {method}
+charge_update = {update}
+nx = 32
+ny = 32
+Lx = {L}
+Ly = {L}
+n_steps = {steps}
+dt = {dt}
+particles_per_cell = 4
+cloud_particles = 4096
+cloud_sigma = 1.0
+cloud_drift = 0.01
+cloud_thermal = 0.005
+c = 1.0
+eps0 = 1.0
+mu0 = 1.0
+n0 = 1.0
+weibel_vx = 0.5
+weibel_vy_max = 0.01
+weibel_sine_perturb = 0.002
+perturbation_mode = 1
+seed = 12345
+output_prefix = {name}
+"""
+
+
+@pytest.mark.parametrize("example", ["weibel", "cloud"])
+@pytest.mark.parametrize("method", ["bdf1", "bdf2"])
+@pytest.mark.parametrize("update", ["conserving", "naive"])
+def test_driver_p2_matches_validation_summary(tmp_path, example, method, update):
+    """The P2 deck driver on each example deck: final total energy and
+    |B_z| against the reference's validation summary (1e-10 / 1e-8), the
+    largest speed (1e-10), the conserving runs' Gauss/gauge/continuity at
+    roundoff and the naive runs' violations to 1e-6."""
+    import json
+    name = f"{example}_{method}_{update}"
+    L = 6.283185307179586 if example == "weibel" else 16.0
+    steps, dt = (200, 0.02) if (example, method) == ("cloud", "bdf2") else (160, 0.05)
+    out = run_driver(tmp_path, "p2", P2_DECK.format(name=name, example=example, method=method, update=update, L=L,
+                                                   steps=steps, dt=dt))
+    assert f"wrote {name}_diagnostics.csv" in out.stdout
+    ref = json.load(open(os.path.join(HERE, "golden", "p2_validation_summary.json")))["runs"][name]
+    rows = list(csv.DictReader(open(tmp_path / f"{name}_diagnostics.csv")))
+    assert len(rows) == steps + 1 and len(rows[0]) == 18
+    assert rows[0]["active_method"] == "initial" and rows[1]["active_method"] == method
+    col = lambda k: np.array([float(r[k]) for r in rows])
+    assert abs(col("total_energy")[-1] - ref["final_total_energy"]) <= 1e-10 * ref["final_total_energy"]
+    assert abs(col("Bz_l2")[-1] - ref["final_Bz_l2"]) <= 1e-8 * ref["final_Bz_l2"]
+    # the summary's maxima skip the t = 0 row (Project2/tools/validate_against_paper.py:59-60)
+    assert abs(col("max_speed")[1:].max() - ref["max_speed"]) <= 1e-10 * ref["max_speed"]
+    for k in ("continuity_rms", "gauge_rms", "gauss_rms"):
+        got, want = col(k)[1:].max(), ref["max_" + k]
+        if update == "conserving":
+            assert got < 1e-13, k
+        else:
+            assert abs(got - want) <= 1e-6 * want, (k, got, want)
+    echo = (tmp_path / f"{name}_config_echo.txt").read_text()
+    assert f"static_ions = {4096 if example == 'cloud' else 0}" in echo
+    sample = list(csv.DictReader(open(tmp_path / f"{name}_particle_sample.csv")))
+    assert len(sample) == 2000 and list(sample[0].keys())[-1] == "gamma"

@@ -2,6 +2,7 @@
 // reference codes' main() time loops on top of the C ABI (include/upic.h).
 //
 //   upic_run p1 DECK      (Project1/ec_pic_serial_demo.cpp main, 1776-1892)
+//   upic_run p2 DECK      (Project2/spectral_pic_part3.cpp main + run_simulation, 1093-1211)
 //   upic_run p3 DECK      (Project3/ec_pic_nyquist_projected.cpp main, 8320-9058)
 //
 // Decks are `key = value` lines with `#` comments, parsed with the
@@ -38,6 +39,7 @@
 #include <iostream>
 #include <limits>
 #include <map>
+#include <random>
 #include <set>
 #include <sstream>
 #include <stdexcept>
@@ -669,15 +671,245 @@ int run_p3(const std::string& deck_path) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// P2 (run_simulation, P2:1093-1189): the explicit spectral BDF code
+// ---------------------------------------------------------------------------
+struct P2Config {  // Config (P2:173-231) with its defaults
+    std::string example = "weibel", method = "bdf1", charge_update = "conserving", output_prefix = "run";
+    int nx = 32, ny = 32, n_steps = 100, particles_per_cell = 4, cloud_particles = 4096, perturbation_mode = 1;
+    int diagnostic_stride = 1, particle_sample_count = 2000;
+    Real Lx = 2.0 * kPi, Ly = 2.0 * kPi, dt = 0.05, c = 1.0, eps0 = 1.0, mu0 = 1.0, n0 = 1.0;
+    Real electron_charge_sign = -1.0, ion_charge_sign = 1.0;
+    Real weibel_vx = 0.5, weibel_vy_max = 0.01, weibel_sine_perturb = 0.002;
+    Real cloud_sigma = 1.0, cloud_drift = 0.01, cloud_thermal = 0.005;
+    unsigned int seed = 12345;
+    bool run_unit_tests = true;
+};
+
+// read_config (P2:233-291): keys are case-insensitive
+P2Config read_p2(const std::string& path) {
+    Deck d;
+    for (const auto& kv : read_deck(path)) d.kv[lower(kv.first)] = kv.second;
+    P2Config c;
+    if (const std::string* v = d.find({"example", "test_case"})) c.example = lower(*v);
+    if (const std::string* v = d.find({"method"})) c.method = lower(*v);
+    if (const std::string* v = d.find({"charge_update"})) c.charge_update = lower(*v);
+    d.get({"nx"}, c.nx); d.get({"ny"}, c.ny); d.get({"lx"}, c.Lx); d.get({"ly"}, c.Ly);
+    d.get({"n_steps"}, c.n_steps); d.get({"dt"}, c.dt);
+    d.get({"particles_per_cell"}, c.particles_per_cell); d.get({"cloud_particles"}, c.cloud_particles);
+    d.get({"c"}, c.c); d.get({"eps0"}, c.eps0); d.get({"mu0"}, c.mu0); d.get({"n0"}, c.n0);
+    d.get({"electron_charge_sign"}, c.electron_charge_sign); d.get({"ion_charge_sign"}, c.ion_charge_sign);
+    d.get({"weibel_vx"}, c.weibel_vx); d.get({"weibel_vy_max"}, c.weibel_vy_max);
+    d.get({"weibel_sine_perturb"}, c.weibel_sine_perturb); d.get({"perturbation_mode"}, c.perturbation_mode);
+    d.get({"cloud_sigma"}, c.cloud_sigma); d.get({"cloud_drift"}, c.cloud_drift);
+    d.get({"cloud_thermal"}, c.cloud_thermal); d.get({"seed"}, c.seed);
+    d.get({"diagnostic_stride"}, c.diagnostic_stride); d.get({"particle_sample_count"}, c.particle_sample_count);
+    if (const std::string* v = d.find({"output_prefix"})) c.output_prefix = *v;
+    d.get_bool({"run_unit_tests"}, c.run_unit_tests);
+    d.warn_unused();
+    auto pow2 = [](int n) { return n > 0 && (n & (n - 1)) == 0; };
+    if (!pow2(c.nx) || !pow2(c.ny))
+        throw std::runtime_error("the internal radix-2 FFT requires nx and ny to be powers of two");
+    if (c.method != "bdf1" && c.method != "bdf2")
+        throw std::runtime_error("method must be bdf1 or bdf2 in this teaching code");
+    if (c.charge_update != "conserving" && c.charge_update != "naive")
+        throw std::runtime_error("charge_update must be conserving or naive");
+    if (c.example != "weibel" && c.example != "cloud" && c.example != "drifting_cloud")
+        throw std::runtime_error("example must be weibel or cloud");
+    return c;
+}
+
+Real p2_gamma(Real vx, Real vy, Real vz, Real c) {  // gamma_from_v (P2:596-600)
+    const Real beta2 = (vx * vx + vy * vy + vz * vz) / (c * c);
+    return 1.0 / std::sqrt(std::max(1.0e-14, 1.0 - beta2));
+}
+
+Real wrap_periodic(Real x, Real L) {  // P2:160-166: into [0, L)
+    x -= std::floor(x / L) * L;
+    if (x >= L) x -= L;
+    if (x < 0.0) x += L;
+    return x;
+}
+
+// quadratic_weights + scatter_scalar (P2:537-562): nearest-node quadratic
+// spline onto the [0, L) grid
+void p2_scatter(std::vector<Real>& f, const P2Config& c, Real x, Real y, Real value) {
+    const Real dx = c.Lx / c.nx, dy = c.Ly / c.ny, dA = dx * dy;
+    int ix[3], iy[3];
+    Real wx[3], wy[3];
+    auto weights = [](Real pos, Real h, int n, int* idx, Real* w) {
+        const Real s = pos / h;
+        const int ic = (int)std::floor(s + 0.5);
+        const Real r = s - (Real)ic;
+        for (int k = 0; k < 3; ++k) idx[k] = (ic - 1 + k + n) % n;
+        w[0] = 0.5 * (0.5 - r) * (0.5 - r);
+        w[1] = 0.75 - r * r;
+        w[2] = 0.5 * (0.5 + r) * (0.5 + r);
+    };
+    weights(x, dx, c.nx, ix, wx);
+    weights(y, dy, c.ny, iy, wy);
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) f[(size_t)ix[a] * c.ny + iy[b]] += value * wx[a] * wy[b] / dA;
+}
+
+// initialize_weibel_particles / initialize_cloud_particles (P2:624-726), with
+// the same libstdc++ generators and draw order: x | v | P (SoA, 3 components,
+// z = 0), q, m and the fixed ion density
+struct P2Load {
+    int64_t np = 0, ions = 0;
+    std::vector<Real> x, v, P, q, m, ion_rho;
+};
+P2Load p2_load(const P2Config& c) {
+    P2Load L;
+    const Real dx = c.Lx / c.nx, dy = c.Ly / c.ny;
+    std::mt19937 rng(c.seed);
+    std::vector<Real> xs, ys, vx, vy;
+    Real qe, me;
+    L.ion_rho.assign((size_t)c.nx * c.ny, 0.0);
+    if (c.example == "weibel") {
+        const int nsub = std::max(1, (int)std::round(std::sqrt((Real)c.particles_per_cell)));
+        const int total = 2 * c.nx * c.ny * nsub * nsub;
+        const Real w = c.n0 * (c.Lx * c.Ly) / (Real)total;
+        qe = c.electron_charge_sign * w;
+        me = w;
+        std::uniform_real_distribution<Real> uy(-c.weibel_vy_max, c.weibel_vy_max);
+        for (int i = 0; i < c.nx; ++i)
+            for (int j = 0; j < c.ny; ++j)
+                for (int a = 0; a < nsub; ++a)
+                    for (int b = 0; b < nsub; ++b) {
+                        const Real x = ((Real)i + ((Real)a + 0.5) / nsub) * dx;
+                        const Real y = ((Real)j + ((Real)b + 0.5) / nsub) * dy;
+                        const Real seed = c.weibel_sine_perturb * c.c * std::sin(2.0 * kPi * c.perturbation_mode * x / c.Lx);
+                        for (int beam = -1; beam <= 1; beam += 2) {
+                            xs.push_back(x);
+                            ys.push_back(y);
+                            vx.push_back((Real)beam * c.weibel_vx * c.c);
+                            vy.push_back(uy(rng) + seed);
+                        }
+                    }
+        std::fill(L.ion_rho.begin(), L.ion_rho.end(), c.n0);
+    } else {
+        const int Np = c.cloud_particles;
+        const Real w = c.n0 * (c.Lx * c.Ly) / (Real)Np;
+        qe = c.electron_charge_sign * w;
+        me = w;
+        const Real qi = c.ion_charge_sign * w;
+        std::normal_distribution<Real> normal(0.0, 1.0);
+        for (int n = 0; n < Np; ++n) {
+            const Real xc = 0.5 * c.Lx + c.cloud_sigma * normal(rng);
+            const Real yc = 0.5 * c.Ly + c.cloud_sigma * normal(rng);
+            const Real x = wrap_periodic(xc, c.Lx), y = wrap_periodic(yc, c.Ly);
+            Real u = c.cloud_drift * c.c + c.cloud_thermal * normal(rng);
+            Real t = c.cloud_drift * c.c + c.cloud_thermal * normal(rng);
+            // cap at 0.2 c (rare large draws)
+            const Real sp = std::sqrt(u * u + t * t + 0.0 * 0.0), vmax = 0.2 * c.c;
+            if (sp > vmax) { u *= (vmax / sp); t *= (vmax / sp); }
+            xs.push_back(x); ys.push_back(y); vx.push_back(u); vy.push_back(t);
+            p2_scatter(L.ion_rho, c, x, y, qi);
+        }
+        L.ions = Np;
+    }
+    const int64_t np = (int64_t)xs.size();
+    L.np = np;
+    L.x.assign(3 * (size_t)np, 0.0); L.v.assign(3 * (size_t)np, 0.0); L.P.assign(3 * (size_t)np, 0.0);
+    L.q.assign((size_t)np, qe); L.m.assign((size_t)np, me);
+    for (int64_t i = 0; i < np; ++i) {
+        L.x[i] = xs[i]; L.x[np + i] = ys[i];
+        L.v[i] = vx[i]; L.v[np + i] = vy[i];
+        const Real g = p2_gamma(vx[i], vy[i], 0.0, c.c);  // P = m gamma v (A = 0)
+        L.P[i] = (me * g) * vx[i] + qe * 0.0;
+        L.P[np + i] = (me * g) * vy[i] + qe * 0.0;
+        L.P[2 * np + i] = (me * g) * 0.0 + qe * 0.0;
+    }
+    return L;
+}
+
+int run_p2(const std::string& deck_path) {
+    const P2Config c = read_p2(deck_path);
+    upic_params p{};
+    p.code = UPIC_CODE_P2;
+    p.nx = c.nx; p.ny = c.ny; p.nz = 1; p.Lx = c.Lx; p.Ly = c.Ly; p.Lz = 1.0; p.dt = c.dt;
+    p.spline_order = 2; p.orbit_quad_order = 1;
+    p.c = c.c; p.eps0 = c.eps0; p.mu0 = c.mu0; p.sigma1 = 1.0; p.sigma2 = 1.0;
+    p.bdf_order = c.method == "bdf2" ? 2 : 1;
+    p.charge_conserving = c.charge_update == "conserving" ? 1 : 0;
+    p.precision = UPIC_PREC_FP64; p.sort_interval = 1;
+    P2Load L = p2_load(c);
+    Run r;
+    r.np = L.np;
+    r.N = c.nx * c.ny;
+    check(upic_ctx_create(&p, &r.ctx), nullptr, "context");
+    check(upic_sim_create(r.ctx, r.np, &r.sim), r.ctx, "simulation");
+    std::vector<Real> fz(3 * (size_t)r.N, 0.0);
+    upic_host_state hs{};
+    hs.np = r.np;
+    hs.x = L.x.data(); hs.v = L.v.data(); hs.v_prev = L.v.data(); hs.P = L.P.data(); hs.vbar_last = L.v.data();
+    hs.q = L.q.data(); hs.m = L.m.data();
+    hs.phi = fz.data(); hs.psi = fz.data(); hs.rho = fz.data(); hs.A = fz.data(); hs.U = fz.data(); hs.J = fz.data();
+    hs.fixed_ion_rho = L.ion_rho.data();
+    check(upic_sim_upload(r.sim, &hs), r.ctx, "upload");
+    check(upic_sim_init_fields(r.sim, c.n0), r.ctx, "initial fields");
+    {  // write_config_echo (P2:964-984)
+        std::ofstream out(c.output_prefix + "_config_echo.txt");
+        out << std::setprecision(17) << "example = " << c.example << "\nmethod = " << c.method
+            << "\ncharge_update = " << c.charge_update << "\nnx = " << c.nx << "\nny = " << c.ny << "\nLx = " << c.Lx
+            << "\nLy = " << c.Ly << "\ndt = " << c.dt << "\nn_steps = " << c.n_steps << "\nc = " << c.c
+            << "\neps0 = " << c.eps0 << "\nmu0 = " << c.mu0 << "\nn0 = " << c.n0 << "\nmobile_electrons = " << L.np
+            << "\nstatic_ions = " << L.ions << "\ncell_area = " << (c.Lx / c.nx) * (c.Ly / c.ny)
+            << "\noutput_prefix = " << c.output_prefix << "\n";
+    }
+    std::ofstream diag(c.output_prefix + "_diagnostics.csv");
+    diag << std::setprecision(17);
+    diag << "step,time,requested_method,active_method,charge_update,"
+         << "gauge_rms,gauss_rms,continuity_rms,Bz_l2,B_l2,E_l2,rho_l2,J_l2,"
+         << "field_energy,kinetic_energy,total_energy,max_speed,mean_gamma\n";
+    auto row = [&](int step, Real t, const char* active) {
+        upic_diagnostics d{};
+        check(upic_sim_diagnostics(r.sim, c.perturbation_mode, &d), r.ctx, "diagnostics");
+        diag << step << ',' << t << ',' << c.method << ',' << active << ',' << c.charge_update << ',' << d.gauge_rms
+             << ',' << d.gauss_rms << ',' << d.continuity_rms << ',' << d.Bz_l2 << ',' << d.B_l2 << ',' << d.E_l2
+             << ',' << d.rho_l2 << ',' << d.J_l2 << ',' << d.field_energy << ',' << d.kinetic_energy << ','
+             << d.total_energy << ',' << d.max_speed << ',' << d.mean_gamma << "\n";
+    };
+    row(0, 0.0, "initial");  // BDF1 with duplicated histories at t = 0
+    for (int step = 1; step <= c.n_steps; ++step) {
+        upic_step_info info{};
+        check(upic_step_p2(r.sim, &info), r.ctx, "step");
+        if (step % std::max(1, c.diagnostic_stride) == 0 || step == c.n_steps)
+            row(step, step * c.dt, c.method.c_str());
+    }
+    {  // write_particle_sample (P2:986-1001)
+        std::vector<Real> x(3 * (size_t)r.np), v(3 * (size_t)r.np), q((size_t)r.np), m((size_t)r.np);
+        upic_host_state ds{};
+        ds.np = r.np;
+        ds.x = x.data(); ds.v = v.data(); ds.q = q.data(); ds.m = m.data();
+        check(upic_sim_download(r.sim, &ds), r.ctx, "download");
+        std::ofstream out(c.output_prefix + "_particle_sample.csv");
+        out << std::setprecision(17) << "id,x,y,x_centered,y_centered,vx,vy,vz,q,m,gamma\n";
+        const int64_t count = std::min<int64_t>(c.particle_sample_count, r.np);
+        for (int64_t i = 0; i < count; ++i) {
+            const Real vx = v[i], vy = v[r.np + i], vz = v[2 * r.np + i];
+            out << i << ',' << x[i] << ',' << x[r.np + i] << ',' << x[i] - 0.5 * c.Lx << ','
+                << x[r.np + i] - 0.5 * c.Ly << ',' << vx << ',' << vy << ',' << vz << ',' << q[i] << ',' << m[i]
+                << ',' << p2_gamma(vx, vy, vz, c.c) << "\n";
+        }
+    }
+    std::cout << "wrote " << c.output_prefix << "_diagnostics.csv\n"
+              << "wrote " << c.output_prefix << "_particle_sample.csv\n"
+              << "wrote " << c.output_prefix << "_config_echo.txt\n";
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc != 3 || (std::string(argv[1]) != "p1" && std::string(argv[1]) != "p3")) {
-        std::cerr << "usage: " << argv[0] << " p1|p3 input_deck.txt\n";
+    const std::string code = argc == 3 ? argv[1] : "";
+    if (code != "p1" && code != "p2" && code != "p3") {
+        std::cerr << "usage: " << argv[0] << " p1|p2|p3 input_deck.txt\n";
         return 2;
     }
     try {
-        return std::string(argv[1]) == "p1" ? run_p1(argv[2]) : run_p3(argv[2]);
+        return code == "p1" ? run_p1(argv[2]) : (code == "p2" ? run_p2(argv[2]) : run_p3(argv[2]));
     } catch (const std::exception& e) {
         std::cerr << "error: " << e.what() << "\n";
         return 1;
