@@ -186,6 +186,14 @@ def test_driver_p3_weibel_eigenmode_follows_theory(tmp_path):
     te = np.array([float(r["total_energy"]) for r in rows])
     assert np.max(np.abs(te - te[0])) / te[0] < 1e-12
     assert "linear theory growth/damping rate" in out.stdout
+    # <prefix>_step_diagnostics.csv (P3:8462-8473): the work identities of
+    # every step close to rounding of the total energy
+    steps = list(csv.DictReader(open(tmp_path / "weibel_eig_step_diagnostics.csv")))
+    assert len(steps) == 20 and len(steps[0]) == 46
+    for r in steps:
+        for k in ("particle_energy_residual", "field_energy_residual", "deposit_gather_work_residual"):
+            assert abs(float(r[k])) < 1e-13 * te[0], k
+        assert float(r["A_chain_abs_rms"]) < 1e-15
 
 
 # The reference's P2 example decks (Project2/examples/*.txt; their key

@@ -13,9 +13,11 @@
 //   <prefix>_config_echo.txt    (P1 write_config_echo 1731-1770)
 //   <prefix>_particle_sample.csv (P1 write_particle_sample 1716-1729)
 // Columns whose quantity this build does not compute (selective-refinement
-// level counts, convergence-rate estimates) are written as nan.  P1 also
-// writes <prefix>_step_diagnostics.csv (R_A and the work residuals of every
-// step, P1:1818-1867) from upic_sim_step_diagnostics.
+// level counts, convergence-rate estimates) are written as nan.  P1 and P3
+// also write <prefix>_step_diagnostics.csv (R_A and the work residuals of
+// every step, P1:1818-1867, P3:8462-8890) from upic_sim_step_diagnostics
+// (nan after a selectively refined P3 step; deck key step_diagnostics =
+// false skips it for P3).
 //
 // Initial conditions: P1 two_stream and landau_* are the reference's
 // deterministic loads (lattice / quiet start, P1:1151-1304) computed here on
@@ -538,6 +540,10 @@ int run_p3(const std::string& deck_path) {
     d.get_str({"precision"}, prec);
     d.get({"learned_guess"}, p.learned_guess);
     d.get({"sort_interval"}, p.sort_interval);
+    // <prefix>_step_diagnostics.csv every step, as the reference (false skips
+    // its extra orbit pass per step; a key of this driver)
+    bool step_diag = true;
+    d.get_bool({"step_diagnostics"}, step_diag);
     d.warn_unused();
     p.split_orbit_at_knots = split ? 1 : 0;
     p.nyquist_projection = proj ? 1 : 0;
@@ -605,6 +611,48 @@ int run_p3(const std::string& deck_path) {
         check(upic_sim_collect_guess_data(r.sim, stride, ((r.np + stride - 1) / stride) * 20), r.ctx, "collect");
     }
     const auto t0 = std::chrono::steady_clock::now();
+    // <prefix>_step_diagnostics.csv (P3:8462-8473, rows 8835-8890): the
+    // nonlinear solve's columns and compute_step_diagnostics
+    std::ofstream stepcsv;
+    if (step_diag) {
+        check(upic_sim_keep_step_start(r.sim, 1), r.ctx, "step diagnostics");
+        stepcsv.open(prefix + "_step_diagnostics.csv");
+        stepcsv << "step,time,nonlinear_iterations,nonlinear_rms_residual,nonlinear_max_particle_residual,"
+                << "anderson_restarts,convergence_method,observed_convergence_rate,"
+                << "estimated_map_contraction,backtracked_steps,minimum_update_alpha,"
+                << "local_particle_iterations_mean,local_particle_iterations_max,uniform_substeps_used,"
+                << "selective_fallback_used,direct_attempt_iterations,selective_refinement_restarts,"
+                << "selective_refined_particles,selective_total_particle_substeps,selective_work_ratio,"
+                << "selective_n1,selective_n2,selective_n4,selective_n8,selective_n16,selective_n32,selective_n64,"
+                << "A_chain_abs_rms,A_chain_rel_rms,A_chain_max_norm,"
+                << "A_chain_mean_norm,A_chain_work_defect,delta_K,delta_W,delta_total_energy,particle_work,"
+                << "grid_work,deposit_gather_work_residual,particle_energy_residual,field_energy_residual,"
+                << "crossing_particles,total_crossings,avg_path_segments,max_path_segments,dt,elapsed_wall_seconds\n";
+    }
+    auto step_row = [&](int step, Real t, const upic_step_info& in) {
+        upic_step_diagnostics sd{};
+        const bool have = upic_sim_step_diagnostics(r.sim, &sd) == UPIC_OK;  // not after a refined step
+        auto v = [&](double x) { return have ? x : kNaN; };
+        const std::string method = in.anderson_steps > 0 ? "anderson" : (in.picard_fallback_steps > 0 ? "picard" : "direct");
+        stepcsv << step << "," << std::setprecision(17) << t << "," << in.iterations << "," << in.final_rms_residual
+                << "," << in.final_max_particle_residual << "," << in.anderson_restarts << "," << method << ","
+                << kNaN << "," << in.estimated_map_contraction << "," << in.backtracked_steps << ","
+                << in.minimum_update_alpha << "," << in.local_iterations_mean << "," << in.local_iterations_max
+                << ",1,0,0,0,0,0,0";
+        for (int k = 0; k < 7; ++k) stepcsv << "," << kNaN;
+        stepcsv << "," << v(sd.A_chain_abs_rms) << "," << v(sd.A_chain_rel_rms) << "," << v(sd.A_chain_max_norm)
+                << "," << v(sd.A_chain_mean_norm) << "," << v(sd.A_chain_work_defect) << "," << v(sd.delta_K) << ","
+                << v(sd.delta_W) << "," << v(sd.delta_total_energy) << "," << v(sd.particle_work) << ","
+                << v(sd.grid_work) << "," << v(sd.deposit_gather_work_residual) << ","
+                << v(sd.particle_energy_residual) << "," << v(sd.field_energy_residual) << ","
+                << (have ? sd.crossing_particles : in.path.crossing_particles) << ","
+                << (have ? sd.total_crossings : in.path.total_segments - in.path.n_particles) << ","
+                << (have ? sd.avg_path_segments
+                         : (in.path.n_particles ? (Real)in.path.total_segments / (Real)in.path.n_particles : 0.0))
+                << "," << (have ? sd.max_path_segments : in.path.max_segments) << "," << p.dt << ","
+                << wall_since(t0) << "\n";
+        stepcsv.flush();
+    };
     std::ofstream diag(prefix + "_diagnostics.csv");
     diag << "step,time,gauss_rms,gauss_max,divE_rms,sigma1_rho_rms,"
          << "gauss_normalization_rms,gauss_relative,"
@@ -658,6 +706,7 @@ int run_p3(const std::string& deck_path) {
     for (int n = 0; n < n_steps; ++n) {
         upic_step_info info{};
         check(upic_step_p3(r.sim, &info), r.ctx, "step");
+        if (step_diag) step_row(n + 1, (n + 1) * p.dt, info);
         const upic_diagnostics g = row(n + 1, (n + 1) * p.dt, info, p.dt);
         if (learned == UPIC_GUESS_LEARNED && n + 1 == 20) {
             check(upic_sim_collect_guess_data(r.sim, 0, 0), r.ctx, "collect");
