@@ -206,3 +206,47 @@ def test_deposit_variants_agree_and_autotune(split):
         assert np.max(np.abs(e[:, 1] - e0[:, 1]) / e0[:, 1]) < 1e-4, variant
         assert np.max(periodic_dist(xv[0], x0[0], p64.Lx)) < 1e-5 * p64.Lx, variant
         assert np.max(periodic_dist(xv[1], x0[1], p64.Ly)) < 1e-5 * p64.Ly, variant
+
+
+def test_mixed_history_follows_particles_through_sorts(orc):
+    """The cell sort moves v_prev and vbar lazily (the commit overwrites
+    them): after every step, in the original particle order, v_prev is the
+    previous step's v exactly and x^{n+1} - x^n = dt vbar_last to FP32
+    position rounding; a step that fails without changing the state leaves
+    v_prev and vbar_last where they were."""
+    lib = gpu()
+    p = with_mixed_precision(p3_deck())
+    n = p.nx * p.ny * 8
+    x, v, q, m = lib.load_particles(p, make_load(LOAD_BIMAXWELL, 8, seed=9, vth=(0.05, 0.2, 0.2)), n)
+    st = orc.init_fields(p, x, v, q, m, 1.0)
+    ctx = lib.Context(p)
+    sim = lib.Simulation(ctx, n)
+    sim.upload(st.copy())
+    prev = sim.download(State(n, st.N))
+    h = p.Lx / p.nx
+    for _ in range(3):
+        sim.step()
+        cur = sim.download(State(n, st.N))
+        assert np.array_equal(cur.v_prev, prev.v)
+        for d, L in ((0, p.Lx), (1, p.Ly)):
+            dx = cur.x.reshape(3, n)[d] - prev.x.reshape(3, n)[d]
+            dx -= L * np.round(dx / L)
+            assert np.max(np.abs(dx - p.dt * cur.vbar_last.reshape(3, n)[d])) <= 1e-5 * h
+        prev = cur
+    # a step that cannot converge (one outer iteration allowed) fails after
+    # the sort and leaves the state as it was
+    sim.close()
+    ctx.close()
+    p_fail = with_mixed_precision(p3_deck())
+    p_fail.nonlinear_max_iter = 1
+    ctx = lib.Context(p_fail)
+    sim = lib.Simulation(ctx, n)
+    sim.upload(prev)
+    before = sim.download(State(n, st.N))
+    with pytest.raises(lib.UpicError):
+        sim.step()
+    after = sim.download(State(n, st.N))
+    for f in ("x", "v", "v_prev", "vbar_last", "P", "q", "m"):
+        assert np.array_equal(getattr(after, f), getattr(before, f)), f
+    sim.close()
+    ctx.close()

@@ -159,7 +159,10 @@ void launch_mixed_expand(hipStream_t s, const GridDev& g, int64_t np, const Mixe
 void launch_mixed_keys(hipStream_t s, const GridDev& g, int64_t np, const MixedState& ms, double dt, int split,
                        int* keys, int* counts);
 void launch_mixed_permute(hipStream_t s, int64_t np, const int* perm, const MixedState& src, const MixedState& dst,
-                          const int64_t* id_src, int64_t* id_dst);
+                          const int64_t* id_src, int64_t* id_dst, bool hist);
+// the v_prev and vbar part of a permutation moved without them (hist = false)
+void launch_mixed_permute_hist(hipStream_t s, int64_t np, const int* perm, const float* vp, const float* vb,
+                               float* vp_out, float* vb_out);
 
 // mlp.hip: the learned-guess network (layout in mlp.hpp) -------------------
 // Training rows are 15 floats: 12 features, then the 3 target components.

@@ -1346,6 +1346,9 @@ __global__ void k_mixed_keys(MGrid g, int64_t np, const int* __restrict__ cell, 
 // All per-particle arrays of the mixed state through one permutation:
 // dst[i] = src[perm[i]] for the cell index, the 3-vectors (offsets, P, v,
 // v_prev, vbar), q, m and the original particle index.
+// HIST: also v_prev and vbar (else the caller permutes them later, or never:
+// a committed step overwrites both)
+template <bool HIST>
 __global__ void k_mixed_permute(int64_t np, const int* __restrict__ perm, MixedState src, MixedState dst,
                                 const int64_t* __restrict__ id_src, int64_t* __restrict__ id_dst) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1357,8 +1360,10 @@ __global__ void k_mixed_permute(int64_t np, const int* __restrict__ perm, MixedS
         dst.off[c * np + i] = src.off[c * np + j];
         dst.P[c * np + i] = src.P[c * np + j];
         dst.v[c * np + i] = src.v[c * np + j];
-        dst.vp[c * np + i] = src.vp[c * np + j];
-        dst.vb[c * np + i] = src.vb[c * np + j];
+        if constexpr (HIST) {
+            dst.vp[c * np + i] = src.vp[c * np + j];
+            dst.vb[c * np + i] = src.vb[c * np + j];
+        }
     }
     dst.q[i] = src.q[j];
     dst.m[i] = src.m[j];
@@ -1366,6 +1371,20 @@ __global__ void k_mixed_permute(int64_t np, const int* __restrict__ perm, MixedS
 }
 
 inline unsigned mblk(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// v_prev and vbar of a permutation whose other fields have been moved
+__global__ void k_mixed_permute_hist(int64_t np, const int* __restrict__ perm, const float* __restrict__ vp,
+                                     const float* __restrict__ vb, float* __restrict__ vp_out,
+                                     float* __restrict__ vb_out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const int64_t j = perm[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        vp_out[c * np + i] = vp[c * np + j];
+        vb_out[c * np + i] = vb[c * np + j];
+    }
+}
 
 __global__ void k_zero_int(int* __restrict__ x) {
     if (threadIdx.x == 0) *x = 0;
@@ -1549,9 +1568,15 @@ void launch_mixed_keys(hipStream_t s, const GridDev& g, int64_t np, const MixedS
 }
 
 void launch_mixed_permute(hipStream_t s, int64_t np, const int* perm, const MixedState& src, const MixedState& dst,
-                          const int64_t* id_src, int64_t* id_dst) {
+                          const int64_t* id_src, int64_t* id_dst, bool hist) {
+    if (np <= 0) return;
+    if (hist) hipLaunchKernelGGL(k_mixed_permute<true>, dim3(mblk(np)), dim3(256), 0, s, np, perm, src, dst, id_src, id_dst);
+    else hipLaunchKernelGGL(k_mixed_permute<false>, dim3(mblk(np)), dim3(256), 0, s, np, perm, src, dst, id_src, id_dst);
+}
+void launch_mixed_permute_hist(hipStream_t s, int64_t np, const int* perm, const float* vp, const float* vb,
+                               float* vp_out, float* vb_out) {
     if (np > 0)
-        hipLaunchKernelGGL(k_mixed_permute, dim3(mblk(np)), dim3(256), 0, s, np, perm, src, dst, id_src, id_dst);
+        hipLaunchKernelGGL(k_mixed_permute_hist, dim3(mblk(np)), dim3(256), 0, s, np, perm, vp, vb, vp_out, vb_out);
 }
 
 }  // namespace upic

@@ -56,6 +56,10 @@ struct upic_sim {
     // demand (diagnostics, download) while fp64_stale is set
     bool mixed = false;
     bool fp64_stale = false;
+    // The cell sort moved the mixed state except v_prev and vbar, which
+    // still follow the order before it (perm); a committed step overwrites
+    // both, anything else that reads them applies perm first.
+    bool hist_perm_pending = false;
     MixedState ms{}, mo{};
     float* ftmp = nullptr;   // 3*np sort scratch
     float* ftmp1 = nullptr;  // np sort scratch
@@ -194,14 +198,28 @@ void deposit_record(upic_sim* s) {
 // FP64 particle arrays -> mixed state
 void mixed_compress(upic_sim* s) {
     upic_ctx* c = s->c;
+    s->hist_perm_pending = false;  // v_prev and vbar rewritten below
     launch_mixed_compress(c->stream, c->g, s->np, s->x, s->P, s->v, s->v_prev, s->vbar_last, s->q, s->m, s->ms);
     HIPCHK(hipGetLastError());
     // the FP64 view shows the state the simulation evolves: the rounded one
     s->fp64_stale = true;
 }
 
+// Move v_prev and vbar into the order of the last cell sort (scratch: ftmp
+// and the map's vbar output, both free outside a map evaluation).
+void apply_hist_perm(upic_sim* s) {
+    if (!s->hist_perm_pending) return;
+    upic_ctx* c = s->c;
+    launch_mixed_permute_hist(c->stream, s->np, s->perm, s->ms.vp, s->ms.vb, s->ftmp, s->mo.vb);
+    HIPCHK(hipGetLastError());
+    std::swap(s->ms.vp, s->ftmp);
+    std::swap(s->ms.vb, s->mo.vb);
+    s->hist_perm_pending = false;
+}
+
 // Bring the FP64 particle arrays up to date with the mixed state.
 void fp64_view(upic_sim* s) {
+    apply_hist_perm(s);
     if (!s->mixed || !s->fp64_stale) return;
     upic_ctx* c = s->c;
     launch_mixed_expand(c->stream, c->g, s->np, s->ms, s->x, s->P, s->v, s->v_prev, s->vbar_last, s->q, s->m);
@@ -329,24 +347,25 @@ bool sort_particles(upic_sim* s) {
         // more Gauss-segment passes, so grouping them keeps the wavefronts of
         // the map and deposit sweeps from running the second pass for a few
         // lanes each.  Both halves stay in cell order.
+        apply_hist_perm(s);  // a step that failed after the last sort
         const bool split = c->opt.split != 0;
         const int nkeys = split ? 2 * ncell : ncell;
         HIPCHK(hipMemsetAsync(s->counts, 0, nkeys * sizeof(int), c->stream));
         launch_mixed_keys(c->stream, c->g, np, s->ms, c->p.dt, split ? 1 : 0, s->keys, s->counts);
         launch_scan_counts(c->stream, nkeys, s->counts, s->offsets, s->scan_tmp);
         launch_scatter_perm(c->stream, np, s->keys, s->offsets, s->perm);
-        // the map's output arrays (free between steps) and three scratch
-        // arrays receive the permuted state in one pass
+        // the map's output arrays (free between steps) and two scratch
+        // arrays receive the permuted state in one pass; v_prev and vbar
+        // are left for the step's commit to overwrite (hist_perm_pending)
         MixedState dst{};
-        dst.cell = s->mo.cell; dst.off = s->mo.off; dst.P = s->mo.P; dst.v = s->mo.v; dst.vb = s->mo.vb;
-        dst.vp = s->ftmp; dst.q = s->ftmp1; dst.m = s->ftmp2;
-        launch_mixed_permute(c->stream, np, s->perm, s->ms, dst, s->id, s->idtmp);
+        dst.cell = s->mo.cell; dst.off = s->mo.off; dst.P = s->mo.P; dst.v = s->mo.v;
+        dst.q = s->ftmp1; dst.m = s->ftmp2;
+        launch_mixed_permute(c->stream, np, s->perm, s->ms, dst, s->id, s->idtmp, false);
+        s->hist_perm_pending = true;
         std::swap(s->ms.cell, s->mo.cell);
         std::swap(s->ms.off, s->mo.off);
         std::swap(s->ms.P, s->mo.P);
         std::swap(s->ms.v, s->mo.v);
-        std::swap(s->ms.vb, s->mo.vb);
-        std::swap(s->ms.vp, s->ftmp);
         std::swap(s->ms.q, s->ftmp1);
         std::swap(s->ms.m, s->ftmp2);
         std::swap(s->id, s->idtmp);
@@ -944,12 +963,14 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
                                      std::to_string(fn.max_particle_scaled) + ")");
     // commit: v_prev <- v, particle outputs in, mesh state from the final map
     if (s->mixed) {
+        // v_prev <- v and vbar <- the map's: both in the current order
         std::swap(s->ms.vp, s->ms.v);
         std::swap(s->ms.v, s->mo.v);
         std::swap(s->ms.cell, s->mo.cell);
         std::swap(s->ms.off, s->mo.off);
         std::swap(s->ms.P, s->mo.P);
         std::swap(s->ms.vb, s->mo.vb);
+        s->hist_perm_pending = false;
         s->fp64_stale = true;
         launch_copy(c->stream, n3, s->J, s->Jprev);
         s->jprev_valid = true;
