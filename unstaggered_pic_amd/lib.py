@@ -101,7 +101,7 @@ def exported_symbols():
     import re
     names = []
     for line in open(hdr):
-        m = re.match(r"^(?:upic_status|const char\*)\s+(upic_\w+)\(", line)
+        m = re.match(r"^(?:upic_status|const char\*|void)\s+(upic_\w+)\(", line)
         if m:
             names.append(m.group(1))
     return names
