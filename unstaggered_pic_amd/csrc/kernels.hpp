@@ -124,6 +124,11 @@ struct P3MArgs {
     int* retry_idx;
     float* retry_gs;
     float* retry_old;              // the queued particles' starting guesses (3*np) or NULL
+    // A^n at each particle's start point (3*np): written by the sweep that
+    // gathers the predictor (a0_out), read by the later sweeps of the step
+    // (a0_in)
+    float* a0_out;
+    const float* a0_in;
 };
 // Picard sweep; with a substep layout it also deposits, otherwise it writes
 // vdep and launch_deposit_mixed deposits (reason and vdep as inputs).

@@ -845,7 +845,8 @@ __device__ __forceinline__ void map_particle(const MGrid& g, const QuadF& qd, in
 }
 
 // GM: initial guess of the inner solve, 0 = v^n (the reference's) or
-// a.guess when given, 1 = physics predictor, 2 = predictor + network;
+// a.guess when given, 1 = physics predictor, 2 = predictor + network,
+// 3 = as 0 with A^n at the start point read from a.a0_in;
 // COLLECT: write (features, converged correction) training rows in the
 // write-out pass.
 // FUSED: deposit inside the sweep (per-particle substep layouts); otherwise
@@ -861,7 +862,8 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
     // features and guess at the start point (all lanes: the network runs on
     // whole wavefronts)
     float feat[12] = {}, dph[3] = {0.0f, 0.0f, 0.0f}, lg[3] = {0.0f, 0.0f, 0.0f}, a0s[3] = {0.0f, 0.0f, 0.0f};
-    if constexpr (GM > 0 || COLLECT) {
+    constexpr bool PRED = GM == 1 || GM == 2;
+    if constexpr (PRED || COLLECT) {
         if (p < a.np && !a.layout) {
             const int c0 = a.cell[p];
             float v0[3];
@@ -869,22 +871,26 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
             guess_features<R, GM == 2 || COLLECT>(g, a.NR, c0 / g.ny, c0 - (c0 / g.ny) * g.ny, a.off[p],
                                                    a.off[a.np + p], v0, a.q[p], a.m[p], (float)a.kappa,
                                                    (float)a.dt, feat, dph, a0s);
+            if (a.a0_out) st3f(a.a0_out, a.np, p, a0s);
         }
         if constexpr (GM == 2) mlp_wave_infer(a.mlp, feat, lg, mlp_lds + (threadIdx.x / 64) * kMlpLdsWave);
     }
     double rsum[2] = {0.0, 0.0}, rmax[2] = {0.0, 0.0};
     int nseg_total = 0;
+    if constexpr (GM == 3) {
+        if (p < a.np) ld3f(a.a0_in, a.np, p, a0s);  // from the step's first sweep
+    }
     if (p < a.np) {
         float gin[3];
         ld3f(a.v, a.np, p, gin);
         if (a.guess) ld3f(a.guess, a.np, p, gin);
-        if constexpr (GM > 0) {
+        if constexpr (PRED) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) gin[d] += dph[d] + (GM == 2 ? lg[d] : 0.0f);
         }
         map_particle<R, COLLECT, FUSED, !FUSED && GM != 2>(g, qd, split, a, p, gin, 0,
                                                            a.retry_n != nullptr && !a.layout, feat, dph,
-                                        (GM > 0 || COLLECT) && !a.layout ? a0s : nullptr, T, rsum, rmax,
+                                        (PRED || COLLECT || GM == 3) && !a.layout ? a0s : nullptr, T, rsum, rmax,
                                         nseg_total);
     }
     if constexpr (FUSED) {
@@ -1427,6 +1433,7 @@ void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int sp
         } else {                                                                                                    \
             if (gm == 2) hipLaunchKernelGGL((k_p3_map_mixed<R_, 2, false, false>), gr, bl, lds, s, mg, q, split, a); \
             else if (gm == 1) hipLaunchKernelGGL((k_p3_map_mixed<R_, 1, false, false>), gr, bl, 0, s, mg, q, split, a); \
+            else if (a.a0_in) hipLaunchKernelGGL((k_p3_map_mixed<R_, 3, false, false>), gr, bl, 0, s, mg, q, split, a); \
             else hipLaunchKernelGGL((k_p3_map_mixed<R_, 0, false, false>), gr, bl, 0, s, mg, q, split, a);         \
         }                                                                                                           \
     } while (0)

@@ -93,6 +93,9 @@ struct upic_sim {
     // (before the Nyquist projection) and its path statistics
     double* Jdep = nullptr;
     bool jdep_valid = false;
+    // A^n at each particle's start point, from this step's first sweep
+    float* a0 = nullptr;
+    bool a0_valid = false;
     unsigned long long jdep_stats[3] = {0, 0, 0};
     // state at the start of the last step, for the step diagnostics
     // (upic_sim_keep_step_start): x | v (3np each) | phi (N) | A | U (3N each)
@@ -150,7 +153,7 @@ void sim_free(upic_sim* s) {
                     &s->mo.off, &s->mo.P, &s->mo.v, &s->mo.vb, &s->ftmp, &s->ftmp1, &s->ftmp2, &s->NRf};
     for (float** p : fl) dfree(*p);
     dfree(s->ms.cell); dfree(s->mo.cell);
-    dfree(s->retry_n); dfree(s->retry_idx); dfree(s->retry_gs); dfree(s->retry_old); dfree(s->Jdep); dfree(s->s0);
+    dfree(s->retry_n); dfree(s->retry_idx); dfree(s->retry_gs); dfree(s->retry_old); dfree(s->Jdep); dfree(s->a0); dfree(s->s0);
     dfree(s->mlp_P); dfree(s->mlp_blob); dfree(s->train_buf); dfree(s->part); dfree(s->vdep);
     for (auto& gx : s->fs_graphs) (void)hipGraphExecDestroy(gx.second);
     s->fs_graphs.clear();
@@ -578,7 +581,7 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     field_solve(s, s->Jfield);
     launch_zero(c->stream, 3ll * N, Jmap);
     zero_red(c);
-    bool incremental = false;
+    bool incremental = false, a0_written = false;
     if (s->mixed) {
         P3MArgs a{};
         a.np = s->np; a.dt = p.dt; a.kappa = p.c; a.rtol = p.local_rtol; a.atol = p.local_atol;
@@ -615,6 +618,12 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
         // A sweep that started every particle from the previous sweep's vbar
         // moves the kept deposit (Jdep) by the particles it changed only.
         incremental = a.guess != nullptr && s->jdep_valid;
+        // A^n at the start point is fixed during the step: the first sweep
+        // (predictor) keeps it for the later sweeps
+        const bool gathers_a0 = a.guess_mode == 1 || (a.guess_mode == 2 && a.mlp) || a.train;
+        a.a0_out = !use_layout && gathers_a0 && !s->a0_valid ? s->a0 : nullptr;
+        a.a0_in = !use_layout && a.guess_mode == 0 && a.train == nullptr && s->a0_valid ? s->a0 : nullptr;
+        a0_written = a.a0_out != nullptr;
         a.retry_old = incremental ? s->retry_old : nullptr;
         s->jdep_valid = false;  // until this evaluation has completed
         timed(c, "p3_map_mixed", [&] { launch_p3_map_mixed(c->stream, c->g, c->qf, c->opt.split, a); });
@@ -675,6 +684,7 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
         }
         for (int k = 0; k < 3; ++k) s->jdep_stats[k] = c->hstats[k];
         s->jdep_valid = true;
+        if (a0_written) s->a0_valid = true;
     }
     if (s->mixed) deposit_record(s);
     if (s->mixed && !use_layout) s->vdep_valid = true;
@@ -993,6 +1003,7 @@ void p3_step(upic_sim* s, upic_step_info* info) {
     s->train_written = false;
     s->vdep_valid = false;
     s->jdep_valid = false;
+    s->a0_valid = false;
     maybe_sort(s);
     keep_step_start(s);
     state_spectra(s);
@@ -1401,6 +1412,7 @@ upic_status upic_sim_create(upic_ctx* c, int64_t np, upic_sim** out) {
             s->retry_gs = dalloc<float>(n3);
             s->retry_old = dalloc<float>(n3);
             s->Jdep = dalloc<double>(3 * (size_t)N);
+            s->a0 = dalloc<float>(n3);
             s->NRf = dalloc<float>(8 * (size_t)N);
         }
         // identity ids
