@@ -267,15 +267,16 @@ def main():
     # q, m (16 B) per particle, writes x, P, v, vbar_macro (96 B) once per
     # step, reads A_n, A_end, grad phi_mid (9 N doubles), writes J (3 N).
     # Mixed: reads cell (4 B), offsets, P, v (3 x 12 B), q, m (8 B) and, with
-    # the learned guess, the cached network correction (12 B, written instead
-    # by the first sweep of a step); every sweep writes the end state cell,
-    # offsets, P, v, vbar_macro (52 B) and the converged vbar for the deposit
-    # (12 B); reads the FP32 node records (32 N B).
+    # the learned guess, the previous sweep's vbar and A^n at the start point
+    # (2 x 12 B, the first sweep of a step writes A^n instead); every sweep
+    # writes the end state cell, offsets, P, v, vbar_macro (52 B) and the
+    # converged vbar for the deposit (12 B); reads the FP32 node records
+    # (32 N B).
     maps = sum(i.map_evaluations for i in infos)
     if prec == 0:
         bytes_per_launch = np_ * 88.0 + (np_ * 96.0 * args.steps) / max(maps, 1) + 12.0 * 8 * N
     else:
-        bytes_per_launch = np_ * (48.0 + (12.0 if learned else 0.0) + 64.0) + 32.0 * N
+        bytes_per_launch = np_ * (48.0 + (24.0 if learned else 0.0) + 64.0) + 32.0 * N
     avg_ms = kms / max(launches, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if launches else None
     # operations of one inner iteration at r = 3, nq = 4 (counted from the

@@ -720,8 +720,8 @@ __device__ __forceinline__ void guess_features(const MGrid& g, const float4* __r
 template <int R, bool COLLECT, bool FUSED, bool FOLD = false>
 __device__ __forceinline__ void map_particle(const MGrid& g, const QuadF& qd, int split, const P3MArgs& a, int64_t p,
                                              const float gin[3], int it0, bool defer, const float feat[12],
-                                             const float dph[3], const float* A0start, const DepTileF& T,
-                                             double rsum[2], double rmax[2], int& nseg_total) {
+                                             const float dph[3], bool has_a0, float a0x, float a0y, float a0z,
+                                             const DepTileF& T, double rsum[2], double rmax[2], int& nseg_total) {
     const int nsub = a.layout ? (int)a.layout[p] : 1;
     const float h = (float)(a.dt / (double)nsub);
     const int c0 = a.cell[p];
@@ -739,8 +739,10 @@ __device__ __forceinline__ void map_particle(const MGrid& g, const QuadF& qd, in
     for (int s = 0; s < nsub && reason == 0; ++s) {
         const float th0 = (float)s * inv_nsub, th1 = (float)(s + 1) * inv_nsub;
         float A0p[3];
-        if (A0start && nsub == 1) {
-            A0p[0] = A0start[0]; A0p[1] = A0start[1]; A0p[2] = A0start[2];
+        // A^n at the start point, by value (kept in registers) when the
+        // caller has it
+        if (has_a0 && nsub == 1) {
+            A0p[0] = a0x; A0p[1] = a0y; A0p[2] = a0z;
         } else {
             gather_interp_f<R>(g, a.NR, ci, cj, tx, ty, th0, A0p);
         }
@@ -853,7 +855,7 @@ __device__ __forceinline__ void map_particle(const MGrid& g, const QuadF& qd, in
 // the converged vbar goes to a.vdep and the deposit kernel deposits it,
 // which keeps the deposit's registers out of the Picard sweep.
 template <int R, int GM, bool COLLECT, bool FUSED>
-__global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
+__device__ __forceinline__ void p3_map_mixed_body(const MGrid& g, const QuadF& qd, int split, const P3MArgs& a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ float tile_mem[kTileX * kTileY * 3];
     extern __shared__ unsigned char mlp_lds[];
@@ -890,13 +892,27 @@ __global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int spl
         }
         map_particle<R, COLLECT, FUSED, !FUSED && GM != 2>(g, qd, split, a, p, gin, 0,
                                                            a.retry_n != nullptr && !a.layout, feat, dph,
-                                        (PRED || COLLECT || GM == 3) && !a.layout ? a0s : nullptr, T, rsum, rmax,
-                                        nseg_total);
+                                        (PRED || COLLECT || GM == 3) && !a.layout, a0s[0], a0s[1], a0s[2], T,
+                                        rsum, rmax, nseg_total);
     }
     if constexpr (FUSED) {
         if (a.J_out) tilef_end(g, T, a.J_out);
     }
     block_partials<2, 2>(rsum, rmax, nseg_total, a.part);
+}
+
+template <int R, int GM, bool COLLECT, bool FUSED>
+__global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
+    p3_map_mixed_body<R, GM, COLLECT, FUSED>(g, qd, split, a);
+}
+
+// The collection sweeps (training rows) hold the features as well: capped
+// at 256 VGPRs (two wavefronts per SIMD) by the compiler instead of one
+// register past it halving their occupancy.
+template <int R, int GM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_p3_map_mixed_collect(
+    MGrid g, QuadF qd, int split, P3MArgs a) {
+    p3_map_mixed_body<R, GM, true, false>(g, qd, split, a);
 }
 
 // The deferred particles of a sweep (uniform layout), packed into full
@@ -921,8 +937,8 @@ __global__ __launch_bounds__(256) void k_p3_map_retry(MGrid g, QuadF qd, int spl
             guess_features<R>(g, a.NR, c0 / g.ny, c0 - (c0 / g.ny) * g.ny, a.off[p], a.off[a.np + p], v0, a.q[p],
                               a.m[p], (float)a.kappa, (float)a.dt, feat, dph, a0s);
         }
-        map_particle<R, COLLECT, false, true>(g, qd, split, a, p, gin, 1, false, feat, dph, nullptr, T, rsum, rmax,
-                                        nseg_total);
+        map_particle<R, COLLECT, false, true>(g, qd, split, a, p, gin, 1, false, feat, dph, false, 0.0f, 0.0f, 0.0f,
+                                              T, rsum, rmax, nseg_total);
     }
     block_partials<2, 2>(rsum, rmax, nseg_total, a.part);
 }
@@ -1427,9 +1443,9 @@ void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int sp
         if (a.layout) {                                                                                             \
             hipLaunchKernelGGL((k_p3_map_mixed<R_, 0, false, true>), gr, bl, 0, s, mg, q, split, a);               \
         } else if (col) {                                                                                           \
-            if (gm == 2) hipLaunchKernelGGL((k_p3_map_mixed<R_, 2, true, false>), gr, bl, lds, s, mg, q, split, a); \
-            else if (gm == 1) hipLaunchKernelGGL((k_p3_map_mixed<R_, 1, true, false>), gr, bl, 0, s, mg, q, split, a); \
-            else hipLaunchKernelGGL((k_p3_map_mixed<R_, 0, true, false>), gr, bl, 0, s, mg, q, split, a);          \
+            if (gm == 2) hipLaunchKernelGGL((k_p3_map_mixed_collect<R_, 2>), gr, bl, lds, s, mg, q, split, a);     \
+            else if (gm == 1) hipLaunchKernelGGL((k_p3_map_mixed_collect<R_, 1>), gr, bl, 0, s, mg, q, split, a);  \
+            else hipLaunchKernelGGL((k_p3_map_mixed_collect<R_, 0>), gr, bl, 0, s, mg, q, split, a);               \
         } else {                                                                                                    \
             if (gm == 2) hipLaunchKernelGGL((k_p3_map_mixed<R_, 2, false, false>), gr, bl, lds, s, mg, q, split, a); \
             else if (gm == 1) hipLaunchKernelGGL((k_p3_map_mixed<R_, 1, false, false>), gr, bl, 0, s, mg, q, split, a); \
