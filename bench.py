@@ -276,7 +276,8 @@ def main():
     if prec == 0:
         bytes_per_launch = np_ * 88.0 + (np_ * 96.0 * args.steps) / max(maps, 1) + 12.0 * 8 * N
     else:
-        bytes_per_launch = np_ * (48.0 + (24.0 if learned else 0.0) + 64.0) + 32.0 * N
+        later = 1.0 - args.steps / max(maps, 1)  # share of sweeps after a step's first
+        bytes_per_launch = np_ * (48.0 + ((12.0 + 12.0 * later) if learned else 0.0) + 64.0) + 32.0 * N
     avg_ms = kms / max(launches, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if launches else None
     # operations of one inner iteration at r = 3, nq = 4 (counted from the
