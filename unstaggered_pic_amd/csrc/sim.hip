@@ -60,6 +60,13 @@ struct upic_sim {
     // still follow the order before it (perm); a committed step overwrites
     // both, anything else that reads them applies perm first.
     bool hist_perm_pending = false;
+    // Starting later sweeps from the previous sweep's vbar makes the map
+    // remember its history within the inner tolerance.  Where the outer
+    // solve then needs more than kVdepMaxMaps evaluations in two steps in a
+    // row (weak net currents such as Landau loads), the simulation stops
+    // using the vbar start: every sweep starts from the predictor.
+    int step_maps = 0, vdep_strikes = 0;
+    bool vdep_off = false;
     MixedState ms{}, mo{};
     float* ftmp = nullptr;   // 3*np sort scratch
     float* ftmp1 = nullptr;  // np sort scratch
@@ -135,6 +142,9 @@ constexpr double kMixedFinalSlack = 2.0;
 // the outer absolute tolerance is also floored at this fraction of the
 // mean gross current density (about 2 FP32 ulps).
 constexpr double kMixedGrossFloor = 1.0e-7;
+// map evaluations per step beyond which the later sweeps stop starting from
+// the previous sweep's vbar (upic_sim::vdep_off)
+constexpr int kVdepMaxMaps = 4;
 
 namespace {
 
@@ -608,10 +618,11 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
         // later ones, whose fields differ by the outer update only, from the
         // converged vbar of the previous sweep (no counterpart in the
         // reference, whose start is always v^n, P3:5318)
-        if (s->guess_mode >= 1 && s->vdep_valid && !use_layout) {
+        if (s->guess_mode >= 1 && s->vdep_valid && !use_layout && !s->vdep_off) {
             a.guess_mode = 0;
             a.guess = s->vdep;
         }
+        ++s->step_maps;
         s->vdep_valid = false;  // until this evaluation has completed
         a.vdep = s->vdep;
         a.retry_n = s->retry_n; a.retry_idx = s->retry_idx; a.retry_gs = s->retry_gs;
@@ -1004,6 +1015,7 @@ void p3_step(upic_sim* s, upic_step_info* info) {
     s->vdep_valid = false;
     s->jdep_valid = false;
     s->a0_valid = false;
+    s->step_maps = 0;
     maybe_sort(s);
     keep_step_start(s);
     state_spectra(s);
@@ -1048,6 +1060,10 @@ void p3_step(upic_sim* s, upic_step_info* info) {
         const int64_t rows = (s->np + s->train_stride - 1) / s->train_stride;
         s->train_n = std::min(s->train_cap, s->train_n + rows);
         s->train_written = false;
+    }
+    if (s->mixed) {
+        s->vdep_strikes = s->step_maps > kVdepMaxMaps ? s->vdep_strikes + 1 : 0;
+        if (s->vdep_strikes >= 2) s->vdep_off = true;
     }
     ++s->steps;
 }
@@ -1467,6 +1483,8 @@ upic_status upic_sim_upload(upic_sim* s, const upic_host_state* st) {
         HIPCHK(hipMemcpy(s->id, ids.data(), s->np * sizeof(int64_t), hipMemcpyHostToDevice));
         s->steps = 0;
         s->start_valid = false;
+        s->vdep_off = false;
+        s->vdep_strikes = 0;
         if (s->mixed) mixed_compress(s);
     });
 }
@@ -1588,6 +1606,8 @@ static upic_status init_fields(upic_sim* s, double n0, const double* A_host, con
             c->sync();
             s->steps = 0;
             s->start_valid = false;
+            s->vdep_off = false;
+            s->vdep_strikes = 0;
             return;
         }
         fp64_view(s);
@@ -1645,6 +1665,8 @@ static upic_status init_fields(upic_sim* s, double n0, const double* A_host, con
         c->sync();
         s->steps = 0;
         s->start_valid = false;
+        s->vdep_off = false;
+        s->vdep_strikes = 0;
     });
 }
 
