@@ -361,7 +361,11 @@ upic_status upic_sim_train_guess(upic_sim* sim, int32_t epochs, int32_t batch, d
 /* Modes 1 and 2 apply to the first outer evaluation of a step; the later
  * evaluations (whose fields differ from it by the outer update only) start
  * each particle from the orbit-averaged velocity it converged to in the
- * previous evaluation.  Mode 0 keeps the reference's v^n throughout. */
+ * previous evaluation, until two steps in a row need more than four outer
+ * evaluations: from then on every evaluation starts from mode 1 or 2 (the
+ * vbar start remembers earlier evaluations within the inner tolerance,
+ * which slows weak-current outer solves).  Mode 0 keeps the reference's v^n
+ * throughout. */
 upic_status upic_sim_set_guess_mode(upic_sim* sim, int32_t mode);
 upic_status upic_sim_guess_params(upic_sim* sim, float* params);          /* UPIC_MLP_NPARAM floats out */
 upic_status upic_sim_set_guess_params(upic_sim* sim, const float* params); /* and in (installs the network) */
