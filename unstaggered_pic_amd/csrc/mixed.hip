@@ -979,33 +979,37 @@ __global__ __launch_bounds__(256) void k_deposit_mixed(MGrid g, QuadF qd, int sp
 template <int R>
 __global__ __launch_bounds__(256) void k_deposit_delta_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
     const int n = *a.retry_n;
-    const int64_t k0 = (int64_t)blockIdx.x * blockDim.x;
-    if (k0 >= n) {  // no queue entries for this workgroup: neutral partials
-        if (threadIdx.x < kPartSlots) a.part[(int64_t)blockIdx.x * kPartSlots + threadIdx.x] = 0.0;
-        return;
-    }
     __shared__ float tile_mem[kTileX * kTileY * 3];
-    const DepTileF T = tilef_begin(g, tile_mem, a.cell, a.np, (int64_t)a.retry_idx[k0]);
     double sums[2] = {0.0, 0.0}, maxs[1] = {0.0};
-    const int64_t k = k0 + threadIdx.x;
-    if (k < n && a.reason[a.retry_idx[k]] == 0) {
-        const int64_t p = a.retry_idx[k];
-        const float h = (float)a.dt;
-        const int c0 = a.cell[p];
-        const int ci = c0 / g.ny, cj = c0 - (c0 / g.ny) * g.ny;
-        const float qs = a.q[p] * a.inv_dV;
-        float vo[3], vn[3];
-        ld3f(a.retry_old, a.np, k, vo);
-        ld3f(a.vdep, a.np, p, vn);
-        const int so = deposit_orbit_f<R>(g, qd, split != 0, ci, cj, a.off[p], a.off[a.np + p], h * vo[0] * g.ihx,
-                                          h * vo[1] * g.ihy, vo, -qs, T, a.J_out);
-        const int sn = deposit_orbit_f<R>(g, qd, split != 0, ci, cj, a.off[p], a.off[a.np + p], h * vn[0] * g.ihx,
-                                          h * vn[1] * g.ihy, vn, qs, T, a.J_out);
-        sums[0] = (double)(sn - so);
-        sums[1] = (double)((sn > 1 ? 1 : 0) - (so > 1 ? 1 : 0));
-        maxs[0] = (double)sn;
+    // A workgroup takes 128 queue entries at a time: the 32 entries of each
+    // wavefront are spread over its 64 lanes, the lower half removing the
+    // old orbit's current and the upper half adding the new one, so a lane
+    // runs one orbit (the queue is short and scattered: parallelism, not
+    // the tile, sets the speed).  Grid-stride; k0 is uniform in the
+    // workgroup, so every thread takes part in each chunk's tile.
+    const int lane = threadIdx.x & 63;
+    const bool add = lane >= 32;
+    for (int64_t k0 = (int64_t)blockIdx.x * 128; k0 < n; k0 += (int64_t)gridDim.x * 128) {
+        const DepTileF T = tilef_begin(g, tile_mem, a.cell, a.np, (int64_t)a.retry_idx[k0]);
+        const int64_t k = k0 + 32 * (threadIdx.x >> 6) + (lane & 31);
+        if (k < n && a.reason[a.retry_idx[k]] == 0) {
+            const int64_t p = a.retry_idx[k];
+            const float h = (float)a.dt;
+            const int c0 = a.cell[p];
+            const int ci = c0 / g.ny, cj = c0 - (c0 / g.ny) * g.ny;
+            const float qs = a.q[p] * a.inv_dV;
+            float vb[3];
+            if (add) ld3f(a.vdep, a.np, p, vb);
+            else ld3f(a.retry_old, a.np, k, vb);
+            const int sg = deposit_orbit_f<R>(g, qd, split != 0, ci, cj, a.off[p], a.off[a.np + p],
+                                              h * vb[0] * g.ihx, h * vb[1] * g.ihy, vb, add ? qs : -qs, T, a.J_out);
+            sums[0] += add ? (double)sg : -(double)sg;
+            sums[1] += sg > 1 ? (add ? 1.0 : -1.0) : 0.0;
+            if (add) maxs[0] = fmax(maxs[0], (double)sg);
+        }
+        tilef_end(g, T, a.J_out);
+        __syncthreads();  // the flush has read the tile before the next chunk clears it
     }
-    tilef_end(g, T, a.J_out);
     block_partials<2, 1>(sums, maxs, 0, a.part);
 }
 
@@ -1496,9 +1500,10 @@ void launch_deposit_delta_mixed(hipStream_t s, const GridDev& g, const QuadF& q,
     mg.nx = g.nx; mg.ny = g.ny; mg.N = g.N; mg.order = g.order;
     mg.hx = (float)g.h[0]; mg.hy = (float)g.h[1];
     mg.ihx = (float)(1.0 / g.h[0]); mg.ihy = (float)(1.0 / g.h[1]);
-    // one workgroup per 256 possible queue entries; those past the queue
-    // length exit at once
-    const unsigned nb = mblk(a.np);
+    // at most kDeltaBlocks workgroups striding over the queue (its length is
+    // on the device): a short queue costs a small grid and a small fold
+    constexpr unsigned kDeltaBlocks = 1024;
+    const unsigned nb = std::min(mblk(a.np), kDeltaBlocks);
     switch (g.order) {
         case 1: hipLaunchKernelGGL(k_deposit_delta_mixed<1>, dim3(nb), dim3(256), 0, s, mg, q, split, a); break;
         case 2: hipLaunchKernelGGL(k_deposit_delta_mixed<2>, dim3(nb), dim3(256), 0, s, mg, q, split, a); break;
