@@ -218,6 +218,20 @@ upic_status upic_gather_orbit(upic_ctx* ctx, const double* x0, const double* dx,
 upic_status upic_orbit_grad_A(upic_ctx* ctx, const double* x0, const double* dx,
                               const double* A_n, const double* A_np1, double theta0,
                               double theta1, int64_t np, double* D);
+/* solve_local_particle_substep (P3:4930-4983) with
+ * evaluate_local_particle_substep (4843-4884): the inner per-particle Picard
+ * solve of one substep [theta0, theta1] of length h from `guess`, with A at
+ * time theta interpolated between A_n and A_end and grad phi_mid = gpm
+ * (P3 contexts; local tolerances, damping and iteration cap from the
+ * parameters; on 2-D grids the z part of gpm is taken as zero, as for any
+ * gradient there).  Device arrays: guess, x0, P0 (3*np), q, m (np), A_n, A_end,
+ * gpm (3N) in; vbar, P_end, v_end (3*np; zero for failed particles),
+ * iterations (np) and reasons (np, LocalParticleFailureReason P3:204-211)
+ * out. */
+upic_status upic_local_substep(upic_ctx* ctx, const double* guess, const double* x0, const double* P0,
+                               const double* q, const double* m, double h, double theta0, double theta1,
+                               const double* A_n, const double* A_end, const double* gpm, int64_t np,
+                               double* vbar, double* P_end, double* v_end, int32_t* iterations, uint8_t* reasons);
 /* gather_vector_at over particles (P1:666-678, P3:2267-2346). out is 3*np. */
 upic_status upic_gather_point(upic_ctx* ctx, const double* pos, const double* field3,
                               int64_t np, double* out);

@@ -317,3 +317,55 @@ def p2_weibel_state(orc, p):
     st.A_hist = np.zeros(15 * nx * ny)
     st.rho_hist = np.zeros(5 * nx * ny)
     return st
+
+
+@pytest.mark.parametrize("order,nz,th", [(1, 1, (0.0, 1.0)), (2, 1, (0.25, 0.5)), (3, 1, (0.0, 1.0)),
+                                         (4, 1, (0.5, 1.0)), (2, 4, (0.0, 1.0)), (3, 4, (0.25, 0.5))])
+def test_local_substep_vs_oracle(orc, order, nz, th):
+    """upic_local_substep (solve_local_particle_substep, P3:4930-4983) on
+    random particles and smooth fields against the oracle: vbar, P_end, v_end
+    to 1e-12, iteration counts and failure reasons exact (a low iteration cap
+    makes some particles fail with maximum_iterations)."""
+    lib = gpu()
+    for max_iter in (64, 3):
+        p = make_params(code=CODE_P3, nx=16, ny=8, nz=nz, Lx=TWO_PI, Ly=4.0, Lz=2.0 if nz > 1 else 1.0, dt=0.2,
+                        spline_order=order, orbit_quad_order=order + 2, nyquist_projection=1, local_rtol=1e-12,
+                        local_atol=1e-14, local_max_iter=max_iter)
+        rng = np.random.default_rng(11 + order)
+        n = 513
+        x0, v = random_particles(rng, p, n, 0.2)
+        vv = v.reshape(3, n)
+        sp = np.sqrt((vv ** 2).sum(0))
+        vv *= np.minimum(1.0, 0.8 / sp)  # below c
+        v = vv.ravel().copy()
+        N = p.nx * p.ny * p.nz
+        # smooth low-mode fields: the inner Picard map is then a contraction
+        # (node noise would make it discontinuous across knots, and two
+        # roundings could settle on different iterates)
+        gx = (np.arange(p.nx) / p.nx - 0.5) * p.Lx
+        gy = (np.arange(p.ny) / p.ny - 0.5) * p.Ly
+        gz = (np.arange(p.nz) / p.nz - 0.5) * p.Lz
+        X, Y, Z = np.meshgrid(gx, gy, gz, indexing="ij")
+        kx, ky, kz = 2 * np.pi / p.Lx, 2 * np.pi / p.Ly, 2 * np.pi / p.Lz
+        def smooth(amp, ph):
+            return np.concatenate([amp * np.sin(kx * X + ky * Y + kz * Z * (p.nz > 1) + ph + c).ravel()
+                                   for c in (0.0, 1.3, 2.6)])
+        An, Ae, gpm = smooth(0.05, 0.0), smooth(0.05, 0.4), smooth(0.2, 1.1)
+        if p.nz == 1:
+            gpm[2 * N:] = 0.0  # a gradient on a 2-D grid has no z part (the 2-D node records keep x, y)
+        q, m = -np.ones(n), np.ones(n)
+        vv = v.reshape(3, n)
+        P0 = (vv / np.sqrt(1 - (vv ** 2).sum(0))).ravel()
+        h = p.dt * (th[1] - th[0])
+        print(f"local substep order={order} nz={nz} theta={th} max_iter={max_iter}", flush=True)
+        ctx = lib.Context(p)
+        d = ctx.dev
+        vb, Pe, ve, it, rs = ctx.local_substep(d(v), d(x0), d(P0), d(q), d(m), h, th[0], th[1], d(An), d(Ae), d(gpm))
+        vbo, Peo, veo, ito, rso = orc.p3_local_solve(p, v, x0, P0, q, m, h, th[0], th[1], An, Ae, gpm)
+        assert np.array_equal(rs.astype(np.int32), rso)
+        assert np.array_equal(it, ito)
+        if max_iter == 3:
+            assert (rso == 5).any()
+        for a, b in ((vb, vbo), (Pe, Peo), (ve, veo)):
+            assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+        ctx.close()

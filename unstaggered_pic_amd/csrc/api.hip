@@ -190,6 +190,36 @@ upic_status upic_orbit_grad_A(upic_ctx* c, const double* x0, const double* dx, c
     });
 }
 
+upic_status upic_local_substep(upic_ctx* c, const double* guess, const double* x0, const double* P0,
+                               const double* q, const double* m, double h, double theta0, double theta1,
+                               const double* A_n, const double* A_end, const double* gpm, int64_t np,
+                               double* vbar, double* P_end, double* v_end, int32_t* iterations, uint8_t* reasons) {
+    if (!c || np < 0 || !guess || !x0 || !P0 || !q || !m || !A_n || !A_end || !gpm || !vbar || !P_end || !v_end ||
+        !iterations || !reasons)
+        return UPIC_EINVAL;
+    if (c->p.code != 3) {
+        c->err = "upic_local_substep needs a P3 context";
+        return UPIC_EINVAL;
+    }
+    return (upic_status)guarded(c, [&] {
+        require_stencil_grid(c->p);
+        const upic_params& p = c->p;
+        P3Args a{};
+        a.np = np; a.dt = h; a.kappa = p.c; a.rtol = p.local_rtol; a.atol = p.local_atol;
+        a.damping = p.local_damping; a.max_iter = p.local_max_iter;
+        a.x = x0; a.P = P0; a.q = q; a.m = m; a.guess = guess;
+        a.A_n = A_n; a.A_end = A_end; a.gpm = gpm;
+        if (c->g.nz == 1) {  // 2-D: the node records the sweeps read
+            double* nr = c->workspace(8 * (size_t)c->N());
+            launch_pack_nodes(c->stream, c->N(), gpm, A_n, A_end, nr);
+            a.NR = nr;
+        }
+        a.vbar_out = vbar; a.P_out = P_end; a.v_out = v_end; a.reason = reasons;
+        launch_p3_local_solve(c->stream, c->g, c->q, c->opt, a, h, theta0, theta1, iterations);
+        HIPCHK(hipGetLastError());
+    });
+}
+
 upic_status upic_gather_point(upic_ctx* c, const double* pos, const double* f3, int64_t np, double* out) {
     if (!c || np < 0) return UPIC_EINVAL;
     return (upic_status)guarded(c, [&] {

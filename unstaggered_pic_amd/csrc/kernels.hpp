@@ -73,6 +73,10 @@ void launch_deposit_scalar_point(hipStream_t s, const GridDev& g, const double* 
 void launch_p1_deposit_guess(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const P1Args& a);
 void launch_p1_push(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const P1Args& a, bool final_);
 void launch_p3_map(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const P3Args& a);
+// solve_local_particle_substep alone (a.x, a.P, a.q, a.m, a.guess in;
+// a.vbar_out, a.P_out, a.v_out, a.reason and iters out; 2-D: a.NR)
+void launch_p3_local_solve(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const P3Args& a,
+                           double h, double th0, double th1, int32_t* iters);
 void launch_p2_advance_deposit(hipStream_t s, const P2Args& a);
 void launch_p2_push(hipStream_t s, const P2Args& a);
 

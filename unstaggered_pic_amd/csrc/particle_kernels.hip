@@ -322,6 +322,42 @@ __device__ __forceinline__ void p3_local_map(const GridDev& g, const QuadDev& qd
     }
 }
 
+// solve_local_particle_substep (P3:4930-4983): damped Picard on vbar from
+// gs over the substep [th0, th1] of length h.  Returns the failure reason
+// (0: converged; vacc and acc then hold the accepted iterate and its map).
+template <int R, bool Z3>
+__device__ __forceinline__ int p3_solve_substep(const GridDev& g, const QuadDev& qd, const OrbitOpt& opt,
+                                                const P3Args& a, double gs[3], const double xpos[3],
+                                                const double Pc[3], const double A0p[3], const double pst[3],
+                                                double qp, double mp, double h, double th0, double th1,
+                                                LocalMap& acc, double vacc[3], int& its, double& scaled) {
+    its = 0;
+    scaled = 0.0;
+    int reason = 5;  // maximum_iterations
+    for (int it = 0; it < a.max_iter; ++it) {
+        LocalMap mpd;
+        p3_local_map<R, Z3>(g, qd, opt, a, gs, xpos, Pc, A0p, pst, qp, mp, h, th0, th1, mpd);
+        ++its;
+        if (!finite3(mpd.vbar)) { reason = 1; break; }
+        const double nm = norm3(mpd.vbar);
+        if (nm >= a.kappa) { reason = 2; break; }
+        const double res[3] = {mpd.vbar[0] - gs[0], mpd.vbar[1] - gs[1], mpd.vbar[2] - gs[2]};
+        const double scale = 0.5 * (nm + norm3(gs));
+        scaled = norm3(res) / (a.atol + a.rtol * scale);
+        if (scaled <= 1.0) {
+            vacc[0] = gs[0]; vacc[1] = gs[1]; vacc[2] = gs[2];
+            acc = mpd;
+            reason = 0;
+            break;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gs[d] = gs[d] + a.damping * res[d];
+        if (!finite3(gs)) { reason = 3; break; }
+        if (norm3(gs) >= a.kappa) { reason = 4; break; }
+    }
+    return reason;
+}
+
 template <int R, bool Z3>
 __global__ __launch_bounds__(256) void k_p3_map(GridDev g, QuadDev qd, OrbitOpt opt, P3Args a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -349,32 +385,11 @@ __global__ __launch_bounds__(256) void k_p3_map(GridDev g, QuadDev qd, OrbitOpt 
             if (a.guess && nsub == 1) ld3(a.guess, a.np, p, gs);
             LocalMap mp_acc;
             double vacc[3];
-            bool conv = false;
             int its = 0;
             double scaled = 0.0;
-            reason = 5;  // maximum_iterations
-            for (int it = 0; it < a.max_iter; ++it) {
-                LocalMap mpd;
-                p3_local_map<R, Z3>(g, qd, opt, a, gs, xpos, Pc, A0p, pst, qp, mp, h, th0, th1, mpd);
-                ++its;
-                if (!finite3(mpd.vbar)) { reason = 1; break; }
-                const double nm = norm3(mpd.vbar);
-                if (nm >= a.kappa) { reason = 2; break; }
-                const double res[3] = {mpd.vbar[0] - gs[0], mpd.vbar[1] - gs[1], mpd.vbar[2] - gs[2]};
-                const double scale = 0.5 * (nm + norm3(gs));
-                scaled = norm3(res) / (a.atol + a.rtol * scale);
-                if (scaled <= 1.0) {
-                    vacc[0] = gs[0]; vacc[1] = gs[1]; vacc[2] = gs[2];
-                    mp_acc = mpd;
-                    conv = true;
-                    reason = 0;
-                    break;
-                }
-#pragma unroll
-                for (int d = 0; d < 3; ++d) gs[d] = gs[d] + a.damping * res[d];
-                if (!finite3(gs)) { reason = 3; break; }
-                if (norm3(gs) >= a.kappa) { reason = 4; break; }
-            }
+            reason = p3_solve_substep<R, Z3>(g, qd, opt, a, gs, xpos, Pc, A0p, pst, qp, mp, h, th0, th1, mp_acc,
+                                             vacc, its, scaled);
+            const bool conv = reason == 0;
             rsum[0] += (double)its;
             rmax[0] = fmax(rmax[0], (double)its);
             rmax[1] = fmax(rmax[1], scaled);
@@ -405,6 +420,35 @@ __global__ __launch_bounds__(256) void k_p3_map(GridDev g, QuadDev qd, OrbitOpt 
         if (a.J_out) tile_end(g, T, a.J_out);
     }
     block_partials<2, 2>(rsum, rmax, nseg_total, a.part);
+}
+
+// The inner solve alone for given particles and fields (one substep of
+// length h over [th0, th1]): vbar, P_end, v_end of converged particles (zero
+// for failed ones, whose reason is set), iterations.
+template <int R, bool Z3>
+__global__ __launch_bounds__(256) void k_p3_local_solve(GridDev g, QuadDev qd, OrbitOpt opt, P3Args a, double h,
+                                                        double th0, double th1, int32_t* __restrict__ iters) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.np) return;
+    double xpos[3], Pc[3], gs[3];
+    ld3(a.x, a.np, p, xpos); ld3(a.P, a.np, p, Pc); ld3(a.guess, a.np, p, gs);
+    const double qp = a.q[p], mp = a.m[p];
+    double A0p[3];
+    if constexpr (Z3) gather_time_interp<R, Z3>(g, a.A_n, a.A_end, th0, xpos, A0p);
+    else gather_interp_rec<R>(g, a.NR, th0, xpos, A0p);
+    const double pst[3] = {Pc[0] - qp * A0p[0], Pc[1] - qp * A0p[1], Pc[2] - qp * A0p[2]};
+    LocalMap acc;
+    double vacc[3];
+    int its = 0;
+    double scaled = 0.0;
+    const int reason = p3_solve_substep<R, Z3>(g, qd, opt, a, gs, xpos, Pc, A0p, pst, qp, mp, h, th0, th1, acc, vacc,
+                                               its, scaled);
+    const double zero[3] = {0.0, 0.0, 0.0};
+    st3(a.vbar_out, a.np, p, reason == 0 ? vacc : zero);
+    st3(a.P_out, a.np, p, reason == 0 ? acc.P_end : zero);
+    st3(a.v_out, a.np, p, reason == 0 ? acc.v_end : zero);
+    iters[p] = its;
+    a.reason[p] = (uint8_t)reason;
 }
 
 // ---------------------------------------------------------------------------
@@ -560,6 +604,11 @@ void launch_p3_map(hipStream_t s, const GridDev& g, const QuadDev& q, const Orbi
     if (a.np <= 0) return;
     UPIC_DISPATCH(g.order, g.nz > 1, hipLaunchKernelGGL((k_p3_map<R, Z>), dim3(nblk(a.np)), dim3(256), 0, s, g, q, o, a));
     launch_finish_partials(s, a.part, nblk(a.np), 2, 2, a.red, a.redmax, a.stats);
+}
+void launch_p3_local_solve(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const P3Args& a,
+                           double h, double th0, double th1, int32_t* iters) {
+    if (a.np <= 0) return;
+    UPIC_DISPATCH(g.order, g.nz > 1, hipLaunchKernelGGL((k_p3_local_solve<R, Z>), dim3(nblk(a.np)), dim3(256), 0, s, g, q, o, a, h, th0, th1, iters));
 }
 
 // Folding of the per-block partials of a sweep (slots 4, 5, 8 are maxima,

@@ -48,6 +48,8 @@ def lib():
             "upic_gather_orbit": [vp, vp, vp, vp, C.c_int64, vp],
             "upic_orbit_grad_A": [vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_int64, vp],
             "upic_gather_point": [vp, vp, vp, C.c_int64, vp],
+            "upic_local_substep": [vp, vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double, vp, vp, vp, C.c_int64,
+                                   vp, vp, vp, vp, vp],
             "upic_deposit_scalar_point": [vp, vp, vp, C.c_int64, C.c_int, vp],
             "upic_fft": [vp, vp, C.c_int],
             "upic_spectral_gradient": [vp, vp, vp], "upic_spectral_divergence": [vp, vp, vp],
@@ -257,6 +259,18 @@ class Context:
                                             n, self.ptr(out)))
         self.sync()
         return out
+
+    def local_substep(self, guess, x0, P0, q, m, h, th0, th1, An, Aend, gpm):
+        """solve_local_particle_substep on device arrays; returns host
+        arrays vbar, P_end, v_end, iterations, reasons."""
+        n = q.numel()
+        vb, Pe, ve = self.empty(3 * n), self.empty(3 * n), self.empty(3 * n)
+        it, rs = DeviceArray(n, np.int32), DeviceArray(n, np.uint8)
+        self._check(lib().upic_local_substep(self.h, self.ptr(guess), self.ptr(x0), self.ptr(P0), self.ptr(q),
+                                             self.ptr(m), h, th0, th1, self.ptr(An), self.ptr(Aend), self.ptr(gpm), n,
+                                             self.ptr(vb), self.ptr(Pe), self.ptr(ve), self.ptr(it), self.ptr(rs)))
+        self.sync()
+        return vb.numpy(), Pe.numpy(), ve.numpy(), it.numpy(), rs.numpy()
 
     def gather_point(self, pos, f3):
         n = pos.numel() // 3
