@@ -259,6 +259,12 @@ upic_status upic_cn_potentials(upic_ctx* ctx, const double* phi, const double* p
 upic_status upic_project_nyquist(upic_ctx* ctx, double* f, int ncomp);
 /* solve_periodic_poisson_for_phi (P1:959-968, P3:3196-3206). */
 upic_status upic_poisson(upic_ctx* ctx, const double* rho, double* phi);
+/* solve_bdf_wave_scalar (P2:829-848), P2 contexts:
+ * [b0/(c^2 dt^2) + k^2] u^ = S^ - sum_{l>=1} b_l u^_{l-1} / (c^2 dt^2) with
+ * b = a * a (convolution) of the BDF1 {1, -1} or BDF2 {3/2, -2, 1/2}
+ * coefficients (P2:769-785).  src (N); hist holds u^n, u^{n-1}, ... as
+ * consecutive N-arrays (2 for BDF1, 4 for BDF2); out (N) may alias src. */
+upic_status upic_bdf_wave(upic_ctx* ctx, const double* src, const double* hist, int order, double* out);
 
 /* ---- Group B: per-step drivers on a device-resident simulation ---------- */
 typedef struct upic_sim upic_sim;

@@ -1892,6 +1892,18 @@ int orc_project_nyquist(const upic_params* p, double* f, int ncomp) {
         for (int c = 0; c < ncomp; ++c) { Field x(f + c * g.N, f + (c + 1) * g.N); project_nyquist(x, g); std::copy(x.begin(), x.end(), f + c * g.N); }
     });
 }
+// solve_bdf_wave_scalar (P2:829-848); hist holds u^n, u^{n-1}, ... (N each)
+int orc_bdf_wave(const upic_params* p, const double* src, const double* hist, int order, double* out) {
+    return guarded([&] {
+        const Ctx C(*p);
+        const int N = C.g.N;
+        const std::vector<Real> a = order == 2 ? std::vector<Real>{1.5, -2.0, 0.5} : std::vector<Real>{1.0, -1.0};
+        std::deque<Field> h;
+        for (std::size_t l = 0; l + 1 < 2 * a.size() - 1; ++l) h.push_back(Field(hist + l * N, hist + (l + 1) * N));
+        const Field u = p2_bdf_wave(Field(src, src + N), h, a, C.p.c, C.p.dt, C.g);
+        std::copy(u.begin(), u.end(), out);
+    });
+}
 int orc_poisson(const upic_params* p, const double* rho, double* phi) {
     return guarded([&] {
         const Grid g = make_grid(*p);

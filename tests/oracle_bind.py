@@ -174,6 +174,11 @@ class Oracle:
         self.check(self.L.orc_poisson(C.byref(p), ptr(rho), ptr(out)))
         return out
 
+    def bdf_wave(self, p, src, hist, order):
+        out = np.zeros(len(src))
+        self.check(self.L.orc_bdf_wave(C.byref(p), ptr(src), ptr(hist), int(order), ptr(out)))
+        return out
+
     def p3_local_solve(self, p, guess, x0, P0, q, m, h, th0, th1, An, Aend, gpm):
         n = len(q)
         vb, Pe, ve = np.zeros(3 * n), np.zeros(3 * n), np.zeros(3 * n)

@@ -369,3 +369,22 @@ def test_local_substep_vs_oracle(orc, order, nz, th):
         for a, b in ((vb, vbo), (Pe, Peo), (ve, veo)):
             assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
         ctx.close()
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_bdf_wave_vs_oracle(orc, order):
+    """upic_bdf_wave (solve_bdf_wave_scalar, P2:829-848) against the oracle's
+    (the one inside its P2 step, pinned by the reference's P2 validation
+    runs) on random sources and histories, 1e-12 of the solution's scale."""
+    lib = gpu()
+    p = make_params(code=CODE_P2, nx=32, ny=16, nz=1, Lx=TWO_PI, Ly=3.0, Lz=1.0, dt=0.05)
+    N = p.nx * p.ny
+    rng = np.random.default_rng(5 + order)
+    src = rng.normal(size=N)
+    hist = rng.normal(size=(2 if order == 1 else 4) * N)
+    ctx = lib.Context(p)
+    d = ctx.dev
+    u = ctx.bdf_wave(d(src), d(hist), order).numpy()
+    uo = orc.bdf_wave(p, src, hist, order)
+    assert np.max(np.abs(u - uo)) <= 1e-12 * np.max(np.abs(uo))
+    ctx.close()

@@ -190,6 +190,34 @@ upic_status upic_orbit_grad_A(upic_ctx* c, const double* x0, const double* dx, c
     });
 }
 
+upic_status upic_bdf_wave(upic_ctx* c, const double* src, const double* hist, int order, double* out) {
+    if (!c || !src || !hist || !out || (order != 1 && order != 2)) return UPIC_EINVAL;
+    if (c->p.code != 2) {
+        c->err = "upic_bdf_wave needs a P2 context";
+        return UPIC_EINVAL;
+    }
+    return (upic_status)guarded(c, [&] {
+        const int N = c->N();
+        const std::vector<double> a = order == 2 ? std::vector<double>{1.5, -2.0, 0.5} : std::vector<double>{1.0, -1.0};
+        std::vector<double> b(2 * a.size() - 1, 0.0);
+        for (size_t i = 0; i < a.size(); ++i)
+            for (size_t j = 0; j < a.size(); ++j) b[i + j] += a[i] * a[j];
+        const double scale = 1.0 / (c->p.c * c->p.c * c->p.dt * c->p.dt);
+        // by linearity: one transform of the combined real right-hand side
+        double* w = c->workspace(3 * (size_t)N);
+        LinComb lc{};
+        lc.k = 0;
+        lc.c[lc.k] = 1.0; lc.f[lc.k++] = src;
+        for (size_t l = 1; l < b.size(); ++l) { lc.c[lc.k] = -scale * b[l]; lc.f[lc.k++] = hist + (l - 1) * (size_t)N; }
+        launch_lincomb(c->stream, N, lc, w);
+        double* zh = w + N;
+        to_spectra(c, w, zh, 1);
+        launch_spectral_div_shift(c->stream, c->sp, scale * b[0], zh, 1);
+        from_spectra(c, zh, out, 1);
+        HIPCHK(hipGetLastError());
+    });
+}
+
 upic_status upic_local_substep(upic_ctx* c, const double* guess, const double* x0, const double* P0,
                                const double* q, const double* m, double h, double theta0, double theta1,
                                const double* A_n, const double* A_end, const double* gpm, int64_t np,

@@ -48,6 +48,7 @@ def lib():
             "upic_gather_orbit": [vp, vp, vp, vp, C.c_int64, vp],
             "upic_orbit_grad_A": [vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_int64, vp],
             "upic_gather_point": [vp, vp, vp, C.c_int64, vp],
+            "upic_bdf_wave": [vp, vp, vp, C.c_int, vp],
             "upic_local_substep": [vp, vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double, vp, vp, vp, C.c_int64,
                                    vp, vp, vp, vp, vp],
             "upic_deposit_scalar_point": [vp, vp, vp, C.c_int64, C.c_int, vp],
@@ -257,6 +258,12 @@ class Context:
         out = self.empty(9 * n)
         self._check(lib().upic_orbit_grad_A(self.h, self.ptr(x0), self.ptr(dx), self.ptr(An), self.ptr(Anp1), th0, th1,
                                             n, self.ptr(out)))
+        self.sync()
+        return out
+
+    def bdf_wave(self, src, hist, order):
+        out = self.empty(self.N())
+        self._check(lib().upic_bdf_wave(self.h, self.ptr(src), self.ptr(hist), int(order), self.ptr(out)))
         self.sync()
         return out
 
