@@ -188,6 +188,14 @@ def test_driver_p3_weibel_eigenmode_follows_theory(tmp_path):
     assert "linear theory growth/damping rate" in out.stdout
     # <prefix>_step_diagnostics.csv (P3:8462-8473): the work identities of
     # every step close to rounding of the total energy
+    # <prefix>_config_echo.txt with the reference's keys (P3:8217-8300), as
+    # Project3/summarize_hpc_run.py and pic_analysis.py read it
+    echo = dict(line.split("=", 1) for line in (tmp_path / "weibel_eig_config_echo.txt").read_text().splitlines())
+    assert echo["test_case"] == "cold_relativistic_weibel" and int(echo["n_steps"]) == 20
+    assert float(echo["target_time"]) == pytest.approx(2.0) and int(echo["particles"]) == 2 * 8 * 32 * 4
+    assert echo["adaptive_dt"] == "false" and echo["uniform_particle_subcycling"] == "false"
+    assert float(echo["auto_theory_growth_rate"]) == pytest.approx(rate, rel=1e-15)
+    assert len(echo) == 80
     steps = list(csv.DictReader(open(tmp_path / "weibel_eig_step_diagnostics.csv")))
     assert len(steps) == 20 and len(steps[0]) == 46
     for r in steps:

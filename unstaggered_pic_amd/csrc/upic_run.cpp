@@ -10,7 +10,8 @@
 // 1178-1435); keys this driver does not use are reported once and ignored,
 // as P1 does.  Outputs, with the reference's file names and column schemas:
 //   <prefix>_diagnostics.csv    (P1 1811-1842; P3 8444-8460 and 8561-8624)
-//   <prefix>_config_echo.txt    (P1 write_config_echo 1731-1770)
+//   <prefix>_config_echo.txt    (P1 write_config_echo 1731-1770, P2 964-984,
+//                                P3 8217-8300)
 //   <prefix>_particle_sample.csv (P1 write_particle_sample 1716-1729)
 // Columns whose quantity this build does not compute (selective-refinement
 // level counts, convergence-rate estimates) are written as nan.  P1 and P3
@@ -466,7 +467,11 @@ int run_p3(const std::string& deck_path) {
     d.get({"nx"}, p.nx); d.get({"ny"}, p.ny); d.get({"nz"}, p.nz);
     d.get({"Lx"}, p.Lx); d.get({"Ly"}, p.Ly); d.get({"Lz"}, p.Lz);
     d.get({"n_steps"}, n_steps); d.get({"dt"}, p.dt);
-    if (const std::string* v = d.find({"test_case", "initial_condition", "example"})) tc = lower(*v);
+    std::string tc_raw = tc;
+    if (const std::string* v = d.find({"test_case", "initial_condition", "example"})) {
+        tc_raw = *v;
+        tc = lower(*v);
+    }
     d.get({"particles_per_cell_pair", "particles_per_cell", "velocity_samples_per_cell"}, ppc);
     d.get({"spline_order"}, p.spline_order); d.get({"orbit_quad_order"}, p.orbit_quad_order);
     d.get_bool({"split_orbit_at_knots"}, split);
@@ -606,6 +611,54 @@ int run_p3(const std::string& deck_path) {
     if (std::abs(theory_growth) > 0.0)
         std::cout << std::scientific << std::setprecision(6) << "linear theory growth/damping rate = " << theory_growth
                   << (use_B ? " using |B_z(k_y)|" : " using |E_x(k_x)|") << "\n";
+    {  // write_config_echo (P3:8217-8300); the features this driver does not
+       // run (adaptive dt, uniform subcycling, field and failure files) are
+       // echoed off with the reference's defaults
+        if (!ref_kind) upic_p3_case_defaults(0, &cs);
+        const Real wB0 = cs.weibel_B0 >= 0.0 ? cs.weibel_B0 : cs.perturbation;
+        const Real lal = cs.landau_alpha >= 0.0 ? cs.landau_alpha : cs.perturbation;
+        auto tf = [](bool b) { return b ? "true" : "false"; };
+        std::ofstream out(prefix + "_config_echo.txt");
+        out << std::setprecision(17);
+        out << "test_case=" << tc_raw << "\ntransverse_diagnostics=false\ntransverse_spectrum_interval=1"
+            << "\ntsi_transverse_seed=" << cs.tsi_transverse_seed << "\ntsi_transverse_mode=" << cs.tsi_transverse_mode
+            << "\nnyquist_projection=" << p.nyquist_projection << "\nnx=" << p.nx << "\nny=" << p.ny << "\nnz=" << p.nz
+            << "\nLx=" << p.Lx << "\nLy=" << p.Ly << "\nLz=" << p.Lz << "\nn_steps=" << n_steps << "\ndt=" << p.dt
+            << "\ntarget_time=" << (Real)n_steps * p.dt << "\nparticles=" << r.np
+            << "\nparticles_per_cell_pair_or_velocity_samples=" << (ref_kind ? cs.particles_per_cell_pair : ppc)
+            << "\nspline_order=" << p.spline_order << "\norbit_quad_order=" << p.orbit_quad_order
+            << "\nsplit_orbit_at_knots=" << tf(p.split_orbit_at_knots != 0) << "\nmacro_time_step=fixed"
+            << "\nadaptive_dt=false\nadaptive_dt_min=" << p.dt / 16.0
+            << "\nadaptive_dt_shrink=0.5\nadaptive_dt_growth=2\nadaptive_dt_growth_interval=4\nadaptive_max_attempts=4"
+            << "\nnonlinear_solver=" << solver << "\nnonlinear_rtol=" << p.nonlinear_rtol
+            << "\nnonlinear_atol=" << p.nonlinear_atol << "\nnonlinear_max_iter=" << p.nonlinear_max_iter
+            << "\nanderson_memory=" << p.anderson_memory << "\nanderson_start=" << p.anderson_start
+            << "\nanderson_damping=" << p.anderson_damping << "\nanderson_regularization=" << p.anderson_regularization
+            << "\nanderson_growth_limit=" << p.anderson_growth_limit
+            << "\nuniform_particle_subcycling=false\nuniform_refine_on_failure=false\nuniform_initial_substeps=1"
+            << "\nuniform_substeps=10\nuniform_max_substeps=20\nuniform_subcycling_start_time=0"
+            << "\nparticle_local_rtol=" << p.local_rtol << "\nparticle_local_atol=" << p.local_atol
+            << "\nparticle_local_max_iter=" << p.local_max_iter << "\nuniform_local_rtol=" << p.local_rtol
+            << "\nuniform_local_atol=" << p.local_atol << "\nuniform_local_max_iter=" << p.local_max_iter
+            << "\nuniform_local_damping=" << p.local_damping
+            << "\nselective_particle_fallback=" << tf(p.selective_particle_fallback != 0)
+            << "\nselective_particle_max_substeps=" << p.selective_particle_max_substeps
+            << "\nselective_particle_dynamic_chunk=4\nselective_failure_diagnostics=false"
+            << "\nselective_failure_diagnostic_limit=16\nparticle_snapshot_times=\nfield_diagnostics=false"
+            << "\nfield_diagnostics_interval=10\nfield_probe_points_per_cell=32\nfield_probe_axis=auto"
+            << "\nfield_grid_snapshots=false\nkappa=" << p.c << "\nsigma1=" << p.sigma1 << "\nsigma2=" << p.sigma2
+            << "\nsigma1_minus_kappa2_sigma2=" << (p.sigma1 - p.c * p.c * p.sigma2) << "\nrho0=" << rho0
+            << "\nv0=" << (ref_kind ? cs.v0 : v0) << "\nperturbation=" << (ref_kind ? cs.perturbation : alpha)
+            << "\ntwo_stream_density_alpha=" << cs.two_stream_density_alpha
+            << "\ntwo_stream_eigenmode=" << tf(cs.two_stream_eigenmode != 0)
+            << "\ntwo_stream_finite_grid_theory=" << tf(cs.two_stream_finite_grid_theory != 0)
+            << "\nweibel_B0=" << wB0 << "\nweibel_eigenmode=" << tf(cs.weibel_eigenmode != 0)
+            << "\nweibel_finite_grid_theory=" << tf(cs.weibel_finite_grid_theory != 0) << "\nlandau_alpha=" << lal
+            << "\nthermal_velocity=" << cs.thermal_velocity << "\nperturbation_mode=" << mode
+            << "\ntheory_omega=" << theory_omega << "\ntheory_gamma=" << theory_gamma
+            << "\nauto_theory_growth_rate=" << theory_growth
+            << "\ndV=" << (p.Lx / p.nx) * (p.Ly / p.ny) * (p.Lz / p.nz) << "\n";
+    }
     if (learned == UPIC_GUESS_LEARNED) {
         const int64_t stride = std::max<int64_t>(1, r.np / 16384);
         check(upic_sim_collect_guess_data(r.sim, stride, ((r.np + stride - 1) / stride) * 20), r.ctx, "collect");
