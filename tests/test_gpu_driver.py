@@ -196,6 +196,9 @@ def test_driver_p3_weibel_eigenmode_follows_theory(tmp_path):
     assert echo["adaptive_dt"] == "false" and echo["uniform_particle_subcycling"] == "false"
     assert float(echo["auto_theory_growth_rate"]) == pytest.approx(rate, rel=1e-15)
     assert len(echo) == 80
+    sample = list(csv.DictReader(open(tmp_path / "weibel_eig_particle_sample.csv")))
+    assert len(sample) == 2 * 8 * 32 * 4 and list(sample[0].keys()) == ["particle", "x", "y", "z", "vx", "vy", "vz",
+                                                                        "q", "m"]
     steps = list(csv.DictReader(open(tmp_path / "weibel_eig_step_diagnostics.csv")))
     assert len(steps) == 20 and len(steps[0]) == 46
     for r in steps:

@@ -12,7 +12,8 @@
 //   <prefix>_diagnostics.csv    (P1 1811-1842; P3 8444-8460 and 8561-8624)
 //   <prefix>_config_echo.txt    (P1 write_config_echo 1731-1770, P2 964-984,
 //                                P3 8217-8300)
-//   <prefix>_particle_sample.csv (P1 write_particle_sample 1716-1729)
+//   <prefix>_particle_sample.csv (P1 write_particle_sample 1716-1729, P2 986-1001,
+//                                P3 9039)
 // Columns whose quantity this build does not compute (selective-refinement
 // level counts, convergence-rate estimates) are written as nan.  P1 and P3
 // also write <prefix>_step_diagnostics.csv (R_A and the work residuals of
@@ -770,6 +771,7 @@ int run_p3(const std::string& deck_path) {
                       << n_steps << " Gauss=" << g.gauss_rms << " TE=" << g.total_energy
                       << " outer=" << info.iterations << " inner=" << info.local_iterations_mean << "\n";
     }
+    write_particle_sample(prefix + "_particle_sample.csv", r, 20000);  // P3:9039, same format as P1
     return 0;
 }
 
