@@ -207,6 +207,72 @@ def test_driver_p3_weibel_eigenmode_follows_theory(tmp_path):
         assert float(r["A_chain_abs_rms"]) < 1e-15
 
 
+# The key values of the reference's manuscript two-stream deck
+# (Project3/inputs/manuscript/tsi_main_dt01_T60.txt), cut from 600 steps to
+# the first 100 (t = 10, the linear phase).
+P3_TSI_DECK = """test_case = cold_relativistic_two_stream
+nx = 32
+ny = 32
+nz = 32
+Lx = 25.132741228718345
+Ly = 25.132741228718345
+Lz = 25.132741228718345
+n_steps = 100
+dt = 0.1
+particles_per_cell_pair = 10
+spline_order = 2
+orbit_quad_order = 8
+split_orbit_at_knots = true
+kappa = 1.0
+sigma1 = 1.0
+sigma2 = 1.0
+rho0 = 1.0
+v0 = 0.9
+perturbation = 0.0
+two_stream_density_alpha = 0.005
+two_stream_eigenmode = true
+two_stream_finite_grid_theory = true
+perturbation_mode = 1
+nonlinear_solver = anderson_quasi_newton
+nonlinear_rtol = 1.0e-11
+nonlinear_atol = 1.0e-13
+nonlinear_max_iter = 80
+anderson_memory = 4
+anderson_start = 2
+anderson_damping = 0.5
+anderson_regularization = 1.0e-12
+anderson_growth_limit = 1.2
+selective_particle_fallback = false
+particle_local_rtol = 1.0e-11
+particle_local_atol = 1.0e-13
+particle_local_max_iter = 64
+particle_local_damping = 1.0
+nyquist_projection = true
+output_prefix = tsi_main
+"""
+
+
+def test_driver_p3_manuscript_two_stream_linear_phase(tmp_path):
+    from unstaggered_pic_amd import lib as U
+    from unstaggered_pic_amd.abi import P3_TWO_STREAM, make_params
+    run_driver(tmp_path, "p3", P3_TSI_DECK)
+    rows = list(csv.DictReader(open(tmp_path / "tsi_main_diagnostics.csv")))
+    assert len(rows) == 101
+    L = 25.132741228718345
+    p = make_params(code=3, nx=32, ny=32, nz=32, Lx=L, Ly=L, Lz=L, spline_order=2)
+    rate = U.p3_initial_state(p, U.p3_case(P3_TWO_STREAM, particles_per_cell_pair=10, v0=0.9,
+                                           two_stream_density_alpha=0.005, two_stream_eigenmode=1,
+                                           two_stream_finite_grid_theory=1))["growth_rate"]
+    assert rate == pytest.approx(0.0910913804, rel=1e-9)
+    assert all(float(r["theory_growth_rate"]) == pytest.approx(rate, rel=1e-15) for r in rows)
+    # |E_x(k_x)| / (E_0 exp(Gamma t)) over t = 0..10: measured within 5e-4
+    ratio = np.array([float(r["mode_over_theory"]) for r in rows])
+    print("two-stream |E_x(k)| / theory:", " ".join(f"{x:.5f}" for x in ratio[::10]), flush=True)
+    assert np.max(np.abs(ratio - 1.0)) < 2e-3
+    te = np.array([float(r["total_energy"]) for r in rows])
+    assert np.max(np.abs(te - te[0])) / te[0] < 1e-12
+
+
 # The reference's P2 example decks (Project2/examples/*.txt; their key
 # values): the driver's runs against Project2/docs/validation_summary.json.
 P2_DECK = """# {name}
