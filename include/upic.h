@@ -232,6 +232,20 @@ upic_status upic_local_substep(upic_ctx* ctx, const double* guess, const double*
                                const double* q, const double* m, double h, double theta0, double theta1,
                                const double* A_n, const double* A_end, const double* gpm, int64_t np,
                                double* vbar, double* P_end, double* v_end, int32_t* iterations, uint8_t* reasons);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): upic_local_substep with the same arguments and results,
+ * calling no hipMalloc/hipFree.  Its 2-D node records (8 doubles per mesh
+ * node) live in a static device array of the library, one per GPU, sized for
+ * meshes of at most 2^18 nodes (512 x 512; 16 MiB); a 2-D context with more
+ * nodes gets UPIC_EINVAL.  3-D contexts use no scratch.  Every call on one
+ * GPU shares that array: calls that overlap on one GPU (two streams, two
+ * threads) race and must be serialised by the caller. */
+upic_status upic_local_substep_noalloc(upic_ctx* ctx, const double* guess, const double* x0, const double* P0,
+                                       const double* q, const double* m, double h, double theta0, double theta1,
+                                       const double* A_n, const double* A_end, const double* gpm, int64_t np,
+                                       double* vbar, double* P_end, double* v_end, int32_t* iterations,
+                                       uint8_t* reasons);
 /* gather_vector_at over particles (P1:666-678, P3:2267-2346). out is 3*np. */
 upic_status upic_gather_point(upic_ctx* ctx, const double* pos, const double* field3,
                               int64_t np, double* out);

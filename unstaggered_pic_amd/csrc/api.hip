@@ -1,9 +1,42 @@
 // api.hip -- C ABI (include/upic.h): context and the kernel-level (group A)
 // entry points.  The per-step drivers are in sim.hip.
+#include <mutex>
+
 #include "api_internal.hpp"
 
 // message of the last failed call that has no context (upic_last_error(NULL))
 thread_local std::string g_create_err;
+
+// Device scratch of upic_local_substep_noalloc: the 2-D packed node records
+// (8 doubles per mesh node) of at most kNoAllocMaxNodes nodes, 16 MiB per
+// GPU, held from module load to process end.  Host code never passes the
+// array's name to a kernel: its device address is resolved once per device
+// (resolve_noalloc_scratch, called by upic_ctx_create with the device
+// current) and kept in g_scratch_dev.
+constexpr size_t kNoAllocMaxNodes = (size_t)1 << 18;  // 512 x 512
+static __device__ __align__(256) double scratch[8 * kNoAllocMaxNodes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxScratchDevices] = {};
+
+void resolve_noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxScratchDevices)
+        throw Error(UPIC_EDEVICE, "device ordinal beyond the scratch table (64 devices)");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return;
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw Error(UPIC_EDEVICE, "hipGetSymbolAddress returned a null scratch address");
+    g_scratch_dev[device] = static_cast<double*>(ptr);
+}
+
+double* noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (device >= 0 && device < kMaxScratchDevices) ? g_scratch_dev[device] : nullptr;
+}
+}  // namespace
 
 extern "C" {
 
@@ -16,6 +49,7 @@ upic_status upic_ctx_create(const upic_params* params, upic_ctx** out) {
         c->p = *params;
         const upic_params& p = c->p;
         HIPCHK(hipGetDevice(&c->device));
+        resolve_noalloc_scratch(c->device);
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         GridDev& g = c->g;
         g.nx = p.nx; g.ny = p.ny; g.nz = p.nz;
@@ -218,15 +252,19 @@ upic_status upic_bdf_wave(upic_ctx* c, const double* src, const double* hist, in
     });
 }
 
-upic_status upic_local_substep(upic_ctx* c, const double* guess, const double* x0, const double* P0,
-                               const double* q, const double* m, double h, double theta0, double theta1,
-                               const double* A_n, const double* A_end, const double* gpm, int64_t np,
-                               double* vbar, double* P_end, double* v_end, int32_t* iterations, uint8_t* reasons) {
+// Body of upic_local_substep(_noalloc).  node_scratch: device memory for the
+// 2-D node records (8 N doubles); NULL = the context workspace (grown with
+// hipMalloc on demand).
+static upic_status local_substep(upic_ctx* c, const char* name, double* node_scratch, const double* guess,
+                                 const double* x0, const double* P0, const double* q, const double* m, double h,
+                                 double theta0, double theta1, const double* A_n, const double* A_end,
+                                 const double* gpm, int64_t np, double* vbar, double* P_end, double* v_end,
+                                 int32_t* iterations, uint8_t* reasons) {
     if (!c || np < 0 || !guess || !x0 || !P0 || !q || !m || !A_n || !A_end || !gpm || !vbar || !P_end || !v_end ||
         !iterations || !reasons)
         return UPIC_EINVAL;
     if (c->p.code != 3) {
-        c->err = "upic_local_substep needs a P3 context";
+        c->err = std::string(name) + " needs a P3 context";
         return UPIC_EINVAL;
     }
     return (upic_status)guarded(c, [&] {
@@ -238,7 +276,7 @@ upic_status upic_local_substep(upic_ctx* c, const double* guess, const double* x
         a.x = x0; a.P = P0; a.q = q; a.m = m; a.guess = guess;
         a.A_n = A_n; a.A_end = A_end; a.gpm = gpm;
         if (c->g.nz == 1) {  // 2-D: the node records the sweeps read
-            double* nr = c->workspace(8 * (size_t)c->N());
+            double* nr = node_scratch ? node_scratch : c->workspace(8 * (size_t)c->N());
             launch_pack_nodes(c->stream, c->N(), gpm, A_n, A_end, nr);
             a.NR = nr;
         }
@@ -246,6 +284,37 @@ upic_status upic_local_substep(upic_ctx* c, const double* guess, const double* x
         launch_p3_local_solve(c->stream, c->g, c->q, c->opt, a, h, theta0, theta1, iterations);
         HIPCHK(hipGetLastError());
     });
+}
+
+upic_status upic_local_substep(upic_ctx* c, const double* guess, const double* x0, const double* P0,
+                               const double* q, const double* m, double h, double theta0, double theta1,
+                               const double* A_n, const double* A_end, const double* gpm, int64_t np,
+                               double* vbar, double* P_end, double* v_end, int32_t* iterations, uint8_t* reasons) {
+    return local_substep(c, "upic_local_substep", nullptr, guess, x0, P0, q, m, h, theta0, theta1, A_n, A_end, gpm,
+                         np, vbar, P_end, v_end, iterations, reasons);
+}
+
+upic_status upic_local_substep_noalloc(upic_ctx* c, const double* guess, const double* x0, const double* P0,
+                                       const double* q, const double* m, double h, double theta0, double theta1,
+                                       const double* A_n, const double* A_end, const double* gpm, int64_t np,
+                                       double* vbar, double* P_end, double* v_end, int32_t* iterations,
+                                       uint8_t* reasons) {
+    if (!c) return UPIC_EINVAL;
+    double* nr = nullptr;
+    if (c->g.nz == 1) {
+        if ((size_t)c->N() > kNoAllocMaxNodes) {
+            c->err = "upic_local_substep_noalloc: a 2-D mesh of " + std::to_string(c->N()) +
+                     " nodes exceeds the static scratch of " + std::to_string(kNoAllocMaxNodes) + " nodes";
+            return UPIC_EINVAL;
+        }
+        nr = noalloc_scratch(c->device);
+        if (!nr) {
+            c->err = "upic_local_substep_noalloc: no scratch address for device " + std::to_string(c->device);
+            return UPIC_EDEVICE;
+        }
+    }
+    return local_substep(c, "upic_local_substep_noalloc", nr, guess, x0, P0, q, m, h, theta0, theta1, A_n, A_end,
+                         gpm, np, vbar, P_end, v_end, iterations, reasons);
 }
 
 upic_status upic_gather_point(upic_ctx* c, const double* pos, const double* f3, int64_t np, double* out) {

@@ -51,6 +51,8 @@ def lib():
             "upic_bdf_wave": [vp, vp, vp, C.c_int, vp],
             "upic_local_substep": [vp, vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double, vp, vp, vp, C.c_int64,
                                    vp, vp, vp, vp, vp],
+            "upic_local_substep_noalloc": [vp, vp, vp, vp, vp, vp, C.c_double, C.c_double, C.c_double, vp, vp, vp,
+                                           C.c_int64, vp, vp, vp, vp, vp],
             "upic_deposit_scalar_point": [vp, vp, vp, C.c_int64, C.c_int, vp],
             "upic_fft": [vp, vp, C.c_int],
             "upic_spectral_gradient": [vp, vp, vp], "upic_spectral_divergence": [vp, vp, vp],
@@ -267,13 +269,15 @@ class Context:
         self.sync()
         return out
 
-    def local_substep(self, guess, x0, P0, q, m, h, th0, th1, An, Aend, gpm):
+    def local_substep(self, guess, x0, P0, q, m, h, th0, th1, An, Aend, gpm, noalloc=False):
         """solve_local_particle_substep on device arrays; returns host
-        arrays vbar, P_end, v_end, iterations, reasons."""
+        arrays vbar, P_end, v_end, iterations, reasons.  noalloc selects
+        upic_local_substep_noalloc (static node-record scratch)."""
         n = q.numel()
         vb, Pe, ve = self.empty(3 * n), self.empty(3 * n), self.empty(3 * n)
         it, rs = DeviceArray(n, np.int32), DeviceArray(n, np.uint8)
-        self._check(lib().upic_local_substep(self.h, self.ptr(guess), self.ptr(x0), self.ptr(P0), self.ptr(q),
+        fn = lib().upic_local_substep_noalloc if noalloc else lib().upic_local_substep
+        self._check(fn(self.h, self.ptr(guess), self.ptr(x0), self.ptr(P0), self.ptr(q),
                                              self.ptr(m), h, th0, th1, self.ptr(An), self.ptr(Aend), self.ptr(gpm), n,
                                              self.ptr(vb), self.ptr(Pe), self.ptr(ve), self.ptr(it), self.ptr(rs)))
         self.sync()
