@@ -395,8 +395,10 @@ upic_status upic_sim_train_guess(upic_sim* sim, int32_t epochs, int32_t batch, d
 /* Modes 1 and 2 apply to the first outer evaluation of a step; the later
  * evaluations (whose fields differ from it by the outer update only) start
  * each particle from the orbit-averaged velocity it converged to in the
- * previous evaluation, until two steps in a row need more than four outer
- * evaluations: from then on every evaluation starts from mode 1 or 2 (the
+ * previous evaluation, until two steps in a row that start from the
+ * extrapolated current 2 J^n - J^{n-1} (every step but a run's first) need
+ * more than four outer evaluations: from then on every evaluation starts
+ * from mode 1 or 2 (the
  * vbar start remembers earlier evaluations within the inner tolerance,
  * which slows weak-current outer solves).  Mode 0 keeps the reference's v^n
  * throughout. */

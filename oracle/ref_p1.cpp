@@ -94,6 +94,37 @@ int ref_p1_init(const upic_params* p, int kind, int ppc, double alpha, double vt
         return 0;
     } catch (const std::exception& e) { ref_err = e.what(); return 1; }
 }
+// The field and history tail of the reference's initialisers (P1:1279-1301,
+// the same steps as the two-stream initialiser's) applied to the particles
+// st->x, v, q, m given by the caller: rho, J, phi, psi, A, U, P and the
+// Taylor start-up v_prev, composed from the reference's own functions.
+int ref_p1_init_from_particles(const upic_params* p, double n0, upic_host_state* st) {
+    try {
+        const Config c = p1_cfg(p); const Grid g = make_grid(c);
+        State s;
+        s.x = ref_load3<Vec3>(st->x, st->np); s.v = ref_load3<Vec3>(st->v, st->np);
+        s.q.assign(st->q, st->q + st->np); s.m.assign(st->m, st->m + st->np);
+        const std::size_t np = s.x.size();
+        s.v_prev.assign(np, Vec3{0.0, 0.0, 0.0});
+        s.P.assign(np, Vec3{0.0, 0.0, 0.0});
+        Field rho_e = deposit_scalar_to_mesh(s.x, s.q, g, c.spline_order, true);
+        s.rho.assign(g.N, 0.0);
+        for (int id = 0; id < g.N; ++id) s.rho[id] = rho_e[id] + n0;
+        subtract_mean(s.rho);
+        s.J = deposit_current_to_mesh(s.x, s.v, s.q, g, c.spline_order);
+        s.phi = solve_periodic_poisson_for_phi(s.rho, g, c.eps0);
+        s.psi.assign(g.N, 0.0);
+        s.A = zero_vecfield(g.N);
+        s.U = zero_vecfield(g.N);
+        for (std::size_t i = 0; i < np; ++i) s.P[i] = mul3(s.m[i], s.v[i]);
+        State tmp = s;
+        const VecField E0 = compute_E(tmp, g);
+        const auto E0p = gather_vector_all(s.x, E0, g, c.spline_order);
+        for (std::size_t i = 0; i < np; ++i) s.v_prev[i] = sub3(s.v[i], mul3(c.dt, mul3(s.q[i] / s.m[i], E0p[i])));
+        p1_store(s, st);
+        return 0;
+    } catch (const std::exception& e) { ref_err = e.what(); return 1; }
+}
 int ref_p1_step(const upic_params* p, const upic_host_state* in, upic_host_state* out, int32_t* iters, double* errs) {
     try {
         const Config c = p1_cfg(p); const Grid g = make_grid(c); const Quadrature qd = gauss_legendre_01(c.orbit_quad_order);

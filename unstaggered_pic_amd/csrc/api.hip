@@ -85,8 +85,6 @@ upic_status upic_ctx_create(const upic_params* params, upic_ctx** out) {
             c->plan.tw[d] = dalloc<double>(n);
             HIPCHK(hipMemcpy(c->plan.tw[d], tw.data(), n * sizeof(double), hipMemcpyHostToDevice));
         }
-        HIPCHK(hipEventCreate(&c->ev0));
-        HIPCHK(hipEventCreate(&c->ev1));
         c->dred = dalloc<double>(64);
         c->dstats = dalloc<unsigned long long>(8);
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hred), 64 * sizeof(double), hipHostMallocDefault));
@@ -110,8 +108,7 @@ upic_status upic_ctx_destroy(upic_ctx* c) {
     dfree(c->work);
     if (c->hred) (void)hipHostFree(c->hred);
     if (c->hstats) (void)hipHostFree(c->hstats);
-    if (c->ev0) (void)hipEventDestroy(c->ev0);
-    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t e : c->all_events) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return UPIC_OK;
@@ -124,9 +121,11 @@ upic_status upic_ctx_sync(upic_ctx* c) {
 
 upic_status upic_ctx_set_timing(upic_ctx* c, int enable) {
     if (!c) return UPIC_EINVAL;
-    c->timing = enable != 0;
-    c->timers.clear();
-    return UPIC_OK;
+    return (upic_status)guarded(c, [&] {
+        c->sync();
+        c->timing = enable != 0;
+        c->timers.clear();
+    });
 }
 
 upic_status upic_ctx_set_graphs(upic_ctx* c, int enable) {
@@ -137,6 +136,8 @@ upic_status upic_ctx_set_graphs(upic_ctx* c, int enable) {
 
 upic_status upic_ctx_timing(upic_ctx* c, const char* name, int64_t* launches, double* total_ms) {
     if (!c || !name || !launches || !total_ms) return UPIC_EINVAL;
+    const int rc = guarded(c, [&] { c->sync(); });
+    if (rc != UPIC_OK) return (upic_status)rc;
     auto it = c->timers.find(name);
     *launches = it == c->timers.end() ? 0 : it->second.launches;
     *total_ms = it == c->timers.end() ? 0.0 : it->second.ms;

@@ -149,13 +149,17 @@ struct upic_ctx {
     // group-A workspace, grown on demand
     double* work = nullptr;
     size_t work_n = 0;
-    // per-kernel-family timing with HIP events on `stream`
+    // per-kernel-family timing with HIP events on `stream`: each timed launch
+    // is bracketed by two events from a pool; the pairs are read at the
+    // stream's next synchronisation (no wait per launch)
     struct KTimer { int64_t launches = 0; double ms = 0.0; };
+    struct PendingTimer { const char* name; hipEvent_t e0, e1; };
     bool timing = false;
     std::map<std::string, KTimer> timers;
+    std::vector<PendingTimer> pending_timers;
+    std::vector<hipEvent_t> event_pool, all_events;
     // replay the field solve of a map evaluation from a captured hipGraph
     bool graphs = false;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     int N() const { return g.N; }
     double* workspace(size_t n) {
@@ -166,7 +170,34 @@ struct upic_ctx {
         }
         return work;
     }
-    void sync() { HIPCHK(hipStreamSynchronize(stream)); }
+    void sync() {
+        HIPCHK(hipStreamSynchronize(stream));
+        resolve_timers();
+    }
+    hipEvent_t take_event() {
+        if (event_pool.empty()) {
+            hipEvent_t e = nullptr;
+            HIPCHK(hipEventCreate(&e));
+            all_events.push_back(e);
+            return e;
+        }
+        hipEvent_t e = event_pool.back();
+        event_pool.pop_back();
+        return e;
+    }
+    // Reads the event pairs of completed timed launches (the stream is idle).
+    void resolve_timers() {
+        for (const PendingTimer& t : pending_timers) {
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, t.e0, t.e1));
+            auto& k = timers[t.name];
+            k.launches += 1;
+            k.ms += ms;
+            event_pool.push_back(t.e0);
+            event_pool.push_back(t.e1);
+        }
+        pending_timers.clear();
+    }
     double c2() const { return p.c * p.c; }
     double phi_src() const { return p.code == 3 ? p.sigma1 : 1.0 / p.eps0; }
     double A_src() const { return p.code == 3 ? p.sigma2 : p.mu0; }
@@ -249,18 +280,16 @@ int guarded(upic_ctx* c, F&& f) {
 
 // Runs `launch` between two events on the context stream when timing is on
 // and accumulates the elapsed time under `name`.
+// `name` must have static storage (a string literal).
 template <class F>
 void timed(upic_ctx* c, const char* name, F&& launch) {
     if (!c->timing) { launch(); return; }
-    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    if (c->pending_timers.size() >= 4096) c->sync();
+    hipEvent_t e0 = c->take_event(), e1 = c->take_event();
+    HIPCHK(hipEventRecord(e0, c->stream));
     launch();
-    HIPCHK(hipEventRecord(c->ev1, c->stream));
-    HIPCHK(hipEventSynchronize(c->ev1));
-    float ms = 0.0f;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    auto& t = c->timers[name];
-    t.launches += 1;
-    t.ms += ms;
+    HIPCHK(hipEventRecord(e1, c->stream));
+    c->pending_timers.push_back({name, e0, e1});
 }
 
 // real fields -> spectra; spectra -> real (destroys the spectra)

@@ -1016,6 +1016,10 @@ void p3_step(upic_sim* s, upic_step_info* info) {
     s->jdep_valid = false;
     s->a0_valid = false;
     s->step_maps = 0;
+    // a step that cannot start from the extrapolated current (the first one
+    // of a run) needs more outer evaluations whatever the inner start: it
+    // does not count against the vbar start
+    const bool extrapolated_start = s->mixed && s->jprev_valid;
     maybe_sort(s);
     keep_step_start(s);
     state_spectra(s);
@@ -1061,7 +1065,7 @@ void p3_step(upic_sim* s, upic_step_info* info) {
         s->train_n = std::min(s->train_cap, s->train_n + rows);
         s->train_written = false;
     }
-    if (s->mixed) {
+    if (s->mixed && extrapolated_start) {
         s->vdep_strikes = s->step_maps > kVdepMaxMaps ? s->vdep_strikes + 1 : 0;
         if (s->vdep_strikes >= 2) s->vdep_off = true;
     }
