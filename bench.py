@@ -10,7 +10,11 @@ deck (Project3/inputs/manuscript/weibel_main_dt02_T100.txt).  Particles are
 generated by the library's seeded loader (synthetic, seed 12345).
 
 Legs of the default run (--form all), each from the same initial state and
-timing the same steps (warmup, then --steps):
+timing the same steps (--spinup start-up steps, --warmup, then --steps; the
+first steps of a run are a start-up transient -- a Taylor-started history
+velocity, a first step without the extrapolated current, and in the mixed
+form a pause of the vbar start of later sweeps -- whose cost per step is
+reported as spinup_ms_per_step):
   * plain FP64: the reference's algorithm and tolerances;
   * mixed precision, guess mode 0: FP32 particle state, push and deposit
     tiles, FP64 mesh, FFT and convergence tests at the FP32 tolerance floors
@@ -324,6 +328,14 @@ def run_leg(args, lib, code, prec, mode, net=None):
             sim.set_guess_params(net)
         sim.set_guess_mode(mode)
     d0 = sim.diagnostics(1)
+    # start-up steps of the run (Taylor-started history, first step without
+    # the extrapolated current), then the warmup
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.spinup):
+        sim.step()
+    ctx.sync()
+    t_spin = time.perf_counter() - t0
     for _ in range(args.warmup):
         sim.step()
     ctx.set_timing(not args.no_timing)
@@ -363,6 +375,7 @@ def run_leg(args, lib, code, prec, mode, net=None):
     return {"p": p, "np": np_, "N": N, "elapsed": el, "infos": infos, "launches": launches, "kms": kms, "dep": dep,
             "kname": kname, "d0": d0, "d_start": d_start, "d1": d1, "dep_choice": dep_choice,
             "upload_seconds": t_upload, "as_reference": asref, "code": code, "prec": prec, "mode": mode,
+            "spinup_ms_per_step": 1e3 * t_spin / args.spinup if args.spinup else None,
             "workload": workload, "ppc": ppc}
 
 
@@ -440,6 +453,7 @@ def compute_roofline(r, args):
 def leg_summary(r, args, traffic_form=None):
     value = r["np"] * args.steps / r["elapsed"]
     out = {"value": value, "ms_per_step": 1e3 * r["elapsed"] / args.steps,
+           "spinup_ms_per_step": r["spinup_ms_per_step"],
            "avg_launch_ms": r["kms"] / max(r["launches"], 1), "physics": physics(r),
            "roofline": roofline(r, args, traffic_form)}
     cr = compute_roofline(r, args)
@@ -462,6 +476,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--spinup", type=int, default=20,
+                    help="start-up steps of each leg's run before the warmup (reported as spinup_ms_per_step)")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="weibel",
                     help="deck; sets the grid side and particles per cell unless --nx/--ppc are given")
     ap.add_argument("--nx", type=int, default=None)
@@ -546,6 +562,7 @@ def main():
         "roofline": summ["roofline"],
         "physics": summ["physics"],
         "host_upload_seconds": r["upload_seconds"],
+        "spinup_steps": args.spinup, "spinup_ms_per_step": r["spinup_ms_per_step"],
     }
     for k in ("compute_roofline", "as_reference", "deposit_kernel_avg_ms", "deposit_variant_sorted_unsorted"):
         if k in summ:

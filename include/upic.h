@@ -395,12 +395,14 @@ upic_status upic_sim_train_guess(upic_sim* sim, int32_t epochs, int32_t batch, d
 /* Modes 1 and 2 apply to the first outer evaluation of a step; the later
  * evaluations (whose fields differ from it by the outer update only) start
  * each particle from the orbit-averaged velocity it converged to in the
- * previous evaluation, until two steps in a row that start from the
- * extrapolated current 2 J^n - J^{n-1} (every step but a run's first) need
- * more than four outer evaluations: from then on every evaluation starts
- * from mode 1 or 2 (the
- * vbar start remembers earlier evaluations within the inner tolerance,
- * which slows weak-current outer solves).  Mode 0 keeps the reference's v^n
+ * previous evaluation.  That start remembers earlier evaluations within the
+ * inner tolerance, which slows some outer solves (weak net currents, the
+ * first steps of a run whose network guesses pass at their first
+ * iteration): when two steps in a row that start from the extrapolated
+ * current 2 J^n - J^{n-1} (every step but a run's first) need more than
+ * four outer evaluations, every evaluation starts from mode 1 or 2 for the
+ * next 16 steps, and the vbar start is then tried again (each relapse
+ * doubles the pause, up to 1024 steps).  Mode 0 keeps the reference's v^n
  * throughout. */
 upic_status upic_sim_set_guess_mode(upic_sim* sim, int32_t mode);
 upic_status upic_sim_guess_params(upic_sim* sim, float* params);          /* UPIC_MLP_NPARAM floats out */

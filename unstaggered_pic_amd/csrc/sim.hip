@@ -9,6 +9,13 @@
 #include "api_internal.hpp"
 #include "mlp.hpp"
 
+// map evaluations per step beyond which the later sweeps stop starting from
+// the previous sweep's vbar (upic_sim::vdep_off_steps), the first pause in
+// steps and its cap
+constexpr int kVdepMaxMaps = 4;
+constexpr int kVdepBackoff0 = 16;
+constexpr int kVdepBackoffMax = 1024;
+
 struct upic_sim {
     upic_ctx* c = nullptr;
     int64_t np = 0;
@@ -63,10 +70,14 @@ struct upic_sim {
     // Starting later sweeps from the previous sweep's vbar makes the map
     // remember its history within the inner tolerance.  Where the outer
     // solve then needs more than kVdepMaxMaps evaluations in two steps in a
-    // row (weak net currents such as Landau loads), the simulation stops
-    // using the vbar start: every sweep starts from the predictor.
+    // row (weak net currents such as Landau loads, or the first steps of a
+    // run whose network guesses are accepted at their first iteration), the
+    // simulation stops using the vbar start for vdep_backoff steps (every
+    // sweep starts from the predictor or the network), then tries it again;
+    // each relapse doubles the pause.
     int step_maps = 0, vdep_strikes = 0;
-    bool vdep_off = false;
+    int vdep_off_steps = 0, vdep_backoff = kVdepBackoff0;
+    bool vdep_used = false;  // this step's later sweeps started from vbar
     MixedState ms{}, mo{};
     float* ftmp = nullptr;   // 3*np sort scratch
     float* ftmp1 = nullptr;  // np sort scratch
@@ -142,9 +153,6 @@ constexpr double kMixedFinalSlack = 2.0;
 // the outer absolute tolerance is also floored at this fraction of the
 // mean gross current density (about 2 FP32 ulps).
 constexpr double kMixedGrossFloor = 1.0e-7;
-// map evaluations per step beyond which the later sweeps stop starting from
-// the previous sweep's vbar (upic_sim::vdep_off)
-constexpr int kVdepMaxMaps = 4;
 
 namespace {
 
@@ -618,9 +626,10 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
         // later ones, whose fields differ by the outer update only, from the
         // converged vbar of the previous sweep (no counterpart in the
         // reference, whose start is always v^n, P3:5318)
-        if (s->guess_mode >= 1 && s->vdep_valid && !use_layout && !s->vdep_off) {
+        if (s->guess_mode >= 1 && s->vdep_valid && !use_layout && s->vdep_off_steps == 0) {
             a.guess_mode = 0;
             a.guess = s->vdep;
+            s->vdep_used = true;
         }
         ++s->step_maps;
         s->vdep_valid = false;  // until this evaluation has completed
@@ -1016,6 +1025,7 @@ void p3_step(upic_sim* s, upic_step_info* info) {
     s->jdep_valid = false;
     s->a0_valid = false;
     s->step_maps = 0;
+    s->vdep_used = false;
     // a step that cannot start from the extrapolated current (the first one
     // of a run) needs more outer evaluations whatever the inner start: it
     // does not count against the vbar start
@@ -1065,9 +1075,17 @@ void p3_step(upic_sim* s, upic_step_info* info) {
         s->train_n = std::min(s->train_cap, s->train_n + rows);
         s->train_written = false;
     }
-    if (s->mixed && extrapolated_start) {
-        s->vdep_strikes = s->step_maps > kVdepMaxMaps ? s->vdep_strikes + 1 : 0;
-        if (s->vdep_strikes >= 2) s->vdep_off = true;
+    if (s->mixed) {
+        if (s->vdep_off_steps > 0) {
+            --s->vdep_off_steps;
+        } else if (extrapolated_start && s->vdep_used) {
+            s->vdep_strikes = s->step_maps > kVdepMaxMaps ? s->vdep_strikes + 1 : 0;
+            if (s->vdep_strikes >= 2) {
+                s->vdep_off_steps = s->vdep_backoff;
+                s->vdep_backoff = std::min(2 * s->vdep_backoff, kVdepBackoffMax);
+                s->vdep_strikes = 0;
+            }
+        }
     }
     ++s->steps;
 }
@@ -1487,8 +1505,9 @@ upic_status upic_sim_upload(upic_sim* s, const upic_host_state* st) {
         HIPCHK(hipMemcpy(s->id, ids.data(), s->np * sizeof(int64_t), hipMemcpyHostToDevice));
         s->steps = 0;
         s->start_valid = false;
-        s->vdep_off = false;
+        s->vdep_off_steps = 0;
         s->vdep_strikes = 0;
+        s->vdep_backoff = kVdepBackoff0;
         if (s->mixed) mixed_compress(s);
     });
 }
@@ -1610,8 +1629,9 @@ static upic_status init_fields(upic_sim* s, double n0, const double* A_host, con
             c->sync();
             s->steps = 0;
             s->start_valid = false;
-            s->vdep_off = false;
+            s->vdep_off_steps = 0;
             s->vdep_strikes = 0;
+            s->vdep_backoff = kVdepBackoff0;
             return;
         }
         fp64_view(s);
@@ -1669,8 +1689,9 @@ static upic_status init_fields(upic_sim* s, double n0, const double* A_host, con
         c->sync();
         s->steps = 0;
         s->start_valid = false;
-        s->vdep_off = false;
+        s->vdep_off_steps = 0;
         s->vdep_strikes = 0;
+        s->vdep_backoff = kVdepBackoff0;
     });
 }
 
