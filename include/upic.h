@@ -121,6 +121,20 @@ typedef struct upic_path_stats {
     int32_t pad;
 } upic_path_stats;
 
+/* The update that produced the accepted outer iterate of a P3 step (the
+ * reference's convergence_method, P3:5898, 6122-6130): its CSV string is
+ * "nested_full_dt_" (or, after a selective refinement, "selective_") followed
+ * by picard_fallback, picard, current_anderson, current_anderson_backtracked,
+ * current_picard or current_picard_backtracked. */
+typedef enum upic_update_method {
+    UPIC_UPDATE_PICARD_FALLBACK = 0,  /* converged at the first evaluation (Anderson solver) */
+    UPIC_UPDATE_PICARD = 1,           /* converged at the first evaluation (Picard solver)   */
+    UPIC_UPDATE_CURRENT_ANDERSON = 2,
+    UPIC_UPDATE_CURRENT_ANDERSON_BACKTRACKED = 3,
+    UPIC_UPDATE_CURRENT_PICARD = 4,
+    UPIC_UPDATE_CURRENT_PICARD_BACKTRACKED = 5
+} upic_update_method;
+
 /* Outcome of one time step (P1 PicardHistoryEntry 1503-1507, P3
  * NonlinearSolveInfo 5493-5519). */
 typedef struct upic_step_info {
@@ -147,6 +161,19 @@ typedef struct upic_step_info {
     double local_scaled_residual_max;
     int64_t local_iterations_total;  /* summed over all map evaluations of the step   */
     upic_path_stats path;
+    /* P3 outer update and selective refinement (NonlinearSolveInfo
+     * P3:5508-5518, set at 6171-6185 and 6346-6356); without a refinement the
+     * full-dt layout counts: one substep per particle, work ratio 1. */
+    int32_t update_method;                   /* upic_update_method                          */
+    int32_t uniform_substeps_used;           /* largest substep count of the accepted layout */
+    int32_t selective_fallback_used;         /* 1 = the step needed the particle refinement  */
+    int32_t direct_attempt_iterations;       /* outer iterations of the failed full-dt try   */
+    int32_t selective_refinement_restarts;   /* substep doublings before the accepted layout */
+    int32_t reserved0;
+    int64_t selective_refined_particles;     /* particles with more than one substep         */
+    int64_t selective_total_particle_substeps;
+    double selective_work_ratio;             /* total substeps / particles                    */
+    int64_t selective_level_counts[7];       /* particles at 1, 2, 4, ..., 64 substeps         */
 } upic_step_info;
 
 /* Per-step diagnostics (P1 Diagnostics 1037-1055, P3 Diagnostics 3469-3499,
@@ -353,10 +380,22 @@ typedef struct upic_step_diagnostics {
 /* Keep (enable != 0) or stop keeping each step's start state (P1, P3;
  * 6 np + 7 N doubles of HBM). */
 upic_status upic_sim_keep_step_start(upic_sim* sim, int enable);
-/* The step diagnostics of the last step; UPIC_EINVAL without a kept start
- * or after a step that needed the selective per-particle refinement (the
- * reference's subcycled variant, P3:4363-4490, is not implemented). */
+/* The step diagnostics of the last step (UPIC_EINVAL without a kept start).
+ * After a P3 step that needed the selective per-particle refinement they are
+ * compute_subcycled_step_diagnostics (P3:4363-4473, dispatched at
+ * 4505-4512): every substep's chain-rule defect (A at the substep's times
+ * theta0, theta1 = s/nsub, (s+1)/nsub), the works over the substep orbits
+ * and the mesh work against the step's current. */
 upic_status upic_sim_step_diagnostics(upic_sim* sim, upic_step_diagnostics* out);
+/* The substep paths of the last P3 step when it was selectively refined
+ * (State::subcycle_offset_last, subcycle_x0_last, subcycle_vbar_last,
+ * P3:3220-3221, filled at 6211-6213): host arrays in the original particle
+ * order -- offsets (np + 1; particle p's substeps are entries
+ * offsets[p] .. offsets[p+1]-1), start points x0 and orbit-averaged
+ * velocities vbar (3 * n_total each, component-major).  *n_total is set to
+ * the number of substeps (0 when the last step was not refined); with
+ * offsets == NULL only *n_total is set. */
+upic_status upic_sim_substep_paths(upic_sim* sim, int64_t* n_total, int64_t* offsets, double* x0, double* vbar);
 /* Device pointers of the resident state (for zero-copy callers).  With
  * UPIC_PREC_MIXED the particle pointers are an FP64 view rebuilt from the
  * FP32 state by this call; writes to them reach the simulation only through

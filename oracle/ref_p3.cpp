@@ -132,6 +132,24 @@ int ref_p3_init(const upic_params* p, int kind, int ppc, double alpha, double vt
         return 0;
     } catch (const std::exception& e) { ref_err = e.what(); return 1; }
 }
+// ref_p3_init with the eigenmode switch of the two-stream / Weibel loads
+// (two_stream_eigenmode, weibel_eigenmode; P3:3689-4108)
+int ref_p3_init_eigen(const upic_params* p, int kind, int ppc, double alpha, double vth, double v0, double pert,
+                      int mode, double n0, int eigenmode, upic_host_state* st) {
+    try {
+        Config c = p3_cfg(p);
+        c.test_case = kind == 0 ? "landau_weak" : (kind == 1 ? "cold_relativistic_two_stream" : "cold_relativistic_weibel");
+        c.particles_per_cell_pair = ppc; c.landau_alpha = alpha; c.thermal_velocity = vth; c.v0 = v0;
+        c.perturbation = pert; c.perturbation_mode = mode; c.n0 = n0;
+        if (kind == 1) { c.two_stream_density_alpha = alpha; c.two_stream_eigenmode = eigenmode != 0; }
+        if (kind == 2) { c.weibel_B0 = alpha; c.weibel_eigenmode = eigenmode != 0; }
+        const Grid g = make_grid(c);
+        const State s = initialize_state(c, g);
+        if (!st->x) { st->np = static_cast<int64_t>(s.x.size()); return 0; }
+        p3_store(s, st);
+        return 0;
+    } catch (const std::exception& e) { ref_err = e.what(); return 1; }
+}
 int ref_p3_step(const upic_params* p, const upic_host_state* in, upic_host_state* out, int32_t* iters, double* info3) {
     try {
         const Config c = p3_cfg(p); const Grid g = make_grid(c); const Quadrature qd = gauss_legendre_01(c.orbit_quad_order);
@@ -161,6 +179,58 @@ int ref_p3_step_diagnostics(const upic_params* p, const upic_host_state* s0, con
                               d.field_energy_residual, (double)d.crossing_particles, (double)d.total_crossings,
                               d.average_path_segments, (double)d.max_path_segments};
         std::memcpy(out17, v, sizeof(v));
+        return 0;
+    } catch (const std::exception& e) { ref_err = e.what(); return 1; }
+}
+static void p3_step_diag17(const StepDiagnostics& d, double* out17) {
+    const double v[17] = {d.A_chain_abs_rms, d.A_chain_rel_rms, d.A_chain_max_norm, d.A_chain_mean_norm,
+                          d.A_chain_work_defect, d.delta_K, d.delta_W, d.delta_total_energy, d.particle_work,
+                          d.grid_work, d.deposit_gather_work_residual, d.particle_energy_residual,
+                          d.field_energy_residual, (double)d.crossing_particles, (double)d.total_crossings,
+                          d.average_path_segments, (double)d.max_path_segments};
+    std::memcpy(out17, v, sizeof(v));
+}
+// compute_step_diagnostics of s0 -> s1 where s1 carries substep paths
+// (offsets np + 1; x0, vbar 3*n component-major): the reference dispatches
+// to compute_subcycled_step_diagnostics (P3:4505-4512)
+int ref_p3_step_diagnostics_paths(const upic_params* p, const upic_host_state* s0, const upic_host_state* s1,
+                                  const int64_t* off, int64_t n, const double* x0, const double* vb, double* out17) {
+    try {
+        const Config c = p3_cfg(p); const Grid g = make_grid(c); const Quadrature qd = gauss_legendre_01(c.orbit_quad_order);
+        const State a = p3_load(s0, g.N, c.dt);
+        State b = p3_load(s1, g.N, c.dt);
+        b.subcycle_offset_last.assign(off, off + s0->np + 1);
+        b.subcycle_x0_last = ref_load3<Vec3>(x0, n);
+        b.subcycle_vbar_last = ref_load3<Vec3>(vb, n);
+        p3_step_diag17(compute_step_diagnostics(a, b, g, c.dt, c.spline_order, qd, c.sigma1, c.sigma2, c.kappa,
+                                                c.split_orbit_at_knots), out17);
+        return 0;
+    } catch (const std::exception& e) { ref_err = e.what(); return 1; }
+}
+// One nonlinear_step_orbit of the reference (selective fallback as the
+// parameters say) with its NonlinearSolveInfo (info16: iterations,
+// converged, uniform_substeps_used, selective_fallback_used,
+// direct_attempt_iterations, selective_refinement_restarts,
+// selective_refined_particles, selective_total_particle_substeps,
+// selective_work_ratio, selective_level_counts[7]) and the step diagnostics
+// of in -> out computed from the reference's own end state (out17).
+int ref_p3_step_refined(const upic_params* p, const upic_host_state* in, upic_host_state* out, double* info16,
+                        double* out17) {
+    try {
+        const Config c = p3_cfg(p); const Grid g = make_grid(c); const Quadrature qd = gauss_legendre_01(c.orbit_quad_order);
+        std::vector<NonlinearHistoryEntry> hist;
+        NonlinearSolveInfo info;
+        const State s0 = p3_load(in, g.N, c.dt);
+        const State s1 = nonlinear_step_orbit(s0, g, c, qd, hist, &info, 0.0, nullptr);
+        p3_store(s1, out);
+        const double v[9] = {(double)info.iterations, info.converged ? 1.0 : 0.0, (double)info.uniform_substeps_used,
+                             info.selective_fallback_used ? 1.0 : 0.0, (double)info.direct_attempt_iterations,
+                             (double)info.selective_refinement_restarts, (double)info.selective_refined_particles,
+                             (double)info.selective_total_particle_substeps, info.selective_work_ratio};
+        std::memcpy(info16, v, sizeof(v));
+        for (int k = 0; k < 7; ++k) info16[9 + k] = (double)info.selective_level_counts[k];
+        p3_step_diag17(compute_step_diagnostics(s0, s1, g, c.dt, c.spline_order, qd, c.sigma1, c.sigma2, c.kappa,
+                                                c.split_orbit_at_knots), out17);
         return 0;
     } catch (const std::exception& e) { ref_err = e.what(); return 1; }
 }

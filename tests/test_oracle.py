@@ -327,6 +327,45 @@ def test_oracle_matches_compiled_reference_p3(orc, ref):
             assert np.array_equal(getattr(out, f), getattr(so, f)), f
 
 
+def refinement_deck():
+    """The reference's forced-refinement check (Project3/tests/test_solver.cpp:
+    48-60): 4^3 cold Weibel, one particle per cell pair, dt 1, linear splines,
+    a single inner iteration at rtol 2e-3, so the full-dt solve fails and the
+    selective particle refinement takes over."""
+    return make_params(code=3, nx=4, ny=4, nz=4, Lx=TWO_PI, Ly=TWO_PI, Lz=TWO_PI, dt=1.0, spline_order=1,
+                       orbit_quad_order=8, nyquist_projection=0, nonlinear_rtol=1e-9, nonlinear_atol=1e-11,
+                       nonlinear_max_iter=32, anderson_damping=0.5, local_rtol=2e-3, local_atol=1e-10,
+                       local_max_iter=1, selective_particle_fallback=1, selective_particle_max_substeps=64)
+
+
+def test_oracle_refined_step_matches_compiled_reference(orc, ref):
+    """The selectively refined P3 step (nonlinear_step_orbit's fallback,
+    P3:6328-6454) from the reference's own forced-refinement state: the
+    oracle's step is bit-identical to the reference's, and the reference's
+    refinement record is what the check in test_solver.cpp asserts
+    (converged, refined particles, more than one substep)."""
+    p = refinement_deck()
+    h = HostState()
+    args = (C.byref(p), 2, 1, C.c_double(0.2), C.c_double(1.0), C.c_double(0.9), C.c_double(0.0), 1,
+            C.c_double(1.0), 0)
+    ref.p3.ref_p3_last_error.restype = C.c_char_p
+    assert ref.p3.ref_p3_init_eigen(*args, C.byref(h)) == 0, ref.p3.ref_p3_last_error()
+    st = State(h.np, 64)
+    assert ref.p3.ref_p3_init_eigen(*args, C.byref(st.host())) == 0
+    out = State(st.np, 64)
+    info = np.zeros(16)
+    diag = np.zeros(17)
+    assert ref.p3.ref_p3_step_refined(C.byref(p), C.byref(st.host()), C.byref(out.host()), ptr(info), ptr(diag)) == 0, \
+        ref.p3.ref_p3_last_error()
+    assert info[1] == 1 and info[3] == 1 and info[6] > 0 and info[2] > 1
+    assert info[7] == sum(info[9 + k] * 2 ** k for k in range(7))
+    so, oi = orc.step(p, st)
+    for f in ("x", "v", "P", "phi", "psi", "A", "U", "J", "rho"):
+        assert np.array_equal(getattr(out, f), getattr(so, f)), f
+    assert oi.iterations == info[0]
+    assert np.all(np.isfinite(diag))
+
+
 # ---------------------------------------------------------------------------
 # Learned-guess network (no reference counterpart): the oracle's FP64 forward
 # and training gradient, pinned by a closed form and by finite differences.

@@ -56,7 +56,17 @@ class StepInfo(C.Structure):
         ("local_iterations_mean", C.c_double), ("local_scaled_residual_max", C.c_double),
         ("local_iterations_total", C.c_int64),
         ("path", PathStats),
+        ("update_method", C.c_int32), ("uniform_substeps_used", C.c_int32),
+        ("selective_fallback_used", C.c_int32), ("direct_attempt_iterations", C.c_int32),
+        ("selective_refinement_restarts", C.c_int32), ("reserved0", C.c_int32),
+        ("selective_refined_particles", C.c_int64), ("selective_total_particle_substeps", C.c_int64),
+        ("selective_work_ratio", C.c_double), ("selective_level_counts", C.c_int64 * 7),
     ]
+
+
+# upic_update_method -> the reference's convergence_method suffix (P3:5898, 6122-6130)
+UPDATE_METHOD_NAMES = ("picard_fallback", "picard", "current_anderson", "current_anderson_backtracked",
+                       "current_picard", "current_picard_backtracked")
 
 
 class Diagnostics(C.Structure):

@@ -26,6 +26,18 @@ struct P1Args {
     double *x_out, *v_out, *P_out, *vprev_out, *vbar_out;  // final mode
 };
 
+// The substep paths of a layout sweep's write-out pass (the reference's
+// State::subcycle_offset_last / subcycle_x0_last / subcycle_vbar_last,
+// P3:3220-3221): substep s of stored particle p is entry off[p] + s of
+// x0 and vbar (component-major, n entries per component).  x0 == NULL: not
+// recorded.
+struct SubPaths {
+    const int64_t* off;
+    double* x0;
+    double* vbar;
+    int64_t n;
+};
+
 // P3 per-particle map arguments.
 struct P3Args {
     int64_t np;
@@ -44,6 +56,7 @@ struct P3Args {
     double* redmax;               // [2]: max local iterations, max final scaled residual
     unsigned long long* stats;
     double* part;                 // per-block partials (kPartSlots per block)
+    SubPaths sub;                 // substep start points and vbar (write-out pass, or empty)
 };
 
 // P2 explicit step arguments.
@@ -133,6 +146,10 @@ struct P3MArgs {
     // (a0_in)
     float* a0_out;
     const float* a0_in;
+    // substep paths of the write-out pass of a layout sweep, as FP64
+    // positions -L/2 + (cell + offset) h (sub.x0 == NULL: not recorded)
+    SubPaths sub;
+    double lo_x, lo_y, lo_z, hx, hy, Lz;
 };
 // Picard sweep; with a substep layout it also deposits, otherwise it writes
 // vdep and launch_deposit_mixed deposits (reason and vdep as inputs).
@@ -284,6 +301,16 @@ void launch_kinetic(hipStream_t s, int64_t np, const double* v, const double* m,
 void launch_step_diag_orbit(hipStream_t s, int64_t np, int midpoint, double dt, const double L[3], const double* x0,
                             const double* v0, const double* v1, const double* vbar_last, double* vbar, double* dx,
                             double* x1);
+// compute_subcycled_step_diagnostics (P3:4363-4473): per particle, its
+// substeps' chain-rule defects, works and path segments over the recorded
+// substep paths; sums14: [0..4) |R|^2, |A_delta|^2, |rhs|^2, |R| (weighted
+// by 1/nsub), [4] work defect, [5] particle work, [6] mesh crossings;
+// max2: [0] max |R|; stats: crossing particles, total segments, max
+// segments per particle.
+void launch_step_diag_subcycled(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, int64_t np,
+                                double dt, const SubPaths& sp, const double* qa, const double* A0, const double* A1,
+                                const double* Umid, const double* Emid, double* part, double* part2, double* sums,
+                                double* max1, unsigned long long* stats);
 void launch_step_diag_particles(hipStream_t s, int64_t np, double dt, const double* q, const double* vbar,
                                 const double* Ubar, const double* D, const double* A0p, const double* A1p,
                                 const double* Ebar, double* part, double* part2, double* sums6, double* max1);

@@ -814,6 +814,16 @@ __device__ __forceinline__ void map_particle(const MGrid& g, const QuadF& qd, in
                 }
             }
         }
+        if (a.write_out && a.sub.x0) {
+            // the substep's start point and vbar, as FP64 positions (the
+            // FP64 view's -L/2 + (cell + offset) h)
+            const int64_t j = a.sub.off[p] + s, n = a.sub.n;
+            a.sub.x0[j] = a.lo_x + ((double)ci + (double)tx) * a.hx;
+            a.sub.x0[n + j] = a.lo_y + ((double)cj + (double)ty) * a.hy;
+            a.sub.x0[2 * n + j] = a.lo_z + (double)tz * a.Lz;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) a.sub.vbar[d * n + j] = (double)vacc[d];
+        }
         const float dux = h * vacc[0] * g.ihx, duy = h * vacc[1] * g.ihy;
         if constexpr (FUSED) {
             if (a.J_out)

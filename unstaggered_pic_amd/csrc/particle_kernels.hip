@@ -394,6 +394,11 @@ __global__ __launch_bounds__(256) void k_p3_map(GridDev g, QuadDev qd, OrbitOpt 
             rmax[0] = fmax(rmax[0], (double)its);
             rmax[1] = fmax(rmax[1], scaled);
             if (!conv) break;
+            if (a.write_out && a.sub.x0) {
+                const int64_t j = a.sub.off[p] + s;
+                st3(a.sub.x0, a.sub.n, j, xpos);
+                st3(a.sub.vbar, a.sub.n, j, vacc);
+            }
             const double dxp[3] = {h * vacc[0], h * vacc[1], h * vacc[2]};
             if (a.J_out) {
                 if constexpr (Z3) nseg_total += deposit_orbit_one<R, Z3>(g, qd, opt, xpos, dxp, vacc, qp / (double)nsub, a.J_out);
@@ -755,6 +760,91 @@ __global__ __launch_bounds__(256) void k_step_diag_particles(int64_t np, double 
     }
     block_partials<4, 1>(s4, mx, 0, part);
     block_partials<2, 0>(w2, nullptr, 0, part2);
+}
+
+// Segments of the knot-split orbit x0 -> x0 + dxp (the entries of
+// path_breakpoints_for_particle minus one, P3:2555-2606).
+__device__ __forceinline__ int orbit_segments(const GridDev& g, const OrbitOpt& opt, const double x0[3],
+                                              const double dxp[3]) {
+    if (!opt.split) return 1;
+    BreakGen bg;
+    breakgen_init(bg, g, x0, dxp, true);
+    int n = 1;
+    while (breakgen_next(bg) < 1.0) ++n;
+    return n;
+}
+
+// compute_subcycled_step_diagnostics (P3:4363-4473): one work-item per
+// particle walks its recorded substeps in order.  For substep s of nsub
+// (h = dt / nsub, theta in [s, s + 1] / nsub): R = A(theta1, x1) -
+// A(theta0, x0) - h (Ubar + D_A(theta0, theta1) vbar), the 1/nsub-weighted
+// norms of R, A_delta and rhs, the work defect -q vbar.R, the particle work
+// h q vbar.Ebar and the orbit's segments.
+template <int R, bool Z3>
+__global__ __launch_bounds__(256) void k_step_diag_subcycled(GridDev g, QuadDev qd, OrbitOpt opt, int64_t np,
+                                                             double dt, SubPaths sp, const double* __restrict__ qa,
+                                                             const double* __restrict__ A0,
+                                                             const double* __restrict__ A1,
+                                                             const double* __restrict__ Umid,
+                                                             const double* __restrict__ Emid,
+                                                             double* __restrict__ part, double* __restrict__ part2) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double s4[4] = {0.0, 0.0, 0.0, 0.0}, mx[1] = {0.0}, w3[3] = {0.0, 0.0, 0.0};
+    double segs = 0.0, crossed = 0.0;
+    if (p < np) {
+        const int64_t b = sp.off[p], e = sp.off[p + 1];
+        const int nsub = (int)(e - b);
+        const double h = dt / (double)nsub, w = 1.0 / (double)nsub;
+        const double qp = qa[p];
+        for (int s = 0; s < nsub; ++s) {
+            double x0[3], vb[3];
+            ld3(sp.x0, sp.n, b + s, x0);
+            ld3(sp.vbar, sp.n, b + s, vb);
+            const double th0 = (double)s / (double)nsub, th1 = (double)(s + 1) / (double)nsub;
+            const double dxp[3] = {h * vb[0], h * vb[1], h * vb[2]};
+            const double x1[3] = {wrap1(x0[0] + dxp[0], g.L[0]), wrap1(x0[1] + dxp[1], g.L[1]),
+                                  wrap1(x0[2] + dxp[2], g.L[2])};
+            double A0p[3], A1p[3], G[3], DA[3][3], Ub[3], Eb[3], Dm[3][3];
+            gather_time_interp<R, Z3>(g, A0, A1, th0, x0, A0p);
+            gather_time_interp<R, Z3>(g, A0, A1, th1, x1, A1p);
+            orbit_gather_grad<R, Z3, false, true>(g, qd, opt, x0, dxp, nullptr, A0, A1, th0, th1, false, G, DA);
+            orbit_gather_grad<R, Z3, true, false>(g, qd, opt, x0, dxp, Umid, nullptr, nullptr, 0.0, 1.0, false, Ub, Dm);
+            orbit_gather_grad<R, Z3, true, false>(g, qd, opt, x0, dxp, Emid, nullptr, nullptr, 0.0, 1.0, false, Eb, Dm);
+            double Rv[3], Ad[3], rhs[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                Ad[a] = A1p[a] - A0p[a];
+                rhs[a] = h * (Ub[a] + (DA[a][0] * vb[0] + DA[a][1] * vb[1] + DA[a][2] * vb[2]));
+                Rv[a] = Ad[a] - rhs[a];
+            }
+            const double r2 = Rv[0] * Rv[0] + Rv[1] * Rv[1] + Rv[2] * Rv[2];
+            const double nr = sqrt(r2);
+            s4[0] += w * r2;
+            s4[1] += w * (Ad[0] * Ad[0] + Ad[1] * Ad[1] + Ad[2] * Ad[2]);
+            s4[2] += w * (rhs[0] * rhs[0] + rhs[1] * rhs[1] + rhs[2] * rhs[2]);
+            s4[3] += w * nr;
+            mx[0] = fmax(mx[0], nr);
+            w3[0] += -qp * (vb[0] * Rv[0] + vb[1] * Rv[1] + vb[2] * Rv[2]);
+            w3[1] += h * qp * (vb[0] * Eb[0] + vb[1] * Eb[1] + vb[2] * Eb[2]);
+            const int nseg = orbit_segments(g, opt, x0, dxp);
+            segs += (double)nseg;
+            w3[2] += (double)(nseg - 1);
+            if (nseg > 1) crossed = 1.0;
+        }
+    }
+    block_partials_stats<4, 1>(s4, mx, crossed, segs, p < np ? segs : 1.0, part);
+    block_partials<3, 0>(w3, nullptr, 0, part2);
+}
+
+void launch_step_diag_subcycled(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, int64_t np,
+                                double dt, const SubPaths& sp, const double* qa, const double* A0, const double* A1,
+                                const double* Umid, const double* Emid, double* part, double* part2, double* sums,
+                                double* max1, unsigned long long* stats) {
+    if (np <= 0) return;
+    UPIC_DISPATCH(g.order, g.nz > 1, hipLaunchKernelGGL((k_step_diag_subcycled<R, Z>), dim3(nblk(np)), dim3(256), 0, s,
+                                                       g, q, o, np, dt, sp, qa, A0, A1, Umid, Emid, part, part2));
+    launch_finish_partials(s, part, nblk(np), 4, 1, sums, max1, stats);
+    launch_finish_partials(s, part2, nblk(np), 3, 0, sums + 4, nullptr, nullptr);
 }
 
 void launch_step_diag_orbit(hipStream_t s, int64_t np, int midpoint, double dt, const double L[3], const double* x0,

@@ -138,6 +138,21 @@ struct upic_sim {
     } tuner;
     // captured field solves, one per source-current buffer
     std::vector<std::pair<const double*, hipGraphExec_t>> fs_graphs;
+    // substep paths of the last step when it was selectively refined (the
+    // reference's State::subcycle_offset_last / _x0_last / _vbar_last,
+    // P3:3220-3221): offsets per stored particle (np + 1), start points and
+    // vbar per substep (3 x sub_n each)
+    int64_t* sub_off = nullptr;
+    double *sub_x0 = nullptr, *sub_vb = nullptr;
+    int64_t sub_n = 0, sub_cap = 0;
+    bool sub_valid = false;
+    // refinement record of the last P3 step (NonlinearSolveInfo
+    // P3:5511-5518)
+    struct RefineInfo {
+        int used = 0, direct_iterations = 0, restarts = 0, max_substeps = 1;
+        int64_t refined = 0, total_substeps = 0;
+        int64_t levels[7] = {};
+    } refine;
 };
 
 // Mixed precision: an FP32 map resolves each node current only to a fraction
@@ -173,6 +188,9 @@ void sim_free(upic_sim* s) {
     dfree(s->ms.cell); dfree(s->mo.cell);
     dfree(s->retry_n); dfree(s->retry_idx); dfree(s->retry_gs); dfree(s->retry_old); dfree(s->Jdep); dfree(s->a0); dfree(s->s0);
     dfree(s->mlp_P); dfree(s->mlp_blob); dfree(s->train_buf); dfree(s->part); dfree(s->vdep);
+    dfree(s->sub_off); dfree(s->sub_x0); dfree(s->sub_vb);
+    s->sub_cap = s->sub_n = 0;
+    s->sub_valid = false;
     for (auto& gx : s->fs_graphs) (void)hipGraphExecDestroy(gx.second);
     s->fs_graphs.clear();
     if (s->tuner.e0) (void)hipEventDestroy(s->tuner.e0);
@@ -584,6 +602,7 @@ Norms residual_norms(upic_sim* s, const double* map, const double* guess) {
 
 struct RefineNeeded : Error {
     std::vector<int64_t> failed;  // stored-order particle indices
+    int iterations = 0;           // outer iterations recorded for the failed attempt
     RefineNeeded(const std::string& m, std::vector<int64_t> f) : Error(UPIC_EREFINE, m), failed(std::move(f)) {}
 };
 
@@ -645,6 +664,11 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
         a.a0_in = !use_layout && a.guess_mode == 0 && a.train == nullptr && s->a0_valid ? s->a0 : nullptr;
         a0_written = a.a0_out != nullptr;
         a.retry_old = incremental ? s->retry_old : nullptr;
+        if (use_layout && write_out && s->sub_x0) {
+            a.sub = SubPaths{s->sub_off, s->sub_x0, s->sub_vb, s->sub_n};
+            a.lo_x = -0.5 * p.Lx; a.lo_y = -0.5 * p.Ly; a.lo_z = -0.5 * p.Lz;
+            a.hx = c->g.h[0]; a.hy = c->g.h[1]; a.Lz = p.Lz;
+        }
         s->jdep_valid = false;  // until this evaluation has completed
         timed(c, "p3_map_mixed", [&] { launch_p3_map_mixed(c->stream, c->g, c->qf, c->opt.split, a); });
         if (!use_layout) {
@@ -688,6 +712,7 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     a.x_out = s->xo; a.P_out = s->Po; a.v_out = s->vo; a.vbar_out = s->vbo;
     a.reason = s->reason;
     a.red = c->dred; a.redmax = c->dred + 2; a.stats = c->dstats; a.part = s->part;
+    if (use_layout && write_out && s->sub_x0) a.sub = SubPaths{s->sub_off, s->sub_x0, s->sub_vb, s->sub_n};
     timed(c, "p3_map", [&] { launch_p3_map(c->stream, c->g, c->q, c->opt, a); });
     }
     launch_project_nyquist(c->stream, c->sp, Jmap, 3);
@@ -886,13 +911,21 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
     Norms prefetched{};  // norms of (Jacc, guess) from the accepted trial
     Norms conv_norms{};
     const int keep = p.anderson_memory + 1;
+    // the update that produced the current iterate (the reference's
+    // last_update_method, P3:5898, 6122-6137)
+    int last_method = picard ? UPIC_UPDATE_PICARD : UPIC_UPDATE_PICARD_FALLBACK;
     for (int it = 0; it < p.nonlinear_max_iter; ++it) {
         Norms nm{};
         if (have_prefetched) {
             have_prefetched = false;
             nm = prefetched;
         } else {
-            eval(s->guess, s->Jacc, false, nm);
+            try {
+                eval(s->guess, s->Jacc, false, nm);
+            } catch (RefineNeeded& e) {
+                e.iterations = it + 1;  // as the reference records it (P3:5963-5966)
+                throw;
+            }
         }
         LinComb lr{2, {1.0, -1.0}, {s->Jacc, s->guess}};
         launch_lincomb(c->stream, n3, lr, s->resid);
@@ -954,6 +987,8 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
         if (!accepted) throw Error(UPIC_ENONLINEAR, "outer current Anderson/Picard backtracking failed at the prescribed macro dt");
         if (as_anderson) ++and_steps; else ++pic_steps;
         if (alpha < 1.0) { ++backtracked; min_alpha = std::min(min_alpha, alpha); }
+        last_method = as_anderson ? (alpha < 1.0 ? UPIC_UPDATE_CURRENT_ANDERSON_BACKTRACKED : UPIC_UPDATE_CURRENT_ANDERSON)
+                                  : (alpha < 1.0 ? UPIC_UPDATE_CURRENT_PICARD_BACKTRACKED : UPIC_UPDATE_CURRENT_PICARD);
         prev_rms = nm.rms_relative;
         std::swap(s->guess, s->trial);
         std::swap(s->Jacc, s->Jtry);
@@ -986,6 +1021,7 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
         info->local_scaled_residual_max = fo.scaled_max;
         info->local_iterations_total = local_total;
         info->path = fo.ps;
+        info->update_method = last_method;
     }
     if (!ok)
         throw Error(UPIC_ENONLINEAR, "final outer macro-current consistency evaluation failed at the prescribed macro dt"
@@ -1017,6 +1053,39 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
     c->sync();
 }
 
+// Uploads a per-particle substep layout (stored order) with its substep
+// offsets, sizes the substep-path buffers and records the layout's
+// statistics (P3:5248-5263, 6172-6184).
+void set_layout(upic_sim* s, const std::vector<int8_t>& lay) {
+    const int64_t np = s->np;
+    std::vector<int64_t> off((size_t)np + 1, 0);
+    auto& r = s->refine;
+    r.refined = 0;
+    r.max_substeps = 1;
+    for (int k = 0; k < 7; ++k) r.levels[k] = 0;
+    for (int64_t i = 0; i < np; ++i) {
+        const int nsub = lay[i];
+        off[i + 1] = off[i] + nsub;
+        if (nsub > 1) ++r.refined;
+        r.max_substeps = std::max(r.max_substeps, nsub);
+        int lev = 0;
+        for (int v = nsub; v > 1; v /= 2) ++lev;
+        ++r.levels[std::min(lev, 6)];
+    }
+    r.total_substeps = off[np];
+    HIPCHK(hipMemcpy(s->layout, lay.data(), (size_t)np, hipMemcpyHostToDevice));
+    if (!s->sub_off) s->sub_off = dalloc<int64_t>((size_t)np + 1);
+    HIPCHK(hipMemcpy(s->sub_off, off.data(), ((size_t)np + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
+    if (r.total_substeps > s->sub_cap) {
+        dfree(s->sub_x0);
+        dfree(s->sub_vb);
+        s->sub_cap = std::max<int64_t>(r.total_substeps, np + np / 4);
+        s->sub_x0 = dalloc<double>(3 * (size_t)s->sub_cap);
+        s->sub_vb = dalloc<double>(3 * (size_t)s->sub_cap);
+    }
+    s->sub_n = r.total_substeps;
+}
+
 // nonlinear_step_orbit (P3:6477-6526) with the selective fallback (6328-6454)
 void p3_step(upic_sim* s, upic_step_info* info) {
     upic_ctx* c = s->c;
@@ -1045,12 +1114,22 @@ void p3_step(upic_sim* s, upic_step_info* info) {
         s->outer_atol = std::max({c->p.nonlinear_atol, c->p.nonlinear_rtol * std::sqrt(c->hred[20] / (double)N),
                                   kMixedGrossFloor * gross});
     }
+    s->sub_valid = false;
+    s->refine = upic_sim::RefineInfo{};
+    s->refine.total_substeps = s->np;
+    s->refine.levels[0] = s->np;
     try {
         p3_step_layout(s, false, info);
     } catch (const RefineNeeded& e) {
         if (!c->p.selective_particle_fallback)
             throw Error(UPIC_ENONLINEAR, std::string("inner particle solve failed over full dt; time and dt unchanged: ") + e.what());
+        // nonlinear_step_selective_particle_fallback (P3:6328-6454): the
+        // failed particles' substeps double until the layout converges; the
+        // reference first repeats the failed full-dt layout, which fails the
+        // same way, so its restart count is the number of doublings
         s->start_refined = true;
+        s->refine.used = 1;
+        s->refine.direct_iterations = e.iterations;
         std::vector<int8_t> lay((size_t)s->np, 1);
         std::vector<int64_t> failed = e.failed;
         for (;;) {
@@ -1061,14 +1140,27 @@ void p3_step(upic_sim* s, upic_step_info* info) {
                                                      " with particles still unconverged; macro dt was not changed");
             }
             for (int64_t i : failed) lay[i] = (int8_t)std::min<int>(c->p.selective_particle_max_substeps, 2 * lay[i]);
-            HIPCHK(hipMemcpy(s->layout, lay.data(), (size_t)s->np, hipMemcpyHostToDevice));
+            ++s->refine.restarts;
+            set_layout(s, lay);
             try {
                 p3_step_layout(s, true, info);
+                s->sub_valid = true;
                 break;
             } catch (const RefineNeeded& e2) {
                 failed = e2.failed;
             }
         }
+    }
+    if (info) {
+        const auto& r = s->refine;
+        info->uniform_substeps_used = r.max_substeps;
+        info->selective_fallback_used = r.used;
+        info->direct_attempt_iterations = r.direct_iterations;
+        info->selective_refinement_restarts = r.restarts;
+        info->selective_refined_particles = r.refined;
+        info->selective_total_particle_substeps = r.total_substeps;
+        info->selective_work_ratio = s->np ? (double)r.total_substeps / (double)s->np : 0.0;
+        for (int k = 0; k < 7; ++k) info->selective_level_counts[k] = r.levels[k];
     }
     if (s->train_written) {  // rows of the final map evaluation of this step
         const int64_t rows = (s->np + s->train_stride - 1) / s->train_stride;
@@ -1727,8 +1819,8 @@ upic_status upic_sim_step_diagnostics(upic_sim* s, upic_step_diagnostics* out) {
         c->err = "no step start kept: enable upic_sim_keep_step_start before the step";
         return UPIC_EINVAL;
     }
-    if (s->start_refined) {
-        c->err = "step diagnostics of a selectively refined step (per-particle substep paths) are not implemented";
+    if (s->start_refined && !s->sub_valid) {
+        c->err = "the last step's substep paths were not recorded (it failed or was interrupted)";
         return UPIC_EINVAL;
     }
     upic_diagnostics d1{};
@@ -1768,6 +1860,47 @@ upic_status upic_sim_step_diagnostics(upic_sim* s, upic_step_diagnostics* out) {
         {
             LinComb em{3, {-0.5, -0.5, -1.0}, {s->gtmp, s->gtmp + 3ll * N, Umid}};
             launch_lincomb(c->stream, 3ll * N, em, Emid);
+        }
+        if (s->start_refined) {
+            // compute_subcycled_step_diagnostics (P3:4363-4473) over the
+            // recorded substep paths; the mesh work uses the step's current
+            launch_zero(c->stream, 8, red);
+            HIPCHK(hipMemsetAsync(c->dstats, 0, 3 * sizeof(unsigned long long), c->stream));
+            const SubPaths sp{s->sub_off, s->sub_x0, s->sub_vb, s->sub_n};
+            launch_step_diag_subcycled(c->stream, c->g, c->q, c->opt, np, dt, sp, s->q, A0, s->A, Umid, Emid, part,
+                                       part2, red, red + 7, c->dstats);
+            const double* xs[1] = {s->J};
+            const double* ys[1] = {Emid};
+            launch_zero(c->stream, 1, c->dred + 48);
+            launch_dot_many(c->stream, 3ll * N, 1, xs, ys, c->dred + 48);
+            HIPCHK(hipMemcpyAsync(c->hred + 32, red, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(c->hred + 48, c->dred + 48, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(c->hstats, c->dstats, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                  c->stream));
+            HIPCHK(hipGetLastError());
+            c->sync();
+            const double* h = c->hred + 32;
+            const double n3 = 3.0 * (double)std::max<int64_t>(1, np);
+            upic_step_diagnostics& o = *out;
+            std::memset(&o, 0, sizeof(o));
+            o.A_chain_abs_rms = std::sqrt(h[0] / n3);
+            o.A_chain_rel_rms = o.A_chain_abs_rms / (0.5 * (std::sqrt(h[1] / n3) + std::sqrt(h[2] / n3)) + 1.0e-300);
+            o.A_chain_max_norm = h[7];
+            o.A_chain_mean_norm = h[3] / (double)std::max<int64_t>(1, np);
+            o.A_chain_work_defect = h[4];
+            o.particle_work = h[5];
+            o.grid_work = dt * c->hred[48] * c->g.dV;
+            o.delta_K = d1.kinetic_energy - s->K0;
+            o.delta_W = d1.field_energy - s->W0;
+            o.delta_total_energy = o.delta_K + o.delta_W;
+            o.deposit_gather_work_residual = o.particle_work - o.grid_work;
+            o.particle_energy_residual = o.delta_K - o.particle_work;
+            o.field_energy_residual = o.delta_W + o.grid_work;
+            o.crossing_particles = (int64_t)c->hstats[0];
+            o.total_crossings = (int64_t)std::llround(h[6]);
+            o.avg_path_segments = np ? (double)c->hstats[1] / (double)np : 1.0;
+            o.max_path_segments = std::max<int32_t>(1, (int32_t)c->hstats[2]);
+            return;
         }
         // orbit residual R_A (compute_RA_orbit_residual)
         launch_gather_orbit(c->stream, c->g, c->q, c->opt, x0, dx, Umid, np, Ubar);
@@ -1810,6 +1943,40 @@ upic_status upic_sim_step_diagnostics(upic_sim* s, upic_step_diagnostics* out) {
         o.total_crossings = segs >= np ? segs - np : 0;
         o.avg_path_segments = np ? (double)segs / (double)np : 1.0;
         o.max_path_segments = std::max<int32_t>(1, (int32_t)c->hstats[2]);
+    });
+}
+
+upic_status upic_sim_substep_paths(upic_sim* s, int64_t* n_total, int64_t* offsets, double* x0, double* vbar) {
+    if (!s || !n_total) return UPIC_EINVAL;
+    *n_total = s->sub_valid ? s->sub_n : 0;
+    if (!offsets || !s->sub_valid) return UPIC_OK;
+    if (!x0 || !vbar) return UPIC_EINVAL;
+    return (upic_status)guarded(s->c, [&] {
+        upic_ctx* c = s->c;
+        const int64_t np = s->np, n = s->sub_n;
+        c->sync();
+        std::vector<int64_t> off((size_t)np + 1), id((size_t)np);
+        std::vector<double> hx(3 * (size_t)n), hv(3 * (size_t)n);
+        HIPCHK(hipMemcpy(off.data(), s->sub_off, off.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(id.data(), s->id, id.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hx.data(), s->sub_x0, hx.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hv.data(), s->sub_vb, hv.size() * sizeof(double), hipMemcpyDeviceToHost));
+        // stored order -> original order
+        std::vector<int64_t> cnt((size_t)np, 0), where((size_t)np, 0);
+        for (int64_t p = 0; p < np; ++p) {
+            cnt[(size_t)id[p]] = off[p + 1] - off[p];
+            where[(size_t)id[p]] = p;
+        }
+        offsets[0] = 0;
+        for (int64_t i = 0; i < np; ++i) offsets[i + 1] = offsets[i] + cnt[i];
+        for (int64_t i = 0; i < np; ++i) {
+            const int64_t src = off[where[i]], dst = offsets[i];
+            for (int64_t k = 0; k < cnt[i]; ++k)
+                for (int d = 0; d < 3; ++d) {
+                    x0[d * n + dst + k] = hx[d * n + src + k];
+                    vbar[d * n + dst + k] = hv[d * n + src + k];
+                }
+        }
     });
 }
 

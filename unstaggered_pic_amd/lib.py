@@ -71,6 +71,7 @@ def lib():
             "upic_sim_init_fields_AU": [vp, C.c_double, vp, vp],
             "upic_sim_keep_step_start": [vp, C.c_int],
             "upic_sim_step_diagnostics": [vp, C.POINTER(StepDiagnostics)],
+            "upic_sim_substep_paths": [vp, C.POINTER(C.c_int64), vp, vp, vp],
             "upic_p3_initial_state": [C.POINTER(Params), C.POINTER(P3Case), C.POINTER(HostState), dp],
             "upic_ctx_set_timing": [vp, C.c_int],
             "upic_device_count": [C.POINTER(C.c_int)], "upic_device_alloc": [C.c_size_t, C.POINTER(vp)],
@@ -413,6 +414,20 @@ class Simulation:
         d = StepDiagnostics()
         self.ctx._check(lib().upic_sim_step_diagnostics(self.h, C.byref(d)))
         return d
+
+    def substep_paths(self):
+        """The last step's substep paths when it was selectively refined:
+        (offsets (np + 1), x0 (3 x n), vbar (3 x n)) in the original
+        particle order, or None."""
+        n = C.c_int64()
+        self.ctx._check(lib().upic_sim_substep_paths(self.h, C.byref(n), None, None, None))
+        if n.value == 0:
+            return None
+        off = np.zeros(self.np + 1, np.int64)
+        x0, vb = np.zeros(3 * n.value), np.zeros(3 * n.value)
+        self.ctx._check(lib().upic_sim_substep_paths(self.h, C.byref(n), off.ctypes.data_as(vp), x0.ctypes.data_as(vp),
+                                                     vb.ctypes.data_as(vp)))
+        return off, x0, vb
 
     def failure_reasons(self):
         r = np.zeros(self.np, np.uint8)
