@@ -153,7 +153,7 @@ typedef struct upic_step_info {
     double final_rms_residual;
     double final_max_particle_residual;
     double final_absolute_rms_residual;
-    double estimated_map_contraction;
+    double estimated_map_contraction;   /* |dG| / |dx| of the last two outer iterates (P3:5983-5996) */
     /* P3 inner solve */
     int32_t local_iterations_max;
     int32_t failed_particles;
@@ -174,6 +174,8 @@ typedef struct upic_step_info {
     int64_t selective_total_particle_substeps;
     double selective_work_ratio;             /* total substeps / particles                    */
     int64_t selective_level_counts[7];       /* particles at 1, 2, 4, ..., 64 substeps         */
+    double observed_convergence_rate;        /* (final / first rms residual)^(1/(iters-1)),
+                                                P3:5535-5547                                    */
 } upic_step_info;
 
 /* Per-step diagnostics (P1 Diagnostics 1037-1055, P3 Diagnostics 3469-3499,

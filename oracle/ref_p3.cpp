@@ -234,6 +234,21 @@ int ref_p3_step_refined(const upic_params* p, const upic_host_state* in, upic_ho
         return 0;
     } catch (const std::exception& e) { ref_err = e.what(); return 1; }
 }
+// The reference program itself (its main, renamed where the source is
+// included): argv[1] is the deck; its output files go where the deck's
+// output_prefix points.
+int ref_p3_main(int argc, char** argv) { return upic_ref_p3_program_main(argc, argv); }
+// read_config + write_config_echo (P3:1178-1435, 8217-8300) of a deck, with
+// the particle count of its initial state
+int ref_p3_config_echo(const char* deck, const char* out) {
+    try {
+        const Config c = read_config(deck);
+        const Grid g = make_grid(c);
+        const State s = initialize_state(c, g);
+        write_config_echo(out, c, g, s.x.size());
+        return 0;
+    } catch (const std::exception& e) { ref_err = e.what(); return 1; }
+}
 int ref_p3_diagnostics(const upic_params* p, const upic_host_state* s, int mode, double* out10) {
     try {
         const Config c = p3_cfg(p); const Grid g = make_grid(c);

@@ -125,6 +125,7 @@ particle_local_rtol = 1.0e-12
 particle_local_atol = 1.0e-14
 particle_local_max_iter = 64
 precision = {prec}
+nyquist_projection = true
 output_prefix = weibel_small
 """
 
@@ -164,6 +165,7 @@ anderson_damping = 0.5
 particle_local_rtol = 1.0e-12
 particle_local_atol = 1.0e-14
 particle_local_max_iter = 64
+nyquist_projection = true
 output_prefix = weibel_eig
 """
 
@@ -337,3 +339,222 @@ def test_driver_p2_matches_validation_summary(tmp_path, example, method, update)
     assert f"static_ions = {4096 if example == 'cloud' else 0}" in echo
     sample = list(csv.DictReader(open(tmp_path / f"{name}_particle_sample.csv")))
     assert len(sample) == 2000 and list(sample[0].keys())[-1] == "gamma"
+
+
+# ---------------------------------------------------------------------------
+# The driver against the reference program itself: the reference's main
+# (compiled from Project3/ec_pic_nyquist_projected.cpp into oracle/_ref,
+# run in-process on the CPU) and upic_run p3 on the same deck write the same
+# <prefix>_diagnostics.csv and <prefix>_step_diagnostics.csv.
+# ---------------------------------------------------------------------------
+# Project3/tests/test_solver.cpp:48-60: one inner iteration at rtol 2e-3
+# forces the selective particle refinement
+P3_FORCED_REFINEMENT_DECK = """test_case = cold_relativistic_weibel
+nx = 4
+ny = 4
+nz = 4
+Lx = 6.283185307179586
+Ly = 6.283185307179586
+Lz = 6.283185307179586
+n_steps = 2
+dt = 1.0
+particles_per_cell_pair = 1
+spline_order = 1
+orbit_quad_order = 8
+v0 = 0.9
+weibel_B0 = 0.2
+weibel_eigenmode = false
+nonlinear_rtol = 1e-9
+nonlinear_atol = 1e-11
+nonlinear_max_iter = 32
+anderson_damping = 0.5
+uniform_local_rtol = 2e-3
+uniform_local_atol = 1e-10
+uniform_local_max_iter = 1
+selective_particle_fallback = true
+"""
+# the reference's manuscript decks (inputs/manuscript/tsi_main_dt01_T60.txt,
+# weibel_main_dt02_T100.txt) reduced as its validate.py does (4^3, two
+# particles per cell pair, six steps)
+P3_MANUSCRIPT_TSI_REDUCED = """test_case = cold_relativistic_two_stream
+nx = 4
+ny = 4
+nz = 4
+Lx = 25.132741228718345
+Ly = 25.132741228718345
+Lz = 25.132741228718345
+n_steps = 6
+dt = 0.1
+particles_per_cell_pair = 2
+spline_order = 2
+orbit_quad_order = 8
+split_orbit_at_knots = true
+kappa = 1.0
+sigma1 = 1.0
+sigma2 = 1.0
+rho0 = 1.0
+v0 = 0.9
+perturbation = 0.0
+two_stream_density_alpha = 0.005
+two_stream_eigenmode = true
+two_stream_finite_grid_theory = true
+perturbation_mode = 1
+nonlinear_solver = anderson_quasi_newton
+nonlinear_rtol = 1.0e-11
+nonlinear_atol = 1.0e-13
+nonlinear_max_iter = 80
+anderson_memory = 4
+anderson_start = 2
+anderson_damping = 0.5
+anderson_regularization = 1.0e-12
+anderson_growth_limit = 1.2
+selective_particle_fallback = false
+particle_local_rtol = 1.0e-11
+particle_local_atol = 1.0e-13
+particle_local_max_iter = 64
+particle_local_damping = 1.0
+nyquist_projection = true
+"""
+P3_MANUSCRIPT_WEIBEL_REDUCED = """test_case = cold_relativistic_weibel
+nx = 4
+ny = 4
+nz = 4
+Lx = 6.283185307179586
+Ly = 6.283185307179586
+Lz = 6.283185307179586
+n_steps = 6
+dt = 0.2
+particles_per_cell_pair = 2
+spline_order = 2
+orbit_quad_order = 8
+split_orbit_at_knots = true
+kappa = 1.0
+sigma1 = 1.0
+sigma2 = 1.0
+rho0 = 1.0
+v0 = 0.9
+weibel_B0 = 1.0e-4
+weibel_eigenmode = true
+weibel_finite_grid_theory = true
+perturbation_mode = 1
+nonlinear_solver = anderson_quasi_newton
+nonlinear_rtol = 1.0e-11
+nonlinear_atol = 1.0e-13
+nonlinear_max_iter = 80
+anderson_memory = 4
+anderson_start = 2
+anderson_damping = 0.5
+anderson_regularization = 1.0e-12
+anderson_growth_limit = 1.2
+uniform_particle_subcycling = false
+selective_particle_fallback = false
+selective_particle_max_substeps = 64
+particle_local_rtol = 1.0e-12
+particle_local_atol = 1.0e-14
+particle_local_max_iter = 64
+particle_local_damping = 1.0
+adaptive_dt = false
+nyquist_projection = true
+"""
+# columns: exact (integers, strings), to 1e-11 relative (state quantities),
+# rounding-level residuals (absolute 1e-10 of the column's scale), skipped
+EXACT = {"step", "nonlinear_iterations", "nonlinear_converged", "anderson_restarts", "anderson_steps",
+         "picard_fallback_steps", "convergence_method", "backtracked_steps", "minimum_update_alpha",
+         "local_particle_iterations_max", "uniform_substeps_used", "selective_fallback_used",
+         "direct_attempt_iterations", "selective_refinement_restarts", "selective_refined_particles",
+         "selective_total_particle_substeps", "selective_work_ratio", "selective_n1", "selective_n2", "selective_n4",
+         "selective_n8", "selective_n16", "selective_n32", "selective_n64", "crossing_particles",
+         "max_path_segments", "total_crossings", "avg_path_segments", "dt", "time"}
+NOISE = {"gauss_rms", "gauss_max", "gauss_relative", "gauge_rms", "gauge_max", "gauge_relative",
+         "nonlinear_rms_residual", "nonlinear_max_particle_residual", "nonlinear_absolute_rms_residual",
+         "observed_convergence_rate", "estimated_map_contraction", "mean_rho", "A_chain_abs_rms", "A_chain_rel_rms",
+         "A_chain_max_norm", "A_chain_mean_norm", "A_chain_work_defect", "delta_total_energy",
+         "deposit_gather_work_residual", "particle_energy_residual", "field_energy_residual", "rho_mode",
+         "E_mode_im", "Bz_y_mode_re", "Bz_y_mode_im", "mode_over_theory"}
+SKIP = {"elapsed_wall_seconds"}
+# a rounding-level residual over its normalisation: compared where the
+# normalisation is resolved (a charge-neutral load has only rounding noise
+# in rho, and the ratio is then noise over noise)
+RATIO = {"gauss_relative": "gauss_normalization_rms", "gauge_relative": "gauge_normalization_rms"}
+
+
+# decided at the inner solves' tolerance (a particle accepting its vbar one
+# iteration earlier or later moves them): the same order of magnitude
+TOL_LEVEL = {"nonlinear_rms_residual", "nonlinear_max_particle_residual", "nonlinear_absolute_rms_residual",
+             "observed_convergence_rate", "estimated_map_contraction"}
+
+
+# energy changes and works of a step: their rounding and the inner
+# tolerance's effect are set by the total energy
+ENERGY_CHANGE = {"delta_K", "delta_W", "delta_total_energy", "particle_work", "grid_work",
+                 "deposit_gather_work_residual", "particle_energy_residual", "field_energy_residual"}
+
+
+def compare_csv(got_path, want_path, energy_scale=0.0):
+    got = list(csv.DictReader(open(got_path)))
+    want = list(csv.DictReader(open(want_path)))
+    assert len(got) == len(want)
+    assert list(got[0].keys()) == list(want[0].keys())
+    cols = list(want[0].keys())
+    scale = {c: max([abs(float(r[c])) for r in want if c not in EXACT | {"convergence_method"}
+                     and np.isfinite(float(r[c]))] or [0.0]) for c in cols if c not in ("convergence_method",)}
+    bad = []
+    for k, (a, b) in enumerate(zip(got, want)):
+        for c in cols:
+            if c in SKIP:
+                continue
+            if c == "convergence_method":
+                if a[c] != b[c]:
+                    bad.append((k, c, a[c], b[c]))
+                continue
+            x, y = float(a[c]), float(b[c])
+            if c in EXACT:
+                ok = x == pytest.approx(y, rel=1e-14, abs=0)
+            elif c in RATIO:
+                ok = float(b[RATIO[c]]) <= 1e-10 or abs(x - y) <= 1e-9
+            elif not np.isfinite(y):
+                ok = str(x) == str(y)
+            elif c in TOL_LEVEL:
+                ok = abs(x - y) <= 1e-9 or (y != 0.0 and 0.01 <= x / y <= 100.0)
+            elif c in ENERGY_CHANGE:
+                ok = abs(x - y) <= 1e-12 * energy_scale
+            else:
+                # 1e-11 relative, 1e-13 absolute (rounding-level entries
+                # such as a Gauss residual or a vanishing mode); the
+                # rounding-level residual columns to 1e-9 of their largest
+                # value
+                ok = abs(x - y) <= 1e-11 * abs(y) + 1e-13 + (1e-9 * scale[c] if c in NOISE else 0.0)
+            if not ok:
+                bad.append((k, c, x, y))
+    assert not bad, bad[:20]
+
+
+@pytest.mark.parametrize("name,deck", [("forced", P3_FORCED_REFINEMENT_DECK), ("tsi", P3_MANUSCRIPT_TSI_REDUCED),
+                                       ("weibel", P3_MANUSCRIPT_WEIBEL_REDUCED)], ids=["forced", "tsi", "weibel"])
+def test_driver_p3_matches_reference_program(tmp_path, name, deck):
+    """Diagnostics and step-diagnostics CSVs of the driver equal the
+    reference program's on the same deck: iteration counts, method strings,
+    refinement columns and path statistics exactly; every other column to
+    1e-11 relative plus 1e-13 absolute (rounding-level entries), the
+    rounding-level residual columns to 1e-9 of their largest value, the
+    step's energy changes and works to 1e-12 of the total energy, and the
+    outer residual norms (set at the inner solves' tolerance) to a factor of
+    100."""
+    import ctypes as C
+    from oracle_bind import Ref
+    ref = Ref()
+    if ref.p3 is None:
+        pytest.skip("reference harness (oracle/_ref) not built")
+    tail = "run_unit_tests = false\nfield_diagnostics = false\nselective_failure_diagnostics = false\n"
+    (tmp_path / "ref").mkdir()
+    (tmp_path / "ref" / "deck.txt").write_text(deck + tail + f"output_prefix = {tmp_path / 'ref' / name}\n")
+    argv = (C.c_char_p * 3)(b"ref", str(tmp_path / "ref" / "deck.txt").encode(), None)
+    assert ref.p3.ref_p3_main(2, argv) == 0
+    run_driver(tmp_path, "p3", deck + tail + f"output_prefix = {name}\n")
+    compare_csv(tmp_path / f"{name}_diagnostics.csv", tmp_path / "ref" / f"{name}_diagnostics.csv")
+    rows = list(csv.DictReader(open(tmp_path / "ref" / f"{name}_diagnostics.csv")))
+    te = max(abs(float(r["total_energy"])) for r in rows)
+    compare_csv(tmp_path / f"{name}_step_diagnostics.csv", tmp_path / "ref" / f"{name}_step_diagnostics.csv", te)
+    echo = (tmp_path / f"{name}_config_echo.txt").read_text().splitlines()
+    ref_echo = (tmp_path / "ref" / f"{name}_config_echo.txt").read_text().splitlines()
+    assert [l for l in echo if not l.startswith("output_prefix")] == [l for l in ref_echo if not l.startswith("output_prefix")]

@@ -441,6 +441,27 @@ void launch_residual_norms(hipStream_t s, int N, const double* map, const double
     hipLaunchKernelGGL(k_residual_norms, dim3(nb), dim3(256), 0, s, N, map, guess, floor_, rtol, atol, out);
 }
 
+// Sums of |g - xl|^2 and |g - xl + r - rl|^2: the outer iterate's change and
+// the map's change between two iterations (estimated_map_contraction,
+// P3:5983-5996).
+__global__ void k_contraction_sums(int64_t n, const double* __restrict__ g, const double* __restrict__ xl,
+                                   const double* __restrict__ r, const double* __restrict__ rl,
+                                   double* __restrict__ out) {
+    double v[2] = {0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double dx = g[i] - xl[i];
+        const double dg = dx + (r[i] - rl[i]);
+        v[0] += dx * dx;
+        v[1] += dg * dg;
+    }
+    block_reduce_add<2>(v, out);
+}
+void launch_contraction_sums(hipStream_t s, int64_t n, const double* g, const double* xl, const double* r,
+                             const double* rl, double* out) {
+    hipLaunchKernelGGL(k_contraction_sums, dim3((unsigned)std::min<int64_t>(gblk(n), 1024)), dim3(256), 0, s, n, g, xl,
+                       r, rl, out);
+}
+
 __global__ void k_sum_squares(int64_t n, const double* __restrict__ f, double* __restrict__ out) {
     double v[1] = {0.0};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) v[0] += f[i] * f[i];

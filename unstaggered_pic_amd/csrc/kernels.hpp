@@ -284,6 +284,9 @@ void launch_diff_norms(hipStream_t s, int64_t n, const double* a, const double* 
 void launch_residual_norms(hipStream_t s, int N, const double* map, const double* guess, double floor_, double rtol,
                            double atol, double* out);
 // Energy-type reductions over grid fields: out += {sum f^2 ...}
+// [0] += sum |g - xl|^2, [1] += sum |g - xl + r - rl|^2 over n entries
+void launch_contraction_sums(hipStream_t s, int64_t n, const double* g, const double* xl, const double* r,
+                             const double* rl, double* out);
 void launch_sum_squares(hipStream_t s, int64_t n, const double* f, double* out);
 void launch_sum(hipStream_t s, int64_t n, const double* f, double* out);
 void launch_max_abs(hipStream_t s, int64_t n, const double* f, double* out);

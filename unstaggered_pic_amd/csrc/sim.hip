@@ -144,6 +144,10 @@ struct upic_sim {
     // vbar per substep (3 x sub_n each)
     int64_t* sub_off = nullptr;
     double *sub_x0 = nullptr, *sub_vb = nullptr;
+    // per outer iteration: sums of |x_k - x_{k-1}|^2 and of the map change
+    // (estimated_map_contraction), 2 x nonlinear_max_iter
+    double* contr = nullptr;
+    double* hcontr = nullptr;  // pinned host mirror
     int64_t sub_n = 0, sub_cap = 0;
     bool sub_valid = false;
     // refinement record of the last P3 step (NonlinearSolveInfo
@@ -188,7 +192,9 @@ void sim_free(upic_sim* s) {
     dfree(s->ms.cell); dfree(s->mo.cell);
     dfree(s->retry_n); dfree(s->retry_idx); dfree(s->retry_gs); dfree(s->retry_old); dfree(s->Jdep); dfree(s->a0); dfree(s->s0);
     dfree(s->mlp_P); dfree(s->mlp_blob); dfree(s->train_buf); dfree(s->part); dfree(s->vdep);
-    dfree(s->sub_off); dfree(s->sub_x0); dfree(s->sub_vb);
+    dfree(s->sub_off); dfree(s->sub_x0); dfree(s->sub_vb); dfree(s->contr);
+    if (s->hcontr) (void)hipHostFree(s->hcontr);
+    s->hcontr = nullptr;
     s->sub_cap = s->sub_n = 0;
     s->sub_valid = false;
     for (auto& gx : s->fs_graphs) (void)hipGraphExecDestroy(gx.second);
@@ -897,7 +903,14 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
     bool converged = false, have_prefetched = false;
     int restarts = 0, and_steps = 0, pic_steps = 0, backtracked = 0, maps = 0, iters = 0;
     double min_alpha = 1.0, contraction = std::numeric_limits<double>::infinity();
+    double first_rms = std::numeric_limits<double>::infinity();
     int64_t local_total = 0;
+    if (!s->contr) {
+        s->contr = dalloc<double>(2 * (size_t)std::max(1, p.nonlinear_max_iter));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&s->hcontr), 2 * (size_t)std::max(1, p.nonlinear_max_iter) *
+                             sizeof(double), hipHostMallocDefault));
+    }
+    launch_zero(c->stream, 2ll * std::max(1, p.nonlinear_max_iter), s->contr);
     MapOut last{};
     // Jacc: current map of the accepted guess.  An evaluation also returns
     // the residual norms of its map against its guess (same synchronisation).
@@ -930,6 +943,10 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
         LinComb lr{2, {1.0, -1.0}, {s->Jacc, s->guess}};
         launch_lincomb(c->stream, n3, lr, s->resid);
         if (nm.nonfinite) throw Error(UPIC_ENONLINEAR, "uniform particle map returned a nonfinite macro current");
+        if (it == 0) first_rms = nm.rms_relative;
+        if (H.len > 0)  // the iterate's and the map's change since the last iteration (P3:5983-5996)
+            launch_contraction_sums(c->stream, n3, s->guess, H.X(H.len - 1), s->resid, H.Rr(H.len - 1),
+                                    s->contr + 2 * it);
         iters = it + 1;
         if (nm.rms_scaled <= 1.0 && nm.max_particle_scaled <= 1.0) {
             converged = true;
@@ -1002,6 +1019,27 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
     if (fast) fn = conv_norms;
     const double slack = s->mixed ? kMixedFinalSlack : 1.0;
     const bool ok = !fn.nonfinite && fn.rms_scaled <= slack && fn.max_particle_scaled <= slack;
+    // the per-iteration change sums, read after the commit's synchronisation
+    HIPCHK(hipMemcpyAsync(s->hcontr, s->contr, 2 * (size_t)iters * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    // estimated_map_contraction: the ratio at the last iteration whose
+    // iterate moved by more than 64 ulps (P3:5983-5996)
+    auto finish_contraction = [&] {
+        const double cnt = (double)n3;
+        for (int k = 1; k < iters; ++k) {
+            const double dx = std::sqrt(s->hcontr[2 * k] / cnt), dg = std::sqrt(s->hcontr[2 * k + 1] / cnt);
+            if (dx > 64.0 * std::numeric_limits<double>::epsilon()) contraction = dg / dx;
+        }
+        if (info) info->estimated_map_contraction = contraction;
+    };
+    // observed_nonlinear_convergence_rate (P3:5535-5547) of the per-iteration
+    // history against the final consistency residual
+    double observed = std::numeric_limits<double>::infinity();
+    if (iters >= 2 && first_rms > 0.0 && fn.rms_relative >= 0.0 && std::isfinite(fn.rms_relative)) {
+        const double ratio = fn.rms_relative / first_rms;
+        observed = fn.rms_relative == 0.0 ? 0.0
+                   : ((ratio > 0.0 && std::isfinite(ratio)) ? std::pow(ratio, 1.0 / (double)(iters - 1))
+                                                            : std::numeric_limits<double>::infinity());
+    }
     if (info) {
         std::memset(info, 0, sizeof(*info));
         info->iterations = iters;
@@ -1017,11 +1055,14 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
         info->final_absolute_rms_residual = fn.rms_absolute;
         info->estimated_map_contraction = contraction;
         info->local_iterations_max = (int32_t)fo.iter_max;
-        info->local_iterations_mean = s->np ? fo.iter_sum / (double)s->np : 0.0;
+        // per particle substep (P3:5407-5410)
+        const int64_t nsubsteps = use_layout ? s->sub_n : s->np;
+        info->local_iterations_mean = nsubsteps ? fo.iter_sum / (double)nsubsteps : 0.0;
         info->local_scaled_residual_max = fo.scaled_max;
         info->local_iterations_total = local_total;
         info->path = fo.ps;
         info->update_method = last_method;
+        info->observed_convergence_rate = observed;
     }
     if (!ok)
         throw Error(UPIC_ENONLINEAR, "final outer macro-current consistency evaluation failed at the prescribed macro dt"
@@ -1042,6 +1083,7 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
         s->jprev_valid = true;
         commit_fields(s, s->Jacc);
         c->sync();
+        finish_contraction();
         return;
     }
     std::swap(s->v_prev, s->v);   // v_prev = old v
@@ -1051,6 +1093,7 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
     std::swap(s->vbar_last, s->vbo);
     commit_fields(s, s->Jacc);
     c->sync();
+    finish_contraction();
 }
 
 // Uploads a per-particle substep layout (stored order) with its substep

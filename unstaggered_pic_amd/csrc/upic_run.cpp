@@ -14,12 +14,13 @@
 //                                P3 8217-8300)
 //   <prefix>_particle_sample.csv (P1 write_particle_sample 1716-1729, P2 986-1001,
 //                                P3 9039)
-// Columns whose quantity this build does not compute (selective-refinement
-// level counts, convergence-rate estimates) are written as nan.  P1 and P3
-// also write <prefix>_step_diagnostics.csv (R_A and the work residuals of
-// every step, P1:1818-1867, P3:8462-8890) from upic_sim_step_diagnostics
-// (nan after a selectively refined P3 step; deck key step_diagnostics =
-// false skips it for P3).
+// P1 and P3 also write <prefix>_step_diagnostics.csv (R_A and the work
+// residuals of every step, P1:1818-1867, P3:8462-8890, the subcycled form
+// after a selectively refined P3 step) from upic_sim_step_diagnostics (deck
+// key step_diagnostics = false skips it for P3).  Not written: P3's
+// transverse, field-probe, field-grid, particle-snapshot, selective-failure
+// and particle-inner-history files; a deck that asks for them gets a
+// warning.
 //
 // Initial conditions: P1 two_stream and landau_* are the reference's
 // deterministic loads (lattice / quiet start, P1:1151-1304) computed here on
@@ -33,6 +34,7 @@
 // Exit codes as the reference: 0 ok, 1 error, 2 usage.
 #include "../../include/upic.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -71,38 +73,50 @@ bool parse_bool(const std::string& v) {
     return s == "1" || s == "true" || s == "yes" || s == "on";
 }
 
+// One deck entry: its value and the line it was last set on.
+struct DeckValue {
+    std::string value;
+    int line = 0;
+};
+
 // key -> value of a deck (later lines win, as in the reference parsers)
-std::map<std::string, std::string> read_deck(const std::string& path) {
+std::map<std::string, DeckValue> read_deck(const std::string& path) {
     std::ifstream in(path);
     if (!in) throw std::runtime_error("could not open input file: " + path);
-    std::map<std::string, std::string> kv;
+    std::map<std::string, DeckValue> kv;
     std::string line;
+    int n = 0;
     while (std::getline(in, line)) {
+        ++n;
         const auto sharp = line.find('#');
         if (sharp != std::string::npos) line = line.substr(0, sharp);
         const auto eq = line.find('=');
         if (eq == std::string::npos) continue;
         const std::string key = trim(line.substr(0, eq));
-        if (!key.empty()) kv[key] = trim(line.substr(eq + 1));
+        if (!key.empty()) kv[key] = DeckValue{trim(line.substr(eq + 1)), n};
     }
     return kv;
 }
 
 struct Deck {
-    std::map<std::string, std::string> kv;
+    std::map<std::string, DeckValue> kv;
     std::set<std::string> used;
     bool has(std::initializer_list<const char*> keys) const {
         for (const char* k : keys)
             if (kv.count(k)) return true;
         return false;
     }
+    // The value of the alias set last in the file: the reference parsers
+    // apply the lines in order, so the latest line of any alias wins.
     const std::string* find(std::initializer_list<const char*> keys) {
-        const std::string* r = nullptr;
+        const DeckValue* r = nullptr;
         for (const char* k : keys) {
             auto it = kv.find(k);
-            if (it != kv.end()) { used.insert(k); r = &it->second; }
+            if (it == kv.end()) continue;
+            used.insert(k);
+            if (!r || it->second.line > r->line) r = &it->second;
         }
-        return r;
+        return r ? &r->value : nullptr;
     }
     template <class T>
     void get(std::initializer_list<const char*> keys, T& out) {
@@ -450,33 +464,30 @@ int run_p3(const std::string& deck_path) {
     d.kv = read_deck(deck_path);
     upic_params p{};
     p.code = UPIC_CODE_P3;
-    // defaults of P3 Config (837-990) for the keys used here
-    p.nx = 16; p.ny = 16; p.nz = 1; p.Lx = 2.0 * kPi; p.Ly = 2.0 * kPi; p.Lz = 2.0 * kPi;
-    p.dt = 0.05; p.spline_order = 2; p.orbit_quad_order = 16; p.split_orbit_at_knots = 1;
-    p.nyquist_projection = 1; p.c = 1.0; p.eps0 = 1.0; p.mu0 = 1.0; p.sigma1 = 1.0; p.sigma2 = 1.0;
+    // the defaults of the reference's Config (P3:837-990)
+    p.nx = 4; p.ny = 4; p.nz = 4; p.Lx = 2.0 * kPi; p.Ly = 2.0 * kPi; p.Lz = 2.0 * kPi;
+    p.dt = 0.00625; p.spline_order = 2; p.orbit_quad_order = 16; p.split_orbit_at_knots = 1;
+    p.nyquist_projection = 0; p.c = 1.0; p.eps0 = 1.0; p.mu0 = 1.0; p.sigma1 = 1.0; p.sigma2 = 1.0;
     p.nonlinear_solver_picard = 0; p.nonlinear_max_iter = 32; p.nonlinear_rtol = 1e-12; p.nonlinear_atol = 1e-13;
     p.anderson_memory = 4; p.anderson_start = 2; p.anderson_damping = 0.7; p.anderson_regularization = 1e-12;
     p.anderson_growth_limit = 1.2; p.local_rtol = 1e-11; p.local_atol = 1e-13; p.local_max_iter = 24;
     p.local_damping = 1.0; p.selective_particle_fallback = 0; p.selective_particle_max_substeps = 64;
     p.precision = UPIC_PREC_FP64; p.sort_interval = 1; p.learned_guess = 0;
-    int n_steps = 10, ppc = 8, mode = 1;
-    std::string tc = "bimaxwell", prefix = "ec_pic", solver = "anderson_quasi_newton", prec = "fp64";
-    bool split = true, proj = true, sel = false;
-    Real rho0 = 1.0, v0 = 0.2, noise = 0.0, alpha = 0.01;
+    int n_steps = 6, ppc = 4, mode = 1;
+    std::string tc = "cold_relativistic_two_stream", prefix = "orbit_smoke", solver = "anderson_quasi_newton",
+                prec = "fp64";
+    bool split = true, proj = false, sel = false;
+    Real rho0 = 1.0, v0 = 0.3, noise = 0.0, alpha = 0.02;
     Real vth[3] = {0.05, 0.2, 0.2};
     uint64_t seed = 12345;
     d.get({"nx"}, p.nx); d.get({"ny"}, p.ny); d.get({"nz"}, p.nz);
     d.get({"Lx"}, p.Lx); d.get({"Ly"}, p.Ly); d.get({"Lz"}, p.Lz);
     d.get({"n_steps"}, n_steps); d.get({"dt"}, p.dt);
-    std::string tc_raw = tc;
-    if (const std::string* v = d.find({"test_case", "initial_condition", "example"})) {
-        tc_raw = *v;
-        tc = lower(*v);
-    }
+    if (const std::string* v = d.find({"test_case", "initial_condition", "example"})) tc = lower(*v);
     d.get({"particles_per_cell_pair", "particles_per_cell", "velocity_samples_per_cell"}, ppc);
     d.get({"spline_order"}, p.spline_order); d.get({"orbit_quad_order"}, p.orbit_quad_order);
     d.get_bool({"split_orbit_at_knots"}, split);
-    d.get_str({"nonlinear_solver"}, solver);
+    if (const std::string* v = d.find({"nonlinear_solver"})) solver = lower(*v);
     d.get({"nonlinear_rtol", "nonlinear_tol", "picard_tol"}, p.nonlinear_rtol);
     d.get({"nonlinear_atol"}, p.nonlinear_atol);
     d.get_bool({"nyquist_projection"}, proj);
@@ -485,13 +496,116 @@ int run_p3(const std::string& deck_path) {
     d.get({"anderson_damping"}, p.anderson_damping);
     d.get({"anderson_regularization"}, p.anderson_regularization);
     d.get({"anderson_growth_limit"}, p.anderson_growth_limit);
-    d.get_bool({"selective_particle_fallback"}, sel);
-    d.get({"selective_particle_max_substeps"}, p.selective_particle_max_substeps);
+    d.get_bool({"selective_particle_fallback", "selective_subcycle_fallback"}, sel);
+    d.get({"selective_particle_max_substeps", "selective_max_substeps"}, p.selective_particle_max_substeps);
     d.get({"particle_local_rtol", "uniform_local_rtol"}, p.local_rtol);
     d.get({"particle_local_atol", "uniform_local_atol"}, p.local_atol);
     d.get({"particle_local_max_iter", "uniform_local_max_iter"}, p.local_max_iter);
     d.get({"particle_local_damping", "uniform_local_damping"}, p.local_damping);
-    d.get({"kappa"}, p.c); d.get({"sigma1"}, p.sigma1); d.get({"sigma2"}, p.sigma2);
+    // kappa, sigma1, sigma2 and their deprecated names, applied in file
+    // order with the reference's warnings (P3:1308-1315)
+    {
+        struct Coef { int line; std::string key, value; };
+        std::vector<Coef> cs;
+        for (const char* k : {"kappa", "c", "sigma1", "sigma2", "eps0", "mu0"}) {
+            auto it = d.kv.find(k);
+            if (it == d.kv.end()) continue;
+            d.used.insert(k);
+            cs.push_back({it->second.line, k, it->second.value});
+        }
+        std::sort(cs.begin(), cs.end(), [](const Coef& a, const Coef& b) { return a.line < b.line; });
+        for (const Coef& cf : cs) {
+            std::istringstream ss(cf.value);
+            if (cf.key == "kappa") ss >> p.c;
+            else if (cf.key == "c") {
+                ss >> p.c;
+                std::cerr << "warning: c is deprecated; use kappa in the nondimensional paper notation\n";
+            } else if (cf.key == "sigma1") ss >> p.sigma1;
+            else if (cf.key == "sigma2") ss >> p.sigma2;
+            else if (cf.key == "eps0") {
+                Real eps0 = 1.0;
+                ss >> eps0;
+                p.sigma1 = 1.0 / eps0;
+                std::cerr << "warning: eps0 is deprecated; use sigma1=1/eps0 in the nondimensional paper notation\n";
+            } else {
+                ss >> p.sigma2;
+                std::cerr << "warning: mu0 is deprecated; use sigma2 in the nondimensional paper notation\n";
+            }
+        }
+    }
+    // keys the reference accepts and ignores (P3:1295-1306)
+    for (const char* k : {"picard_relaxation", "picard_omega", "relaxed_picard_omega", "picard_relaxation_min",
+                          "picard_omega_min", "picard_relaxation_shrink", "picard_relaxation_growth",
+                          "picard_adaptive_relaxation", "adaptive_picard_relaxation", "picard_current_in_convergence",
+                          "picard_converge_current", "final_consistency_sweep", "picard_require_converged",
+                          "reject_unconverged_picard"})
+        if (d.kv.count(k)) { d.used.insert(k); std::cerr << "warning: obsolete Picard control ignored: " << k << "\n"; }
+    for (const char* k : {"adaptive_dt_max", "dt_max", "adaptive_max_accepted_steps", "max_accepted_steps",
+                          "adaptive_energy_rtol", "energy_accept_rtol", "adaptive_finish_exactly"})
+        if (d.kv.count(k)) {
+            d.used.insert(k);
+            std::cerr << "warning: unsupported adaptive time-step control ignored: " << k << "\n";
+        }
+    // controls of the reference's features outside this driver: read (and
+    // echoed) as the reference reads them; the manuscript runs need the
+    // adaptive dt and uniform subcycling off (P3:8344-8345), and outputs
+    // this driver does not write are named in a warning
+    bool adaptive = false, uniform_sub = false, uniform_refine = false, transverse = false, field_diag = true,
+         field_grid = false, fail_diag = true;
+    Real transverse_interval = 1.0, field_interval = 10.0, adaptive_dt_min = 0.0, adaptive_shrink = 0.5,
+         adaptive_growth = 2.0, uniform_start = 0.0;
+    int adaptive_growth_interval = 4, adaptive_attempts = 4, uniform_initial = 1, uniform_substeps = 10,
+        uniform_max = 20, dynamic_chunk = 4, fail_limit = 16, probe_points = 32;
+    std::string probe_axis = "auto", snapshot_times_raw;
+    std::vector<Real> snapshot_times;
+    d.get_bool({"adaptive_dt", "adaptive_time_step", "adaptive_timestep", "adaptive_reject_on_unconverged"}, adaptive);
+    d.get({"adaptive_dt_min", "dt_min"}, adaptive_dt_min);
+    d.get({"adaptive_dt_shrink", "dt_shrink"}, adaptive_shrink);
+    d.get({"adaptive_dt_growth", "dt_growth"}, adaptive_growth);
+    d.get({"adaptive_dt_growth_interval", "dt_growth_interval"}, adaptive_growth_interval);
+    d.get({"adaptive_max_attempts", "max_step_attempts"}, adaptive_attempts);
+    d.get_bool({"uniform_particle_subcycling", "uniform_subcycling"}, uniform_sub);
+    d.get_bool({"uniform_refine_on_failure", "particle_refine_on_failure"}, uniform_refine);
+    d.get({"uniform_initial_substeps", "particle_initial_substeps"}, uniform_initial);
+    d.get({"uniform_substeps", "particle_substeps"}, uniform_substeps);
+    d.get({"uniform_max_substeps", "particle_max_substeps"}, uniform_max);
+    d.get({"uniform_subcycling_start_time", "particle_subcycling_start_time"}, uniform_start);
+    d.get({"selective_particle_dynamic_chunk", "selective_dynamic_chunk"}, dynamic_chunk);
+    d.get_bool({"selective_failure_diagnostics", "failed_particle_diagnostics"}, fail_diag);
+    d.get({"selective_failure_diagnostic_limit", "failed_particle_diagnostic_limit"}, fail_limit);
+    d.get_bool({"field_diagnostics", "write_field_diagnostics"}, field_diag);
+    d.get({"field_diagnostics_interval", "field_output_interval"}, field_interval);
+    d.get({"field_probe_points_per_cell", "field_samples_per_cell"}, probe_points);
+    if (const std::string* v = d.find({"field_probe_axis"})) probe_axis = lower(*v);
+    d.get_bool({"field_grid_snapshots", "write_full_field_snapshots"}, field_grid);
+    d.get_bool({"transverse_diagnostics"}, transverse);
+    d.get({"transverse_spectrum_interval"}, transverse_interval);
+    if (const std::string* v = d.find({"particle_snapshot_times"})) {
+        std::string times = *v;
+        std::replace(times.begin(), times.end(), ',', ' ');
+        std::istringstream values(times);
+        Real t;
+        while (values >> t) {
+            if (!std::isfinite(t) || t < 0.0) throw std::runtime_error("particle_snapshot_times must be finite and nonnegative");
+            snapshot_times.push_back(t);
+        }
+        if (!values.eof()) throw std::runtime_error("invalid particle_snapshot_times list");
+        std::sort(snapshot_times.begin(), snapshot_times.end());
+        snapshot_times.erase(std::unique(snapshot_times.begin(), snapshot_times.end()), snapshot_times.end());
+    }
+    if (adaptive || uniform_sub)
+        throw std::runtime_error("manuscript runs require adaptive_dt=false, uniform_particle_subcycling=false");
+    {
+        std::vector<std::string> skipped;
+        if (transverse) skipped.push_back("transverse_diagnostics (<prefix>_transverse_spectrum.csv, _velocity_moments.csv)");
+        if (field_diag) skipped.push_back("field_diagnostics (<prefix>_field_probe*.csv, _knot_jumps.csv)");
+        if (field_grid) skipped.push_back("field_grid_snapshots (<prefix>_field_grid_t*.csv)");
+        if (fail_diag) skipped.push_back("selective_failure_diagnostics (<prefix>_selective_failures.csv)");
+        if (!snapshot_times.empty()) skipped.push_back("particle_snapshot_times (<prefix>_particles_t*.csv)");
+        for (const std::string& k : skipped)
+            std::cerr << "warning: " << k << " is requested; upic_run does not write these files\n";
+        if (d.has({"run_unit_tests"})) d.used.insert("run_unit_tests");
+    }
     // the reference's initial conditions (is_two_stream_case / is_weibel_case
     // / is_landau_case, P3:1042-1105)
     int ref_kind = 0;
@@ -511,8 +625,7 @@ int run_p3(const std::string& deck_path) {
     d.get({"theory_omega", "landau_theory_omega"}, theory_omega);
     if (ref_kind) {
         upic_p3_case_defaults(ref_kind, &cs);
-        cs.particles_per_cell_pair = d.has({"particles_per_cell_pair", "particles_per_cell", "velocity_samples_per_cell"})
-                                         ? ppc : cs.particles_per_cell_pair;
+        cs.particles_per_cell_pair = ppc;
         d.get({"n0", "rho0", "background_density"}, cs.n0); d.get({"v0"}, cs.v0);
         d.get({"perturbation"}, cs.perturbation); d.get({"perturbation_mode"}, cs.perturbation_mode);
         d.get({"two_stream_density_alpha", "density_perturbation", "alpha_x"}, cs.two_stream_density_alpha);
@@ -520,23 +633,23 @@ int run_p3(const std::string& deck_path) {
         d.get({"landau_alpha", "landau_density_perturbation"}, cs.landau_alpha);
         d.get({"thermal_velocity", "vth"}, cs.thermal_velocity);
         d.get({"tsi_transverse_seed"}, cs.tsi_transverse_seed); d.get({"tsi_transverse_mode"}, cs.tsi_transverse_mode);
-        bool b = true;
-        if (d.has({"finite_grid_theory"})) {
-            d.get_bool({"finite_grid_theory"}, b);
-            cs.two_stream_finite_grid_theory = cs.weibel_finite_grid_theory = b ? 1 : 0;
+        // finite_grid_theory sets both theory switches; the line set last wins
+        {
+            bool b = cs.two_stream_finite_grid_theory != 0;
+            d.get_bool({"two_stream_finite_grid_theory", "finite_grid_theory"}, b);
+            cs.two_stream_finite_grid_theory = b;
+            b = cs.weibel_finite_grid_theory != 0;
+            d.get_bool({"weibel_finite_grid_theory", "finite_grid_theory"}, b);
+            cs.weibel_finite_grid_theory = b;
         }
-        b = cs.two_stream_eigenmode != 0;
+        bool b = cs.two_stream_eigenmode != 0;
         d.get_bool({"two_stream_eigenmode", "initialize_two_stream_eigenmode"}, b); cs.two_stream_eigenmode = b;
-        b = cs.two_stream_finite_grid_theory != 0;
-        d.get_bool({"two_stream_finite_grid_theory"}, b); cs.two_stream_finite_grid_theory = b;
         b = cs.weibel_eigenmode != 0;
         d.get_bool({"weibel_eigenmode", "initialize_weibel_eigenmode"}, b); cs.weibel_eigenmode = b;
-        b = cs.weibel_finite_grid_theory != 0;
-        d.get_bool({"weibel_finite_grid_theory"}, b); cs.weibel_finite_grid_theory = b;
         rho0 = cs.n0;
         mode = cs.perturbation_mode;
     } else {
-        d.get({"rho0", "n0"}, rho0); d.get({"v0"}, v0); d.get({"perturbation_mode"}, mode);
+        d.get({"rho0", "n0", "background_density"}, rho0); d.get({"v0"}, v0); d.get({"perturbation_mode"}, mode);
         d.get({"perturbation", "landau_alpha"}, alpha);
         d.get({"vth_x"}, vth[0]); d.get({"vth_y"}, vth[1]); d.get({"vth_z"}, vth[2]);
         if (const std::string* v = d.find({"vth", "thermal_velocity"})) vth[0] = vth[1] = vth[2] = std::atof(v->c_str());
@@ -591,63 +704,53 @@ int run_p3(const std::string& deck_path) {
         hs.x = P.x.data(); hs.v = P.v.data(); hs.q = P.q.data(); hs.m = P.m.data();
         check(upic_load_particles(&p, &ls, &hs), nullptr, "particle load");
     }
-    check(upic_ctx_create(&p, &r.ctx), nullptr, "context");
-    check(upic_sim_create(r.ctx, r.np, &r.sim), r.ctx, "simulation");
-    std::vector<Real> zeros(3 * (size_t)r.np, 0.0), fz(3 * (size_t)r.N, 0.0);
-    hs.v_prev = zeros.data(); hs.P = zeros.data(); hs.vbar_last = zeros.data();
-    hs.phi = fz.data(); hs.psi = fz.data(); hs.rho = fz.data(); hs.A = fz.data(); hs.U = fz.data(); hs.J = fz.data();
-    check(upic_sim_upload(r.sim, &hs), r.ctx, "upload");
-    if (ref_kind == UPIC_P3_WEIBEL)
-        check(upic_sim_init_fields_AU(r.sim, rho0, A0.data(), U0.data()), r.ctx, "initial fields");
-    else
-        check(upic_sim_init_fields(r.sim, rho0), r.ctx, "initial fields");
     // linear-theory reference (P3:8427-8433): the selected growth rate times
     // the initial |E_x(k_x)| (|B_z(k_y)| for Weibel)
     const Real theory_growth = std::abs(theory_gamma) > 0.0 ? theory_gamma : growth;
-    upic_diagnostics d0{};
-    check(upic_sim_diagnostics(r.sim, mode, &d0), r.ctx, "diagnostics");
-    const bool use_B = ref_kind == UPIC_P3_WEIBEL;
-    const Real theory_amp0 = use_B ? d0.Bz_y_mode_abs : d0.E_mode_abs;
-    const bool has_theory = (std::abs(theory_growth) > 0.0 || std::abs(theory_omega) > 0.0) && theory_amp0 > 0.0;
-    if (std::abs(theory_growth) > 0.0)
-        std::cout << std::scientific << std::setprecision(6) << "linear theory growth/damping rate = " << theory_growth
-                  << (use_B ? " using |B_z(k_y)|" : " using |E_x(k_x)|") << "\n";
-    {  // write_config_echo (P3:8217-8300); the features this driver does not
-       // run (adaptive dt, uniform subcycling, field and failure files) are
-       // echoed off with the reference's defaults
+    // the echo is written before any device call, as the reference writes it
+    // before the time loop
+    {  // write_config_echo (P3:8217-8300), every key as the deck set it
         if (!ref_kind) upic_p3_case_defaults(0, &cs);
         const Real wB0 = cs.weibel_B0 >= 0.0 ? cs.weibel_B0 : cs.perturbation;
         const Real lal = cs.landau_alpha >= 0.0 ? cs.landau_alpha : cs.perturbation;
         auto tf = [](bool b) { return b ? "true" : "false"; };
         std::ofstream out(prefix + "_config_echo.txt");
         out << std::setprecision(17);
-        out << "test_case=" << tc_raw << "\ntransverse_diagnostics=false\ntransverse_spectrum_interval=1"
+        out << "test_case=" << tc << "\ntransverse_diagnostics=" << tf(transverse)
+            << "\ntransverse_spectrum_interval=" << transverse_interval
             << "\ntsi_transverse_seed=" << cs.tsi_transverse_seed << "\ntsi_transverse_mode=" << cs.tsi_transverse_mode
             << "\nnyquist_projection=" << p.nyquist_projection << "\nnx=" << p.nx << "\nny=" << p.ny << "\nnz=" << p.nz
             << "\nLx=" << p.Lx << "\nLy=" << p.Ly << "\nLz=" << p.Lz << "\nn_steps=" << n_steps << "\ndt=" << p.dt
             << "\ntarget_time=" << (Real)n_steps * p.dt << "\nparticles=" << r.np
-            << "\nparticles_per_cell_pair_or_velocity_samples=" << (ref_kind ? cs.particles_per_cell_pair : ppc)
+            << "\nparticles_per_cell_pair_or_velocity_samples=" << ppc
             << "\nspline_order=" << p.spline_order << "\norbit_quad_order=" << p.orbit_quad_order
             << "\nsplit_orbit_at_knots=" << tf(p.split_orbit_at_knots != 0) << "\nmacro_time_step=fixed"
-            << "\nadaptive_dt=false\nadaptive_dt_min=" << p.dt / 16.0
-            << "\nadaptive_dt_shrink=0.5\nadaptive_dt_growth=2\nadaptive_dt_growth_interval=4\nadaptive_max_attempts=4"
+            << "\nadaptive_dt=false\nadaptive_dt_min=" << (adaptive_dt_min > 0.0 ? adaptive_dt_min : p.dt / 16.0)
+            << "\nadaptive_dt_shrink=" << adaptive_shrink << "\nadaptive_dt_growth=" << adaptive_growth
+            << "\nadaptive_dt_growth_interval=" << adaptive_growth_interval
+            << "\nadaptive_max_attempts=" << adaptive_attempts
             << "\nnonlinear_solver=" << solver << "\nnonlinear_rtol=" << p.nonlinear_rtol
             << "\nnonlinear_atol=" << p.nonlinear_atol << "\nnonlinear_max_iter=" << p.nonlinear_max_iter
             << "\nanderson_memory=" << p.anderson_memory << "\nanderson_start=" << p.anderson_start
             << "\nanderson_damping=" << p.anderson_damping << "\nanderson_regularization=" << p.anderson_regularization
             << "\nanderson_growth_limit=" << p.anderson_growth_limit
-            << "\nuniform_particle_subcycling=false\nuniform_refine_on_failure=false\nuniform_initial_substeps=1"
-            << "\nuniform_substeps=10\nuniform_max_substeps=20\nuniform_subcycling_start_time=0"
+            << "\nuniform_particle_subcycling=false\nuniform_refine_on_failure=" << tf(uniform_refine)
+            << "\nuniform_initial_substeps=" << uniform_initial << "\nuniform_substeps=" << uniform_substeps
+            << "\nuniform_max_substeps=" << uniform_max << "\nuniform_subcycling_start_time=" << uniform_start
             << "\nparticle_local_rtol=" << p.local_rtol << "\nparticle_local_atol=" << p.local_atol
             << "\nparticle_local_max_iter=" << p.local_max_iter << "\nuniform_local_rtol=" << p.local_rtol
             << "\nuniform_local_atol=" << p.local_atol << "\nuniform_local_max_iter=" << p.local_max_iter
             << "\nuniform_local_damping=" << p.local_damping
             << "\nselective_particle_fallback=" << tf(p.selective_particle_fallback != 0)
             << "\nselective_particle_max_substeps=" << p.selective_particle_max_substeps
-            << "\nselective_particle_dynamic_chunk=4\nselective_failure_diagnostics=false"
-            << "\nselective_failure_diagnostic_limit=16\nparticle_snapshot_times=\nfield_diagnostics=false"
-            << "\nfield_diagnostics_interval=10\nfield_probe_points_per_cell=32\nfield_probe_axis=auto"
-            << "\nfield_grid_snapshots=false\nkappa=" << p.c << "\nsigma1=" << p.sigma1 << "\nsigma2=" << p.sigma2
+            << "\nselective_particle_dynamic_chunk=" << dynamic_chunk
+            << "\nselective_failure_diagnostics=" << tf(fail_diag)
+            << "\nselective_failure_diagnostic_limit=" << fail_limit << "\nparticle_snapshot_times=";
+        for (Real t : snapshot_times) out << t << ' ';
+        out << "\nfield_diagnostics=" << tf(field_diag) << "\nfield_diagnostics_interval=" << field_interval
+            << "\nfield_probe_points_per_cell=" << probe_points << "\nfield_probe_axis=" << probe_axis
+            << "\nfield_grid_snapshots=" << tf(field_grid) << "\nkappa=" << p.c << "\nsigma1=" << p.sigma1
+            << "\nsigma2=" << p.sigma2
             << "\nsigma1_minus_kappa2_sigma2=" << (p.sigma1 - p.c * p.c * p.sigma2) << "\nrho0=" << rho0
             << "\nv0=" << (ref_kind ? cs.v0 : v0) << "\nperturbation=" << (ref_kind ? cs.perturbation : alpha)
             << "\ntwo_stream_density_alpha=" << cs.two_stream_density_alpha
@@ -660,6 +763,24 @@ int run_p3(const std::string& deck_path) {
             << "\nauto_theory_growth_rate=" << theory_growth
             << "\ndV=" << (p.Lx / p.nx) * (p.Ly / p.ny) * (p.Lz / p.nz) << "\n";
     }
+    check(upic_ctx_create(&p, &r.ctx), nullptr, "context");
+    check(upic_sim_create(r.ctx, r.np, &r.sim), r.ctx, "simulation");
+    std::vector<Real> zeros(3 * (size_t)r.np, 0.0), fz(3 * (size_t)r.N, 0.0);
+    hs.v_prev = zeros.data(); hs.P = zeros.data(); hs.vbar_last = zeros.data();
+    hs.phi = fz.data(); hs.psi = fz.data(); hs.rho = fz.data(); hs.A = fz.data(); hs.U = fz.data(); hs.J = fz.data();
+    check(upic_sim_upload(r.sim, &hs), r.ctx, "upload");
+    if (ref_kind == UPIC_P3_WEIBEL)
+        check(upic_sim_init_fields_AU(r.sim, rho0, A0.data(), U0.data()), r.ctx, "initial fields");
+    else
+        check(upic_sim_init_fields(r.sim, rho0), r.ctx, "initial fields");
+    upic_diagnostics d0{};
+    check(upic_sim_diagnostics(r.sim, mode, &d0), r.ctx, "diagnostics");
+    const bool use_B = ref_kind == UPIC_P3_WEIBEL;
+    const Real theory_amp0 = use_B ? d0.Bz_y_mode_abs : d0.E_mode_abs;
+    const bool has_theory = (std::abs(theory_growth) > 0.0 || std::abs(theory_omega) > 0.0) && theory_amp0 > 0.0;
+    if (std::abs(theory_growth) > 0.0)
+        std::cout << std::scientific << std::setprecision(6) << "linear theory growth/damping rate = " << theory_growth
+                  << (use_B ? " using |B_z(k_y)|" : " using |E_x(k_x)|") << "\n";
     if (learned == UPIC_GUESS_LEARNED) {
         const int64_t stride = std::max<int64_t>(1, r.np / 16384);
         check(upic_sim_collect_guess_data(r.sim, stride, ((r.np + stride - 1) / stride) * 20), r.ctx, "collect");
@@ -683,28 +804,34 @@ int run_p3(const std::string& deck_path) {
                 << "grid_work,deposit_gather_work_residual,particle_energy_residual,field_energy_residual,"
                 << "crossing_particles,total_crossings,avg_path_segments,max_path_segments,dt,elapsed_wall_seconds\n";
     }
-    auto step_row = [&](int step, Real t, const upic_step_info& in) {
-        upic_step_diagnostics sd{};
-        const bool have = upic_sim_step_diagnostics(r.sim, &sd) == UPIC_OK;  // not after a refined step
-        auto v = [&](double x) { return have ? x : kNaN; };
-        const std::string method = in.anderson_steps > 0 ? "anderson" : (in.picard_fallback_steps > 0 ? "picard" : "direct");
+    // the solve's convergence_method string (P3:6501-6502, 6185)
+    auto method_name = [](const upic_step_info& in) {
+        static const char* names[] = {"picard_fallback", "picard", "current_anderson", "current_anderson_backtracked",
+                                      "current_picard", "current_picard_backtracked"};
+        const int k = (in.update_method >= 0 && in.update_method < 6) ? in.update_method : 0;
+        return std::string(in.selective_fallback_used ? "selective_" : "nested_full_dt_") + names[k];
+    };
+    // the NonlinearSolveInfo columns shared by both CSVs, from
+    // local_particle_iterations_mean to selective_n64 (P3:8604-8620)
+    auto solve_cols = [](std::ostream& o, const upic_step_info& in) {
+        o << "," << in.local_iterations_mean << "," << in.local_iterations_max << "," << in.uniform_substeps_used << ","
+          << in.selective_fallback_used << "," << in.direct_attempt_iterations << ","
+          << in.selective_refinement_restarts << "," << in.selective_refined_particles << ","
+          << in.selective_total_particle_substeps << "," << in.selective_work_ratio;
+        for (int k = 0; k < 7; ++k) o << "," << in.selective_level_counts[k];
+    };
+    auto step_row = [&](int step, Real t, const upic_step_info& in, const upic_step_diagnostics& sd) {
         stepcsv << step << "," << std::setprecision(17) << t << "," << in.iterations << "," << in.final_rms_residual
-                << "," << in.final_max_particle_residual << "," << in.anderson_restarts << "," << method << ","
-                << kNaN << "," << in.estimated_map_contraction << "," << in.backtracked_steps << ","
-                << in.minimum_update_alpha << "," << in.local_iterations_mean << "," << in.local_iterations_max
-                << ",1,0,0,0,0,0,0";
-        for (int k = 0; k < 7; ++k) stepcsv << "," << kNaN;
-        stepcsv << "," << v(sd.A_chain_abs_rms) << "," << v(sd.A_chain_rel_rms) << "," << v(sd.A_chain_max_norm)
-                << "," << v(sd.A_chain_mean_norm) << "," << v(sd.A_chain_work_defect) << "," << v(sd.delta_K) << ","
-                << v(sd.delta_W) << "," << v(sd.delta_total_energy) << "," << v(sd.particle_work) << ","
-                << v(sd.grid_work) << "," << v(sd.deposit_gather_work_residual) << ","
-                << v(sd.particle_energy_residual) << "," << v(sd.field_energy_residual) << ","
-                << (have ? sd.crossing_particles : in.path.crossing_particles) << ","
-                << (have ? sd.total_crossings : in.path.total_segments - in.path.n_particles) << ","
-                << (have ? sd.avg_path_segments
-                         : (in.path.n_particles ? (Real)in.path.total_segments / (Real)in.path.n_particles : 0.0))
-                << "," << (have ? sd.max_path_segments : in.path.max_segments) << "," << p.dt << ","
-                << wall_since(t0) << "\n";
+                << "," << in.final_max_particle_residual << "," << in.anderson_restarts << "," << method_name(in)
+                << "," << in.observed_convergence_rate << "," << in.estimated_map_contraction << ","
+                << in.backtracked_steps << "," << in.minimum_update_alpha;
+        solve_cols(stepcsv, in);
+        stepcsv << "," << sd.A_chain_abs_rms << "," << sd.A_chain_rel_rms << "," << sd.A_chain_max_norm << ","
+                << sd.A_chain_mean_norm << "," << sd.A_chain_work_defect << "," << sd.delta_K << "," << sd.delta_W
+                << "," << sd.delta_total_energy << "," << sd.particle_work << "," << sd.grid_work << ","
+                << sd.deposit_gather_work_residual << "," << sd.particle_energy_residual << ","
+                << sd.field_energy_residual << "," << sd.crossing_particles << "," << sd.total_crossings << ","
+                << sd.avg_path_segments << "," << sd.max_path_segments << "," << p.dt << "," << wall_since(t0) << "\n";
         stepcsv.flush();
     };
     std::ofstream diag(prefix + "_diagnostics.csv");
@@ -724,12 +851,14 @@ int run_p3(const std::string& deck_path) {
          << "selective_refined_particles,selective_total_particle_substeps,selective_work_ratio,"
          << "selective_n1,selective_n2,selective_n4,selective_n8,selective_n16,selective_n32,selective_n64,"
          << "crossing_particles,avg_path_segments,max_path_segments,dt,elapsed_wall_seconds\n";
-    auto row = [&](int step, Real t, const upic_step_info& in, Real step_dt) {
+    // ps: the step's path statistics (crossing particles, average and
+    // maximum segments; the step diagnostics' when they are computed,
+    // P3:8839-8849)
+    auto row = [&](int step, Real t, const upic_step_info& in, const std::string& method, Real crossing, Real avg_seg,
+                   Real max_seg, Real step_dt) {
         upic_diagnostics g{};
         check(upic_sim_diagnostics(r.sim, mode, &g), r.ctx, "diagnostics");
         const Real gn = g.divE_rms + g.rho_rms, an = g.psi_over_c2_rms + g.divA_rms;
-        const std::string method = in.iterations == 0 ? "none"
-                                   : (in.anderson_steps > 0 ? "anderson" : (in.picard_fallback_steps > 0 ? "picard" : "direct"));
         Real th_abs = kNaN, th_ratio = kNaN;
         if (has_theory) {
             th_abs = theory_amp0 * std::exp(theory_growth * t);
@@ -744,24 +873,37 @@ int run_p3(const std::string& deck_path) {
              << theory_growth << "," << th_abs << "," << th_ratio << "," << g.mean_rho << "," << in.iterations << ","
              << in.converged << "," << in.final_rms_residual << "," << in.final_max_particle_residual << ","
              << in.final_absolute_rms_residual << "," << in.anderson_restarts << "," << in.anderson_steps << ","
-             << in.picard_fallback_steps << "," << method << "," << kNaN << "," << in.estimated_map_contraction
-             << "," << in.backtracked_steps << "," << in.minimum_update_alpha << "," << in.local_iterations_mean
-             << "," << in.local_iterations_max << ",1,0,0,0,0,0,0";
-        for (int k = 0; k < 7; ++k) diag << "," << kNaN;
-        diag << "," << in.path.crossing_particles << ","
-             << (in.path.n_particles ? (Real)in.path.total_segments / (Real)in.path.n_particles : 0.0) << ","
-             << in.path.max_segments << "," << step_dt << "," << wall_since(t0) << "\n";
+             << in.picard_fallback_steps << "," << method << "," << in.observed_convergence_rate << ","
+             << in.estimated_map_contraction << "," << in.backtracked_steps << "," << in.minimum_update_alpha;
+        solve_cols(diag, in);
+        diag << "," << crossing << "," << avg_seg << "," << max_seg << "," << step_dt << "," << wall_since(t0) << "\n";
         diag.flush();
         return g;
     };
-    upic_step_info none{};
-    none.path.max_segments = 1;
-    row(0, 0.0, none, 0.0);
+    // row 0: the reference's initial NonlinearSolveInfo (P3:8626-8633)
+    upic_step_info init{};
+    init.converged = 1;
+    init.final_rms_residual = init.final_max_particle_residual = init.final_absolute_rms_residual = kNaN;
+    init.observed_convergence_rate = init.estimated_map_contraction = std::numeric_limits<Real>::infinity();
+    init.minimum_update_alpha = 1.0;
+    init.selective_work_ratio = 1.0;
+    row(0, 0.0, init, "initial_state", 0, 0.0, 1, 0.0);
     for (int n = 0; n < n_steps; ++n) {
         upic_step_info info{};
         check(upic_step_p3(r.sim, &info), r.ctx, "step");
-        if (step_diag) step_row(n + 1, (n + 1) * p.dt, info);
-        const upic_diagnostics g = row(n + 1, (n + 1) * p.dt, info, p.dt);
+        const Real t = (n + 1) * p.dt;
+        const std::string method = method_name(info);
+        upic_diagnostics g{};
+        if (step_diag) {
+            upic_step_diagnostics sd{};
+            check(upic_sim_step_diagnostics(r.sim, &sd), r.ctx, "step diagnostics");
+            step_row(n + 1, t, info, sd);
+            g = row(n + 1, t, info, method, (Real)sd.crossing_particles, sd.avg_path_segments,
+                    (Real)sd.max_path_segments, p.dt);
+        } else {
+            const Real avg = info.path.n_particles ? (Real)info.path.total_segments / (Real)info.path.n_particles : 0.0;
+            g = row(n + 1, t, info, method, (Real)info.path.crossing_particles, avg, (Real)info.path.max_segments, p.dt);
+        }
         if (learned == UPIC_GUESS_LEARNED && n + 1 == 20) {
             check(upic_sim_collect_guess_data(r.sim, 0, 0), r.ctx, "collect");
             check(upic_sim_train_guess(r.sim, 15, 4096, 2e-3, 5, nullptr), r.ctx, "training");
@@ -793,7 +935,10 @@ struct P2Config {  // Config (P2:173-231) with its defaults
 // read_config (P2:233-291): keys are case-insensitive
 P2Config read_p2(const std::string& path) {
     Deck d;
-    for (const auto& kv : read_deck(path)) d.kv[lower(kv.first)] = kv.second;
+    for (const auto& kv : read_deck(path)) {
+        DeckValue& slot = d.kv[lower(kv.first)];
+        if (kv.second.line >= slot.line) slot = kv.second;
+    }
     P2Config c;
     if (const std::string* v = d.find({"example", "test_case"})) c.example = lower(*v);
     if (const std::string* v = d.find({"method"})) c.method = lower(*v);
