@@ -210,7 +210,11 @@ __device__ __forceinline__ double breakgen_next(BreakGen& bg) {
         if (s1 < s) { s = s1; d = 1; }
         if (s2 < s) { s = s2; d = 2; }
         if (s > 1.5) { bg.done = true; bg.last = 1.0; return 1.0; }
-        bg.ax[d].m += bg.ax[d].step;
+        // advance the chosen axis with constant indices only: an index known
+        // only at run time would keep the whole generator in scratch memory
+        if (d == 0) bg.ax[0].m += bg.ax[0].step;
+        else if (d == 1) bg.ax[1].m += bg.ax[1].step;
+        else bg.ax[2].m += bg.ax[2].step;
         if (!(s > tol && s < 1.0 - tol)) continue;
         if (fabs(s - bg.last) > 1.0e-12) { bg.last = s; return s; }
     }

@@ -2,10 +2,10 @@
 //
 // Mesh data a particle push reads at a node are packed into one 64-byte
 // record (grad phi_mid x, y; A^n x, y, z; A^{n+1} x, y, z) so a node costs
-// four 16-byte loads.  Gauss points are processed in chunks of four: while the
-// stencil base is the same for the whole chunk (always, between knots), the
-// node loop is outermost and each record is loaded once per chunk, with the
-// quadrature folded into five node weights
+// four 16-byte loads.  Gauss points are processed in chunks of kChunk: while
+// the stencil base is the same for the whole chunk (always, between knots),
+// the node loop is outermost and each record is loaded once per chunk, with
+// the quadrature folded into five node weights
 //   W = sum w S, Gx0 = sum w (1-theta) dS/dx, Gx1 = sum w theta dS/dx, (y alike)
 // so that gather += W grad_phi and D[c][.] += A0_c G.0 + A1_c G.1.
 // This is the same sum as the reference's point-by-point loops
@@ -25,6 +25,10 @@
 namespace upic {
 
 constexpr int kTile = 8;
+// Gauss points per chunk of the FP64 2-D orbit pass: their stencil weights
+// are live together (2 points: 32 doubles at cubic order; 4 points measured
+// slower in the two-wavefront sweep)
+constexpr int kChunk = 2;
 
 struct NodeRec {  // 8 doubles
     double gx, gy, a0x, a0y, a0z, a1x, a1y, a1z;
@@ -77,11 +81,12 @@ __device__ __forceinline__ int orbit_gg_2d(const GridDev& g, const QuadDev& q, c
         const double sb = opt.split ? breakgen_next(bg) : 1.0;
         const double len = sb - sa;
         if (len > 0.0) {
-            for (int q0 = 0; q0 < q.n; q0 += 4) {
-                double wx[4][R + 1], dwx[4][R + 1], wy[4][R + 1], dwy[4][R + 1], wq[4], th[4];
-                int bx[4], by[4];
+            for (int q0 = 0; q0 < q.n; q0 += kChunk) {
+                double wx[kChunk][R + 1], dwx[kChunk][R + 1], wy[kChunk][R + 1], dwy[kChunk][R + 1], wq[kChunk],
+                    th[kChunk];
+                int bx[kChunk], by[kChunk];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
+                for (int k = 0; k < kChunk; ++k) {
                     const int iq = q0 + k < q.n ? q0 + k : q0;
                     const double s = sa + len * q.s[iq];
                     wq[k] = (q0 + k < q.n) ? len * q.w[iq] : 0.0;
@@ -102,27 +107,33 @@ __device__ __forceinline__ int orbit_gg_2d(const GridDev& g, const QuadDev& q, c
                     }
                     th[k] = p1_theta ? s : th0 + (th1 - th0) * s;
                 }
-                const bool same = bx[1] == bx[0] && bx[2] == bx[0] && bx[3] == bx[0] && by[1] == by[0] &&
-                                  by[2] == by[0] && by[3] == by[0];
+                bool same = true;
+#pragma unroll
+                for (int k = 1; k < kChunk; ++k) same = same && bx[k] == bx[0] && by[k] == by[0];
                 if (same) {
 #pragma unroll
                     for (int a = 0; a <= R; ++a) {
                         const int ia = wrap_idx(bx[0] + a, g.nx);
+                        // x factors of this stencil row's node weights
+                        double u[kChunk], du[kChunk], tdu[kChunk], tu[kChunk];
+#pragma unroll
+                        for (int k = 0; k < kChunk; ++k) {
+                            u[k] = wq[k] * wx[k][a];
+                            du[k] = wq[k] * dwx[k][a];
+                            tu[k] = th[k] * u[k];
+                            tdu[k] = th[k] * du[k];
+                        }
 #pragma unroll
                         for (int b = 0; b <= R; ++b) {
                             const int ib = wrap_idx(by[0] + b, g.ny);
                             double W = 0.0, Gx0 = 0.0, Gx1 = 0.0, Gy0 = 0.0, Gy1 = 0.0;
 #pragma unroll
-                            for (int k = 0; k < 4; ++k) {
-                                const double ws = wq[k] * wy[k][b];
-                                const double wd = wq[k] * dwy[k][b];
-                                const double sx_ = dwx[k][a] * ws;  // w dS/dx
-                                const double sy_ = wx[k][a] * wd;   // w dS/dy
-                                W += wx[k][a] * ws;
-                                Gx1 += th[k] * sx_;
-                                Gx0 += sx_;
-                                Gy1 += th[k] * sy_;
-                                Gy0 += sy_;
+                            for (int k = 0; k < kChunk; ++k) {
+                                W += u[k] * wy[k][b];
+                                Gx0 += du[k] * wy[k][b];   // w dS/dx
+                                Gx1 += tdu[k] * wy[k][b];
+                                Gy0 += u[k] * dwy[k][b];   // w dS/dy
+                                Gy1 += tu[k] * dwy[k][b];
                             }
                             Gx0 -= Gx1;  // sum w (1 - theta) dS/dx
                             Gy0 -= Gy1;
@@ -142,7 +153,7 @@ __device__ __forceinline__ int orbit_gg_2d(const GridDev& g, const QuadDev& q, c
                     // the stencil changes inside the chunk (rare): point by
                     // point, recomputing each point's stencil in a rolled loop
 #pragma unroll 1
-                    for (int k = 0; k < 4 && q0 + k < q.n; ++k) {
+                    for (int k = 0; k < kChunk && q0 + k < q.n; ++k) {
                         const double s = sa + len * q.s[q0 + k];
                         const double wk = len * q.w[q0 + k];
                         const double thk = p1_theta ? s : th0 + (th1 - th0) * s;
