@@ -911,14 +911,18 @@ __device__ __forceinline__ void p3_map_mixed_body(const MGrid& g, const QuadF& q
     block_partials<2, 2>(rsum, rmax, nseg_total, a.part);
 }
 
+// The sweeps are capped at 256 registers (VGPRs + AGPRs, two wavefronts
+// per SIMD): left to itself the compiler can settle one register past it and
+// halve their occupancy (1.39 against 2.50 ms per 16.8M-particle cubic
+// sweep).
 template <int R, int GM, bool COLLECT, bool FUSED>
-__global__ __launch_bounds__(256) void k_p3_map_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_p3_map_mixed(MGrid g, QuadF qd,
+                                                                                            int split, P3MArgs a) {
     p3_map_mixed_body<R, GM, COLLECT, FUSED>(g, qd, split, a);
 }
 
-// The collection sweeps (training rows) hold the features as well: capped
-// at 256 VGPRs (two wavefronts per SIMD) by the compiler instead of one
-// register past it halving their occupancy.
+// The collection sweeps (training rows) hold the features as well; the same
+// cap.
 template <int R, int GM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_p3_map_mixed_collect(
     MGrid g, QuadF qd, int split, P3MArgs a) {
@@ -927,9 +931,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
 // The deferred particles of a sweep (uniform layout), packed into full
 // wavefronts: queue entries [0, *retry_n), grid-stride.  Picks the Picard
-// iteration up at its second step from the queued guess.
+// iteration up at its second step from the queued guess.  Capped at two
+// wavefronts per SIMD like the sweeps.
 template <int R, bool COLLECT>
-__global__ __launch_bounds__(256) void k_p3_map_retry(MGrid g, QuadF qd, int split, P3MArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_p3_map_retry(MGrid g, QuadF qd,
+                                                                                            int split, P3MArgs a) {
     const int n = *a.retry_n;
     double rsum[2] = {0.0, 0.0}, rmax[2] = {0.0, 0.0};
     int nseg_total = 0;

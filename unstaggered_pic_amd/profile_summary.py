@@ -3,14 +3,20 @@ FETCH_SIZE / WRITE_SIZE, --output-format csv) into the markdown tables kept
 with the sources.
 
   python -m unstaggered_pic_amd.profile_summary --trace DIR [--after NAME]
-         [--steps K] [--fetch DIR] [--write DIR] [--copy-mib 1024]
+         [--nth-last N] [--steps K] [--fetch DIR] [--write DIR] [--copy-mib 1024]
          [--kernel SUBSTRING[,SUBSTRING]] [--json-out FILE] [--counters DIR,DIR]
          --command "..."
 
 HBM traffic per dispatch = FETCH_SIZE x (copy bytes / FETCH_SIZE of the
 calibration copy) + WRITE_SIZE x (copy bytes / WRITE_SIZE of the copy): the
 bench's --calibrate-copy-mib copy is a kernel that reads and writes a known
-number of bytes, so the two factors convert the counters to bytes.
+number of bytes, so the two factors convert the counters to bytes.  With
+a trace, each kernel's HBM rate is that traffic over its mean duration in
+the summarised region.
+
+FLOP rates come from the SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F{64,32} counters
+(wavefront instructions: 64 lanes, 2 flops per FMA) over the GPU-active time
+of the dispatch.
 """
 import argparse
 import collections
@@ -49,13 +55,13 @@ def counters_by_name(dirname):
     return {k: {c: (sum(v[-4:]) / len(v[-4:]), len(v)) for c, v in d.items()} for k, d in agg.items()}
 
 
-def trace_stats(dirname, after):
+def trace_stats(dirname, after, nth_last=1):
     """Per-kernel (calls, total ns) over the dispatches that start after the
-    last dispatch whose name contains `after`, from the kernel trace."""
+    nth-last dispatch whose name contains `after`, from the kernel trace."""
     rows = list(csv.DictReader(open(_one(dirname, "*kernel_trace.csv"))))
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
     marks = [i for i, r in enumerate(rows) if after in r["Kernel_Name"]]
-    tail = rows[marks[-1] + 1:] if marks else rows
+    tail = rows[marks[-min(nth_last, len(marks))] + 1:] if marks else rows
     agg = collections.defaultdict(lambda: [0, 0])
     for r in tail:
         agg[r["Kernel_Name"]][0] += 1
@@ -69,6 +75,8 @@ def main():
     ap.add_argument("--trace", required=True)
     ap.add_argument("--after", default=None,
                     help="summarise only the dispatches after the last one whose name contains this")
+    ap.add_argument("--nth-last", type=int, default=1,
+                    help="start after the nth-last dispatch matching --after instead of the last")
     ap.add_argument("--steps", type=int, default=0, help="time steps in the summarised region (per-step totals)")
     ap.add_argument("--json-out", default=None, help="write the per-dispatch HBM traffic of the kernels as JSON")
     ap.add_argument("--counters", default=None,
@@ -80,10 +88,13 @@ def main():
     ap.add_argument("--command", default="")
     ap.add_argument("--top", type=int, default=12)
     a = ap.parse_args()
+    mean_ns = {}
     if a.after:
-        agg, span = trace_stats(a.trace, a.after)
+        agg, span = trace_stats(a.trace, a.after, a.nth_last)
+        mean_ns = {short(k): t / n for k, (n, t) in agg.items()}
         busy = sum(v[1] for v in agg.values())
-        print(f"Dispatches after the last `{a.after}`: GPU busy {busy / 1e6:.2f} ms over a span of {span / 1e6:.2f} ms"
+        where = "the last" if a.nth_last == 1 else f"the {a.nth_last}th-last"
+        print(f"Dispatches after {where} `{a.after}`: GPU busy {busy / 1e6:.2f} ms over a span of {span / 1e6:.2f} ms"
               + (f" ({a.steps} steps: {busy / 1e6 / a.steps:.2f} ms busy per step)" if a.steps else "") + ".")
         print()
         print("| kernel | calls | total ms | average us | % of busy |" + (" ms per step |" if a.steps else ""))
@@ -111,16 +122,20 @@ def main():
         print(f"Calibration copy of {a.copy_mib} MiB: FETCH_SIZE {fc:.0f} KiB (factor {sf:.4f}), "
               f"WRITE_SIZE {wc:.0f} KiB (factor {sw:.4f}).")
         print()
-        print("| kernel | dispatches | fetch MB/dispatch | write MB/dispatch | HBM MB/dispatch |")
-        print("|---|---|---|---|---|")
+        print("| kernel | dispatches | fetch MB/dispatch | write MB/dispatch | HBM MB/dispatch |"
+              + (" mean us (trace) | HBM GB/s |" if mean_ns else ""))
+        print("|---|---|---|---|---|" + ("---|---|" if mean_ns else ""))
         out = {"copy_factors": {"fetch": sf, "write": sw}, "kernels": {}}
         for k in sorted(f):
             if not any(x in k for x in a.kernel.split(",")) or k not in w:
                 continue
             fm = sum(f[k]) / len(f[k]) * sf * 1024 / 1e6
             wm = sum(w[k]) / len(w[k]) * sw * 1024 / 1e6
-            print(f"| `{short(k)}` | {len(f[k])} | "
-                  f"{fm:.1f} | {wm:.1f} | {fm + wm:.1f} |")
+            line = f"| `{short(k)}` | {len(f[k])} | {fm:.1f} | {wm:.1f} | {fm + wm:.1f} |"
+            if mean_ns:
+                t = mean_ns.get(short(k))
+                line += f" {t / 1e3:.1f} | {(fm + wm) * 1e6 / t:.0f} |" if t else " - | - |"
+            print(line)
             out["kernels"][short(k)] = {"dispatches": len(f[k]), "fetch_bytes": fm * 1e6, "write_bytes": wm * 1e6}
         if a.json_out:
             import json
@@ -133,9 +148,16 @@ def main():
                 if any(x in k for x in a.kernel.split(",")):
                     res[k].update(vals)
         print()
-        print("| kernel | dispatches | VALU / wavefront | VMEM loads / wavefront | LDS / wavefront | "
-              "VALU busy | GPU-active ms |")
-        print("|---|---|---|---|---|---|---|")
+        print("| kernel | dispatches | VALU / wavefront | VMEM loads / wavefront | VMEM stores / wavefront | "
+              "LDS / wavefront | VALU busy | GPU-active ms | FP64 TFLOP/s | FP32 TFLOP/s |")
+        print("|---|---|---|---|---|---|---|---|---|---|")
+
+        def flops(d, t):
+            names = [f"SQ_INSTS_VALU_{op}_F{t}" for op in ("FMA", "MUL", "ADD", "TRANS")]
+            if not all(n in d for n in names):
+                return None
+            return 64.0 * (2.0 * d[names[0]][0] + d[names[1]][0] + d[names[2]][0] + d[names[3]][0])
+
         for k in sorted(res):
             d = res[k]
             if "SQ_WAVES" not in d or "GRBM_GUI_ACTIVE" not in d:
@@ -144,10 +166,18 @@ def main():
             # GRBM_GUI_ACTIVE sums the 8 XCDs; a wave64 VALU op occupies a SIMD
             # (1024 per GPU) for 4 cycles; 2.4 GHz
             cyc = d["GRBM_GUI_ACTIVE"][0] / 8.0
-            busy = d["SQ_INSTS_VALU"][0] * 4.0 / 1024.0 / cyc
-            print(f"| `{short(k)}` | {d['SQ_WAVES'][1]} | {d['SQ_INSTS_VALU'][0] / w:.0f} | "
-                  f"{d['SQ_INSTS_VMEM_RD'][0] / w:.0f} | {d['SQ_INSTS_LDS'][0] / w:.0f} | {busy:.2f} | "
-                  f"{cyc / 2.4e6:.3f} |")
+            sec = cyc / 2.4e9
+
+            def per_wave(n):
+                return f"{d[n][0] / w:.0f}" if n in d else "-"
+
+            busy = f"{d['SQ_INSTS_VALU'][0] * 4.0 / 1024.0 / cyc:.2f}" if "SQ_INSTS_VALU" in d else "-"
+            f64, f32 = flops(d, 64), flops(d, 32)
+            print(f"| `{short(k)}` | {d['SQ_WAVES'][1]} | {per_wave('SQ_INSTS_VALU')} | "
+                  f"{per_wave('SQ_INSTS_VMEM_RD')} | {per_wave('SQ_INSTS_VMEM_WR')} | {per_wave('SQ_INSTS_LDS')} | "
+                  f"{busy} | {sec * 1e3:.3f} | "
+                  + (f"{f64 / sec / 1e12:.2f}" if f64 is not None else "-") + " | "
+                  + (f"{f32 / sec / 1e12:.2f}" if f32 is not None else "-") + " |")
     if a.command:
         print()
         print("Command: `" + a.command + "`")
