@@ -31,7 +31,7 @@ def _compile(src, defines=(), tag=""):
 
 
 def build(verbose=False, defines=(), out=None):
-    """Compile and link the library.  `defines` (e.g. ["UPIC_CHUNK=2"]) and
+    """Compile and link the library.  `defines` (e.g. ["UPIC_MIXED_SCHED=0"]) and
     `out` build a tuning variant next to the default library; the loader
     picks a variant up through the UPIC_LIBRARY environment variable."""
     out = out or OUT

@@ -85,9 +85,9 @@ upic_status upic_ctx_create(const upic_params* params, upic_ctx** out) {
             c->plan.tw[d] = dalloc<double>(n);
             HIPCHK(hipMemcpy(c->plan.tw[d], tw.data(), n * sizeof(double), hipMemcpyHostToDevice));
         }
-        c->dred = dalloc<double>(64);
+        c->dred = dalloc<double>(kRedSlots);
         c->dstats = dalloc<unsigned long long>(8);
-        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hred), 64 * sizeof(double), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hred), kRedSlots * sizeof(double), hipHostMallocDefault));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hstats), 8 * sizeof(unsigned long long), hipHostMallocDefault));
     });
     if (rc != UPIC_OK) {

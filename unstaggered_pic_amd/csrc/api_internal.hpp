@@ -131,6 +131,17 @@ inline QuadF fp32_rule(const QuadDev& q) {
     return r;
 }
 
+// Slots of a context's reduction area (dred, mirrored in hred): [0, 8) the
+// particle sweeps' sums and maxima, [8, 14) residual norms, 14 the absolute
+// outer tolerance of the step, [16, 64) Gram entries and other sums,
+// [64, 74) the device outer-solve control (Anderson status, nonfinite flag,
+// coefficients), [80, 86) the norms of the step's initial predictor.
+constexpr int kRedSlots = 128;
+constexpr int kRedAtol = 14;
+constexpr int kRedGram = 16;
+constexpr int kRedCtl = 64;
+constexpr int kRedPredictor = 80;
+
 struct upic_ctx {
     upic_params p{};
     GridDev g{};
@@ -142,7 +153,7 @@ struct upic_ctx {
     hipStream_t stream = nullptr;
     int device = 0;
     std::string err;
-    double* dred = nullptr;               // small device reduction area (64 doubles)
+    double* dred = nullptr;               // small device reduction area (kRedSlots doubles)
     unsigned long long* dstats = nullptr; // 8 counters
     double* hred = nullptr;               // pinned mirror
     unsigned long long* hstats = nullptr;

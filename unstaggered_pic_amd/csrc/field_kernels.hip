@@ -417,7 +417,8 @@ void launch_dot_many(hipStream_t s, int64_t n, int npairs, const double* const* 
 }
 
 __global__ void k_residual_norms(int N, const double* __restrict__ map, const double* __restrict__ guess, double floor_,
-                                 double rtol, double atol, double* __restrict__ out) {
+                                 double rtol, double atol_v, const double* __restrict__ atol_dev, double* __restrict__ out) {
+    const double atol = atol_dev ? *atol_dev : atol_v;
     double sv[3] = {0.0, 0.0, 0.0}, mv[3] = {0.0, 0.0, 0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         const double m0 = map[i], m1 = map[N + i], m2 = map[2 * N + i];
@@ -436,9 +437,155 @@ __global__ void k_residual_norms(int N, const double* __restrict__ map, const do
     block_reduce_max<3>(mv, out + 3);
 }
 void launch_residual_norms(hipStream_t s, int N, const double* map, const double* guess, double floor_, double rtol,
-                           double atol, double* out) {
+                           double atol, double* out, const double* atol_dev) {
     const unsigned nb = (unsigned)std::min<int64_t>(gblk(N), 1024);
-    hipLaunchKernelGGL(k_residual_norms, dim3(nb), dim3(256), 0, s, N, map, guess, floor_, rtol, atol, out);
+    hipLaunchKernelGGL(k_residual_norms, dim3(nb), dim3(256), 0, s, N, map, guess, floor_, rtol, atol, atol_dev, out);
+}
+
+__global__ void k_outer_atol(const double* __restrict__ sums, int N, double dV, double atol, double rtol,
+                             double floor_gross, double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    const double gross = sums[1] / ((double)N * dV);
+    out[0] = fmax(fmax(atol, rtol * sqrt(sums[0] / (double)N)), floor_gross * gross);
+}
+void launch_outer_atol(hipStream_t s, const double* sums, int N, double dV, double atol, double rtol,
+                       double floor_gross, double* out) {
+    hipLaunchKernelGGL(k_outer_atol, dim3(1), dim3(64), 0, s, sums, N, dV, atol, rtol, floor_gross, out);
+}
+
+// Gaussian elimination with partial pivoting and the reference's rejection
+// rules (solve_small_dense, P3:5706-5742): a zero, nonfinite or
+// 100-epsilon-relative pivot, a pivot ratio below 1e-12 or a nonfinite
+// solution fails.  Products and sums are rounded separately (no
+// contraction), as in the host restatement of the same routine.
+__device__ bool small_dense_solve(int n, double (&A)[8][8], double (&b)[8], double (&x)[8]) {
+#pragma clang fp contract(off)
+    double scale = 0.0;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) scale = fmax(scale, fabs(A[i][j]));
+    if (!(scale > 0.0) || !isfinite(scale)) return false;
+    double largest = 0.0, smallest = INFINITY;
+    for (int col = 0; col < n; ++col) {
+        int pr = col;
+        for (int row = col + 1; row < n; ++row)
+            if (fabs(A[row][col]) > fabs(A[pr][col])) pr = row;
+        const double piv = fabs(A[pr][col]);
+        if (!isfinite(piv) || piv <= 100.0 * __DBL_EPSILON__ * scale) return false;
+        largest = fmax(largest, piv);
+        smallest = fmin(smallest, piv);
+        if (pr != col) {
+            for (int j = 0; j < n; ++j) {
+                const double t = A[pr][j];
+                A[pr][j] = A[col][j];
+                A[col][j] = t;
+            }
+            const double t = b[pr];
+            b[pr] = b[col];
+            b[col] = t;
+        }
+        for (int row = col + 1; row < n; ++row) {
+            const double f = A[row][col] / A[col][col];
+            for (int j = col; j < n; ++j) A[row][j] -= f * A[col][j];
+            b[row] -= f * b[col];
+        }
+    }
+    if (smallest <= 1.0e-12 * largest) return false;
+    for (int row = n - 1; row >= 0; --row) {
+        double rhs = b[row];
+        for (int j = row + 1; j < n; ++j) rhs -= A[row][j] * x[j];
+        x[row] = rhs / A[row][row];
+        if (!isfinite(x[row])) return false;
+    }
+    return true;
+}
+
+// The Gram solve of anderson_type2_candidate (P3:5763-5823) in one lane:
+// trace check, the unregularised system must be solvable, then the system
+// with reg x trace/m on its diagonal gives gamma.
+__global__ void k_anderson_gamma(int m, const double* __restrict__ gram, double reg, double* __restrict__ ctl) {
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0) return;
+    double G[8][8], rhs[8], x[8];
+    ctl[0] = 0.0;
+    ctl[1] = 0.0;
+    for (int i = 0; i < m; ++i) rhs[i] = gram[i];
+    int k = m;
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j <= i; ++j) {
+            const double v = gram[k++];
+            G[i][j] = v;
+            G[j][i] = v;
+        }
+    double trace = 0.0;
+    for (int i = 0; i < m; ++i) trace += G[i][i];
+    if (!(trace > 0.0) || !isfinite(trace)) return;
+    double A[8][8], b[8];
+    for (int i = 0; i < m; ++i) {
+        b[i] = rhs[i];
+        for (int j = 0; j < m; ++j) A[i][j] = G[i][j];
+    }
+    if (!small_dense_solve(m, A, b, x)) return;
+    const double rs = trace / (double)m;
+    for (int i = 0; i < m; ++i) {
+        b[i] = rhs[i];
+        for (int j = 0; j < m; ++j) A[i][j] = G[i][j];
+        A[i][i] += reg * rs;
+    }
+    if (!small_dense_solve(m, A, b, x)) return;
+    for (int i = 0; i < m; ++i) ctl[2 + i] = x[i];
+    ctl[0] = 1.0;
+}
+void launch_anderson_gamma(hipStream_t s, int m, const double* gram, double reg, double* ctl) {
+    hipLaunchKernelGGL(k_anderson_gamma, dim3(1), dim3(64), 0, s, m, gram, reg, ctl);
+}
+
+// The accumulation is that of k_lincomb (s += coef f, in term order), so a
+// combination with the same coefficients gives the same bits.
+__global__ void k_lincomb_ctl(int64_t n, LinCombCtl lc, double* __restrict__ ctl, int check_finite,
+                              double* __restrict__ out) {
+    if (ctl[0] == 0.0) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < lc.k; ++k) {
+        const double ck = lc.g[k] >= 0 ? (-ctl[lc.g[k]]) * lc.c[k] : lc.c[k];
+        s += ck * lc.f[k][i];
+    }
+    out[i] = s;
+    if (check_finite && !isfinite(s)) ctl[1] = 1.0;
+}
+void launch_lincomb_ctl(hipStream_t s, int64_t n, const LinCombCtl& lc, double* ctl, int check_finite, double* out) {
+    if (n > 0) hipLaunchKernelGGL(k_lincomb_ctl, dim3(gblk(n)), dim3(256), 0, s, n, lc, ctl, check_finite, out);
+}
+
+__global__ void k_trial_select(int64_t n, const double* __restrict__ guess, const double* __restrict__ resid,
+                               double* __restrict__ cand, double one, double damping, double alpha,
+                               int use_anderson, const double* __restrict__ ctl, double* __restrict__ trial) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bool take = use_anderson && ctl[0] != 0.0 && ctl[1] == 0.0;
+    const double g = guess[i];
+    double full;
+    if (take) {
+        full = cand[i];
+    } else {
+        double s = 0.0;
+        s += one * g;
+        s += damping * resid[i];
+        full = s;
+        cand[i] = full;
+    }
+    double t = 0.0;
+    t += one * g;
+    t += alpha * full;
+    t += (-alpha) * g;
+    trial[i] = t;
+}
+void launch_trial_select(hipStream_t s, int64_t n, const double* guess, const double* resid, double* cand,
+                         double damping, double alpha, int use_anderson, const double* ctl, double* trial) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_trial_select, dim3(gblk(n)), dim3(256), 0, s, n, guess, resid, cand, 1.0, damping, alpha,
+                           use_anderson, ctl, trial);
 }
 
 // Sums of |g - xl|^2 and |g - xl + r - rl|^2: the outer iterate's change and

@@ -281,8 +281,29 @@ void launch_diff_norms(hipStream_t s, int64_t n, const double* a, const double* 
 // map, guess are 3*N component-major; out (pre-zeroed):
 //   [0] sum |r|^2, [1] sum |map|^2, [2] sum |guess|^2,
 //   [3] max |r|/max(floor, vs), [4] max |r|/(atol + rtol vs), [5] nonfinite count
+// atol_dev (nullable) holds the absolute tolerance on the device instead.
 void launch_residual_norms(hipStream_t s, int N, const double* map, const double* guess, double floor_, double rtol,
-                           double atol, double* out);
+                           double atol, double* out, const double* atol_dev = nullptr);
+// out[0] = max(atol, rtol sqrt(sums[0] / N), floor_gross sums[1] / (N dV)):
+// the absolute outer tolerance of a mixed-precision step from the squared
+// current and the gross particle current, on the device.
+void launch_outer_atol(hipStream_t s, const double* sums, int N, double dV, double atol, double rtol,
+                       double floor_gross, double* out);
+// Outer-solve control on the device (anderson_type2_candidate,
+// P3:5763-5823).  ctl: [0] 1 when the Anderson coefficients were formed,
+// [1] 1 when the candidate holds a nonfinite entry, [2, 2 + m) gamma.
+// launch_anderson_gamma solves the Gram system (gram: m right-hand sides
+// then the lower triangle row by row) as solve_small_dense does (P3:5706-5742);
+// launch_lincomb_ctl forms out = sum_t coef_t f_t with coef_t = c_t, or
+// -gamma_{g_t} c_t where g_t >= 0, unless ctl[0] is 0, flagging a nonfinite
+// result in ctl[1] when check_finite; launch_trial_select writes
+// full = cand if use_anderson and ctl[0] = 1 and ctl[1] = 0, else guess +
+// damping resid (into cand too), and trial = guess + alpha full - alpha guess.
+struct LinCombCtl { int k; double c[8]; int g[8]; const double* f[8]; };
+void launch_anderson_gamma(hipStream_t s, int m, const double* gram, double reg, double* ctl);
+void launch_lincomb_ctl(hipStream_t s, int64_t n, const LinCombCtl& lc, double* ctl, int check_finite, double* out);
+void launch_trial_select(hipStream_t s, int64_t n, const double* guess, const double* resid, double* cand,
+                         double damping, double alpha, int use_anderson, const double* ctl, double* trial);
 // Energy-type reductions over grid fields: out += {sum f^2 ...}
 // [0] += sum |g - xl|^2, [1] += sum |g - xl + r - rl|^2 over n entries
 void launch_contraction_sums(hipStream_t s, int64_t n, const double* g, const double* xl, const double* r,

@@ -83,7 +83,10 @@ struct upic_sim {
     float* ftmp1 = nullptr;  // np sort scratch
     float* ftmp2 = nullptr;  // np sort scratch
     float* NRf = nullptr;   // FP32 node records (8N)
-    double outer_atol = 0.0;  // absolute outer tolerance in force for this step
+    // the step's absolute outer tolerance is computed on the device
+    // (mixed precision; dred[kRedAtol]) instead of nonlinear_atol
+    bool atol_dev = false;
+    bool predictor_check = false;  // the predictor's finiteness is read with the next evaluation
     double* part = nullptr;   // per-block partials of the particle sweeps
     // learned initial guess (mixed precision): mode, network, training rows
     int guess_mode = 0;
@@ -576,20 +579,21 @@ struct Norms {
 // host are queued by residual_norms_launch; residual_norms_finish reads them
 // after the next synchronisation of the stream.
 double norms_floor(const upic_params& p) { return 64.0 * std::numeric_limits<double>::epsilon() * p.c; }
-double norms_atol(const upic_sim* s) { return s->outer_atol > 0.0 ? s->outer_atol : s->c->p.nonlinear_atol; }
+// The norms and, for a device-side tolerance, the tolerance itself
+// (dred[kRedAtol], next to the norms) reach the host in one copy.
 void residual_norms_launch(upic_sim* s, const double* map, const double* guess) {
     upic_ctx* c = s->c;
-    launch_zero(c->stream, 8, c->dred + 8);
-    launch_residual_norms(c->stream, c->N(), map, guess, norms_floor(c->p), c->p.nonlinear_rtol, norms_atol(s),
-                          c->dred + 8);
-    HIPCHK(hipMemcpyAsync(c->hred + 8, c->dred + 8, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    launch_zero(c->stream, 6, c->dred + 8);
+    launch_residual_norms(c->stream, c->N(), map, guess, norms_floor(c->p), c->p.nonlinear_rtol, c->p.nonlinear_atol,
+                          c->dred + 8, s->atol_dev ? c->dred + kRedAtol : nullptr);
+    HIPCHK(hipMemcpyAsync(c->hred + 8, c->dred + 8, 7 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
 }
 Norms residual_norms_finish(upic_sim* s) {
     upic_ctx* c = s->c;
     const upic_params& p = c->p;
     const double* h = c->hred + 8;
     const double cnt = 3.0 * (double)c->N();
-    const double floor_ = norms_floor(p), atol = norms_atol(s);
+    const double floor_ = norms_floor(p), atol = s->atol_dev ? c->hred[kRedAtol] : p.nonlinear_atol;
     Norms n;
     n.rms_absolute = std::sqrt(h[0] / cnt);
     const double vrms = 0.5 * (std::sqrt(h[1] / cnt) + std::sqrt(h[2] / cnt));
@@ -623,7 +627,7 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     launch_project_nyquist(c->stream, c->sp, s->Jfield, 3);
     field_solve(s, s->Jfield);
     launch_zero(c->stream, 3ll * N, Jmap);
-    zero_red(c);
+    zero_red(c, 8);  // the sweep's sums and maxima (slot kRedAtol keeps the step's tolerance)
     bool incremental = false, a0_written = false;
     if (s->mixed) {
         P3MArgs a{};
@@ -726,6 +730,10 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     // the residual of this map against its guess, read in the same synchronisation
     if (norms) residual_norms_launch(s, Jmap, guess);
     read_red(c, 4);
+    if (s->predictor_check) {  // queued with the step's predictor, read in this synchronisation
+        s->predictor_check = false;
+        if (c->hred[kRedPredictor + 5] != 0.0) throw Error(UPIC_ENONLINEAR, "nonfinite initial macro-current predictor");
+    }
     if (norms) *norms = residual_norms_finish(s);
     if (s->mixed && !use_layout) {
         if (incremental) {  // path statistics of the kept deposit, moved by the changed particles
@@ -818,8 +826,14 @@ struct History {
     void assign1(const double* x, const double* r) { len = 0; push(x, r, 9); }
 };
 
-// anderson_type2_candidate (P3:5763-5823)
-bool anderson_candidate(upic_sim* s, History& H, int memory, double damping, double reg, double* cand) {
+// anderson_type2_candidate (P3:5763-5823), queued without a
+// synchronisation: the differences, their Gram entries, the Gram solve
+// (launch_anderson_gamma) and cand = x_last + damping r_last -
+// sum_j gamma_j (dX_j + damping dR_j) all run on the device, which records
+// in dred[kRedCtl] whether the coefficients were formed and whether cand is
+// finite; the host reads both with the next evaluation.  Returns false
+// when there is no history to mix (nothing queued).
+bool anderson_launch(upic_sim* s, History& H, int memory, double damping, double reg, double* cand) {
     upic_ctx* c = s->c;
     const int64_t n = 3ll * c->N();
     const int available = H.len - 1;
@@ -837,45 +851,32 @@ bool anderson_candidate(upic_sim* s, History& H, int memory, double damping, dou
     for (int i = 0; i < m; ++i)
         for (int j = 0; j <= i; ++j) { xs.push_back(s->dR[i]); ys.push_back(s->dR[j]); }
     const int npairs = (int)xs.size();
-    launch_zero(c->stream, npairs, c->dred + 16);
-    launch_dot_many(c->stream, n, npairs, xs.data(), ys.data(), c->dred + 16);
-    HIPCHK(hipMemcpyAsync(c->hred + 16, c->dred + 16, npairs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    c->sync();
-    std::vector<std::vector<double>> gram(m, std::vector<double>(m, 0.0));
-    std::vector<double> rhs(m, 0.0);
-    double trace = 0.0;
-    int k = m;
-    for (int i = 0; i < m; ++i) rhs[i] = c->hred[16 + i];
-    for (int i = 0; i < m; ++i)
-        for (int j = 0; j <= i; ++j) {
-            const double v = c->hred[16 + k++];
-            gram[i][j] = v;
-            gram[j][i] = v;
-        }
-    for (int i = 0; i < m; ++i) trace += gram[i][i];
-    if (!(trace > 0.0) || !std::isfinite(trace)) return false;
-    std::vector<double> unreg;
-    if (!solve_small_dense(gram, rhs, unreg)) return false;
-    const double rs = trace / (double)m;
-    for (int i = 0; i < m; ++i) gram[i][i] += reg * rs;
-    std::vector<double> gamma;
-    if (!solve_small_dense(gram, rhs, gamma)) return false;
-    // cand = x_last + damping r_last - sum_j gamma_j (dX_j + damping dR_j)
-    LinComb lc{};
+    double* ctl = c->dred + kRedCtl;
+    launch_zero(c->stream, npairs, c->dred + kRedGram);
+    launch_dot_many(c->stream, n, npairs, xs.data(), ys.data(), c->dred + kRedGram);
+    launch_anderson_gamma(c->stream, m, c->dred + kRedGram, reg, ctl);
+    LinCombCtl lc{};
     lc.k = 0;
-    lc.c[lc.k] = 1.0; lc.f[lc.k++] = H.X(H.len - 1);
-    lc.c[lc.k] = damping; lc.f[lc.k++] = H.Rr(H.len - 1);
+    auto term = [&](LinCombCtl& l, double cf, int g, const double* f) {
+        l.c[l.k] = cf;
+        l.g[l.k] = g;
+        l.f[l.k++] = f;
+    };
+    term(lc, 1.0, -1, H.X(H.len - 1));
+    term(lc, damping, -1, H.Rr(H.len - 1));
     for (int j = 0; j < m && lc.k + 2 <= 8; ++j) {
-        lc.c[lc.k] = -gamma[j]; lc.f[lc.k++] = s->dX[j];
-        lc.c[lc.k] = -gamma[j] * damping; lc.f[lc.k++] = s->dR[j];
+        term(lc, 1.0, 2 + j, s->dX[j]);
+        term(lc, damping, 2 + j, s->dR[j]);
     }
-    launch_lincomb(c->stream, n, lc, cand);
-    if (m > 3) {  // memory 4: remaining terms
-        for (int j = 3; j < m; ++j) {
-            LinComb l2{3, {1.0, -gamma[j], -gamma[j] * damping}, {cand, s->dX[j], s->dR[j]}};
-            launch_lincomb(c->stream, n, l2, cand);
-        }
+    launch_lincomb_ctl(c->stream, n, lc, ctl, m <= 3 ? 1 : 0, cand);
+    for (int j = 3; j < m; ++j) {  // memory above 3: remaining terms
+        LinCombCtl l2{};
+        term(l2, 1.0, -1, cand);
+        term(l2, 1.0, 2 + j, s->dX[j]);
+        term(l2, damping, 2 + j, s->dR[j]);
+        launch_lincomb_ctl(c->stream, n, l2, ctl, j == m - 1 ? 1 : 0, cand);
     }
+    HIPCHK(hipMemcpyAsync(c->hred + kRedCtl, ctl, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     return true;
 }
 
@@ -897,7 +898,13 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
     } else {
         launch_copy(c->stream, n3, s->J, s->guess);
     }
-    if (!finite_vec(s, s->guess)) throw Error(UPIC_ENONLINEAR, "nonfinite initial macro-current predictor");
+    // the predictor's finiteness is read with the first evaluation
+    launch_zero(c->stream, 6, c->dred + kRedPredictor);
+    launch_residual_norms(c->stream, N, s->guess, s->guess, norms_floor(p), p.nonlinear_rtol, p.nonlinear_atol,
+                          c->dred + kRedPredictor);
+    HIPCHK(hipMemcpyAsync(c->hred + kRedPredictor, c->dred + kRedPredictor, 6 * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    s->predictor_check = true;
     History H(s);
     double prev_rms = std::numeric_limits<double>::infinity();
     bool converged = false, have_prefetched = false;
@@ -958,47 +965,64 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
             ++restarts;
         }
         H.push(s->guess, s->resid, keep);
-        bool used_anderson = false;
+        const double damping = picard ? 1.0 : p.anderson_damping;
+        bool tried_anderson = false;
         if (!picard && it >= p.anderson_start && H.len >= 2) {
-            used_anderson = anderson_candidate(s, H, p.anderson_memory, p.anderson_damping, p.anderson_regularization, s->cand);
-            if (!used_anderson || !finite_vec(s, s->cand)) {
+            tried_anderson = anderson_launch(s, H, p.anderson_memory, p.anderson_damping, p.anderson_regularization,
+                                             s->cand);
+            if (!tried_anderson) {
                 H.assign1(s->guess, s->resid);
                 ++restarts;
-                used_anderson = false;
             }
         }
-        // backtracking (P3:6044-6084); accepted guess -> trial, its map -> Jtry
-        auto backtrack = [&](const double* full, double& alpha_out) {
-            for (int bt = 0; bt <= 6; ++bt) {
+        // backtracking (P3:6044-6084) over the candidate in cand from step
+        // bt0 on; accepted guess -> trial, its map -> Jtry
+        auto accept = [&](const Norms& tn, double alpha) {
+            const bool dec = tn.rms_scaled <= (1.0 - 1.0e-4 * alpha) * nm.rms_scaled;
+            const bool safe = tn.max_particle_scaled <= p.anderson_growth_limit * nm.max_particle_scaled;
+            return dec && safe;
+        };
+        auto backtrack = [&](int bt0, double& alpha_out) {
+            for (int bt = bt0; bt <= 6; ++bt) {
                 const double alpha = std::ldexp(1.0, -bt);
                 // guess + alpha (full - guess)
-                LinComb lt{3, {1.0, alpha, -alpha}, {s->guess, full, s->guess}};
+                LinComb lt{3, {1.0, alpha, -alpha}, {s->guess, s->cand, s->guess}};
                 launch_lincomb(c->stream, n3, lt, s->trial);
-                // full (a checked Anderson candidate, or guess + damping x
+                // full (a finite Anderson candidate, or guess + damping x
                 // residual of finite vectors) and guess are finite, so the
                 // alpha = 1 trial is too; shorter steps are checked
                 if (bt > 0 && !finite_vec(s, s->trial)) continue;
                 Norms tn{};
                 eval(s->trial, s->Jtry, false, tn);
                 prefetched = tn;
-                const bool dec = tn.rms_scaled <= (1.0 - 1.0e-4 * alpha) * nm.rms_scaled;
-                const bool safe = tn.max_particle_scaled <= p.anderson_growth_limit * nm.max_particle_scaled;
-                if (dec && safe) { alpha_out = alpha; return true; }
+                if (accept(tn, alpha)) { alpha_out = alpha; return true; }
             }
             return false;
         };
-        bool accepted = false, as_anderson = false;
-        double alpha = 0.0;
-        if (used_anderson) {
-            accepted = backtrack(s->cand, alpha);
-            as_anderson = accepted;
-            if (!accepted) { H.assign1(s->guess, s->resid); ++restarts; }
+        // The full step of the first candidate -- the Anderson candidate when
+        // the device formed it and it is finite, else guess + damping x
+        // residual -- is evaluated before the host learns which one it was:
+        // the Anderson status arrives with the evaluation's norms.
+        launch_trial_select(c->stream, n3, s->guess, s->resid, s->cand, damping, 1.0, tried_anderson ? 1 : 0,
+                            c->dred + kRedCtl, s->trial);
+        Norms tn0{};
+        eval(s->trial, s->Jtry, false, tn0);
+        prefetched = tn0;
+        const bool used_anderson = tried_anderson && c->hred[kRedCtl] != 0.0 && c->hred[kRedCtl + 1] == 0.0;
+        if (tried_anderson && !used_anderson) {  // no usable candidate: the history restarts
+            H.assign1(s->guess, s->resid);
+            ++restarts;
         }
-        if (!accepted) {
-            const double damping = picard ? 1.0 : p.anderson_damping;
+        double alpha = 1.0;
+        bool accepted = accept(tn0, 1.0);
+        if (!accepted) accepted = backtrack(1, alpha);
+        bool as_anderson = accepted && used_anderson;
+        if (!accepted && used_anderson) {
+            H.assign1(s->guess, s->resid);
+            ++restarts;
             LinComb lp{2, {1.0, damping}, {s->guess, s->resid}};
             launch_lincomb(c->stream, n3, lp, s->cand);
-            accepted = backtrack(s->cand, alpha);
+            accepted = backtrack(0, alpha);
             as_anderson = false;
         }
         if (!accepted) throw Error(UPIC_ENONLINEAR, "outer current Anderson/Picard backtracking failed at the prescribed macro dt");
@@ -1145,17 +1169,16 @@ void p3_step(upic_sim* s, upic_step_info* info) {
     maybe_sort(s);
     keep_step_start(s);
     state_spectra(s);
-    s->outer_atol = 0.0;
-    if (s->mixed) {
+    s->atol_dev = false;
+    s->predictor_check = false;
+    if (s->mixed) {  // the floored absolute tolerance, on the device (read with the norms)
         const int N = c->N();
         launch_zero(c->stream, 2, c->dred + 20);
         launch_sum_squares(c->stream, 3ll * N, s->J, c->dred + 20);
         launch_mixed_gross(c->stream, s->np, s->ms, s->part, c->dred + 21);
-        HIPCHK(hipMemcpyAsync(c->hred + 20, c->dred + 20, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        c->sync();
-        const double gross = c->hred[21] / ((double)N * c->g.dV);
-        s->outer_atol = std::max({c->p.nonlinear_atol, c->p.nonlinear_rtol * std::sqrt(c->hred[20] / (double)N),
-                                  kMixedGrossFloor * gross});
+        launch_outer_atol(c->stream, c->dred + 20, N, c->g.dV, c->p.nonlinear_atol, c->p.nonlinear_rtol,
+                          kMixedGrossFloor, c->dred + kRedAtol);
+        s->atol_dev = true;
     }
     s->sub_valid = false;
     s->refine = upic_sim::RefineInfo{};
