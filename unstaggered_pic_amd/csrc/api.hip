@@ -85,10 +85,12 @@ upic_status upic_ctx_create(const upic_params* params, upic_ctx** out) {
             c->plan.tw[d] = dalloc<double>(n);
             HIPCHK(hipMemcpy(c->plan.tw[d], tw.data(), n * sizeof(double), hipMemcpyHostToDevice));
         }
-        c->dred = dalloc<double>(kRedSlots);
-        c->dstats = dalloc<unsigned long long>(8);
-        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hred), kRedSlots * sizeof(double), hipHostMallocDefault));
-        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hstats), 8 * sizeof(unsigned long long), hipHostMallocDefault));
+        // the 8 path-statistic counters follow the reduction slots, so that
+        // one copy brings both to the host
+        c->dred = dalloc<double>(kRedSlots + 8);
+        c->dstats = reinterpret_cast<unsigned long long*>(c->dred + kRedSlots);
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hred), (kRedSlots + 8) * sizeof(double), hipHostMallocDefault));
+        c->hstats = reinterpret_cast<unsigned long long*>(c->hred + kRedSlots);
     });
     if (rc != UPIC_OK) {
         g_create_err = c->err;
@@ -103,11 +105,9 @@ upic_status upic_ctx_destroy(upic_ctx* c) {
     if (!c) return UPIC_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int d = 0; d < 3; ++d) dfree(c->plan.tw[d]);
-    dfree(c->dred);
-    dfree(c->dstats);
+    dfree(c->dred);  // dstats lies in the same allocation, hstats in hred's
     dfree(c->work);
     if (c->hred) (void)hipHostFree(c->hred);
-    if (c->hstats) (void)hipHostFree(c->hstats);
     for (hipEvent_t e : c->all_events) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

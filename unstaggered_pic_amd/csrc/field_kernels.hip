@@ -327,18 +327,22 @@ void launch_spectral_continuity(hipStream_t s, const SpecArgs& sp, double dt, co
 // threads take adjacent lines, which are adjacent in memory.
 constexpr int kProjLines = 16, kProjThreads = 16;
 template <bool ROWS>
-__global__ __launch_bounds__(256) void k_project_axis(double* __restrict__ f, int N, int n, int stride, int lines) {
+// src: the field read (the same as f for the in-place pass); each element
+// is read and written by the same thread.
+__global__ __launch_bounds__(256) void k_project_axis(double* f, const double* src, int N, int n, int stride,
+                                                      int lines) {
     __shared__ double part[kProjLines][kProjThreads + 1];
     const int t = threadIdx.x;
     const int li = ROWS ? t / kProjThreads : t % kProjLines;
     const int ch = ROWS ? t % kProjThreads : t / kProjLines;
     const int l = blockIdx.x * kProjLines + li;
     double* F = f + (int64_t)blockIdx.y * N;
+    const double* S = src + (int64_t)blockIdx.y * N;
     double sum = 0.0;
     int base = 0;
     if (l < lines) {
         base = (l / stride) * (n * stride) + (l % stride);
-        for (int j = ch; j < n; j += kProjThreads) sum += (j & 1 ? -1.0 : 1.0) * F[base + j * stride];
+        for (int j = ch; j < n; j += kProjThreads) sum += (j & 1 ? -1.0 : 1.0) * S[base + j * stride];
     }
     part[li][ch] = sum;
     __syncthreads();
@@ -346,22 +350,38 @@ __global__ __launch_bounds__(256) void k_project_axis(double* __restrict__ f, in
     for (int k = 0; k < kProjThreads; ++k) tot += part[li][k];
     const double mean = tot / n;
     if (l < lines)
-        for (int j = ch; j < n; j += kProjThreads) F[base + j * stride] -= (j & 1 ? -mean : mean);
+        for (int j = ch; j < n; j += kProjThreads) F[base + j * stride] = S[base + j * stride] - (j & 1 ? -mean : mean);
 }
-void launch_project_nyquist(hipStream_t s, const SpecArgs& sp, double* f, int batch) {
-    if (!sp.projection) return;
+void launch_project_nyquist_from(hipStream_t s, const SpecArgs& sp, const double* src, double* f, int batch) {
     const int dims[3] = {sp.nx, sp.ny, sp.nz};
     const int strides[3] = {sp.ny * sp.nz, sp.nz, 1};
-    for (int ax = 0; ax < 3; ++ax) {
-        const int n = dims[ax];
-        if (n <= 1 || n % 2) continue;
-        const int lines = sp.N / n;
-        const dim3 gr((lines + kProjLines - 1) / kProjLines, batch), bl(kProjLines * kProjThreads);
-        if (strides[ax] == 1)
-            hipLaunchKernelGGL(k_project_axis<true>, gr, bl, 0, s, f, sp.N, n, strides[ax], lines);
-        else
-            hipLaunchKernelGGL(k_project_axis<false>, gr, bl, 0, s, f, sp.N, n, strides[ax], lines);
+    const double* from = src;
+    if (sp.projection) {
+        for (int ax = 0; ax < 3; ++ax) {
+            const int n = dims[ax];
+            if (n <= 1 || n % 2) continue;
+            const int lines = sp.N / n;
+            const dim3 gr((lines + kProjLines - 1) / kProjLines, batch), bl(kProjLines * kProjThreads);
+            if (strides[ax] == 1)
+                hipLaunchKernelGGL(k_project_axis<true>, gr, bl, 0, s, f, from, sp.N, n, strides[ax], lines);
+            else
+                hipLaunchKernelGGL(k_project_axis<false>, gr, bl, 0, s, f, from, sp.N, n, strides[ax], lines);
+            from = f;
+        }
     }
+    if (from != f) launch_copy(s, (int64_t)batch * sp.N, src, f);
+}
+void launch_project_nyquist(hipStream_t s, const SpecArgs& sp, double* f, int batch) {
+    launch_project_nyquist_from(s, sp, f, f, batch);
+}
+
+__global__ void k_zero2(double* __restrict__ a, int na, double* __restrict__ b, int nb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < na) a[i] = 0.0;
+    else if (i < na + nb) b[i - na] = 0.0;
+}
+void launch_zero2(hipStream_t s, double* a, int na, double* b, int nb) {
+    hipLaunchKernelGGL(k_zero2, dim3((unsigned)((na + nb + 255) / 256)), dim3(256), 0, s, a, na, b, nb);
 }
 
 // ---------------------------------------------------------------------------

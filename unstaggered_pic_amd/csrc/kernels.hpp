@@ -268,6 +268,10 @@ void launch_spectral_continuity(hipStream_t s, const SpecArgs& sp, double dt, co
                                 double* rhoh);
 // Real-space Nyquist projector (project_nyquist, P3:2044-2061) on `batch` fields
 void launch_project_nyquist(hipStream_t s, const SpecArgs& sp, double* f, int batch);
+// The same into f from src (a copy when no axis is projected)
+void launch_project_nyquist_from(hipStream_t s, const SpecArgs& sp, const double* src, double* f, int batch);
+// a[0, na) = 0 and b[0, nb) = 0 in one launch
+void launch_zero2(hipStream_t s, double* a, int na, double* b, int nb);
 // Subtract the mean of each of `batch` real fields (subtract_mean, P1:215-219)
 void launch_subtract_mean(hipStream_t s, double* f, int batch, int N, double* scratch);
 

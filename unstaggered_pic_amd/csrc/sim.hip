@@ -294,8 +294,9 @@ void state_spectra(upic_sim* s) {
 }
 
 // Field solve of one map evaluation from a real current (3N): continuity,
-// CN potentials, grad phi_mid.  Leaves the end-of-step spectra in S1 and the
-// real A_end and grad phi_mid in AG.
+// CN potentials, grad phi_mid.  Leaves the end-of-step spectra of rho, phi,
+// psi and U in S1 (fields 3..5, 12..14) and the real A_end and grad phi_mid
+// in AG.
 void field_solve_body(upic_sim* s, const double* Jreal) {
     upic_ctx* c = s->c;
     const int N = c->N();
@@ -316,13 +317,11 @@ void field_solve_body(upic_sim* s, const double* Jreal) {
     a.phi_end_h = Zf(s->S1, 4, N);
     a.psi_end_h = Zf(s->S1, 5, N);
     a.A_end_h = Zf(s->S1, 6, N);
-    a.U_end_h = Zf(s->S1, 9, N);
-    a.gpm_h = Zf(s->S1, 12, N);
+    a.gpm_h = Zf(s->S1, 9, N);
+    a.U_end_h = Zf(s->S1, 12, N);
     launch_spectral_solve(c->stream, a);
-    // A_end and gpm are adjacent in S1 (fields 6..8 and 12..14): copy both to scratch
-    launch_copy(c->stream, 6ll * N, Zf(s->S1, 6, N), Zf(s->zs, 0, N));
-    launch_copy(c->stream, 6ll * N, Zf(s->S1, 12, N), Zf(s->zs, 3, N));
-    from_spectra(c, s->zs, s->AG, 6);
+    // A_end and gpm are adjacent in S1 (fields 6..11): transformed back in place
+    from_spectra(c, Zf(s->S1, 6, N), s->AG, 6);
     if (s->mixed) launch_pack_nodes_f32(c->stream, N, s->AG + 3 * (size_t)N, s->A, s->AG, s->NRf);
     else if (c->g.nz == 1) launch_pack_nodes(c->stream, N, s->AG + 3 * (size_t)N, s->A, s->AG, s->NR);
 }
@@ -355,7 +354,7 @@ void commit_fields(upic_sim* s, const double* Jfinal) {
     upic_ctx* c = s->c;
     const int N = c->N();
     launch_copy(c->stream, 2ll * N * 3, Zf(s->S1, 3, N), Zf(s->zs, 0, N));  // rho_e, phi_e, psi_e
-    launch_copy(c->stream, 6ll * N, Zf(s->S1, 9, N), Zf(s->zs, 3, N));      // U_e
+    launch_copy(c->stream, 6ll * N, Zf(s->S1, 12, N), Zf(s->zs, 3, N));     // U_e
     from_spectra(c, s->zs, s->gtmp, 6);
     launch_copy(c->stream, N, s->gtmp, s->rho);
     launch_copy(c->stream, N, s->gtmp + N, s->phi);
@@ -365,13 +364,14 @@ void commit_fields(upic_sim* s, const double* Jfinal) {
     if (Jfinal != s->J) launch_copy(c->stream, 3ll * N, Jfinal, s->J);
 }
 
+// Clears slots [0, n) and the path statistics in one launch.
 void zero_red(upic_ctx* c, int n = 16) {
-    launch_zero(c->stream, n, c->dred);
-    HIPCHK(hipMemsetAsync(c->dstats, 0, 3 * sizeof(unsigned long long), c->stream));
+    launch_zero2(c->stream, c->dred, n, reinterpret_cast<double*>(c->dstats), 3);
 }
-void read_red(upic_ctx* c, int n = 16) {
-    HIPCHK(hipMemcpyAsync(c->hred, c->dred, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(c->hstats, c->dstats, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+// One copy of the whole reduction area and the path statistics (what every
+// kernel since the last read left there), then the synchronisation.
+void read_red(upic_ctx* c) {
+    HIPCHK(hipMemcpyAsync(c->hred, c->dred, (kRedSlots + 3) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     c->sync();
 }
 void fill_path(upic_ctx* c, int64_t np, upic_path_stats& ps) {
@@ -511,7 +511,7 @@ void p1_step(upic_sim* s, upic_step_info* info) {
         a.J_out = s->Jb[nxt];
         timed(c, "p1_push", [&] { launch_p1_push(c->stream, c->g, c->q, c->opt, a, false); });
         HIPCHK(hipGetLastError());
-        read_red(c, 8);
+        read_red(c);
         ++iters;
         const double* h = c->hred;
         const double cnt = 3.0 * (double)np;
@@ -579,14 +579,13 @@ struct Norms {
 // host are queued by residual_norms_launch; residual_norms_finish reads them
 // after the next synchronisation of the stream.
 double norms_floor(const upic_params& p) { return 64.0 * std::numeric_limits<double>::epsilon() * p.c; }
-// The norms and, for a device-side tolerance, the tolerance itself
-// (dred[kRedAtol], next to the norms) reach the host in one copy.
-void residual_norms_launch(upic_sim* s, const double* map, const double* guess) {
+// The norms land in slots [8, 14) (zeroed here unless the caller has), and
+// reach the host with the next read_red, as does a device-side tolerance.
+void residual_norms_launch(upic_sim* s, const double* map, const double* guess, bool zero = true) {
     upic_ctx* c = s->c;
-    launch_zero(c->stream, 6, c->dred + 8);
+    if (zero) launch_zero(c->stream, 6, c->dred + 8);
     launch_residual_norms(c->stream, c->N(), map, guess, norms_floor(c->p), c->p.nonlinear_rtol, c->p.nonlinear_atol,
                           c->dred + 8, s->atol_dev ? c->dred + kRedAtol : nullptr);
-    HIPCHK(hipMemcpyAsync(c->hred + 8, c->dred + 8, 7 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
 }
 Norms residual_norms_finish(upic_sim* s) {
     upic_ctx* c = s->c;
@@ -606,7 +605,7 @@ Norms residual_norms_finish(upic_sim* s) {
 }
 Norms residual_norms(upic_sim* s, const double* map, const double* guess) {
     residual_norms_launch(s, map, guess);
-    s->c->sync();
+    read_red(s->c);
     return residual_norms_finish(s);
 }
 
@@ -623,11 +622,15 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     upic_ctx* c = s->c;
     const upic_params& p = c->p;
     const int N = c->N();
-    launch_copy(c->stream, 3ll * N, guess, s->Jfield);
-    launch_project_nyquist(c->stream, c->sp, s->Jfield, 3);
+    launch_project_nyquist_from(c->stream, c->sp, guess, s->Jfield, 3);
     field_solve(s, s->Jfield);
-    launch_zero(c->stream, 3ll * N, Jmap);
-    zero_red(c, 8);  // the sweep's sums and maxima (slot kRedAtol keeps the step's tolerance)
+    // a mixed-precision uniform sweep writes no current (the deposit that
+    // follows fills Jmap)
+    if (!s->mixed || use_layout) launch_zero(c->stream, 3ll * N, Jmap);
+    // the sweep's sums and maxima, the incremental deposit's path-statistic
+    // changes [4, 7) and the residual norms [8, 14); slot kRedAtol keeps the
+    // step's tolerance
+    zero_red(c, 14);
     bool incremental = false, a0_written = false;
     if (s->mixed) {
         P3MArgs a{};
@@ -685,12 +688,9 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
             P3MArgs ad = a;
             ad.J_out = s->Jdep;
             if (incremental) {
-                launch_zero(c->stream, 3, c->dred + 40);
                 timed(c, "deposit_mixed", [&] {
-                    launch_deposit_delta_mixed(c->stream, c->g, c->qf, c->opt.split, ad, c->dred + 40);
+                    launch_deposit_delta_mixed(c->stream, c->g, c->qf, c->opt.split, ad, c->dred + 4);
                 });
-                HIPCHK(hipMemcpyAsync(c->hred + 40, c->dred + 40, 3 * sizeof(double), hipMemcpyDeviceToHost,
-                                      c->stream));
             } else {
                 launch_zero(c->stream, 3ll * N, s->Jdep);
                 const int variant = deposit_pick(s);
@@ -707,7 +707,6 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
                 });
                 if (s->tuner.pending >= 0) HIPCHK(hipEventRecord(s->tuner.e1, c->stream));
             }
-            launch_copy(c->stream, 3ll * N, s->Jdep, Jmap);
         }
     } else {
     P3Args a{};
@@ -725,11 +724,12 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     if (use_layout && write_out && s->sub_x0) a.sub = SubPaths{s->sub_off, s->sub_x0, s->sub_vb, s->sub_n};
     timed(c, "p3_map", [&] { launch_p3_map(c->stream, c->g, c->q, c->opt, a); });
     }
-    launch_project_nyquist(c->stream, c->sp, Jmap, 3);
+    if (s->mixed && !use_layout) launch_project_nyquist_from(c->stream, c->sp, s->Jdep, Jmap, 3);
+    else launch_project_nyquist(c->stream, c->sp, Jmap, 3);
     HIPCHK(hipGetLastError());
     // the residual of this map against its guess, read in the same synchronisation
-    if (norms) residual_norms_launch(s, Jmap, guess);
-    read_red(c, 4);
+    if (norms) residual_norms_launch(s, Jmap, guess, false);
+    read_red(c);
     if (s->predictor_check) {  // queued with the step's predictor, read in this synchronisation
         s->predictor_check = false;
         if (c->hred[kRedPredictor + 5] != 0.0) throw Error(UPIC_ENONLINEAR, "nonfinite initial macro-current predictor");
@@ -737,9 +737,9 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     if (norms) *norms = residual_norms_finish(s);
     if (s->mixed && !use_layout) {
         if (incremental) {  // path statistics of the kept deposit, moved by the changed particles
-            c->hstats[0] = (unsigned long long)((double)s->jdep_stats[0] + c->hred[41]);
-            c->hstats[1] = (unsigned long long)((double)s->jdep_stats[1] + c->hred[40]);
-            c->hstats[2] = std::max<unsigned long long>(s->jdep_stats[2], (unsigned long long)c->hred[42]);
+            c->hstats[0] = (unsigned long long)((double)s->jdep_stats[0] + c->hred[5]);
+            c->hstats[1] = (unsigned long long)((double)s->jdep_stats[1] + c->hred[4]);
+            c->hstats[2] = std::max<unsigned long long>(s->jdep_stats[2], (unsigned long long)c->hred[6]);
         }
         for (int k = 0; k < 3; ++k) s->jdep_stats[k] = c->hstats[k];
         s->jdep_valid = true;
@@ -831,7 +831,7 @@ struct History {
 // (launch_anderson_gamma) and cand = x_last + damping r_last -
 // sum_j gamma_j (dX_j + damping dR_j) all run on the device, which records
 // in dred[kRedCtl] whether the coefficients were formed and whether cand is
-// finite; the host reads both with the next evaluation.  Returns false
+// finite; the host reads both with the next evaluation (read_red).  Returns false
 // when there is no history to mix (nothing queued).
 bool anderson_launch(upic_sim* s, History& H, int memory, double damping, double reg, double* cand) {
     upic_ctx* c = s->c;
@@ -876,7 +876,6 @@ bool anderson_launch(upic_sim* s, History& H, int memory, double damping, double
         term(l2, damping, 2 + j, s->dR[j]);
         launch_lincomb_ctl(c->stream, n, l2, ctl, j == m - 1 ? 1 : 0, cand);
     }
-    HIPCHK(hipMemcpyAsync(c->hred + kRedCtl, ctl, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     return true;
 }
 
@@ -902,8 +901,6 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
     launch_zero(c->stream, 6, c->dred + kRedPredictor);
     launch_residual_norms(c->stream, N, s->guess, s->guess, norms_floor(p), p.nonlinear_rtol, p.nonlinear_atol,
                           c->dred + kRedPredictor);
-    HIPCHK(hipMemcpyAsync(c->hred + kRedPredictor, c->dred + kRedPredictor, 6 * sizeof(double), hipMemcpyDeviceToHost,
-                          c->stream));
     s->predictor_check = true;
     History H(s);
     double prev_rms = std::numeric_limits<double>::infinity();
