@@ -219,6 +219,17 @@ upic_status upic_ctx_timing(upic_ctx* ctx, const char* name, int64_t* launches, 
  * captured at its first use per simulation and source buffer.  Off by
  * default; results are identical to the eager launches. */
 upic_status upic_ctx_set_graphs(upic_ctx* ctx, int enable);
+/* Deterministic mode, the counterpart of the reference's deposit into
+ * per-thread meshes summed in a fixed order (deposit_current_to_mesh,
+ * P3:2177-2216; the orbit deposit alike): with it on, an FP64 P3 step gives
+ * the same bits on every run from the same state.  The orbit current is
+ * accumulated as exact 64-bit fixed-point integer words (order-independent
+ * sums, converted to FP64 once per map evaluation), the cell sort keeps the
+ * particles of a cell in their storage order, and norms, Gram entries and
+ * diagnostics sums run as one workgroup each.  Off by default (slower);
+ * results agree with the default mode to roundoff.  The mixed-precision
+ * sweep's FP32 tiles are not covered. */
+upic_status upic_ctx_set_deterministic(upic_ctx* ctx, int enable);
 /* Gauss-Legendre nodes/weights on [0,1] used by the context (host arrays of
  * orbit_quad_order entries). */
 upic_status upic_quadrature(const upic_ctx* ctx, double* s, double* w);

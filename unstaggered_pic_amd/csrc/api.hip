@@ -134,6 +134,12 @@ upic_status upic_ctx_set_graphs(upic_ctx* c, int enable) {
     return UPIC_OK;
 }
 
+upic_status upic_ctx_set_deterministic(upic_ctx* c, int enable) {
+    if (!c) return UPIC_EINVAL;
+    c->det = enable != 0;
+    return UPIC_OK;
+}
+
 upic_status upic_ctx_timing(upic_ctx* c, const char* name, int64_t* launches, double* total_ms) {
     if (!c || !name || !launches || !total_ms) return UPIC_EINVAL;
     const int rc = guarded(c, [&] { c->sync(); });

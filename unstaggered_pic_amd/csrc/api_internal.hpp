@@ -171,6 +171,10 @@ struct upic_ctx {
     std::vector<hipEvent_t> event_pool, all_events;
     // replay the field solve of a map evaluation from a captured hipGraph
     bool graphs = false;
+    // deterministic mode (upic_ctx_set_deterministic): fixed-point deposit in
+    // the FP64 P3 sweep, within-cell particle order by storage order in the
+    // FP64 cell sort, one-workgroup reductions
+    bool det = false;
 
     int N() const { return g.N; }
     double* workspace(size_t n) {
@@ -276,6 +280,9 @@ inline void validate(const upic_params& p) {
 
 template <class F>
 int guarded(upic_ctx* c, F&& f) {
+    // the context's reduction mode holds for the launches of this call
+    // (on this host thread)
+    const FixedOrderScope fo(c && c->det);
     try {
         f();
         if (c) c->err.clear();

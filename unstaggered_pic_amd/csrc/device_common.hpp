@@ -42,6 +42,15 @@ __device__ __forceinline__ int wave_claim(int* __restrict__ ctr, int key, bool a
     return slot;
 }
 
+// Target of the deterministic (fixed-point) current deposit: two 64-bit
+// integer words per mesh entry and the scale 2^s (fix_add, particles.hpp).
+struct FixAcc {
+    unsigned long long* hi;
+    unsigned long long* lo;
+    double scale;
+};
+constexpr double kFixLoScale = 1099511627776.0;  // 2^40
+
 // Grid description passed by value to every kernel.
 struct GridDev {
     int nx, ny, nz, N;

@@ -404,6 +404,20 @@ void launch_subtract_mean(hipStream_t s, double* f, int batch, int N, double* sc
     hipLaunchKernelGGL(k_sub_mean, dim3(gblk(N), batch), dim3(256), 0, s, f, N, scratch);
 }
 
+namespace {
+thread_local bool t_fixed_order = false;
+// launch shape of a grid-stride reduction over n items with at most cap
+// workgroups of 256
+struct RedShape { dim3 grid, block; };
+RedShape red_shape(int64_t n, int64_t cap) {
+    if (t_fixed_order) return {dim3(1), dim3(1024)};
+    return {dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(gblk(n), cap))), dim3(256)};
+}
+}  // namespace
+bool fixed_order_reductions() { return t_fixed_order; }
+FixedOrderScope::FixedOrderScope(bool on) : prev(t_fixed_order) { t_fixed_order = on; }
+FixedOrderScope::~FixedOrderScope() { t_fixed_order = prev; }
+
 __global__ void k_diff_norms(int64_t n, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out) {
     double v[3] = {0.0, 0.0, 0.0};
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -413,8 +427,8 @@ __global__ void k_diff_norms(int64_t n, const double* __restrict__ a, const doub
     block_reduce_add<3>(v, out);
 }
 void launch_diff_norms(hipStream_t s, int64_t n, const double* a, const double* b, double* out3) {
-    const unsigned nb = (unsigned)std::min<int64_t>(gblk(n), 2048);
-    hipLaunchKernelGGL(k_diff_norms, dim3(nb), dim3(256), 0, s, n, a, b, out3);
+    const RedShape r = red_shape(n, 2048);
+    hipLaunchKernelGGL(k_diff_norms, r.grid, r.block, 0, s, n, a, b, out3);
 }
 
 struct DotPairs { const double* x[16]; const double* y[16]; };
@@ -431,8 +445,8 @@ void launch_dot_many(hipStream_t s, int64_t n, int npairs, const double* const* 
         DotPairs dp;
         const int cnt = std::min(16, npairs - base);
         for (int k = 0; k < cnt; ++k) { dp.x[k] = xs[base + k]; dp.y[k] = ys[base + k]; }
-        const unsigned nb = (unsigned)std::min<int64_t>(gblk(n), 512);
-        hipLaunchKernelGGL(k_dot_many, dim3(nb, cnt), dim3(256), 0, s, n, cnt, dp, out + base);
+        const RedShape r = red_shape(n, 512);
+        hipLaunchKernelGGL(k_dot_many, dim3(r.grid.x, cnt), r.block, 0, s, n, cnt, dp, out + base);
     }
 }
 
@@ -458,8 +472,8 @@ __global__ void k_residual_norms(int N, const double* __restrict__ map, const do
 }
 void launch_residual_norms(hipStream_t s, int N, const double* map, const double* guess, double floor_, double rtol,
                            double atol, double* out, const double* atol_dev) {
-    const unsigned nb = (unsigned)std::min<int64_t>(gblk(N), 1024);
-    hipLaunchKernelGGL(k_residual_norms, dim3(nb), dim3(256), 0, s, N, map, guess, floor_, rtol, atol, atol_dev, out);
+    const RedShape r = red_shape(N, 1024);
+    hipLaunchKernelGGL(k_residual_norms, r.grid, r.block, 0, s, N, map, guess, floor_, rtol, atol, atol_dev, out);
 }
 
 __global__ void k_outer_atol(const double* __restrict__ sums, int N, double dV, double atol, double rtol,
@@ -625,8 +639,8 @@ __global__ void k_contraction_sums(int64_t n, const double* __restrict__ g, cons
 }
 void launch_contraction_sums(hipStream_t s, int64_t n, const double* g, const double* xl, const double* r,
                              const double* rl, double* out) {
-    hipLaunchKernelGGL(k_contraction_sums, dim3((unsigned)std::min<int64_t>(gblk(n), 1024)), dim3(256), 0, s, n, g, xl,
-                       r, rl, out);
+    const RedShape sh = red_shape(n, 1024);
+    hipLaunchKernelGGL(k_contraction_sums, sh.grid, sh.block, 0, s, n, g, xl, r, rl, out);
 }
 
 __global__ void k_sum_squares(int64_t n, const double* __restrict__ f, double* __restrict__ out) {
@@ -645,10 +659,12 @@ __global__ void k_max_abs(int64_t n, const double* __restrict__ f, double* __res
     block_reduce_max<1>(v, out);
 }
 void launch_sum_squares(hipStream_t s, int64_t n, const double* f, double* out) {
-    hipLaunchKernelGGL(k_sum_squares, dim3((unsigned)std::min<int64_t>(gblk(n), 1024)), dim3(256), 0, s, n, f, out);
+    const RedShape r = red_shape(n, 1024);
+    hipLaunchKernelGGL(k_sum_squares, r.grid, r.block, 0, s, n, f, out);
 }
 void launch_sum(hipStream_t s, int64_t n, const double* f, double* out) {
-    hipLaunchKernelGGL(k_sum, dim3((unsigned)std::min<int64_t>(gblk(n), 1024)), dim3(256), 0, s, n, f, out);
+    const RedShape r = red_shape(n, 1024);
+    hipLaunchKernelGGL(k_sum, r.grid, r.block, 0, s, n, f, out);
 }
 void launch_max_abs(hipStream_t s, int64_t n, const double* f, double* out) {
     hipLaunchKernelGGL(k_max_abs, dim3((unsigned)std::min<int64_t>(gblk(n), 1024)), dim3(256), 0, s, n, f, out);
@@ -676,7 +692,8 @@ __global__ void k_kinetic(int64_t np, const double* __restrict__ v, const double
     block_reduce_max<1>(mx, out + 2);
 }
 void launch_kinetic(hipStream_t s, int64_t np, const double* v, const double* m, double c, int relativistic, double* out3) {
-    hipLaunchKernelGGL(k_kinetic, dim3((unsigned)std::min<int64_t>(gblk(np), 2048)), dim3(256), 0, s, np, v, m, c, relativistic, out3);
+    const RedShape r = red_shape(np, 2048);
+    hipLaunchKernelGGL(k_kinetic, r.grid, r.block, 0, s, np, v, m, c, relativistic, out3);
 }
 
 // ---------------------------------------------------------------------------
@@ -761,6 +778,30 @@ __global__ void k_scatter_perm(int64_t np, const int* __restrict__ keys, int* __
     const int key = active ? keys[p] : 0;
     const int pos = wave_claim(cursor, key, active);
     if (active) perm[pos] = (int)p;
+}
+// One workgroup per cell: each entry's rank is the number of entries of the
+// cell with a smaller old index (entries are distinct), read from LDS for
+// cells of up to kStableLds entries and from HBM beyond.
+constexpr int kStableLds = 4096;
+__global__ __launch_bounds__(256) void k_stable_cells(int ncell, const int* __restrict__ cell_end,
+                                                      const int* __restrict__ perm, int* __restrict__ out) {
+    __shared__ int v[kStableLds];
+    const int c = blockIdx.x;
+    const int b = c ? cell_end[c - 1] : 0, e = cell_end[c], n = e - b;
+    const bool lds = n <= kStableLds;
+    if (lds)
+        for (int i = threadIdx.x; i < n; i += blockDim.x) v[i] = perm[b + i];
+    __syncthreads();
+    const int* src = lds ? v : perm + b;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int x = src[i];
+        int r = 0;
+        for (int j = 0; j < n; ++j) r += src[j] < x ? 1 : 0;
+        out[b + r] = x;
+    }
+}
+void launch_stable_cells(hipStream_t s, int ncell, const int* cell_end, const int* perm, int* out) {
+    if (ncell > 0) hipLaunchKernelGGL(k_stable_cells, dim3(ncell), dim3(256), 0, s, ncell, cell_end, perm, out);
 }
 __global__ void k_permute(int64_t np, int ncomp, const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ perm) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

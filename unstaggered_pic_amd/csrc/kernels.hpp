@@ -57,6 +57,7 @@ struct P3Args {
     unsigned long long* stats;
     double* part;                 // per-block partials (kPartSlots per block)
     SubPaths sub;                 // substep start points and vbar (write-out pass, or empty)
+    FixAcc fix;                   // deterministic deposit: fixed-point target instead of J_out (fix.hi set)
 };
 
 // P2 explicit step arguments.
@@ -70,6 +71,26 @@ struct P2Args {
     double* rho;                  // naive charge deposit target or NULL
     const double* F;              // 11 gathered scalar fields (push)
 };
+
+// Reduction mode of the calling host thread.  While a FixedOrderScope(true)
+// lives, every grid-stride reduction launcher (norms, dot products, sums,
+// kinetic energy) runs one workgroup of 1024 work-items, whose fixed-order
+// tree gives the same bits on every run; otherwise many workgroups combine
+// their sums with atomics, in whatever order they finish.
+bool fixed_order_reductions();
+struct FixedOrderScope {
+    bool prev;
+    explicit FixedOrderScope(bool on);
+    ~FixedOrderScope();
+};
+
+// FP64 cell sort, deterministic mode: each cell's range of perm (the end of
+// cell c at cell_end[c]) is put in ascending order of the old index, into out.
+void launch_stable_cells(hipStream_t s, int ncell, const int* cell_end, const int* perm, int* out);
+
+// Deterministic deposit: out[i] = (hi[i] + lo[i] 2^-40) / scale for n
+// entries, and both words cleared for the next accumulation.
+void launch_fix_to_double(hipStream_t s, int64_t n, const FixAcc& fx, double* out);
 
 // particle_kernels.hip
 void launch_deposit_orbit(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const double* x0,
@@ -339,6 +360,15 @@ void launch_step_diag_subcycled(hipStream_t s, const GridDev& g, const QuadDev& 
                                 double dt, const SubPaths& sp, const double* qa, const double* A0, const double* A1,
                                 const double* Umid, const double* Emid, double* part, double* part2, double* sums,
                                 double* max1, unsigned long long* stats);
+// The same step diagnostics on a 2-D mesh in one fused orbit pass: packs
+// (U_mid, E_mid, A^n, A^{n+1}) into NR12 (12 N doubles), accumulates sums6
+// and max1 as above, deposits the orbit current of q vbar into Jbar and the
+// path statistics into stats.
+void launch_step_diag_2d(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, int64_t np,
+                         int midpoint, double dt, const double* x0, const double* v0, const double* v1,
+                         const double* vbar_last, const double* qa, const double* Umid, const double* Emid,
+                         const double* A0, const double* A1, double* NR12, double* Jbar, double* part, double* part2,
+                         double* sums6, double* max1, unsigned long long* stats);
 void launch_step_diag_particles(hipStream_t s, int64_t np, double dt, const double* q, const double* vbar,
                                 const double* Ubar, const double* D, const double* A0p, const double* A1p,
                                 const double* Ebar, double* part, double* part2, double* sums6, double* max1);

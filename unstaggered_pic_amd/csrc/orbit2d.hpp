@@ -17,7 +17,8 @@
 // node falls inside it, else straight to HBM with FP64 atomics; the tile is
 // added to the mesh once per workgroup.  With cell-sorted particles a
 // workgroup's particles sit in one or two cells, so nearly every deposit
-// lands in LDS.
+// lands in LDS.  The deterministic mode keeps the same path with exact
+// fixed-point words (FixAcc) in place of the FP64 atomics.
 #pragma once
 
 #include "particles.hpp"
@@ -204,11 +205,16 @@ __device__ __forceinline__ int orbit_gg_2d(const GridDev& g, const QuadDev& q, c
     return nseg;
 }
 
-// Workgroup LDS tile for the current deposit (2-D).
+// Workgroup LDS tile for the current deposit (2-D).  In the deterministic
+// mode (fx.hi set) the tile holds the two fixed-point words of FixAcc per
+// entry and the deposit goes to fx instead of J.
 struct DepTile {
-    double* t;   // kTile*kTile*3 doubles in LDS
+    double* t;   // kTile*kTile*3 doubles in LDS (2 x that many words in the fixed-point mode)
     int ox, oy;  // wrapped mesh index of tile node (0,0)
+    FixAcc fx;
 };
+// LDS doubles a deposit tile needs in either mode
+constexpr int kTileLds = 2 * kTile * kTile * 3;
 
 __device__ __forceinline__ void tile_add(const GridDev& g, const DepTile& T, double* __restrict__ J, int ia, int ib,
                                          double j0, double j1, double j2) {
@@ -216,7 +222,23 @@ __device__ __forceinline__ void tile_add(const GridDev& g, const DepTile& T, dou
     if (dx < 0) dx += g.nx;
     int dy = ib - T.oy;
     if (dy < 0) dy += g.ny;
-    if (dx < kTile && dy < kTile) {
+    const bool in = dx < kTile && dy < kTile;
+    if (T.fx.hi) {
+        if (in) {
+            unsigned long long* c = reinterpret_cast<unsigned long long*>(T.t) + 3 * (dx * kTile + dy);
+            constexpr int lo = kTile * kTile * 3;
+            fix_add(c, c + lo, j0, T.fx.scale);
+            fix_add(c + 1, c + 1 + lo, j1, T.fx.scale);
+            fix_add(c + 2, c + 2 + lo, j2, T.fx.scale);
+        } else {
+            const int id = ia * g.ny + ib;
+            fix_add(T.fx.hi + id, T.fx.lo + id, j0, T.fx.scale);
+            fix_add(T.fx.hi + g.N + id, T.fx.lo + g.N + id, j1, T.fx.scale);
+            fix_add(T.fx.hi + 2 * g.N + id, T.fx.lo + 2 * g.N + id, j2, T.fx.scale);
+        }
+        return;
+    }
+    if (in) {
         double* c = T.t + 3 * (dx * kTile + dy);
         atomic_add(c, j0);
         atomic_add(c + 1, j1);
@@ -279,9 +301,10 @@ __device__ __forceinline__ int deposit_orbit_2d(const GridDev& g, const QuadDev&
 
 // Tile set-up: origin two nodes below the stencil base of the workgroup's
 // first particle; zeroed; must be called by every thread of the workgroup.
+// With fx.hi set the tile takes fixed-point words (lds holds kTileLds).
 template <int R>
 __device__ __forceinline__ DepTile tile_begin(const GridDev& g, double* lds, const double* __restrict__ x, int64_t np,
-                                              int64_t p_first) {
+                                              int64_t p_first, FixAcc fx = FixAcc{}) {
     __shared__ int org[2];
     if (threadIdx.x == 0) {
         int ox = 0, oy = 0;
@@ -297,13 +320,30 @@ __device__ __forceinline__ DepTile tile_begin(const GridDev& g, double* lds, con
         org[0] = ox;
         org[1] = oy;
     }
-    for (int i = threadIdx.x; i < kTile * kTile * 3; i += blockDim.x) lds[i] = 0.0;
+    const int nz = fx.hi ? kTileLds : kTile * kTile * 3;
+    for (int i = threadIdx.x; i < nz; i += blockDim.x) lds[i] = 0.0;
     __syncthreads();
-    return DepTile{lds, org[0], org[1]};
+    return DepTile{lds, org[0], org[1], fx};
 }
 
 __device__ __forceinline__ void tile_end(const GridDev& g, const DepTile& T, double* __restrict__ J) {
     __syncthreads();
+    if (T.fx.hi) {
+        const unsigned long long* w = reinterpret_cast<const unsigned long long*>(T.t);
+        constexpr int lo = kTile * kTile * 3;
+        for (int i = threadIdx.x; i < lo; i += blockDim.x) {
+            const unsigned long long h = w[i], l = w[lo + i];
+            if (h == 0 && l == 0) continue;
+            const int comp = i % 3, cell = i / 3;
+            const int dx = cell / kTile, dy = cell % kTile;
+            if (dx >= g.nx || dy >= g.ny) continue;
+            const int ia = (T.ox + dx) % g.nx, ib = (T.oy + dy) % g.ny;
+            const size_t id = (size_t)comp * g.N + ia * g.ny + ib;
+            if (h) atomicAdd(T.fx.hi + id, h);
+            if (l) atomicAdd(T.fx.lo + id, l);
+        }
+        return;
+    }
     for (int i = threadIdx.x; i < kTile * kTile * 3; i += blockDim.x) {
         const double v = T.t[i];
         if (v != 0.0) {

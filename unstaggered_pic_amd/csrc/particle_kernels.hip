@@ -366,9 +366,9 @@ template <int R, bool Z3>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_p3_map(GridDev g, QuadDev qd,
                                                                                       OrbitOpt opt, P3Args a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    __shared__ double tile_mem[kTile * kTile * 3];
+    __shared__ double tile_mem[kTileLds];
     DepTile T{};
-    if constexpr (!Z3) T = tile_begin<R>(g, tile_mem, a.x, a.np, (int64_t)blockIdx.x * blockDim.x);
+    if constexpr (!Z3) T = tile_begin<R>(g, tile_mem, a.x, a.np, (int64_t)blockIdx.x * blockDim.x, a.fix);
     // red: [0] sum of local iterations, [1] failed particles; redm: [0] max
     // local iterations, [1] max final scaled residual
     double rsum[2] = {0.0, 0.0}, rmax[2] = {0.0, 0.0};
@@ -406,7 +406,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             }
             const double dxp[3] = {h * vacc[0], h * vacc[1], h * vacc[2]};
             if (a.J_out) {
-                if constexpr (Z3) nseg_total += deposit_orbit_one<R, Z3>(g, qd, opt, xpos, dxp, vacc, qp / (double)nsub, a.J_out);
+                if constexpr (Z3) nseg_total += deposit_orbit_one<R, Z3>(g, qd, opt, xpos, dxp, vacc, qp / (double)nsub, a.J_out, a.fix);
                 else nseg_total += deposit_orbit_2d<R>(g, qd, opt, xpos, dxp, vacc, qp / (double)nsub, T, a.J_out);
             }
 #pragma unroll
@@ -934,6 +934,20 @@ void launch_init_momenta(hipStream_t s, const GridDev& g, int64_t np, int code, 
 }  // namespace upic
 
 namespace upic {
+__global__ void k_fix_to_double(int64_t n, unsigned long long* __restrict__ hi, unsigned long long* __restrict__ lo,
+                                double inv_scale, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long h = (long long)hi[i], l = (long long)lo[i];
+    out[i] = ((double)h + (double)l * (1.0 / kFixLoScale)) * inv_scale;
+    hi[i] = 0ull;
+    lo[i] = 0ull;
+}
+void launch_fix_to_double(hipStream_t s, int64_t n, const FixAcc& fx, double* out) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_fix_to_double, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, fx.hi, fx.lo,
+                           1.0 / fx.scale, out);
+}
 __global__ void k_pack_nodes(int N, const double* __restrict__ gpm, const double* __restrict__ A0,
                              const double* __restrict__ A1, double* __restrict__ NR) {
     const int id = blockIdx.x * blockDim.x + threadIdx.x;

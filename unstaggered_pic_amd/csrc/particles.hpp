@@ -74,6 +74,24 @@ __device__ __forceinline__ void gather3(const GridDev& g, const double* __restri
 
 __device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
 
+// Exact fixed-point accumulation of a current (the deterministic deposit,
+// which the reference obtains from per-thread meshes summed in a fixed order,
+// P3:2194-2216): a value x enters as two integers, hi = floor(x 2^s) and lo =
+// the remaining fraction times 2^40, each added with a 64-bit integer atomic.
+// Integer sums do not depend on the order of the additions, so the result is
+// bitwise reproducible.  With 2^s chosen so that every node sum stays below
+// 2^61 / 2^s in magnitude, neither word overflows (lo < 2^40 per term), and
+// the part of each value below 2^-(s+40) that is dropped is far below FP64
+// resolution of the node sums (FixAcc, device_common.hpp).
+__device__ __forceinline__ void fix_add(unsigned long long* hi, unsigned long long* lo, double x, double scale) {
+    const double y = x * scale;  // exact: a power-of-two scaling
+    const double f = floor(y);
+    const unsigned long long h = (unsigned long long)(long long)f;
+    const unsigned long long l = (unsigned long long)(long long)((y - f) * kFixLoScale);  // y - f exact, in [0, 1)
+    if (h) atomicAdd(hi, h);
+    if (l) atomicAdd(lo, l);
+}
+
 // Orbit traversal: for each knot-split segment (P1:858-906, P3:2689-2857)
 // and each Gauss point, calls f(s, w, xq) with xq the path point (wrapped
 // for P1).  Returns the number of segments (br.size() - 1).
@@ -116,7 +134,7 @@ __device__ __forceinline__ int for_each_orbit_point(const GridDev& g, const Quad
 template <int R, bool Z3>
 __device__ __forceinline__ int deposit_orbit_one(const GridDev& g, const QuadDev& q, const OrbitOpt& opt,
                                                  const double x0[3], const double dxp[3], const double vel[3],
-                                                 double charge, double* __restrict__ J) {
+                                                 double charge, double* __restrict__ J, FixAcc fx = FixAcc{}) {
     constexpr int NZ = Z3 ? R + 1 : 1;
     double W[R + 1][R + 1][NZ];
     int base[3] = {-1, -1, -1};
@@ -136,6 +154,12 @@ __device__ __forceinline__ int deposit_orbit_one(const GridDev& g, const QuadDev
                     const int ic = Z3 ? wrap_idx(base[2] + c, g.nz) : 0;
                     const int id = (ia * g.ny + ib) * g.nz + ic;
                     const double w = W[a][b][c] * qs;
+                    if (fx.hi) {
+                        fix_add(fx.hi + id, fx.lo + id, w * vel[0], fx.scale);
+                        fix_add(fx.hi + N + id, fx.lo + N + id, w * vel[1], fx.scale);
+                        fix_add(fx.hi + 2 * N + id, fx.lo + 2 * N + id, w * vel[2], fx.scale);
+                        continue;
+                    }
                     atomic_add(&J[id], w * vel[0]);
                     atomic_add(&J[N + id], w * vel[1]);
                     atomic_add(&J[2 * N + id], w * vel[2]);

@@ -153,6 +153,11 @@ struct upic_sim {
     double* hcontr = nullptr;  // pinned host mirror
     int64_t sub_n = 0, sub_cap = 0;
     bool sub_valid = false;
+    // deterministic deposit (upic_ctx_set_deterministic): the fixed-point
+    // words of the current (2 x 3N) and their scale 2^s, from a bound of the
+    // node sums (set at the first deterministic sweep after an upload)
+    unsigned long long *fix_hi = nullptr, *fix_lo = nullptr;
+    double fix_scale = 0.0;
     // refinement record of the last P3 step (NonlinearSolveInfo
     // P3:5511-5518)
     struct RefineInfo {
@@ -196,6 +201,7 @@ void sim_free(upic_sim* s) {
     dfree(s->retry_n); dfree(s->retry_idx); dfree(s->retry_gs); dfree(s->retry_old); dfree(s->Jdep); dfree(s->a0); dfree(s->s0);
     dfree(s->mlp_P); dfree(s->mlp_blob); dfree(s->train_buf); dfree(s->part); dfree(s->vdep);
     dfree(s->sub_off); dfree(s->sub_x0); dfree(s->sub_vb); dfree(s->contr);
+    dfree(s->fix_hi); dfree(s->fix_lo);
     if (s->hcontr) (void)hipHostFree(s->hcontr);
     s->hcontr = nullptr;
     s->sub_cap = s->sub_n = 0;
@@ -424,6 +430,10 @@ bool sort_particles(upic_sim* s) {
     launch_cell_keys(c->stream, c->g, np, s->x, s->keys, s->counts);
     launch_scan_counts(c->stream, ncell, s->counts, s->offsets, s->scan_tmp);
     launch_scatter_perm(c->stream, np, s->keys, s->offsets, s->perm);
+    if (c->det) {  // within a cell, the particles keep their storage order (keys are free now)
+        launch_stable_cells(c->stream, ncell, s->offsets, s->perm, s->keys);
+        std::swap(s->perm, s->keys);
+    }
     double** arr3[] = {&s->x, &s->v, &s->v_prev, &s->P, &s->vbar_last};
     for (double** a : arr3) {
         launch_permute(c->stream, np, 3, *a, s->ptmp, s->perm);
@@ -615,6 +625,33 @@ struct RefineNeeded : Error {
     RefineNeeded(const std::string& m, std::vector<int64_t> f) : Error(UPIC_EREFINE, m), failed(std::move(f)) {}
 };
 
+// The fixed-point target of the deterministic deposit.  A particle adds at
+// most |q| |vbar| / dV < |q| c / dV to any node current (its node weights sum
+// to one and the relativistic vbar stays below c), so every node sum is
+// bounded by B = np max|q| c / dV; the scale 2^s puts 2 B below 2^61.
+FixAcc fixed_point_target(upic_sim* s) {
+    upic_ctx* c = s->c;
+    const int64_t n3 = 3ll * c->N();
+    if (!s->fix_hi) {
+        s->fix_hi = dalloc<unsigned long long>((size_t)n3);
+        s->fix_lo = dalloc<unsigned long long>((size_t)n3);
+        HIPCHK(hipMemsetAsync(s->fix_hi, 0, (size_t)n3 * sizeof(unsigned long long), c->stream));
+        HIPCHK(hipMemsetAsync(s->fix_lo, 0, (size_t)n3 * sizeof(unsigned long long), c->stream));
+    }
+    if (!(s->fix_scale > 0.0)) {
+        launch_zero(c->stream, 1, c->dred + kRedGram);
+        launch_max_abs(c->stream, s->np, s->q, c->dred + kRedGram);
+        double qmax = 0.0;
+        HIPCHK(hipMemcpyAsync(&qmax, c->dred + kRedGram, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        c->sync();
+        const double B = 2.0 * (double)s->np * qmax * c->p.c / c->g.dV;
+        int e = 0;
+        std::frexp(std::max(B, 1e-300), &e);  // B < 2^e
+        s->fix_scale = std::ldexp(1.0, std::min(61 - e, 1000));
+    }
+    return FixAcc{s->fix_hi, s->fix_lo, s->fix_scale};
+}
+
 // evaluate_selective_current_map_at_layout (P3:5235-5431): guess (3N) ->
 // J_map (3N).  write_out stores the particle end state in xo/Po/vo/vbo.
 MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bool use_layout,
@@ -722,7 +759,9 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     a.reason = s->reason;
     a.red = c->dred; a.redmax = c->dred + 2; a.stats = c->dstats; a.part = s->part;
     if (use_layout && write_out && s->sub_x0) a.sub = SubPaths{s->sub_off, s->sub_x0, s->sub_vb, s->sub_n};
+    if (c->det) a.fix = fixed_point_target(s);
     timed(c, "p3_map", [&] { launch_p3_map(c->stream, c->g, c->q, c->opt, a); });
+    if (a.fix.hi) launch_fix_to_double(c->stream, 3ll * N, a.fix, Jmap);
     }
     if (s->mixed && !use_layout) launch_project_nyquist_from(c->stream, c->sp, s->Jdep, Jmap, 3);
     else launch_project_nyquist(c->stream, c->sp, Jmap, 3);
@@ -1637,6 +1676,7 @@ upic_status upic_sim_upload(upic_sim* s, const upic_host_state* st) {
     upic_ctx* c = s->c;
     return (upic_status)guarded(c, [&] {
         s->jprev_valid = false;
+        s->fix_scale = 0.0;  // charges may change: the deterministic deposit rescales
         fp64_view(s);  // arrays the caller leaves out keep their current values
         const int N = c->N();
         const size_t n3 = 3 * (size_t)s->np * sizeof(double), n1 = (size_t)s->np * sizeof(double);
@@ -1900,15 +1940,22 @@ upic_status upic_sim_step_diagnostics(upic_sim* s, upic_step_diagnostics* out) {
         const double* A0 = phi0 + N;
         const double* U0 = phi0 + 4ll * N;
         const int64_t pb = (partial_blocks(np) + 1) * kPartSlots;
-        double* w = c->workspace(30 * (size_t)np + 6 * (size_t)N + 2 * (size_t)pb);
+        // the fused 2-D pass needs no per-particle arrays (12N of node
+        // records instead); the 3-D and substep passes keep 30 np
+        const bool fused = c->g.nz == 1 && !s->start_refined;
+        const size_t per_particle = fused ? 12 * (size_t)N : 30 * (size_t)np;
+        double* w = c->workspace(per_particle + 6 * (size_t)N + 2 * (size_t)pb);
         double *vbar = w, *dx = w + 3 * np, *x1 = w + 6 * np, *Ubar = w + 9 * np, *D = w + 12 * np;
         double *A0p = w + 21 * np, *A1p = w + 24 * np, *Ebar = w + 27 * np;
-        double *Umid = w + 30 * np, *Emid = Umid + 3ll * N, *part = Emid + 3ll * N, *part2 = part + pb;
+        double* NR12 = w;
+        double *Umid = w + per_particle, *Emid = Umid + 3ll * N, *part = Emid + 3ll * N, *part2 = part + pb;
         double* Jbar = s->gtmp + 6ll * N;
         double* red = c->dred + 32;  // the diagnostics' accumulators, free again
         const double L[3] = {p.Lx, p.Ly, p.Lz};
         // orbit of the step: vbar (P1 midpoint, P3 the accepted vbar_last)
-        launch_step_diag_orbit(c->stream, np, p.code == 1 ? 1 : 0, dt, L, x0, v0, s->v, s->vbar_last, vbar, dx, x1);
+        if (!fused)
+            launch_step_diag_orbit(c->stream, np, p.code == 1 ? 1 : 0, dt, L, x0, v0, s->v, s->vbar_last, vbar, dx,
+                                   x1);
         {
             LinComb um{2, {0.5, 0.5}, {U0, s->U}};
             launch_lincomb(c->stream, 3ll * N, um, Umid);
@@ -1965,18 +2012,25 @@ upic_status upic_sim_step_diagnostics(upic_sim* s, upic_step_diagnostics* out) {
             o.max_path_segments = std::max<int32_t>(1, (int32_t)c->hstats[2]);
             return;
         }
-        // orbit residual R_A (compute_RA_orbit_residual)
-        launch_gather_orbit(c->stream, c->g, c->q, c->opt, x0, dx, Umid, np, Ubar);
-        launch_grad_A(c->stream, c->g, c->q, c->opt, x0, dx, A0, s->A, 0.0, 1.0, p.code == 1 ? 1 : 0, np, D);
-        launch_gather_point(c->stream, c->g, x0, A0, np, A0p);
-        launch_gather_point(c->stream, c->g, x1, s->A, np, A1p);
-        // works: particles against Ebar, mesh against the orbit current
-        launch_gather_orbit(c->stream, c->g, c->q, c->opt, x0, dx, Emid, np, Ebar);
         launch_zero(c->stream, 3ll * N, Jbar);
         HIPCHK(hipMemsetAsync(c->dstats, 0, 3 * sizeof(unsigned long long), c->stream));
-        launch_deposit_orbit(c->stream, c->g, c->q, c->opt, x0, dx, vbar, s->q, np, Jbar, c->dstats);
         launch_zero(c->stream, 8, red);
-        launch_step_diag_particles(c->stream, np, dt, s->q, vbar, Ubar, D, A0p, A1p, Ebar, part, part2, red, red + 6);
+        if (fused) {
+            launch_step_diag_2d(c->stream, c->g, c->q, c->opt, np, p.code == 1 ? 1 : 0, dt, x0, v0, s->v,
+                                s->vbar_last, s->q, Umid, Emid, A0, s->A, NR12, Jbar, part, part2, red, red + 6,
+                                c->dstats);
+        } else {
+            // orbit residual R_A (compute_RA_orbit_residual)
+            launch_gather_orbit(c->stream, c->g, c->q, c->opt, x0, dx, Umid, np, Ubar);
+            launch_grad_A(c->stream, c->g, c->q, c->opt, x0, dx, A0, s->A, 0.0, 1.0, p.code == 1 ? 1 : 0, np, D);
+            launch_gather_point(c->stream, c->g, x0, A0, np, A0p);
+            launch_gather_point(c->stream, c->g, x1, s->A, np, A1p);
+            // works: particles against Ebar, mesh against the orbit current
+            launch_gather_orbit(c->stream, c->g, c->q, c->opt, x0, dx, Emid, np, Ebar);
+            launch_deposit_orbit(c->stream, c->g, c->q, c->opt, x0, dx, vbar, s->q, np, Jbar, c->dstats);
+            launch_step_diag_particles(c->stream, np, dt, s->q, vbar, Ubar, D, A0p, A1p, Ebar, part, part2, red,
+                                       red + 6);
+        }
         const double* xs[1] = {Jbar};
         const double* ys[1] = {Emid};
         launch_dot_many(c->stream, 3ll * N, 1, xs, ys, red + 7);

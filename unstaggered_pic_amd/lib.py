@@ -85,6 +85,7 @@ def lib():
             "upic_sim_set_guess_mode": [vp, C.c_int32],
             "upic_sim_set_deposit_variant": [vp, C.c_int32],
             "upic_ctx_set_graphs": [vp, C.c_int],
+            "upic_ctx_set_deterministic": [vp, C.c_int],
             "upic_ensemble_step": [vp, C.c_int32, C.c_int32, vp, vp],
             "upic_sim_deposit_variant": [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
             "upic_sim_guess_params": [vp, vp], "upic_sim_set_guess_params": [vp, vp],
@@ -182,6 +183,11 @@ class Context:
     def set_graphs(self, on=True):
         """Replay each map evaluation's field solve from a captured hipGraph."""
         self._check(lib().upic_ctx_set_graphs(self.h, int(on)))
+
+    def set_deterministic(self, on=True):
+        """Bitwise-reproducible FP64 P3 steps (fixed-point deposit, stable
+        cell order, one-workgroup reductions)."""
+        self._check(lib().upic_ctx_set_deterministic(self.h, int(on)))
 
     def timing(self, name):
         n, ms = C.c_int64(), C.c_double()
