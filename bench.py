@@ -36,14 +36,17 @@ a bounded sample of the same workload).
 
 One step = one upic_step_p3 (upic_step_p1) call.  The timed region covers
 exactly --steps steps bracketed by device synchronisation; inputs are
-resident in HBM before it starts.
+resident in HBM before it starts.  With --graphs 1 (default) each P3 step's
+outer solve runs as one device-driven hipGraph (upic_ctx_set_graphs).
 
 Roofline (HBM, the north-star bookkeeping): B_alg per particle-step is one
 read and one write of the persistent particle state plus 192 N_g / N_p
 bytes of mesh traffic (72.75 B mixed P3, 128.75 B FP64 P3, 177.9 B FP64 P1
 on these grids).  `roofline.achieved` = B_alg x the particles one launch of
 the dominant kernel processes / that kernel's mean launch time (HIP events
-on the context stream); `roofline.step_level` = B_alg x particle-steps/s.
+on the context stream, over --timing-steps steps run right after the timed
+ones; event-bracketed launches take the host-driven outer loop, whose
+kernels are the same); `roofline.step_level` = B_alg x particle-steps/s.
 `traffic` is the rocprofv3-measured HBM bytes per launch of that kernel
 (profiles/); `traffic_model` is the kernel's own byte count (it also moves
 the sweep-to-sweep scratch).
@@ -338,7 +341,6 @@ def run_leg(args, lib, code, prec, mode, net=None):
     t_spin = time.perf_counter() - t0
     for _ in range(args.warmup):
         sim.step()
-    ctx.set_timing(not args.no_timing)
     d_start = sim.diagnostics(1)
     infos = []
     ctx.sync()
@@ -347,12 +349,21 @@ def run_leg(args, lib, code, prec, mode, net=None):
         infos.append(sim.step())
     ctx.sync()
     el = time.perf_counter() - t0
-    kname = "p1_push" if code == 1 else ("p3_map" if prec == 0 else "p3_map_mixed")
-    launches, kms = ctx.timing(kname)
-    dep = ctx.timing("deposit_mixed") if prec == 1 else (0, 0.0)
     d1 = sim.diagnostics(1)
     dep_choice = sim.deposit_variant() if prec == 1 else None
-    ctx.set_timing(False)
+    # per-kernel HIP-event timing on the context stream over a few more steps
+    # (event-bracketed launches run the host-driven outer loop; the kernels
+    # are the same)
+    kname = "p1_push" if code == 1 else ("p3_map" if prec == 0 else "p3_map_mixed")
+    launches, kms, dep, tsteps, tinfos = 0, 0.0, (0, 0.0), 0, []
+    if not args.no_timing and args.timing_steps > 0:
+        ctx.set_timing(True)
+        for _ in range(args.timing_steps):
+            tinfos.append(sim.step())
+        tsteps = args.timing_steps
+        launches, kms = ctx.timing(kname)
+        dep = ctx.timing("deposit_mixed") if prec == 1 else (0, 0.0)
+        ctx.set_timing(False)
     # the reference's scope: compute_diagnostics and compute_step_diagnostics
     # after every step (P3:8835-8850, P1:1847-1848)
     asref = None
@@ -373,6 +384,7 @@ def run_leg(args, lib, code, prec, mode, net=None):
     sim.close()
     ctx.close()
     return {"p": p, "np": np_, "N": N, "elapsed": el, "infos": infos, "launches": launches, "kms": kms, "dep": dep,
+            "timing_steps": tsteps, "timing_infos": tinfos,
             "kname": kname, "d0": d0, "d_start": d_start, "d1": d1, "dep_choice": dep_choice,
             "upload_seconds": t_upload, "as_reference": asref, "code": code, "prec": prec, "mode": mode,
             "spinup_ms_per_step": 1e3 * t_spin / args.spinup if args.spinup else None,
@@ -443,9 +455,10 @@ def compute_roofline(r, args):
     infos = r["infos"]
     segs = np.mean([i.path.total_segments / max(i.path.n_particles, 1) for i in infos])
     flops_iter = segs * 4 * (2 * 4 * 24 + 16 * (2 * 15 + 6 * 2 + 3)) + 16 * 6 + 250
-    local_total = sum(i.local_iterations_total for i in infos)
+    tinfos = r["timing_infos"]
+    local_total = sum(i.local_iterations_total for i in tinfos)
     peak = FP64_PEAK_TFLOPS if r["prec"] == 0 else FP32_PEAK_TFLOPS
-    tflops = (flops_iter * local_total / args.steps) / (r["kms"] * 1e-3 / args.steps) / 1e12
+    tflops = (flops_iter * local_total) / (r["kms"] * 1e-3) / 1e12
     return {"bound": "fp64_valu" if r["prec"] == 0 else "fp32_valu", "achieved": tflops, "peak": peak,
             "unit": "TFLOP/s", "frac": tflops / peak, "flops_per_inner_iteration": flops_iter}
 
@@ -495,7 +508,10 @@ def main():
     ap.add_argument("--p1-ppc", type=int, default=100)
     ap.add_argument("--no-p1", action="store_true", help="skip the P1 Landau leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--graphs", type=int, default=1, help="replay the field solves from hipGraphs (1) or not (0)")
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="1: each P3 step's outer solve as one device-driven hipGraph (0: host-driven loop)")
+    ap.add_argument("--timing-steps", type=int, default=4,
+                    help="steps after the timed ones whose kernels are timed with HIP events")
     ap.add_argument("--no-timing", action="store_true", help="skip the per-kernel event timing")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (a known-size kernel for HBM counters)")

@@ -213,11 +213,20 @@ const char* upic_last_error(const upic_ctx* ctx);
  * and milliseconds since the last upic_ctx_set_timing call. */
 upic_status upic_ctx_set_timing(upic_ctx* ctx, int enable);
 upic_status upic_ctx_timing(upic_ctx* ctx, const char* name, int64_t* launches, double* total_ms);
-/* No reference counterpart: replay the field solve of every map evaluation
- * (forward FFTs of the current, the k-space solve, inverse FFTs, node
- * record packing -- about a dozen kernels on fixed buffers) from a hipGraph
- * captured at its first use per simulation and source buffer.  Off by
- * default; results are identical to the eager launches. */
+/* No reference counterpart: hipGraph replay.  A uniform-layout P3 step
+ * (FP64 or mixed precision, Anderson memory <= 4) runs its whole outer solve
+ * as one captured graph: device controller kernels make the outer loop's
+ * decisions (Anderson, backtracking, restarts, convergence) between gated
+ * map evaluations, so the step needs one launch and one synchronisation;
+ * graphs are cached per buffer configuration.  A step the controller does
+ * not finish within the unrolled evaluations continues two at a time, and
+ * one it reports failed (particle failures, backtracking, nonfinite
+ * currents) runs again on the host-driven path, as the reference would.
+ * Other steps (P1, and the map evaluations of the host-driven path) replay
+ * their field solves from graphs.  enable: 0 off (default), 1 on with the
+ * number of unrolled evaluations adapted to the steps, n >= 2 on with n
+ * unrolled evaluations.  Results agree with the host-driven path to
+ * roundoff (the same decisions; sums in another order). */
 upic_status upic_ctx_set_graphs(upic_ctx* ctx, int enable);
 /* Deterministic mode, the counterpart of the reference's deposit into
  * per-thread meshes summed in a fixed order (deposit_current_to_mesh,

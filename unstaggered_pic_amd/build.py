@@ -10,7 +10,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libupic.so")
-SOURCES = ["particle_kernels.hip", "diag_kernels.hip", "mixed.hip", "mlp.hip", "field_kernels.hip", "api.hip", "sim.hip", "init.hip"]
+SOURCES = ["particle_kernels.hip", "diag_kernels.hip", "outer_dev.hip", "mixed.hip", "mlp.hip", "field_kernels.hip", "api.hip", "sim.hip", "init.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-std=c++17", "-O3", "-fPIC", "--offload-arch=gfx950", "-munsafe-fp-atomics",

@@ -130,7 +130,9 @@ upic_status upic_ctx_set_timing(upic_ctx* c, int enable) {
 
 upic_status upic_ctx_set_graphs(upic_ctx* c, int enable) {
     if (!c) return UPIC_EINVAL;
+    if (enable < 0) return UPIC_EINVAL;
     c->graphs = enable != 0;
+    c->graph_slots = enable > 1 ? enable : 0;
     return UPIC_OK;
 }
 

@@ -169,8 +169,11 @@ struct upic_ctx {
     std::map<std::string, KTimer> timers;
     std::vector<PendingTimer> pending_timers;
     std::vector<hipEvent_t> event_pool, all_events;
-    // replay the field solve of a map evaluation from a captured hipGraph
+    // replay the field solve of a map evaluation, or a whole P3 outer solve,
+    // from a captured hipGraph; graph_slots > 0 fixes the number of map
+    // evaluations a step graph unrolls (0: adapted to the steps)
     bool graphs = false;
+    int graph_slots = 0;
     // deterministic mode (upic_ctx_set_deterministic): fixed-point deposit in
     // the FP64 P3 sweep, within-cell particle order by storage order in the
     // FP64 cell sort, one-workgroup reductions

@@ -51,6 +51,9 @@ struct FixAcc {
 };
 constexpr double kFixLoScale = 1099511627776.0;  // 2^40
 
+// A gated kernel returns at once when its gate (device int) reads 0.
+#define UPIC_GATE_CLOSED(g) ((g) != nullptr && *(g) == 0)
+
 // Grid description passed by value to every kernel.
 struct GridDev {
     int nx, ny, nz, N;

@@ -2,6 +2,7 @@
 // k-space field solves, projector, reductions, and the particle cell sort.
 #include "kernels.hpp"
 #include "particles.hpp"
+#include "dense_solve.hpp"
 
 namespace upic {
 
@@ -13,7 +14,9 @@ namespace upic {
 // neighbouring addresses.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_fft_axis(double* __restrict__ data, int N, int n, int logn, int stride,
-                                                  int lines, int lpb, const double* __restrict__ tw, int inverse) {
+                                                  int lines, int lpb, const double* __restrict__ tw, int inverse,
+                                                  const int* __restrict__ gate) {
+    if (UPIC_GATE_CLOSED(gate)) return;
     extern __shared__ double lds[];  // lpb * n complex
     double2* a = reinterpret_cast<double2*>(lds);
     const int field = blockIdx.y;
@@ -81,18 +84,20 @@ void launch_fft(hipStream_t s, const FFTPlan& plan, double* data, int batch, boo
         const int blocks = (lines + lpb - 1) / lpb;
         const size_t lds = (size_t)lpb * n * sizeof(double2);
         hipLaunchKernelGGL(k_fft_axis, dim3(blocks, batch), dim3(256), lds, s, data, N, n, logn, strides[ax], lines, lpb,
-                           plan.tw[ax], inverse ? 1 : 0);
+                           plan.tw[ax], inverse ? 1 : 0, current_gate());
     }
 }
 
 // ---------------------------------------------------------------------------
 // Elementwise helpers
 // ---------------------------------------------------------------------------
-__global__ void k_r2c(const double* __restrict__ re, double* __restrict__ z, int64_t n) {
+__global__ void k_r2c(const double* __restrict__ re, double* __restrict__ z, int64_t n, const int* __restrict__ gate) {
+    if (UPIC_GATE_CLOSED(gate)) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) { z[2 * i] = re[i]; z[2 * i + 1] = 0.0; }
 }
-__global__ void k_c2r(const double* __restrict__ z, double* __restrict__ re, int64_t n) {
+__global__ void k_c2r(const double* __restrict__ z, double* __restrict__ re, int64_t n, const int* __restrict__ gate) {
+    if (UPIC_GATE_CLOSED(gate)) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) re[i] = z[2 * i];
 }
@@ -100,29 +105,31 @@ __global__ void k_axpby(int64_t n, double a, const double* __restrict__ x, doubl
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = (x ? a * x[i] : 0.0) + b * y[i];
 }
-__global__ void k_copy(int64_t n, const double* __restrict__ x, double* __restrict__ y) {
+__global__ void k_copy(int64_t n, const double* __restrict__ x, double* __restrict__ y, const int* __restrict__ gate) {
+    if (UPIC_GATE_CLOSED(gate)) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = x[i];
 }
-__global__ void k_fill(int64_t n, double v, double* __restrict__ y) {
+__global__ void k_fill(int64_t n, double v, double* __restrict__ y, const int* __restrict__ gate) {
+    if (UPIC_GATE_CLOSED(gate)) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = v;
 }
 static inline unsigned gblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 void launch_real_to_complex(hipStream_t s, const double* re, double* z, int64_t n) {
-    if (n > 0) hipLaunchKernelGGL(k_r2c, dim3(gblk(n)), dim3(256), 0, s, re, z, n);
+    if (n > 0) hipLaunchKernelGGL(k_r2c, dim3(gblk(n)), dim3(256), 0, s, re, z, n, current_gate());
 }
 void launch_complex_real(hipStream_t s, const double* z, double* re, int64_t n) {
-    if (n > 0) hipLaunchKernelGGL(k_c2r, dim3(gblk(n)), dim3(256), 0, s, z, re, n);
+    if (n > 0) hipLaunchKernelGGL(k_c2r, dim3(gblk(n)), dim3(256), 0, s, z, re, n, current_gate());
 }
 void launch_axpby(hipStream_t s, int64_t n, double a, const double* x, double b, double* y) {
     if (n > 0) hipLaunchKernelGGL(k_axpby, dim3(gblk(n)), dim3(256), 0, s, n, a, x, b, y);
 }
 void launch_copy(hipStream_t s, int64_t n, const double* x, double* y) {
-    if (n > 0) hipLaunchKernelGGL(k_copy, dim3(gblk(n)), dim3(256), 0, s, n, x, y);
+    if (n > 0) hipLaunchKernelGGL(k_copy, dim3(gblk(n)), dim3(256), 0, s, n, x, y, current_gate());
 }
 void launch_fill(hipStream_t s, int64_t n, double v, double* y) {
-    if (n > 0) hipLaunchKernelGGL(k_fill, dim3(gblk(n)), dim3(256), 0, s, n, v, y);
+    if (n > 0) hipLaunchKernelGGL(k_fill, dim3(gblk(n)), dim3(256), 0, s, n, v, y, current_gate());
 }
 void launch_zero(hipStream_t s, int64_t n, double* y) { launch_fill(s, n, 0.0, y); }
 
@@ -172,7 +179,8 @@ __device__ __forceinline__ void st2(double* p, int id, double2 v) { reinterpret_
 // i * k * z
 __device__ __forceinline__ double2 ik(double k, double2 z) { return make_double2(-k * z.y, k * z.x); }
 
-__global__ void k_spectral_solve(SolveArgs a) {
+__global__ void k_spectral_solve(SolveArgs a, const int* __restrict__ gate) {
+    if (UPIC_GATE_CLOSED(gate)) return;
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     const int N = a.sp.N;
     if (id >= N) return;
@@ -220,7 +228,7 @@ __global__ void k_spectral_solve(SolveArgs a) {
     }
 }
 void launch_spectral_solve(hipStream_t s, const SolveArgs& a) {
-    hipLaunchKernelGGL(k_spectral_solve, dim3(gblk(a.sp.N)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_spectral_solve, dim3(gblk(a.sp.N)), dim3(256), 0, s, a, current_gate());
 }
 
 __global__ void k_spectral_grad(SpecArgs sp, const double* __restrict__ fh, double* __restrict__ o) {
@@ -330,7 +338,8 @@ template <bool ROWS>
 // src: the field read (the same as f for the in-place pass); each element
 // is read and written by the same thread.
 __global__ __launch_bounds__(256) void k_project_axis(double* f, const double* src, int N, int n, int stride,
-                                                      int lines) {
+                                                      int lines, const int* __restrict__ gate) {
+    if (UPIC_GATE_CLOSED(gate)) return;
     __shared__ double part[kProjLines][kProjThreads + 1];
     const int t = threadIdx.x;
     const int li = ROWS ? t / kProjThreads : t % kProjLines;
@@ -363,9 +372,11 @@ void launch_project_nyquist_from(hipStream_t s, const SpecArgs& sp, const double
             const int lines = sp.N / n;
             const dim3 gr((lines + kProjLines - 1) / kProjLines, batch), bl(kProjLines * kProjThreads);
             if (strides[ax] == 1)
-                hipLaunchKernelGGL(k_project_axis<true>, gr, bl, 0, s, f, from, sp.N, n, strides[ax], lines);
+                hipLaunchKernelGGL(k_project_axis<true>, gr, bl, 0, s, f, from, sp.N, n, strides[ax], lines,
+                                   current_gate());
             else
-                hipLaunchKernelGGL(k_project_axis<false>, gr, bl, 0, s, f, from, sp.N, n, strides[ax], lines);
+                hipLaunchKernelGGL(k_project_axis<false>, gr, bl, 0, s, f, from, sp.N, n, strides[ax], lines,
+                                   current_gate());
             from = f;
         }
     }
@@ -375,13 +386,14 @@ void launch_project_nyquist(hipStream_t s, const SpecArgs& sp, double* f, int ba
     launch_project_nyquist_from(s, sp, f, f, batch);
 }
 
-__global__ void k_zero2(double* __restrict__ a, int na, double* __restrict__ b, int nb) {
+__global__ void k_zero2(double* __restrict__ a, int na, double* __restrict__ b, int nb, const int* __restrict__ gate) {
+    if (UPIC_GATE_CLOSED(gate)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < na) a[i] = 0.0;
     else if (i < na + nb) b[i - na] = 0.0;
 }
 void launch_zero2(hipStream_t s, double* a, int na, double* b, int nb) {
-    hipLaunchKernelGGL(k_zero2, dim3((unsigned)((na + nb + 255) / 256)), dim3(256), 0, s, a, na, b, nb);
+    hipLaunchKernelGGL(k_zero2, dim3((unsigned)((na + nb + 255) / 256)), dim3(256), 0, s, a, na, b, nb, current_gate());
 }
 
 // ---------------------------------------------------------------------------
@@ -406,6 +418,7 @@ void launch_subtract_mean(hipStream_t s, double* f, int batch, int N, double* sc
 
 namespace {
 thread_local bool t_fixed_order = false;
+thread_local const int* t_gate = nullptr;
 // launch shape of a grid-stride reduction over n items with at most cap
 // workgroups of 256
 struct RedShape { dim3 grid, block; };
@@ -415,6 +428,9 @@ RedShape red_shape(int64_t n, int64_t cap) {
 }
 }  // namespace
 bool fixed_order_reductions() { return t_fixed_order; }
+const int* current_gate() { return t_gate; }
+GateScope::GateScope(const int* g) : prev(t_gate) { t_gate = g; }
+GateScope::~GateScope() { t_gate = prev; }
 FixedOrderScope::FixedOrderScope(bool on) : prev(t_fixed_order) { t_fixed_order = on; }
 FixedOrderScope::~FixedOrderScope() { t_fixed_order = prev; }
 
@@ -451,7 +467,9 @@ void launch_dot_many(hipStream_t s, int64_t n, int npairs, const double* const* 
 }
 
 __global__ void k_residual_norms(int N, const double* __restrict__ map, const double* __restrict__ guess, double floor_,
-                                 double rtol, double atol_v, const double* __restrict__ atol_dev, double* __restrict__ out) {
+                                 double rtol, double atol_v, const double* __restrict__ atol_dev, double* __restrict__ out,
+                                 const int* __restrict__ gate) {
+    if (UPIC_GATE_CLOSED(gate)) return;
     const double atol = atol_dev ? *atol_dev : atol_v;
     double sv[3] = {0.0, 0.0, 0.0}, mv[3] = {0.0, 0.0, 0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
@@ -473,7 +491,8 @@ __global__ void k_residual_norms(int N, const double* __restrict__ map, const do
 void launch_residual_norms(hipStream_t s, int N, const double* map, const double* guess, double floor_, double rtol,
                            double atol, double* out, const double* atol_dev) {
     const RedShape r = red_shape(N, 1024);
-    hipLaunchKernelGGL(k_residual_norms, r.grid, r.block, 0, s, N, map, guess, floor_, rtol, atol, atol_dev, out);
+    hipLaunchKernelGGL(k_residual_norms, r.grid, r.block, 0, s, N, map, guess, floor_, rtol, atol, atol_dev, out,
+                       current_gate());
 }
 
 __global__ void k_outer_atol(const double* __restrict__ sums, int N, double dV, double atol, double rtol,
@@ -487,87 +506,11 @@ void launch_outer_atol(hipStream_t s, const double* sums, int N, double dV, doub
     hipLaunchKernelGGL(k_outer_atol, dim3(1), dim3(64), 0, s, sums, N, dV, atol, rtol, floor_gross, out);
 }
 
-// Gaussian elimination with partial pivoting and the reference's rejection
-// rules (solve_small_dense, P3:5706-5742): a zero, nonfinite or
-// 100-epsilon-relative pivot, a pivot ratio below 1e-12 or a nonfinite
-// solution fails.  Products and sums are rounded separately (no
-// contraction), as in the host restatement of the same routine.
-__device__ bool small_dense_solve(int n, double (&A)[8][8], double (&b)[8], double (&x)[8]) {
-#pragma clang fp contract(off)
-    double scale = 0.0;
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) scale = fmax(scale, fabs(A[i][j]));
-    if (!(scale > 0.0) || !isfinite(scale)) return false;
-    double largest = 0.0, smallest = INFINITY;
-    for (int col = 0; col < n; ++col) {
-        int pr = col;
-        for (int row = col + 1; row < n; ++row)
-            if (fabs(A[row][col]) > fabs(A[pr][col])) pr = row;
-        const double piv = fabs(A[pr][col]);
-        if (!isfinite(piv) || piv <= 100.0 * __DBL_EPSILON__ * scale) return false;
-        largest = fmax(largest, piv);
-        smallest = fmin(smallest, piv);
-        if (pr != col) {
-            for (int j = 0; j < n; ++j) {
-                const double t = A[pr][j];
-                A[pr][j] = A[col][j];
-                A[col][j] = t;
-            }
-            const double t = b[pr];
-            b[pr] = b[col];
-            b[col] = t;
-        }
-        for (int row = col + 1; row < n; ++row) {
-            const double f = A[row][col] / A[col][col];
-            for (int j = col; j < n; ++j) A[row][j] -= f * A[col][j];
-            b[row] -= f * b[col];
-        }
-    }
-    if (smallest <= 1.0e-12 * largest) return false;
-    for (int row = n - 1; row >= 0; --row) {
-        double rhs = b[row];
-        for (int j = row + 1; j < n; ++j) rhs -= A[row][j] * x[j];
-        x[row] = rhs / A[row][row];
-        if (!isfinite(x[row])) return false;
-    }
-    return true;
-}
-
-// The Gram solve of anderson_type2_candidate (P3:5763-5823) in one lane:
-// trace check, the unregularised system must be solvable, then the system
-// with reg x trace/m on its diagonal gives gamma.
+// The Gram solve of anderson_type2_candidate (P3:5763-5823) in one lane
+// (anderson_gamma, dense_solve.hpp).
 __global__ void k_anderson_gamma(int m, const double* __restrict__ gram, double reg, double* __restrict__ ctl) {
-#pragma clang fp contract(off)
     if (threadIdx.x != 0) return;
-    double G[8][8], rhs[8], x[8];
-    ctl[0] = 0.0;
-    ctl[1] = 0.0;
-    for (int i = 0; i < m; ++i) rhs[i] = gram[i];
-    int k = m;
-    for (int i = 0; i < m; ++i)
-        for (int j = 0; j <= i; ++j) {
-            const double v = gram[k++];
-            G[i][j] = v;
-            G[j][i] = v;
-        }
-    double trace = 0.0;
-    for (int i = 0; i < m; ++i) trace += G[i][i];
-    if (!(trace > 0.0) || !isfinite(trace)) return;
-    double A[8][8], b[8];
-    for (int i = 0; i < m; ++i) {
-        b[i] = rhs[i];
-        for (int j = 0; j < m; ++j) A[i][j] = G[i][j];
-    }
-    if (!small_dense_solve(m, A, b, x)) return;
-    const double rs = trace / (double)m;
-    for (int i = 0; i < m; ++i) {
-        b[i] = rhs[i];
-        for (int j = 0; j < m; ++j) A[i][j] = G[i][j];
-        A[i][i] += reg * rs;
-    }
-    if (!small_dense_solve(m, A, b, x)) return;
-    for (int i = 0; i < m; ++i) ctl[2 + i] = x[i];
-    ctl[0] = 1.0;
+    anderson_gamma(m, gram, reg, ctl);
 }
 void launch_anderson_gamma(hipStream_t s, int m, const double* gram, double reg, double* ctl) {
     hipLaunchKernelGGL(k_anderson_gamma, dim3(1), dim3(64), 0, s, m, gram, reg, ctl);

@@ -58,6 +58,7 @@ struct P3Args {
     double* part;                 // per-block partials (kPartSlots per block)
     SubPaths sub;                 // substep start points and vbar (write-out pass, or empty)
     FixAcc fix;                   // deterministic deposit: fixed-point target instead of J_out (fix.hi set)
+    const int* gate;              // the launch runs only while *gate != 0 (NULL: always)
 };
 
 // P2 explicit step arguments.
@@ -70,6 +71,19 @@ struct P2Args {
     double* J;                    // 3*N deposit target
     double* rho;                  // naive charge deposit target or NULL
     const double* F;              // 11 gathered scalar fields (push)
+};
+
+// Gate of the calling host thread's launches: while a GateScope(g) lives,
+// the kernels of the gated launchers (FFT and the k-space solve, element-wise
+// copies and fills, the Nyquist projection, node packing, the particle
+// sweeps, deposits and their partial folds, the residual norms) return at
+// once when *g == 0.  The device-driven outer solve gates every map
+// evaluation on its controller's request; nullptr (the default) = ungated.
+const int* current_gate();
+struct GateScope {
+    const int* prev;
+    explicit GateScope(const int* g);
+    ~GateScope();
 };
 
 // Reduction mode of the calling host thread.  While a FixedOrderScope(true)
@@ -87,6 +101,16 @@ struct FixedOrderScope {
 // FP64 cell sort, deterministic mode: each cell's range of perm (the end of
 // cell c at cell_end[c]) is put in ascending order of the old index, into out.
 void launch_stable_cells(hipStream_t s, int ncell, const int* cell_end, const int* perm, int* out);
+
+// Device-driven outer solve (outer_dev.hip, outer.hpp).
+struct OuterDev;
+struct OuterCfg;
+struct OuterVecs;
+void launch_outer_init(hipStream_t s, OuterDev* o, int update_method);
+// pass 1, the Gram fold and solve, passes 2a/2b and the evaluation gate
+void launch_outer_prepare(hipStream_t s, OuterDev* o, const OuterCfg& cf, const OuterVecs& v, double damping);
+void launch_outer_post(hipStream_t s, OuterDev* o, const OuterCfg& cf, int incremental);
+void launch_outer_commit_gate(hipStream_t s, OuterDev* o, int* commit);
 
 // Deterministic deposit: out[i] = (hi[i] + lo[i] 2^-40) / scale for n
 // entries, and both words cleared for the next accumulation.
@@ -171,6 +195,7 @@ struct P3MArgs {
     // positions -L/2 + (cell + offset) h (sub.x0 == NULL: not recorded)
     SubPaths sub;
     double lo_x, lo_y, lo_z, hx, hy, Lz;
+    const int* gate;               // the launch runs only while *gate != 0 (NULL: always)
 };
 // Picard sweep; with a substep layout it also deposits, otherwise it writes
 // vdep and launch_deposit_mixed deposits (reason and vdep as inputs).

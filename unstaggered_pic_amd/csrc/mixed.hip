@@ -918,6 +918,7 @@ __device__ __forceinline__ void p3_map_mixed_body(const MGrid& g, const QuadF& q
 template <int R, int GM, bool COLLECT, bool FUSED>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_p3_map_mixed(MGrid g, QuadF qd,
                                                                                             int split, P3MArgs a) {
+    if (UPIC_GATE_CLOSED(a.gate)) return;
     p3_map_mixed_body<R, GM, COLLECT, FUSED>(g, qd, split, a);
 }
 
@@ -926,6 +927,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 template <int R, int GM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_p3_map_mixed_collect(
     MGrid g, QuadF qd, int split, P3MArgs a) {
+    if (UPIC_GATE_CLOSED(a.gate)) return;
     p3_map_mixed_body<R, GM, true, false>(g, qd, split, a);
 }
 
@@ -936,6 +938,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 template <int R, bool COLLECT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_p3_map_retry(MGrid g, QuadF qd,
                                                                                             int split, P3MArgs a) {
+    if (UPIC_GATE_CLOSED(a.gate)) return;
     const int n = *a.retry_n;
     double rsum[2] = {0.0, 0.0}, rmax[2] = {0.0, 0.0};
     int nseg_total = 0;
@@ -966,6 +969,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // inner solve deposit nothing (the step is then rejected anyway).
 template <int R>
 __global__ __launch_bounds__(256) void k_deposit_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
+    if (UPIC_GATE_CLOSED(a.gate)) return;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ float tile_mem[kTileX * kTileY * 3];
     const DepTileF T = tilef_begin(g, tile_mem, a.cell, a.np, (int64_t)blockIdx.x * blockDim.x);
@@ -994,6 +998,7 @@ __global__ __launch_bounds__(256) void k_deposit_mixed(MGrid g, QuadF qd, int sp
 // [1] crossing count change; maxs: [0] largest new segment count.
 template <int R>
 __global__ __launch_bounds__(256) void k_deposit_delta_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
+    if (UPIC_GATE_CLOSED(a.gate)) return;
     const int n = *a.retry_n;
     __shared__ float tile_mem[kTileX * kTileY * 3];
     double sums[2] = {0.0, 0.0}, maxs[1] = {0.0};
@@ -1042,6 +1047,7 @@ constexpr int kCellWaves = 4, kCellSlices = 8, kCTR = 8, kCTC = 72;
 template <int R>
 __global__ __launch_bounds__(256) void k_deposit_cells_mixed(MGrid g, QuadF qd, int split, P3MArgs a,
                                                              const int* __restrict__ cell_end) {
+    if (UPIC_GATE_CLOSED(a.gate)) return;
     __shared__ float tile[kCTR * kCTC * 3];
     const int c0 = blockIdx.x * 64;
     const int i0 = c0 / g.ny, j0 = c0 - (c0 / g.ny) * g.ny;
@@ -1162,6 +1168,7 @@ __global__ __launch_bounds__(256) void k_deposit_cells_mixed(MGrid g, QuadF qd, 
 // orbits that cross a knot, deposit with per-lane atomics.
 template <int R>
 __global__ __launch_bounds__(256) void k_deposit_wave_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
+    if (UPIC_GATE_CLOSED(a.gate)) return;
     constexpr int NW = (R + 1) * (R + 1);
     constexpr int ROW = NW + 4;  // W (NW) | qs v (3) | pad
     __shared__ float tile_mem[kTileX * kTileY * 3];
@@ -1288,7 +1295,8 @@ __global__ __launch_bounds__(256) void k_deposit_wave_mixed(MGrid g, QuadF qd, i
 
 // FP32 node records from the FP64 fields
 __global__ void k_pack_nodes_f32(int N, const double* __restrict__ gpm, const double* __restrict__ A0,
-                                 const double* __restrict__ A1, float4* __restrict__ NR) {
+                                 const double* __restrict__ A1, float4* __restrict__ NR, const int* __restrict__ gate) {
+    if (UPIC_GATE_CLOSED(gate)) return;
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= N) return;
     NR[2 * id] = make_float4((float)gpm[id], (float)gpm[N + id], (float)A0[id], (float)A0[N + id]);
@@ -1428,7 +1436,8 @@ __global__ void k_mixed_permute_hist(int64_t np, const int* __restrict__ perm, c
     }
 }
 
-__global__ void k_zero_int(int* __restrict__ x) {
+__global__ void k_zero_int(int* __restrict__ x, const int* __restrict__ gate) {
+    if (UPIC_GATE_CLOSED(gate)) return;
     if (threadIdx.x == 0) *x = 0;
 }
 
@@ -1447,9 +1456,11 @@ __global__ __launch_bounds__(256) void k_mixed_gross(int64_t np, const float* __
 
 }  // namespace
 
-void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& a) {
-    if (a.np <= 0) return;
-    if (a.retry_n && !a.layout) hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, a.retry_n);
+void launch_p3_map_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& args) {
+    if (args.np <= 0) return;
+    P3MArgs a = args;
+    a.gate = current_gate();
+    if (a.retry_n && !a.layout) hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, a.retry_n, current_gate());
     MGrid mg;
     mg.nx = g.nx; mg.ny = g.ny; mg.N = g.N; mg.order = g.order;
     mg.hx = (float)g.h[0]; mg.hy = (float)g.h[1];
@@ -1509,9 +1520,11 @@ void launch_mixed_gross(hipStream_t s, int64_t np, const MixedState& ms, double*
     launch_finish_partials(s, part, mblk(np), 1, 0, out, nullptr, nullptr);
 }
 
-void launch_deposit_delta_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& a,
+void launch_deposit_delta_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& args,
                                 double* out) {
-    if (a.np <= 0 || !a.retry_n || !a.retry_old) return;
+    if (args.np <= 0 || !args.retry_n || !args.retry_old) return;
+    P3MArgs a = args;
+    a.gate = current_gate();
     MGrid mg;
     mg.nx = g.nx; mg.ny = g.ny; mg.N = g.N; mg.order = g.order;
     mg.hx = (float)g.h[0]; mg.hy = (float)g.h[1];
@@ -1536,9 +1549,11 @@ bool deposit_variant_usable(const GridDev& g, int variant, bool cells_fresh) {
     return true;
 }
 
-void launch_deposit_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& a,
+void launch_deposit_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int split, const P3MArgs& args,
                           const int* cell_end, int variant) {
-    if (a.np <= 0 || a.layout || !a.J_out) return;
+    if (args.np <= 0 || args.layout || !args.J_out) return;
+    P3MArgs a = args;
+    a.gate = current_gate();
     if (!deposit_variant_usable(g, variant, cell_end != nullptr)) variant = 0;
     MGrid mg;
     mg.nx = g.nx; mg.ny = g.ny; mg.N = g.N; mg.order = g.order;
@@ -1586,7 +1601,7 @@ void launch_deposit_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int s
 
 void launch_pack_nodes_f32(hipStream_t s, int N, const double* gpm, const double* A_n, const double* A_end, float* NR) {
     hipLaunchKernelGGL(k_pack_nodes_f32, dim3(mblk(N)), dim3(256), 0, s, N, gpm, A_n, A_end,
-                       reinterpret_cast<float4*>(NR));
+                       reinterpret_cast<float4*>(NR), current_gate());
 }
 
 void launch_mixed_compress(hipStream_t s, const GridDev& g, int64_t np, const double* x, const double* P,

@@ -216,6 +216,7 @@ struct DepTile {
 // LDS doubles a deposit tile needs in either mode
 constexpr int kTileLds = 2 * kTile * kTile * 3;
 
+template <bool FIX = false>
 __device__ __forceinline__ void tile_add(const GridDev& g, const DepTile& T, double* __restrict__ J, int ia, int ib,
                                          double j0, double j1, double j2) {
     int dx = ia - T.ox;
@@ -223,7 +224,7 @@ __device__ __forceinline__ void tile_add(const GridDev& g, const DepTile& T, dou
     int dy = ib - T.oy;
     if (dy < 0) dy += g.ny;
     const bool in = dx < kTile && dy < kTile;
-    if (T.fx.hi) {
+    if constexpr (FIX) {
         if (in) {
             unsigned long long* c = reinterpret_cast<unsigned long long*>(T.t) + 3 * (dx * kTile + dy);
             constexpr int lo = kTile * kTile * 3;
@@ -251,8 +252,9 @@ __device__ __forceinline__ void tile_add(const GridDev& g, const DepTile& T, dou
     }
 }
 
-// Orbit current deposit of one particle into the tile (2-D).
-template <int R>
+// Orbit current deposit of one particle into the tile (2-D); FIX: the
+// tile holds fixed-point words.
+template <int R, bool FIX = false>
 __device__ __forceinline__ int deposit_orbit_2d(const GridDev& g, const QuadDev& q, const OrbitOpt& opt,
                                                 const double x0[3], const double dxp[3], const double vel[3],
                                                 double charge, const DepTile& T, double* __restrict__ J) {
@@ -269,7 +271,7 @@ __device__ __forceinline__ int deposit_orbit_2d(const GridDev& g, const QuadDev&
             for (int b = 0; b <= R; ++b) {
                 const int ib = wrap_idx(by + b, g.ny);
                 const double w = W[a][b] * qs;
-                tile_add(g, T, J, ia, ib, w * vel[0], w * vel[1], w * vel[2]);
+                tile_add<FIX>(g, T, J, ia, ib, w * vel[0], w * vel[1], w * vel[2]);
             }
         }
         have = false;
@@ -302,7 +304,7 @@ __device__ __forceinline__ int deposit_orbit_2d(const GridDev& g, const QuadDev&
 // Tile set-up: origin two nodes below the stencil base of the workgroup's
 // first particle; zeroed; must be called by every thread of the workgroup.
 // With fx.hi set the tile takes fixed-point words (lds holds kTileLds).
-template <int R>
+template <int R, bool FIX = false>
 __device__ __forceinline__ DepTile tile_begin(const GridDev& g, double* lds, const double* __restrict__ x, int64_t np,
                                               int64_t p_first, FixAcc fx = FixAcc{}) {
     __shared__ int org[2];
@@ -320,15 +322,16 @@ __device__ __forceinline__ DepTile tile_begin(const GridDev& g, double* lds, con
         org[0] = ox;
         org[1] = oy;
     }
-    const int nz = fx.hi ? kTileLds : kTile * kTile * 3;
+    constexpr int nz = FIX ? kTileLds : kTile * kTile * 3;
     for (int i = threadIdx.x; i < nz; i += blockDim.x) lds[i] = 0.0;
     __syncthreads();
     return DepTile{lds, org[0], org[1], fx};
 }
 
+template <bool FIX = false>
 __device__ __forceinline__ void tile_end(const GridDev& g, const DepTile& T, double* __restrict__ J) {
     __syncthreads();
-    if (T.fx.hi) {
+    if constexpr (FIX) {
         const unsigned long long* w = reinterpret_cast<const unsigned long long*>(T.t);
         constexpr int lo = kTile * kTile * 3;
         for (int i = threadIdx.x; i < lo; i += blockDim.x) {

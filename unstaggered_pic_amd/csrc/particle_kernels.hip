@@ -362,13 +362,15 @@ __device__ __forceinline__ int p3_solve_substep(const GridDev& g, const QuadDev&
 // scratch (which stays in cache) but hides latency better than one
 // wavefront holding all of it in VGPRs and AGPRs (19.5 against 24.4 ms per
 // 16.8M-particle cubic sweep).
-template <int R, bool Z3>
+// FIX: the deterministic deposit (fixed-point words in a.fix).
+template <int R, bool Z3, bool FIX>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_p3_map(GridDev g, QuadDev qd,
                                                                                       OrbitOpt opt, P3Args a) {
+    if (UPIC_GATE_CLOSED(a.gate)) return;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    __shared__ double tile_mem[kTileLds];
+    __shared__ double tile_mem[FIX ? kTileLds : kTile * kTile * 3];
     DepTile T{};
-    if constexpr (!Z3) T = tile_begin<R>(g, tile_mem, a.x, a.np, (int64_t)blockIdx.x * blockDim.x, a.fix);
+    if constexpr (!Z3) T = tile_begin<R, FIX>(g, tile_mem, a.x, a.np, (int64_t)blockIdx.x * blockDim.x, a.fix);
     // red: [0] sum of local iterations, [1] failed particles; redm: [0] max
     // local iterations, [1] max final scaled residual
     double rsum[2] = {0.0, 0.0}, rmax[2] = {0.0, 0.0};
@@ -407,7 +409,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             const double dxp[3] = {h * vacc[0], h * vacc[1], h * vacc[2]};
             if (a.J_out) {
                 if constexpr (Z3) nseg_total += deposit_orbit_one<R, Z3>(g, qd, opt, xpos, dxp, vacc, qp / (double)nsub, a.J_out, a.fix);
-                else nseg_total += deposit_orbit_2d<R>(g, qd, opt, xpos, dxp, vacc, qp / (double)nsub, T, a.J_out);
+                else nseg_total += deposit_orbit_2d<R, FIX>(g, qd, opt, xpos, dxp, vacc, qp / (double)nsub, T, a.J_out);
             }
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
@@ -427,7 +429,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         }
     }
     if constexpr (!Z3) {
-        if (a.J_out) tile_end(g, T, a.J_out);
+        if (a.J_out) tile_end<FIX>(g, T, a.J_out);
     }
     block_partials<2, 2>(rsum, rmax, nseg_total, a.part);
 }
@@ -610,9 +612,14 @@ void launch_p1_push(hipStream_t s, const GridDev& g, const QuadDev& q, const Orb
         launch_finish_partials(s, a.part, nblk(a.np), 3, 0, a.red, nullptr, a.stats);
     }
 }
-void launch_p3_map(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const P3Args& a) {
-    if (a.np <= 0) return;
-    UPIC_DISPATCH(g.order, g.nz > 1, hipLaunchKernelGGL((k_p3_map<R, Z>), dim3(nblk(a.np)), dim3(256), 0, s, g, q, o, a));
+void launch_p3_map(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const P3Args& args) {
+    if (args.np <= 0) return;
+    P3Args a = args;
+    a.gate = current_gate();
+    if (a.fix.hi)
+        UPIC_DISPATCH(g.order, g.nz > 1, hipLaunchKernelGGL((k_p3_map<R, Z, true>), dim3(nblk(a.np)), dim3(256), 0, s, g, q, o, a));
+    else
+        UPIC_DISPATCH(g.order, g.nz > 1, hipLaunchKernelGGL((k_p3_map<R, Z, false>), dim3(nblk(a.np)), dim3(256), 0, s, g, q, o, a));
     launch_finish_partials(s, a.part, nblk(a.np), 2, 2, a.red, a.redmax, a.stats);
 }
 void launch_p3_local_solve(hipStream_t s, const GridDev& g, const QuadDev& q, const OrbitOpt& o, const P3Args& a,
@@ -663,7 +670,8 @@ __device__ __forceinline__ void fold_partials(const double* __restrict__ part, i
 // First stage for many blocks: workgroup g folds blocks [g per, (g+1) per)
 // into slot g of `out`.
 __global__ __launch_bounds__(256) void k_fold_partials(const double* __restrict__ part, int64_t nb, int64_t per,
-                                                       double* __restrict__ out) {
+                                                       double* __restrict__ out, const int* __restrict__ gate) {
+    if (UPIC_GATE_CLOSED(gate)) return;
     const int64_t b0 = (int64_t)blockIdx.x * per;
     double t[kPartSlots];
     fold_partials(part, b0, b0 + per < nb ? b0 + per : nb, t);
@@ -676,7 +684,9 @@ __global__ __launch_bounds__(256) void k_fold_partials(const double* __restrict_
 // the accumulators.
 __global__ __launch_bounds__(256) void k_finish_partials(const double* __restrict__ part, int64_t nb, int ns, int nm,
                                                          double* __restrict__ sum, double* __restrict__ mx,
-                                                         unsigned long long* __restrict__ stats) {
+                                                         unsigned long long* __restrict__ stats,
+                                                         const int* __restrict__ gate) {
+    if (UPIC_GATE_CLOSED(gate)) return;
     double t[kPartSlots];
     fold_partials(part, 0, nb, t);
     if (threadIdx.x == 0) {
@@ -697,14 +707,15 @@ void launch_finish_partials(hipStream_t s, const double* part, int64_t nblocks, 
     if (nblocks <= 0) return;
     constexpr int64_t kFoldMin = 2048;
     if (nblocks <= kFoldMin) {
-        hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(256), 0, s, part, nblocks, ns, nm, sum, mx, stats);
+        hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(256), 0, s, part, nblocks, ns, nm, sum, mx, stats,
+                           current_gate());
         return;
     }
     const int64_t per = (nblocks + kFoldBlocks - 1) / kFoldBlocks;
     const int64_t g = (nblocks + per - 1) / per;
     double* out = const_cast<double*>(part) + nblocks * kPartSlots;
-    hipLaunchKernelGGL(k_fold_partials, dim3((unsigned)g), dim3(256), 0, s, part, nblocks, per, out);
-    hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(256), 0, s, out, g, ns, nm, sum, mx, stats);
+    hipLaunchKernelGGL(k_fold_partials, dim3((unsigned)g), dim3(256), 0, s, part, nblocks, per, out, current_gate());
+    hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(256), 0, s, out, g, ns, nm, sum, mx, stats, current_gate());
 }
 // Step diagnostics (compute_step_diagnostics, P1:1384-1439 / P3:4501-4570)
 __global__ __launch_bounds__(256) void k_step_diag_orbit(int64_t np, int midpoint, double dt, double Lx, double Ly,
@@ -935,7 +946,8 @@ void launch_init_momenta(hipStream_t s, const GridDev& g, int64_t np, int code, 
 
 namespace upic {
 __global__ void k_fix_to_double(int64_t n, unsigned long long* __restrict__ hi, unsigned long long* __restrict__ lo,
-                                double inv_scale, double* __restrict__ out) {
+                                double inv_scale, double* __restrict__ out, const int* __restrict__ gate) {
+    if (UPIC_GATE_CLOSED(gate)) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const long long h = (long long)hi[i], l = (long long)lo[i];
@@ -946,10 +958,11 @@ __global__ void k_fix_to_double(int64_t n, unsigned long long* __restrict__ hi, 
 void launch_fix_to_double(hipStream_t s, int64_t n, const FixAcc& fx, double* out) {
     if (n > 0)
         hipLaunchKernelGGL(k_fix_to_double, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, fx.hi, fx.lo,
-                           1.0 / fx.scale, out);
+                           1.0 / fx.scale, out, current_gate());
 }
 __global__ void k_pack_nodes(int N, const double* __restrict__ gpm, const double* __restrict__ A0,
-                             const double* __restrict__ A1, double* __restrict__ NR) {
+                             const double* __restrict__ A1, double* __restrict__ NR, const int* __restrict__ gate) {
+    if (UPIC_GATE_CLOSED(gate)) return;
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= N) return;
     double2* o = reinterpret_cast<double2*>(NR + (size_t)8 * id);
@@ -959,6 +972,6 @@ __global__ void k_pack_nodes(int N, const double* __restrict__ gpm, const double
     o[3] = make_double2(A1[N + id], A1[2 * N + id]);
 }
 void launch_pack_nodes(hipStream_t s, int N, const double* gpm, const double* A_n, const double* A_end, double* NR) {
-    hipLaunchKernelGGL(k_pack_nodes, dim3((N + 255) / 256), dim3(256), 0, s, N, gpm, A_n, A_end, NR);
+    hipLaunchKernelGGL(k_pack_nodes, dim3((N + 255) / 256), dim3(256), 0, s, N, gpm, A_n, A_end, NR, current_gate());
 }
 }  // namespace upic

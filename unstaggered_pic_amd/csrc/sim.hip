@@ -7,6 +7,7 @@
 #include <atomic>
 #include <thread>
 #include "api_internal.hpp"
+#include "outer.hpp"
 #include "mlp.hpp"
 
 // map evaluations per step beyond which the later sweeps stop starting from
@@ -153,6 +154,18 @@ struct upic_sim {
     double* hcontr = nullptr;  // pinned host mirror
     int64_t sub_n = 0, sub_cap = 0;
     bool sub_valid = false;
+    // device-driven outer solve (outer.hpp): the control block, its pinned
+    // mirror, the commit gate, the vector passes' partials, the captured
+    // step graphs (by configuration) and the number of slots a step graph
+    // unrolls (adapted to the evaluations steps need)
+    OuterDev* outer = nullptr;
+    OuterDev* h_outer = nullptr;
+    int* commit_gate = nullptr;
+    double* opart = nullptr;
+    struct OuterGraph { uint64_t key; hipGraphExec_t exec; };
+    std::vector<OuterGraph> outer_graphs;
+    int outer_slots = 0;
+    bool capturing = false;  // a step is being captured (field solves are recorded inline)
     // deterministic deposit (upic_ctx_set_deterministic): the fixed-point
     // words of the current (2 x 3N) and their scale 2^s, from a bound of the
     // node sums (set at the first deterministic sweep after an upload)
@@ -208,6 +221,15 @@ void sim_free(upic_sim* s) {
     s->sub_valid = false;
     for (auto& gx : s->fs_graphs) (void)hipGraphExecDestroy(gx.second);
     s->fs_graphs.clear();
+    for (auto& gx : s->outer_graphs) (void)hipGraphExecDestroy(gx.exec);
+    s->outer_graphs.clear();
+    if (s->outer) (void)hipFree(s->outer);
+    if (s->commit_gate) (void)hipFree(s->commit_gate);
+    if (s->h_outer) (void)hipHostFree(s->h_outer);
+    s->outer = nullptr;
+    s->commit_gate = nullptr;
+    s->h_outer = nullptr;
+    dfree(s->opart);
     if (s->tuner.e0) (void)hipEventDestroy(s->tuner.e0);
     if (s->tuner.e1) (void)hipEventDestroy(s->tuner.e1);
     s->tuner.e0 = s->tuner.e1 = nullptr;
@@ -337,7 +359,7 @@ void field_solve_body(upic_sim* s, const double* Jreal) {
 // source buffer and replayed with one launch (it holds no memset nodes).
 void field_solve(upic_sim* s, const double* Jreal) {
     upic_ctx* c = s->c;
-    if (!c->graphs) { field_solve_body(s, Jreal); return; }
+    if (!c->graphs || s->capturing) { field_solve_body(s, Jreal); return; }
     for (auto& gx : s->fs_graphs)
         if (gx.first == Jreal) {
             HIPCHK(hipGraphLaunch(gx.second, c->stream));
@@ -654,8 +676,14 @@ FixAcc fixed_point_target(upic_sim* s) {
 
 // evaluate_selective_current_map_at_layout (P3:5235-5431): guess (3N) ->
 // J_map (3N).  write_out stores the particle end state in xo/Po/vo/vbo.
-MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bool use_layout,
-              Norms* norms = nullptr) {
+// The launches of a map evaluation (no synchronisation); the host flags of
+// the step (vdep_valid, jdep_valid, a0_valid) are left as the evaluation
+// will leave them once complete.  Returns what p3_map_finish needs.
+struct MapLaunch {
+    bool incremental = false, a0_written = false;
+};
+MapLaunch p3_map_launch(upic_sim* s, const double* guess, double* Jmap, bool write_out, bool use_layout,
+                        bool norms) {
     upic_ctx* c = s->c;
     const upic_params& p = c->p;
     const int N = c->N();
@@ -768,6 +796,22 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     HIPCHK(hipGetLastError());
     // the residual of this map against its guess, read in the same synchronisation
     if (norms) residual_norms_launch(s, Jmap, guess, false);
+    if (s->mixed && !use_layout) {
+        s->jdep_valid = true;
+        if (a0_written) s->a0_valid = true;
+        s->vdep_valid = true;
+    }
+    MapLaunch ml;
+    ml.incremental = incremental;
+    ml.a0_written = a0_written;
+    return ml;
+}
+
+// The evaluation's results after its synchronisation: predictor check,
+// norms, path statistics (the kept deposit's, moved by an incremental
+// deposit), the autotuner's timing, failed particles (RefineNeeded).
+MapOut p3_map_finish(upic_sim* s, const MapLaunch& ml, bool use_layout, Norms* norms) {
+    upic_ctx* c = s->c;
     read_red(c);
     if (s->predictor_check) {  // queued with the step's predictor, read in this synchronisation
         s->predictor_check = false;
@@ -775,17 +819,14 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
     }
     if (norms) *norms = residual_norms_finish(s);
     if (s->mixed && !use_layout) {
-        if (incremental) {  // path statistics of the kept deposit, moved by the changed particles
+        if (ml.incremental) {  // path statistics of the kept deposit, moved by the changed particles
             c->hstats[0] = (unsigned long long)((double)s->jdep_stats[0] + c->hred[5]);
             c->hstats[1] = (unsigned long long)((double)s->jdep_stats[1] + c->hred[4]);
             c->hstats[2] = std::max<unsigned long long>(s->jdep_stats[2], (unsigned long long)c->hred[6]);
         }
         for (int k = 0; k < 3; ++k) s->jdep_stats[k] = c->hstats[k];
-        s->jdep_valid = true;
-        if (a0_written) s->a0_valid = true;
     }
     if (s->mixed) deposit_record(s);
-    if (s->mixed && !use_layout) s->vdep_valid = true;
     MapOut o;
     o.iter_sum = c->hred[0];
     o.failed = (int64_t)c->hred[1];
@@ -801,6 +842,12 @@ MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bo
         throw RefineNeeded("selective local particle solve failed for " + std::to_string(f.size()) + " particles", f);
     }
     return o;
+}
+
+MapOut p3_map(upic_sim* s, const double* guess, double* Jmap, bool write_out, bool use_layout,
+              Norms* norms = nullptr) {
+    const MapLaunch ml = p3_map_launch(s, guess, Jmap, write_out, use_layout, norms != nullptr);
+    return p3_map_finish(s, ml, use_layout, norms);
 }
 
 bool finite_vec(upic_sim* s, const double* v) {
@@ -1156,6 +1203,280 @@ void p3_step_layout(upic_sim* s, bool use_layout, upic_step_info* info) {
     finish_contraction();
 }
 
+// ---------------------------------------------------------------------------
+// Device-driven outer solve (outer.hpp): the uniform-layout step of
+// p3_step_layout as one captured graph -- init, a fixed number of slots
+// (controller passes + a gated map evaluation each) and the gated commit --
+// replayed with one launch and one synchronisation.  The controller makes the
+// host loop's decisions in the same order; a step it does not finish within
+// the unrolled slots continues slot by slot, and a step it reports failed
+// (particle failures, backtracking, nonfinite currents) is run again from
+// its start by the host loop, which reproduces the failure and its handling.
+// ---------------------------------------------------------------------------
+constexpr int kOuterSlotsMin = 3, kOuterSlotsMax = 12, kOuterGraphCache = 8;
+
+bool device_outer_ok(upic_sim* s) {
+    upic_ctx* c = s->c;
+    const upic_params& p = c->p;
+    if (!c->graphs || c->timing || p.code != 3) return false;
+    if (p.anderson_memory < 0 || p.anderson_memory > kOuterMaxMemory || p.nonlinear_max_iter < 1) return false;
+    if (s->train_stride > 0 && s->train_n < s->train_cap) return false;  // training rows are collected eagerly
+    if (s->mixed) {
+        const auto& t = s->tuner;
+        if (t.forced < 0 && t.chosen[s->cells_fresh ? 1 : 0] < 0) return false;  // the deposit autotuner is still timing
+    }
+    return true;
+}
+
+OuterCfg outer_cfg(upic_sim* s) {
+    upic_ctx* c = s->c;
+    const upic_params& p = c->p;
+    OuterCfg cf{};
+    cf.picard = p.nonlinear_solver_picard != 0;
+    cf.anderson_start = p.anderson_start;
+    cf.memory = p.anderson_memory;
+    cf.max_iter = p.nonlinear_max_iter;
+    cf.fast = s->mixed ? 1 : 0;
+    cf.damping = p.anderson_damping;
+    cf.growth = p.anderson_growth_limit;
+    cf.reg = p.anderson_regularization;
+    cf.rtol = p.nonlinear_rtol;
+    cf.atol = p.nonlinear_atol;
+    cf.floor_ = norms_floor(p);
+    cf.slack = s->mixed ? kMixedFinalSlack : 1.0;
+    cf.cnt = 3.0 * (double)c->N();
+    cf.atol_dev = s->atol_dev ? c->dred + kRedAtol : nullptr;
+    cf.predictor = c->dred + kRedPredictor;
+    cf.red = c->dred;
+    cf.dstats = c->dstats;
+    cf.contr = s->contr;
+    return cf;
+}
+
+OuterVecs outer_vecs(upic_sim* s) {
+    OuterVecs v{};
+    v.n = 3ll * s->c->N();
+    v.guess = s->guess; v.trial = s->trial; v.resid = s->resid; v.cand = s->cand; v.Jacc = s->Jacc; v.Jtry = s->Jtry;
+    for (int i = 0; i < 9; ++i) { v.xh[i] = s->xh[i]; v.rh[i] = s->rh[i]; }
+    for (int i = 0; i < kOuterMaxMemory; ++i) { v.dX[i] = s->dX[i]; v.dR[i] = s->dR[i]; }
+    v.part = s->opart;
+    return v;
+}
+
+// Launches of `slots` slots (after the init when with_init) and the gated
+// commit.  The evaluations' configurations follow the host flags of the step
+// (the first evaluation's, then the later ones'), as the host loop's would.
+void enqueue_outer(upic_sim* s, int slots, bool with_init) {
+    upic_ctx* c = s->c;
+    const upic_params& p = c->p;
+    const int N = c->N();
+    const OuterCfg cf = outer_cfg(s);
+    const OuterVecs v = outer_vecs(s);
+    const double damping = cf.picard ? 1.0 : p.anderson_damping;
+    if (with_init) launch_outer_init(c->stream, s->outer, cf.picard ? UPIC_UPDATE_PICARD : UPIC_UPDATE_PICARD_FALLBACK);
+    for (int k = 0; k < slots; ++k) {
+        launch_outer_prepare(c->stream, s->outer, cf, v, damping);
+        MapLaunch ml;
+        {
+            const GateScope gs(&s->outer->gate);
+            ml = p3_map_launch(s, s->trial, s->Jtry, true, false, true);
+        }
+        launch_outer_post(c->stream, s->outer, cf, ml.incremental ? 1 : 0);
+    }
+    launch_outer_commit_gate(c->stream, s->outer, s->commit_gate);
+    const GateScope gs(s->commit_gate);
+    if (s->mixed) launch_copy(c->stream, 3ll * N, s->J, s->Jprev);
+    commit_fields(s, s->Jtry);
+    HIPCHK(hipGetLastError());
+}
+
+uint64_t outer_key(upic_sim* s, int slots) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](uint64_t x) {
+        for (int b = 0; b < 8; ++b) {
+            h ^= (x >> (8 * b)) & 0xffu;
+            h *= 1099511628211ull;
+        }
+    };
+    auto ptr = [&](const void* q) { mix((uint64_t)(uintptr_t)q); };
+    const MixedState* ms[2] = {&s->ms, &s->mo};
+    for (const MixedState* m : ms) {
+        ptr(m->cell); ptr(m->off); ptr(m->P); ptr(m->v); ptr(m->vp); ptr(m->vb); ptr(m->q); ptr(m->m);
+    }
+    const void* ps[] = {s->x, s->v, s->v_prev, s->P, s->vbar_last, s->q, s->m, s->xo, s->vo, s->Po, s->vbo,
+                        s->guess, s->trial, s->Jacc, s->Jtry, s->resid, s->cand, s->Jfield, s->J, s->Jprev,
+                        s->Jdep, s->vdep, s->a0, s->A, s->AG, s->NR, s->NRf, s->S0, s->S1, s->zs, s->gtmp,
+                        s->offsets, s->mlp_blob, s->fix_hi};
+    for (const void* q : ps) ptr(q);
+    const int order = s->cells_fresh ? 1 : 0;
+    const int variant = s->tuner.forced >= 0 ? s->tuner.forced : s->tuner.chosen[order];
+    const int64_t flags[] = {slots, s->mixed, s->guess_mode, s->mlp_ready, s->vdep_off_steps == 0, s->cells_fresh,
+                             variant, s->c->det, s->atol_dev};
+    for (int64_t f : flags) mix((uint64_t)f);
+    return h;
+}
+
+// Runs the step's outer solve on the device.  Returns false (nothing
+// committed, the step's host flags reset) when the host loop must run it.
+bool p3_step_device(upic_sim* s, upic_step_info* info) {
+    upic_ctx* c = s->c;
+    const upic_params& p = c->p;
+    const int N = c->N();
+    const int64_t n3 = 3ll * N;
+    const bool fast = s->mixed;
+    if (!s->outer) {
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&s->outer), sizeof(OuterDev)));
+        HIPCHK(hipMalloc(reinterpret_cast<void**>(&s->commit_gate), sizeof(int)));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&s->h_outer), sizeof(OuterDev), hipHostMallocDefault));
+        s->opart = dalloc<double>((size_t)kOuterPassBlocks * kOuterPartSlots);
+    }
+    if (!s->contr) {
+        s->contr = dalloc<double>(2 * (size_t)std::max(1, p.nonlinear_max_iter));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&s->hcontr), 2 * (size_t)std::max(1, p.nonlinear_max_iter) *
+                             sizeof(double), hipHostMallocDefault));
+    }
+    if (c->det) (void)fixed_point_target(s);  // its scale is read once, outside the capture
+    if (c->graph_slots > 0) s->outer_slots = c->graph_slots;
+    else if (s->outer_slots == 0) s->outer_slots = fast ? 5 : 8;
+    // the initial macro-current predictor (as p3_step_layout) and its norms
+    if (fast && s->jprev_valid) {
+        LinComb le{2, {2.0, -1.0}, {s->J, s->Jprev}};
+        launch_lincomb(c->stream, n3, le, s->guess);
+    } else {
+        launch_copy(c->stream, n3, s->J, s->guess);
+    }
+    launch_zero(c->stream, 6, c->dred + kRedPredictor);
+    launch_residual_norms(c->stream, N, s->guess, s->guess, norms_floor(p), p.nonlinear_rtol, p.nonlinear_atol,
+                          c->dred + kRedPredictor);
+    launch_zero(c->stream, 2ll * std::max(1, p.nonlinear_max_iter), s->contr);
+    // the step graph of this configuration
+    const int slots = s->outer_slots;
+    const uint64_t key = outer_key(s, slots);
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : s->outer_graphs)
+        if (g.key == key) exec = g.exec;
+    s->vdep_valid = s->jdep_valid = s->a0_valid = false;
+    if (!exec) {
+        hipGraph_t gph = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        s->capturing = true;
+        try {
+            enqueue_outer(s, slots, true);
+        } catch (...) {
+            s->capturing = false;
+            hipGraph_t junk = nullptr;
+            (void)hipStreamEndCapture(c->stream, &junk);
+            if (junk) (void)hipGraphDestroy(junk);
+            throw;
+        }
+        s->capturing = false;
+        HIPCHK(hipStreamEndCapture(c->stream, &gph));
+        const hipError_t e = hipGraphInstantiate(&exec, gph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gph);
+        HIPCHK(e);
+        if ((int)s->outer_graphs.size() >= kOuterGraphCache) {
+            (void)hipGraphExecDestroy(s->outer_graphs.front().exec);
+            s->outer_graphs.erase(s->outer_graphs.begin());
+        }
+        s->outer_graphs.push_back({key, exec});
+    }
+    HIPCHK(hipGraphLaunch(exec, c->stream));
+    // the host flags as the graph's first evaluation leaves them, for any
+    // slots launched after it
+    s->vdep_valid = s->jdep_valid = fast;
+    s->a0_valid = fast && (s->guess_mode == 1 || (s->guess_mode == 2 && s->mlp_ready));
+    auto read = [&] {
+        HIPCHK(hipMemcpyAsync(s->h_outer, s->outer, sizeof(OuterDev), hipMemcpyDeviceToHost, c->stream));
+        c->sync();
+    };
+    read();
+    // a step that needs more evaluations than the graph unrolls: two slots
+    // at a time (eagerly)
+    int extra = 0;
+    while (s->h_outer->status == kOuterRunning && extra < 4 * std::max(1, p.nonlinear_max_iter)) {
+        enqueue_outer(s, 2, false);
+        extra += 2;
+        read();
+    }
+    const OuterDev& o = *s->h_outer;
+    if (o.status != kOuterConverged) {
+        s->vdep_valid = s->jdep_valid = s->a0_valid = false;
+        s->step_maps = 0;
+        s->vdep_used = false;
+        return false;
+    }
+    // slots of the next steps' graphs: grow to this step's evaluations and
+    // one spare at once, shrink by one when two or more went unused
+    int next = slots;
+    if (o.maps + 1 > slots) next = o.maps + 1;
+    else if (o.maps + 2 < slots) next = slots - 1;
+    if (c->graph_slots == 0) s->outer_slots = std::min(kOuterSlotsMax, std::max(kOuterSlotsMin, next));
+    s->step_maps = o.maps;
+    s->vdep_used = fast && s->guess_mode >= 1 && s->vdep_off_steps == 0 && o.maps > 1;
+    // contraction sums of the iterations (estimated_map_contraction, P3:5983-5996)
+    const int iters = o.iters;
+    HIPCHK(hipMemcpy(s->hcontr, s->contr, 2 * (size_t)iters * sizeof(double), hipMemcpyDeviceToHost));
+    double contraction = std::numeric_limits<double>::infinity();
+    for (int k = 1; k < iters; ++k) {
+        const double dx = std::sqrt(s->hcontr[2 * k] / (double)n3), dg = std::sqrt(s->hcontr[2 * k + 1] / (double)n3);
+        if (dx > 64.0 * std::numeric_limits<double>::epsilon()) contraction = dg / dx;
+    }
+    const OuterNorms& fn = o.conv;
+    double observed = std::numeric_limits<double>::infinity();
+    if (iters >= 2 && o.first_rms > 0.0 && fn.rms_rel >= 0.0 && std::isfinite(fn.rms_rel)) {
+        const double ratio = fn.rms_rel / o.first_rms;
+        observed = fn.rms_rel == 0.0 ? 0.0
+                   : ((ratio > 0.0 && std::isfinite(ratio)) ? std::pow(ratio, 1.0 / (double)(iters - 1))
+                                                            : std::numeric_limits<double>::infinity());
+    }
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->iterations = iters;
+        info->converged = 1;
+        info->anderson_restarts = o.restarts;
+        info->anderson_steps = o.and_steps;
+        info->picard_fallback_steps = o.pic_steps;
+        info->backtracked_steps = o.backtracked;
+        info->map_evaluations = o.maps;
+        info->minimum_update_alpha = o.min_alpha;
+        info->final_rms_residual = fn.rms_rel;
+        info->final_max_particle_residual = fn.max_rel;
+        info->final_absolute_rms_residual = fn.rms_abs;
+        info->estimated_map_contraction = contraction;
+        info->local_iterations_max = (int32_t)o.iter_max;
+        info->local_iterations_mean = s->np ? o.iter_sum / (double)s->np : 0.0;
+        info->local_scaled_residual_max = o.scaled_max;
+        info->local_iterations_total = (int64_t)o.local_total;
+        upic_path_stats& ps = info->path;
+        ps.n_particles = s->np;
+        ps.crossing_particles = (int64_t)o.stats[0];
+        ps.total_segments = (int64_t)o.stats[1];
+        ps.max_segments = std::max<int32_t>(1, (int32_t)o.stats[2]);
+        info->update_method = o.last_method;
+        info->observed_convergence_rate = observed;
+    }
+    // the commit's kernels ran in the graph (gated); the particle state
+    // moves by pointer
+    if (fast) {
+        std::swap(s->ms.vp, s->ms.v);
+        std::swap(s->ms.v, s->mo.v);
+        std::swap(s->ms.cell, s->mo.cell);
+        std::swap(s->ms.off, s->mo.off);
+        std::swap(s->ms.P, s->mo.P);
+        std::swap(s->ms.vb, s->mo.vb);
+        s->hist_perm_pending = false;
+        s->fp64_stale = true;
+        s->jprev_valid = true;
+    } else {
+        std::swap(s->v_prev, s->v);
+        std::swap(s->v, s->vo);
+        std::swap(s->x, s->xo);
+        std::swap(s->P, s->Po);
+        std::swap(s->vbar_last, s->vbo);
+    }
+    return true;
+}
+
 // Uploads a per-particle substep layout (stored order) with its substep
 // offsets, sizes the substep-path buffers and records the layout's
 // statistics (P3:5248-5263, 6172-6184).
@@ -1221,7 +1542,7 @@ void p3_step(upic_sim* s, upic_step_info* info) {
     s->refine.total_substeps = s->np;
     s->refine.levels[0] = s->np;
     try {
-        p3_step_layout(s, false, info);
+        if (!(device_outer_ok(s) && p3_step_device(s, info))) p3_step_layout(s, false, info);
     } catch (const RefineNeeded& e) {
         if (!c->p.selective_particle_fallback)
             throw Error(UPIC_ENONLINEAR, std::string("inner particle solve failed over full dt; time and dt unchanged: ") + e.what());

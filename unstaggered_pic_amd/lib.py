@@ -181,7 +181,9 @@ class Context:
         self._check(lib().upic_ctx_set_timing(self.h, int(on)))
 
     def set_graphs(self, on=True):
-        """Replay each map evaluation's field solve from a captured hipGraph."""
+        """hipGraph replay: a uniform P3 step's whole outer solve as one graph
+        (device-controlled, gated map evaluations), else the field solves.
+        on: False/0 off, True/1 on, n >= 2 on with n unrolled evaluations."""
         self._check(lib().upic_ctx_set_graphs(self.h, int(on)))
 
     def set_deterministic(self, on=True):
