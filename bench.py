@@ -496,6 +496,10 @@ def main():
     ap.add_argument("--nx", type=int, default=None)
     ap.add_argument("--ppc", type=int, default=None)
     ap.add_argument("--members", type=int, default=16, help="ensemble members (--form ensemble)")
+    ap.add_argument("--baseline-steps", type=int, default=20,
+                    help="timed steps of the sequential and threaded ensemble baselines (--form ensemble)")
+    ap.add_argument("--fp64-steps", type=int, default=10,
+                    help="steps of the per-member energy comparison against FP64 (--form ensemble)")
     ap.add_argument("--form", choices=["all", "both", "fp64", "mixed", "predictor", "learned", "ensemble"],
                     default="all", help="all: the FP64 and three mixed legs, the P1 leg and the CPU legs; "
                                         "both: FP64 and mixed + learned; value is the last P3 leg")

@@ -169,7 +169,10 @@ typedef struct upic_step_info {
     int32_t selective_fallback_used;         /* 1 = the step needed the particle refinement  */
     int32_t direct_attempt_iterations;       /* outer iterations of the failed full-dt try   */
     int32_t selective_refinement_restarts;   /* substep doublings before the accepted layout */
-    int32_t reserved0;
+    int32_t graph_launch;                    /* no reference counterpart: 0 = host-driven outer
+                                                loop, 1 = the step's own device-driven graph,
+                                                2 = one graph launch for the whole ensemble
+                                                (upic_ctx_set_graphs, upic_ensemble_step)       */
     int64_t selective_refined_particles;     /* particles with more than one substep         */
     int64_t selective_total_particle_substeps;
     double selective_work_ratio;             /* total substeps / particles                    */
@@ -470,11 +473,13 @@ upic_status upic_sim_guess_params(upic_sim* sim, float* params);          /* UPI
 upic_status upic_sim_set_guess_params(upic_sim* sim, const float* params); /* and in (installs the network) */
 /* ---- Deposit autotuner (mixed precision) --------------------------------- */
 /* No reference counterpart (the reference deposits inside its per-particle
- * loop, P3:5380-5409).  The mixed-precision current deposit has four
+ * loop, P3:5380-5409).  The mixed-precision current deposit has six
  * variants: 0 particle order with a wavefront reduction of the common
- * stencil window, 1 one workgroup per 64 cells, 2 eight workgroups per 64
- * cells, 3 particle order with per-particle atomics; 1 and 2 need a freshly
- * cell-sorted particle order and ny % 64 == 0.  By default (variant -1) the
+ * stencil window into an 8 x 32-node LDS tile, 1 one workgroup per 64
+ * cells, 2 eight workgroups per 64 cells, 3 particle order with
+ * per-particle atomics, 4 and 5 the wavefront reduction into 4 x 64 and
+ * 16 x 16-node tiles; 1 and 2 need a freshly cell-sorted particle order and
+ * ny % 64 == 0.  By default (variant -1) the
  * first map evaluations time every usable variant twice, one variant per
  * evaluation, and then keep the fastest, separately for sorted and unsorted
  * particle orders.  A variant >= 0 pins it (an unusable pin runs variant 0).

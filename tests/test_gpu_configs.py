@@ -289,11 +289,16 @@ def test_c5_learned_guess_vs_oracle(orc):
     assert learned < plain
 
 
-def test_c4_ensemble_members_vs_oracle(orc):
+@pytest.mark.parametrize("graphs", [0, 1])
+def test_c4_ensemble_members_vs_oracle(orc, graphs):
     """An uneven ensemble (Landau and Weibel members, grid sides 16 and 32,
     4 to 16 particles per cell) advanced together by upic_ensemble_step in
     the mixed form with the predictor guess and the selective refinement on;
-    each member against its own FP64 oracle run over 5 steps (mixed bounds)."""
+    each member against its own FP64 oracle run over 5 steps (graphs 0,
+    host threads) or 12 steps (graphs 1: once the deposit autotuners have
+    timed their variants -- one timing per step, in the step's first
+    evaluation -- every step of the whole ensemble is one graph launch),
+    mixed bounds."""
     lib = gpu()
     members = []
     for kind, side, ppc, vth, amp, seed in (("landau", 16, 8, 0.05, 5e-3, 3), ("weibel", 32, 4, 0.2, 0.0, 4),
@@ -310,8 +315,10 @@ def test_c4_ensemble_members_vs_oracle(orc):
         pm = with_mixed_precision(p64)
         pm.learned_guess = 1
         ctx, sim = upload(lib, pm, st)
+        ctx.set_graphs(graphs)
         members.append((p64, st, ctx, sim))
-    steps = 5
+    steps = 12 if graphs else 5
+    print(f"c4 ensemble: graphs {graphs}, {steps} steps", flush=True)
     rows = [[m[3].diagnostics(1)] for m in members]
     infos = [[] for _ in members]
     for _ in range(steps):
@@ -319,6 +326,8 @@ def test_c4_ensemble_members_vs_oracle(orc):
             infos[k].append(i)
         for k, m in enumerate(members):
             rows[k].append(m[3].diagnostics(1))
+    if graphs:  # the last steps ran as one launch for the whole ensemble
+        assert all(i.graph_launch == 2 for inf in infos for i in inf[-3:])
     for k, (p64, st, ctx, sim) in enumerate(members):
         _, orows = orc.run(p64, st, steps)
         check_mixed_vs_oracle(rows[k], infos[k], orows, f"member {k}")

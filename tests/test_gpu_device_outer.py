@@ -85,6 +85,8 @@ def test_fp64_device_steps_match_oracle_and_host_loop(orc, solver, graphs):
         assert abs(rows[k].field_energy - orows[k, 2]) / abs(orows[k, 2]) < 1e-10, k
     assert [i.iterations for i in infos] == [int(r) for r in orows[1:, 6]]
     assert [i.map_evaluations for i in infos] == [int(r) for r in orows[1:, 18]]
+    assert all(i.graph_launch == 1 for i in infos)
+    assert all(h.graph_launch == 0 for h in host)
     for d, h in zip(infos, host):
         for f in INFO_EXACT:
             assert getattr(d, f) == getattr(h, f), f

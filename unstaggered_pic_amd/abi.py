@@ -58,7 +58,7 @@ class StepInfo(C.Structure):
         ("path", PathStats),
         ("update_method", C.c_int32), ("uniform_substeps_used", C.c_int32),
         ("selective_fallback_used", C.c_int32), ("direct_attempt_iterations", C.c_int32),
-        ("selective_refinement_restarts", C.c_int32), ("reserved0", C.c_int32),
+        ("selective_refinement_restarts", C.c_int32), ("graph_launch", C.c_int32),
         ("selective_refined_particles", C.c_int64), ("selective_total_particle_substeps", C.c_int64),
         ("selective_work_ratio", C.c_double), ("selective_level_counts", C.c_int64 * 7),
         ("observed_convergence_rate", C.c_double),

@@ -380,9 +380,13 @@ __device__ __forceinline__ int orbit_gg_f(const MGrid& g, const QuadF& q, bool s
 // workgroup whose particles are spread over several cells (few particles per
 // cell, or the group of knot-crossing orbits) covers a run of cells along y.
 constexpr int kTileX = 8, kTileY = 32;
+// The tile's shape (tx x ty nodes, tx ty = kTileX kTileY) is a launch
+// parameter of the wave deposit, which the deposit autotuner varies; an
+// empty tile (tx = 0) sends everything to the mesh.
 struct DepTileF {
     float* t;
     int ox, oy;
+    int tx, ty;
 };
 
 __device__ __forceinline__ void tilef_add(const MGrid& g, const DepTileF& T, double* __restrict__ J, int ia, int ib,
@@ -391,8 +395,8 @@ __device__ __forceinline__ void tilef_add(const MGrid& g, const DepTileF& T, dou
     if (dx < 0) dx += g.nx;
     int dy = ib - T.oy;
     if (dy < 0) dy += g.ny;
-    if (dx < kTileX && dy < kTileY) {
-        float* c = T.t + 3 * (dx * kTileY + dy);
+    if (dx < T.tx && dy < T.ty) {
+        float* c = T.t + 3 * (dx * T.ty + dy);
         atomicAdd(c, j0);
         atomicAdd(c + 1, j1);
         atomicAdd(c + 2, j2);
@@ -411,7 +415,7 @@ __device__ __forceinline__ void tilef_add1(const MGrid& g, const DepTileF& T, do
     if (dx < 0) dx += g.nx;
     int dy = ib - T.oy;
     if (dy < 0) dy += g.ny;
-    if (dx < kTileX && dy < kTileY) atomicAdd(T.t + 3 * (dx * kTileY + dy) + comp, v);
+    if (dx < T.tx && dy < T.ty) atomicAdd(T.t + 3 * (dx * T.ty + dy) + comp, v);
     else unsafeAtomicAdd(&J[(size_t)comp * g.N + ia * g.ny + ib], (double)v);
 }
 
@@ -479,14 +483,17 @@ __device__ __forceinline__ int deposit_orbit_f(const MGrid& g, const QuadF& q, b
     return nseg;
 }
 
+// Tile set-up: origin tx/2 - 1 cells before the workgroup's first particle
+// cell in x and 3 before it in y (the particles of a workgroup run forward
+// along y); zeroed; must be called by every thread of the workgroup.
 __device__ __forceinline__ DepTileF tilef_begin(const MGrid& g, float* lds, const int* __restrict__ cell, int64_t np,
-                                                int64_t p_first) {
+                                                int64_t p_first, int tx = kTileX, int ty = kTileY) {
     __shared__ int org[2];
     if (threadIdx.x == 0) {
         int ox = 0, oy = 0;
         if (p_first < np) {
             const int c = cell[p_first];
-            ox = wrapn(c / g.ny - 3, g.nx);
+            ox = wrapn(c / g.ny - (tx / 2 - 1), g.nx);
             oy = wrapn(c % g.ny - 3, g.ny);
         }
         org[0] = ox;
@@ -494,16 +501,16 @@ __device__ __forceinline__ DepTileF tilef_begin(const MGrid& g, float* lds, cons
     }
     for (int i = threadIdx.x; i < kTileX * kTileY * 3; i += blockDim.x) lds[i] = 0.0f;
     __syncthreads();
-    return DepTileF{lds, org[0], org[1]};
+    return DepTileF{lds, org[0], org[1], tx, ty};
 }
 
 __device__ __forceinline__ void tilef_end(const MGrid& g, const DepTileF& T, double* __restrict__ J) {
     __syncthreads();
-    for (int i = threadIdx.x; i < kTileX * kTileY * 3; i += blockDim.x) {
+    for (int i = threadIdx.x; i < T.tx * T.ty * 3; i += blockDim.x) {
         const float v = T.t[i];
         if (v != 0.0f) {
             const int comp = i % 3, cell = i / 3;
-            const int dx = cell / kTileY, dy = cell % kTileY;
+            const int dx = cell / T.ty, dy = cell % T.ty;
             if (dx >= g.nx || dy >= g.ny) continue;
             const int ia = (T.ox + dx) % g.nx, ib = (T.oy + dy) % g.ny;
             unsafeAtomicAdd(&J[(size_t)comp * g.N + ia * g.ny + ib], (double)v);
@@ -1167,14 +1174,14 @@ __global__ __launch_bounds__(256) void k_deposit_cells_mixed(MGrid g, QuadF qd, 
 // window differs from the wave's leading lane, and the earlier windows of
 // orbits that cross a knot, deposit with per-lane atomics.
 template <int R>
-__global__ __launch_bounds__(256) void k_deposit_wave_mixed(MGrid g, QuadF qd, int split, P3MArgs a) {
+__global__ __launch_bounds__(256) void k_deposit_wave_mixed(MGrid g, QuadF qd, int split, P3MArgs a, int tx, int ty) {
     if (UPIC_GATE_CLOSED(a.gate)) return;
     constexpr int NW = (R + 1) * (R + 1);
     constexpr int ROW = NW + 4;  // W (NW) | qs v (3) | pad
     __shared__ float tile_mem[kTileX * kTileY * 3];
     __shared__ float red[4][64 * ROW];
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const DepTileF T = tilef_begin(g, tile_mem, a.cell, a.np, (int64_t)blockIdx.x * blockDim.x);
+    const DepTileF T = tilef_begin(g, tile_mem, a.cell, a.np, (int64_t)blockIdx.x * blockDim.x, tx, ty);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     float W[R + 1][R + 1];
 #pragma unroll
@@ -1543,7 +1550,10 @@ void launch_deposit_delta_mixed(hipStream_t s, const GridDev& g, const QuadF& q,
     launch_finish_partials(s, a.part, nb, 2, 1, out, out + 2, nullptr);
 }
 
-int deposit_variants() { return 4; }
+// 0: wavefront-reduced deposit on an 8 x 32 tile, 1 and 2: cell-mapped
+// deposits (1 or kCellSlices workgroups per 64 cells), 3: per-lane atomics,
+// 4 and 5: the wavefront-reduced deposit on 4 x 64 and 16 x 16 tiles
+int deposit_variants() { return 6; }
 bool deposit_variant_usable(const GridDev& g, int variant, bool cells_fresh) {
     if (variant == 1 || variant == 2) return cells_fresh && g.ny % 64 == 0 && g.nx >= kCTR;
     return true;
@@ -1559,13 +1569,15 @@ void launch_deposit_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int s
     mg.nx = g.nx; mg.ny = g.ny; mg.N = g.N; mg.order = g.order;
     mg.hx = (float)g.h[0]; mg.hy = (float)g.h[1];
     mg.ihx = (float)(1.0 / g.h[0]); mg.ihy = (float)(1.0 / g.h[1]);
-    if (variant == 0) {
+    if (variant == 0 || variant == 4 || variant == 5) {
         const dim3 gr(mblk(a.np)), bl(256);
+        const int tx = variant == 4 ? 4 : variant == 5 ? 16 : kTileX;
+        const int ty = kTileX * kTileY / tx;
         switch (g.order) {
-            case 1: hipLaunchKernelGGL(k_deposit_wave_mixed<1>, gr, bl, 0, s, mg, q, split, a); break;
-            case 2: hipLaunchKernelGGL(k_deposit_wave_mixed<2>, gr, bl, 0, s, mg, q, split, a); break;
-            case 3: hipLaunchKernelGGL(k_deposit_wave_mixed<3>, gr, bl, 0, s, mg, q, split, a); break;
-            case 4: hipLaunchKernelGGL(k_deposit_wave_mixed<4>, gr, bl, 0, s, mg, q, split, a); break;
+            case 1: hipLaunchKernelGGL(k_deposit_wave_mixed<1>, gr, bl, 0, s, mg, q, split, a, tx, ty); break;
+            case 2: hipLaunchKernelGGL(k_deposit_wave_mixed<2>, gr, bl, 0, s, mg, q, split, a, tx, ty); break;
+            case 3: hipLaunchKernelGGL(k_deposit_wave_mixed<3>, gr, bl, 0, s, mg, q, split, a, tx, ty); break;
+            case 4: hipLaunchKernelGGL(k_deposit_wave_mixed<4>, gr, bl, 0, s, mg, q, split, a, tx, ty); break;
             default: break;
         }
         launch_finish_partials(s, a.part, mblk(a.np), 0, 0, nullptr, nullptr, a.stats);

@@ -5,6 +5,7 @@
 //   P2 run_simulation step body    (Project2/spectral_pic_part3.cpp:1133-1189)
 // and the per-step diagnostics (P1:1057-1087, P3:3521-3574).
 #include <atomic>
+#include <mutex>
 #include <thread>
 #include "api_internal.hpp"
 #include "outer.hpp"
@@ -135,8 +136,9 @@ struct upic_sim {
         static constexpr int kTuneReps = 2;
         int forced = -1;
         int chosen[2] = {-1, -1};
-        int count[2][4] = {};
-        float best_ms[2][4] = {};
+        static constexpr int kMaxVariants = 8;
+        int count[2][kMaxVariants] = {};
+        float best_ms[2][kMaxVariants] = {};
         int pending = -1, pending_order = 0;
         hipEvent_t e0 = nullptr, e1 = nullptr;
     } tuner;
@@ -162,10 +164,12 @@ struct upic_sim {
     OuterDev* h_outer = nullptr;
     int* commit_gate = nullptr;
     double* opart = nullptr;
-    struct OuterGraph { uint64_t key; hipGraphExec_t exec; };
+    struct OuterGraph { uint64_t key; hipGraph_t graph; hipGraphExec_t exec; };
     std::vector<OuterGraph> outer_graphs;
-    int outer_slots = 0;
+    int outer_slots = 0, outer_step_slots = 0;
     bool capturing = false;  // a step is being captured (field solves are recorded inline)
+    hipEvent_t ens_ev = nullptr;  // orders the member's stream and an ensemble launch
+    uint64_t serial = 0;          // unique per simulation of the process (ensemble graph keys)
     // deterministic deposit (upic_ctx_set_deterministic): the fixed-point
     // words of the current (2 x 3N) and their scale 2^s, from a bound of the
     // node sums (set at the first deterministic sweep after an upload)
@@ -221,11 +225,16 @@ void sim_free(upic_sim* s) {
     s->sub_valid = false;
     for (auto& gx : s->fs_graphs) (void)hipGraphExecDestroy(gx.second);
     s->fs_graphs.clear();
-    for (auto& gx : s->outer_graphs) (void)hipGraphExecDestroy(gx.exec);
+    for (auto& gx : s->outer_graphs) {
+        (void)hipGraphExecDestroy(gx.exec);
+        (void)hipGraphDestroy(gx.graph);
+    }
     s->outer_graphs.clear();
     if (s->outer) (void)hipFree(s->outer);
     if (s->commit_gate) (void)hipFree(s->commit_gate);
     if (s->h_outer) (void)hipHostFree(s->h_outer);
+    if (s->ens_ev) (void)hipEventDestroy(s->ens_ev);
+    s->ens_ev = nullptr;
     s->outer = nullptr;
     s->commit_gate = nullptr;
     s->h_outer = nullptr;
@@ -359,7 +368,10 @@ void field_solve_body(upic_sim* s, const double* Jreal) {
 // source buffer and replayed with one launch (it holds no memset nodes).
 void field_solve(upic_sim* s, const double* Jreal) {
     upic_ctx* c = s->c;
-    if (!c->graphs || s->capturing) { field_solve_body(s, Jreal); return; }
+    // inline while a step is captured, or under a gate (a gated launch of
+    // the device-driven step outside its graph): the cached field-solve
+    // graphs hold ungated launches
+    if (!c->graphs || s->capturing || current_gate()) { field_solve_body(s, Jreal); return; }
     for (auto& gx : s->fs_graphs)
         if (gx.first == Jreal) {
             HIPCHK(hipGraphLaunch(gx.second, c->stream));
@@ -1316,9 +1328,9 @@ uint64_t outer_key(upic_sim* s, int slots) {
     return h;
 }
 
-// Runs the step's outer solve on the device.  Returns false (nothing
-// committed, the step's host flags reset) when the host loop must run it.
-bool p3_step_device(upic_sim* s, upic_step_info* info) {
+// Launches the step's predictor (and sets up the controller's buffers on
+// first use); returns the key of the step graph's configuration.
+uint64_t device_predict(upic_sim* s) {
     upic_ctx* c = s->c;
     const upic_params& p = c->p;
     const int N = c->N();
@@ -1349,54 +1361,81 @@ bool p3_step_device(upic_sim* s, upic_step_info* info) {
     launch_residual_norms(c->stream, N, s->guess, s->guess, norms_floor(p), p.nonlinear_rtol, p.nonlinear_atol,
                           c->dred + kRedPredictor);
     launch_zero(c->stream, 2ll * std::max(1, p.nonlinear_max_iter), s->contr);
-    // the step graph of this configuration
     const int slots = s->outer_slots;
-    const uint64_t key = outer_key(s, slots);
-    hipGraphExec_t exec = nullptr;
-    for (auto& g : s->outer_graphs)
-        if (g.key == key) exec = g.exec;
+    s->outer_step_slots = slots;
     s->vdep_valid = s->jdep_valid = s->a0_valid = false;
-    if (!exec) {
-        hipGraph_t gph = nullptr;
-        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        s->capturing = true;
-        try {
-            enqueue_outer(s, slots, true);
-        } catch (...) {
-            s->capturing = false;
-            hipGraph_t junk = nullptr;
-            (void)hipStreamEndCapture(c->stream, &junk);
-            if (junk) (void)hipGraphDestroy(junk);
-            throw;
-        }
-        s->capturing = false;
-        HIPCHK(hipStreamEndCapture(c->stream, &gph));
-        const hipError_t e = hipGraphInstantiate(&exec, gph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(gph);
-        HIPCHK(e);
-        if ((int)s->outer_graphs.size() >= kOuterGraphCache) {
-            (void)hipGraphExecDestroy(s->outer_graphs.front().exec);
-            s->outer_graphs.erase(s->outer_graphs.begin());
-        }
-        s->outer_graphs.push_back({key, exec});
-    }
-    HIPCHK(hipGraphLaunch(exec, c->stream));
-    // the host flags as the graph's first evaluation leaves them, for any
-    // slots launched after it
+    return outer_key(s, slots);
+}
+
+// The host flags as a step graph's first evaluation leaves them, for any
+// slots launched after the graph.
+void after_first_evaluation(upic_sim* s) {
+    const bool fast = s->mixed;
     s->vdep_valid = s->jdep_valid = fast;
     s->a0_valid = fast && (s->guess_mode == 1 || (s->guess_mode == 2 && s->mlp_ready));
-    auto read = [&] {
-        HIPCHK(hipMemcpyAsync(s->h_outer, s->outer, sizeof(OuterDev), hipMemcpyDeviceToHost, c->stream));
-        c->sync();
-    };
-    read();
+}
+
+// The step graph of the configuration `key` (captured on first use; the
+// graph and its instance stay cached).
+const upic_sim::OuterGraph& member_graph(upic_sim* s, uint64_t key) {
+    upic_ctx* c = s->c;
+    for (size_t k = 0; k < s->outer_graphs.size(); ++k)
+        if (s->outer_graphs[k].key == key) {
+            after_first_evaluation(s);
+            return s->outer_graphs[k];
+        }
+    const int slots = s->outer_step_slots;
+    hipGraph_t gph = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    s->capturing = true;
+    try {
+        enqueue_outer(s, slots, true);
+    } catch (...) {
+        s->capturing = false;
+        hipGraph_t junk = nullptr;
+        (void)hipStreamEndCapture(c->stream, &junk);
+        if (junk) (void)hipGraphDestroy(junk);
+        throw;
+    }
+    s->capturing = false;
+    HIPCHK(hipStreamEndCapture(c->stream, &gph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, gph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(gph);
+        HIPCHK(e);
+    }
+    if ((int)s->outer_graphs.size() >= kOuterGraphCache) {
+        (void)hipGraphExecDestroy(s->outer_graphs.front().exec);
+        (void)hipGraphDestroy(s->outer_graphs.front().graph);
+        s->outer_graphs.erase(s->outer_graphs.begin());
+    }
+    s->outer_graphs.push_back({key, gph, exec});
+    after_first_evaluation(s);
+    return s->outer_graphs.back();
+}
+
+void read_outer(upic_sim* s) {
+    HIPCHK(hipMemcpyAsync(s->h_outer, s->outer, sizeof(OuterDev), hipMemcpyDeviceToHost, s->c->stream));
+    s->c->sync();
+}
+
+// After the step graph has run and its control block has been read into
+// h_outer: the slots a step still needs, then the commit's pointer moves and
+// the step record.  Returns false (nothing committed, the step's host flags
+// reset) when the host loop must run the step.
+bool device_complete(upic_sim* s, upic_step_info* info) {
+    upic_ctx* c = s->c;
+    const upic_params& p = c->p;
+    const int64_t n3 = 3ll * c->N();
+    const bool fast = s->mixed;
     // a step that needs more evaluations than the graph unrolls: two slots
     // at a time (eagerly)
     int extra = 0;
     while (s->h_outer->status == kOuterRunning && extra < 4 * std::max(1, p.nonlinear_max_iter)) {
         enqueue_outer(s, 2, false);
         extra += 2;
-        read();
+        read_outer(s);
     }
     const OuterDev& o = *s->h_outer;
     if (o.status != kOuterConverged) {
@@ -1407,6 +1446,7 @@ bool p3_step_device(upic_sim* s, upic_step_info* info) {
     }
     // slots of the next steps' graphs: grow to this step's evaluations and
     // one spare at once, shrink by one when two or more went unused
+    const int slots = s->outer_step_slots;
     int next = slots;
     if (o.maps + 1 > slots) next = o.maps + 1;
     else if (o.maps + 2 < slots) next = slots - 1;
@@ -1477,6 +1517,18 @@ bool p3_step_device(upic_sim* s, upic_step_info* info) {
     return true;
 }
 
+// The step's outer solve on the device: its graph, one launch, one
+// synchronisation.  Returns false when the host loop must run the step.
+bool p3_step_device(upic_sim* s, upic_step_info* info) {
+    const uint64_t key = device_predict(s);
+    const upic_sim::OuterGraph& g = member_graph(s, key);
+    HIPCHK(hipGraphLaunch(g.exec, s->c->stream));
+    read_outer(s);
+    if (!device_complete(s, info)) return false;
+    if (info) info->graph_launch = 1;
+    return true;
+}
+
 // Uploads a per-particle substep layout (stored order) with its substep
 // offsets, sizes the substep-path buffers and records the layout's
 // statistics (P3:5248-5263, 6172-6184).
@@ -1497,6 +1549,7 @@ void set_layout(upic_sim* s, const std::vector<int8_t>& lay) {
         ++r.levels[std::min(lev, 6)];
     }
     r.total_substeps = off[np];
+    s->c->sync();  // no launch of the stream may still read the previous layout
     HIPCHK(hipMemcpy(s->layout, lay.data(), (size_t)np, hipMemcpyHostToDevice));
     if (!s->sub_off) s->sub_off = dalloc<int64_t>((size_t)np + 1);
     HIPCHK(hipMemcpy(s->sub_off, off.data(), ((size_t)np + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
@@ -1510,8 +1563,11 @@ void set_layout(upic_sim* s, const std::vector<int8_t>& lay) {
     s->sub_n = r.total_substeps;
 }
 
-// nonlinear_step_orbit (P3:6477-6526) with the selective fallback (6328-6454)
-void p3_step(upic_sim* s, upic_step_info* info) {
+// Start of a P3 step: the step's host flags, the cell sort, the kept start
+// state, the step-start spectra and (mixed precision) the floored absolute
+// tolerance on the device.  Returns whether the step starts from the
+// extrapolated current.
+bool p3_step_begin(upic_sim* s) {
     upic_ctx* c = s->c;
     s->train_written = false;
     s->vdep_valid = false;
@@ -1541,15 +1597,21 @@ void p3_step(upic_sim* s, upic_step_info* info) {
     s->refine = upic_sim::RefineInfo{};
     s->refine.total_substeps = s->np;
     s->refine.levels[0] = s->np;
+    return extrapolated_start;
+}
+
+// The host-driven outer solve of a P3 step: the uniform layout, then on
+// particle failures nonlinear_step_selective_particle_fallback
+// (P3:6328-6454): the failed particles' substeps double until the layout
+// converges; the reference first repeats the failed full-dt layout, which
+// fails the same way, so its restart count is the number of doublings.
+void p3_step_host(upic_sim* s, upic_step_info* info) {
+    upic_ctx* c = s->c;
     try {
-        if (!(device_outer_ok(s) && p3_step_device(s, info))) p3_step_layout(s, false, info);
+        p3_step_layout(s, false, info);
     } catch (const RefineNeeded& e) {
         if (!c->p.selective_particle_fallback)
             throw Error(UPIC_ENONLINEAR, std::string("inner particle solve failed over full dt; time and dt unchanged: ") + e.what());
-        // nonlinear_step_selective_particle_fallback (P3:6328-6454): the
-        // failed particles' substeps double until the layout converges; the
-        // reference first repeats the failed full-dt layout, which fails the
-        // same way, so its restart count is the number of doublings
         s->start_refined = true;
         s->refine.used = 1;
         s->refine.direct_iterations = e.iterations;
@@ -1574,6 +1636,11 @@ void p3_step(upic_sim* s, upic_step_info* info) {
             }
         }
     }
+}
+
+// End of a P3 step: the refinement record, the training rows, the vbar
+// start's backoff, the step count.
+void p3_step_end(upic_sim* s, upic_step_info* info, bool extrapolated_start) {
     if (info) {
         const auto& r = s->refine;
         info->uniform_substeps_used = r.max_substeps;
@@ -1603,6 +1670,13 @@ void p3_step(upic_sim* s, upic_step_info* info) {
         }
     }
     ++s->steps;
+}
+
+// nonlinear_step_orbit (P3:6477-6526) with the selective fallback (6328-6454)
+void p3_step(upic_sim* s, upic_step_info* info) {
+    const bool ext = p3_step_begin(s);
+    if (!(device_outer_ok(s) && p3_step_device(s, info))) p3_step_host(s, info);
+    p3_step_end(s, info, ext);
 }
 
 // ---------------------------------------------------------------------------
@@ -1653,6 +1727,7 @@ void train_guess(upic_sim* s, int epochs, int batch, double lr, uint64_t seed, d
     if (epochs < 1 || batch < 1 || !(lr > 0.0)) throw Error(UPIC_EINVAL, "epochs, batch and lr must be positive");
     if (!s->mlp_P) s->mlp_P = dalloc<float>(kMlpNParam);
     const std::vector<float> P0 = mlp_init_params(seed);
+    c->sync();  // a sweep in flight may still read the parameters
     HIPCHK(hipMemcpy(s->mlp_P, P0.data(), kMlpNParam * sizeof(float), hipMemcpyHostToDevice));
     double* acc = c->dred + 32;
     launch_zero(c->stream, 27, acc);
@@ -1879,12 +1954,141 @@ void p2_diagnostics(upic_sim* s, upic_diagnostics* out) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Batched ensembles in one graph launch
+// ---------------------------------------------------------------------------
+// The device-driven steps of all eligible members, captured as parallel
+// branches of one graph (fork from and join to a master stream; each branch
+// is the member's step graph on the member's stream) and replayed with one
+// hipGraphLaunch and one synchronisation for the whole ensemble.  Graphs are
+// cached by the members' configurations.  Members the controller does not
+// finish continue or fall back on their own, as in a single step.
+struct EnsembleCache {
+    std::mutex mu;
+    std::map<int, hipStream_t> master;  // per device
+    std::vector<std::pair<uint64_t, hipGraphExec_t>> graphs;
+};
+EnsembleCache& ensemble_cache() {
+    static EnsembleCache* c = new EnsembleCache();  // lives until the process ends
+    return *c;
+}
+constexpr size_t kEnsembleGraphCache = 8;
+
+void ensemble_step_graph(upic_sim* const* sims, int32_t n, upic_step_info* infos, upic_status* rc) {
+    std::vector<char> ok((size_t)n, 0), dev((size_t)n, 0), ext((size_t)n, 0);
+    std::vector<uint64_t> keys((size_t)n, 0);
+    for (int32_t i = 0; i < n; ++i) {
+        upic_sim* m = sims[i];
+        rc[i] = (upic_status)guarded(m->c, [&] {
+            ext[i] = p3_step_begin(m);
+            ok[i] = 1;
+            if (device_outer_ok(m)) {
+                keys[i] = device_predict(m);
+                dev[i] = 1;
+            }
+        });
+    }
+    std::vector<int32_t> members;
+    for (int32_t i = 0; i < n; ++i)
+        if (ok[i] && dev[i]) members.push_back(i);
+    if (!members.empty()) {
+        const int device = sims[members[0]]->c->device;
+        uint64_t ek = 1469598103934665603ull;
+        for (int32_t i : members) {
+            for (uint64_t x : {sims[i]->serial, keys[i]})
+                for (int b = 0; b < 8; ++b) {
+                    ek ^= (x >> (8 * b)) & 0xffu;
+                    ek *= 1099511628211ull;
+                }
+        }
+        EnsembleCache& ec = ensemble_cache();
+        std::lock_guard<std::mutex> lk(ec.mu);
+        const upic_status grc = (upic_status)guarded(nullptr, [&] {
+            HIPCHK(hipSetDevice(device));
+            hipStream_t& ms = ec.master[device];
+            if (!ms) HIPCHK(hipStreamCreateWithFlags(&ms, hipStreamNonBlocking));
+            // the members' predictors run before the graph
+            for (int32_t i : members) {
+                upic_sim* m = sims[i];
+                if (!m->ens_ev) HIPCHK(hipEventCreateWithFlags(&m->ens_ev, hipEventDisableTiming));
+                HIPCHK(hipEventRecord(m->ens_ev, m->c->stream));
+                HIPCHK(hipStreamWaitEvent(ms, m->ens_ev, 0));
+            }
+            hipGraphExec_t exec = nullptr;
+            for (auto& g : ec.graphs)
+                if (g.first == ek) exec = g.second;
+            if (!exec) {
+                hipEvent_t fork = nullptr;
+                HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+                HIPCHK(hipStreamBeginCapture(ms, hipStreamCaptureModeThreadLocal));
+                hipGraph_t gph = nullptr;
+                try {
+                    HIPCHK(hipEventRecord(fork, ms));
+                    for (int32_t i : members) {
+                        upic_sim* m = sims[i];
+                        HIPCHK(hipStreamWaitEvent(m->c->stream, fork, 0));
+                        m->capturing = true;
+                        enqueue_outer(m, m->outer_step_slots, true);
+                        m->capturing = false;
+                        HIPCHK(hipEventRecord(m->ens_ev, m->c->stream));
+                        HIPCHK(hipStreamWaitEvent(ms, m->ens_ev, 0));
+                    }
+                } catch (...) {
+                    for (int32_t i : members) sims[i]->capturing = false;
+                    (void)hipStreamEndCapture(ms, &gph);
+                    if (gph) (void)hipGraphDestroy(gph);
+                    (void)hipEventDestroy(fork);
+                    throw;
+                }
+                HIPCHK(hipStreamEndCapture(ms, &gph));
+                (void)hipEventDestroy(fork);
+                const hipError_t e = hipGraphInstantiate(&exec, gph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gph);
+                HIPCHK(e);
+                if (ec.graphs.size() >= kEnsembleGraphCache) {
+                    (void)hipGraphExecDestroy(ec.graphs.front().second);
+                    ec.graphs.erase(ec.graphs.begin());
+                }
+                ec.graphs.emplace_back(ek, exec);
+            }
+            HIPCHK(hipGraphLaunch(exec, ms));
+            for (int32_t i : members) {
+                upic_sim* m = sims[i];
+                after_first_evaluation(m);
+                HIPCHK(hipMemcpyAsync(m->h_outer, m->outer, sizeof(OuterDev), hipMemcpyDeviceToHost, ms));
+            }
+            HIPCHK(hipStreamSynchronize(ms));
+        });
+        if (grc != UPIC_OK) {
+            for (int32_t i : members) {
+                rc[i] = grc;
+                ok[i] = 0;
+            }
+        }
+    }
+    for (int32_t i = 0; i < n; ++i) {
+        if (!ok[i]) continue;
+        upic_sim* m = sims[i];
+        upic_step_info* inf = infos ? infos + i : nullptr;
+        rc[i] = (upic_status)guarded(m->c, [&] {
+            if (dev[i] && device_complete(m, inf)) {
+                if (inf) inf->graph_launch = 2;
+            } else {
+                p3_step_host(m, inf);
+            }
+            p3_step_end(m, inf, ext[i] != 0);
+        });
+    }
+}
+
 extern "C" {
 
 upic_status upic_sim_create(upic_ctx* c, int64_t np, upic_sim** out) {
     if (!c || !out || np < 0) return UPIC_EINVAL;
     *out = nullptr;
     upic_sim* s = new upic_sim();
+    static std::atomic<uint64_t> serials{0};
+    s->serial = ++serials;
     const int rc = guarded(c, [&] {
         if (np > (int64_t)1 << 31) throw Error(UPIC_EINVAL, "at most 2^31 particles per simulation");
         require_stencil_grid(c->p);
@@ -1999,6 +2203,11 @@ upic_status upic_sim_upload(upic_sim* s, const upic_host_state* st) {
         s->jprev_valid = false;
         s->fix_scale = 0.0;  // charges may change: the deterministic deposit rescales
         fp64_view(s);  // arrays the caller leaves out keep their current values
+        // the copies below run on the null stream, which does not wait for
+        // the context's non-blocking stream: the expansion above (and any
+        // step still in flight) must have finished before they overwrite
+        // the same arrays
+        c->sync();
         const int N = c->N();
         const size_t n3 = 3 * (size_t)s->np * sizeof(double), n1 = (size_t)s->np * sizeof(double);
         auto up = [&](double* d, const double* h, size_t bytes) {
@@ -2090,23 +2299,29 @@ upic_status upic_ensemble_step(upic_sim* const* sims, int32_t n, int32_t threads
                 return UPIC_EINVAL;
             }
     }
-    std::atomic<int32_t> next{0};
     std::vector<upic_status> rc((size_t)n, UPIC_OK);
-    auto work = [&] {
-        for (;;) {
-            const int32_t i = next.fetch_add(1);
-            if (i >= n) break;
-            upic_sim* m = sims[i];
-            upic_step_info* inf = infos ? infos + i : nullptr;
-            const int code = m->c->p.code;
-            rc[(size_t)i] = code == 1 ? upic_step_p1(m, inf) : code == 2 ? upic_step_p2(m, inf) : upic_step_p3(m, inf);
-        }
-    };
-    const int32_t nt = std::max<int32_t>(1, std::min<int32_t>(n, threads > 0 ? threads : 16));
-    std::vector<std::thread> pool;
-    for (int32_t t = 1; t < nt; ++t) pool.emplace_back(work);
-    work();
-    for (auto& t : pool) t.join();
+    bool graph_path = n > 0;
+    for (int32_t i = 0; i < n; ++i) graph_path = graph_path && sims[i]->c->p.code == 3 && sims[i]->c->graphs;
+    if (graph_path) {
+        ensemble_step_graph(sims, n, infos, rc.data());
+    } else {
+        std::atomic<int32_t> next{0};
+        auto work = [&] {
+            for (;;) {
+                const int32_t i = next.fetch_add(1);
+                if (i >= n) break;
+                upic_sim* m = sims[i];
+                upic_step_info* inf = infos ? infos + i : nullptr;
+                const int code = m->c->p.code;
+                rc[(size_t)i] = code == 1 ? upic_step_p1(m, inf) : code == 2 ? upic_step_p2(m, inf) : upic_step_p3(m, inf);
+            }
+        };
+        const int32_t nt = std::max<int32_t>(1, std::min<int32_t>(n, threads > 0 ? threads : 16));
+        std::vector<std::thread> pool;
+        for (int32_t t = 1; t < nt; ++t) pool.emplace_back(work);
+        work();
+        for (auto& t : pool) t.join();
+    }
     upic_status first = UPIC_OK;
     for (int32_t i = 0; i < n; ++i) {
         if (status) status[i] = rc[(size_t)i];
@@ -2613,6 +2828,7 @@ upic_status upic_sim_set_guess_params(upic_sim* s, const float* params) {
     return (upic_status)guarded(s->c, [&] {
         if (!s->mixed) throw Error(UPIC_EINVAL, "the learned guess is implemented for the mixed-precision P3 step");
         if (!s->mlp_P) s->mlp_P = dalloc<float>(kMlpNParam);
+        s->c->sync();  // a sweep in flight may still read the parameters
         HIPCHK(hipMemcpy(s->mlp_P, params, kMlpNParam * sizeof(float), hipMemcpyHostToDevice));
         mlp_install(s);
     });

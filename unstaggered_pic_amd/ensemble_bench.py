@@ -6,11 +6,16 @@ cubic splines at dt = 0.05), mixed precision with the physics-predictor
 guess and the selective particle refinement on, each on its own context
 with graph-replayed field solves.
 
-The same members are timed twice from the same state: stepped one after the
-other (one member on the GPU at a time), and advanced together by
-upic_ensemble_step.  Prints one JSON line: aggregate particle-steps/s of
-both, their ratio, and the largest relative total-energy change of any
-member over the timed steps.
+The same members are timed three times from the same state: stepped one
+after the other (one member on the GPU at a time, each step its own
+device-driven graph), advanced together by upic_ensemble_step with host
+threads (graphs off), and advanced together by upic_ensemble_step with
+graphs on (every ensemble step one hipGraph launch).  Then the mixed and the
+FP64 forms of every member run --fp64-steps steps from the same initial
+state for the per-member energy error.  Prints one JSON line: aggregate
+particle-steps/s of each, the ratio of the graph ensemble to the
+sequential steps, the largest relative total-energy change of any member
+over the timed steps and the per-member errors against FP64.
 
   python bench.py --form ensemble [--members 16] [--steps 5] [--warmup 2]
 """
@@ -31,7 +36,7 @@ def member_specs(n, seed=2024):
     return specs
 
 
-def build(lib, spec):
+def build(lib, spec, mixed=True):
     from unstaggered_pic_amd.abi import (CODE_P3, LOAD_BIMAXWELL, LOAD_LANDAU, make_load, make_params,
                                          with_mixed_precision)
     s = spec["side"]
@@ -51,8 +56,9 @@ def build(lib, spec):
                         local_max_iter=64, sort_interval=1, selective_particle_fallback=1)
         load = make_load(LOAD_BIMAXWELL, spec["ppc"], seed=spec["seed"],
                          vth=(0.25 * spec["vth"], spec["vth"], spec["vth"]))
-    p = with_mixed_precision(p)
-    p.learned_guess = 1
+    if mixed:
+        p = with_mixed_precision(p)
+        p.learned_guess = 1
     n = s * s * spec["ppc"]
     x, v, q, m = lib.load_particles(p, load, n)
     return p, n, (x, v, q, m)
@@ -77,11 +83,13 @@ def run(args):
             print(f"# loaded {i + 1}/{len(specs)} members", file=sys.stderr, flush=True)
     total = sum(n for _, n, _ in loads)
 
-    def fresh():
+    def fresh(graphs=True, fp64=False):
         ms = []
-        for p, n, (x, v, q, m) in loads:
+        for k, (p, n, (x, v, q, m)) in enumerate(loads):
+            if fp64:
+                p = build(lib, specs[k], mixed=False)[0]
             ctx = lib.Context(p)
-            ctx.set_graphs(True)
+            ctx.set_graphs(graphs)
             sim = lib.Simulation(ctx, n)
             h = _Host(n, p.nx * p.ny)
             h.x[:], h.v[:], h.q[:], h.m[:] = x, v, q, m
@@ -91,18 +99,19 @@ def run(args):
         return ms
 
     out = {}
-    for mode in ("sequential", "ensemble"):
-        ms = fresh()
+    for mode in ("sequential", "threads", "ensemble"):
+        ms = fresh(graphs=mode != "threads")
         print(f"# {mode}: {len(ms)} members on the GPU", file=sys.stderr, flush=True)
         sims = [s for _, s in ms]
         for _ in range(args.warmup):
-            lib.ensemble_step(sims, threads=16) if mode == "ensemble" else [s.step() for s in sims]
+            lib.ensemble_step(sims, threads=16) if mode != "sequential" else [s.step() for s in sims]
         e0 = [s.diagnostics(1).total_energy for s in sims]
         for c, _ in ms:
             c.sync()
+        steps = args.steps if mode == "ensemble" else min(args.steps, args.baseline_steps)
         t0 = time.perf_counter()
-        for _ in range(args.steps):
-            if mode == "ensemble":
+        for _ in range(steps):
+            if mode != "sequential":
                 lib.ensemble_step(sims, threads=16)
             else:
                 for s in sims:
@@ -111,11 +120,32 @@ def run(args):
             c.sync()
         el = time.perf_counter() - t0
         drift = max(abs(s.diagnostics(1).total_energy - a) / abs(a) for s, a in zip(sims, e0))
-        print(f"# {mode}: {1e3 * el / args.steps:.1f} ms per ensemble step", file=sys.stderr, flush=True)
-        out[mode] = {"value": total * args.steps / el, "ms_per_step": 1e3 * el / args.steps, "max_rel_energy_change": drift}
+        print(f"# {mode}: {1e3 * el / steps:.1f} ms per ensemble step", file=sys.stderr, flush=True)
+        out[mode] = {"value": total * steps / el, "ms_per_step": 1e3 * el / steps, "steps": steps,
+                     "max_rel_energy_change": drift}
         for c, s in ms:
             s.close()
             c.close()
+    # per-member energy error against the FP64 form of the same member: both
+    # ensembles from the same initial state for --fp64-steps steps
+    hist = {}
+    for form in ("mixed", "fp64"):
+        ms = fresh(graphs=True, fp64=form == "fp64")
+        sims = [s for _, s in ms]
+        h = [[(d.total_energy, d.field_energy)] for d in (s.diagnostics(1) for s in sims)]
+        for _ in range(args.fp64_steps):
+            lib.ensemble_step(sims, threads=16)
+            for k, s in enumerate(sims):
+                d = s.diagnostics(1)
+                h[k].append((d.total_energy, d.field_energy))
+        hist[form] = np.array(h)
+        print(f"# {form}: {args.fp64_steps} steps for the energy comparison", file=sys.stderr, flush=True)
+        for c, s in ms:
+            s.close()
+            c.close()
+    hm, hf = hist["mixed"], hist["fp64"]
+    te_err = np.max(np.abs(hm[:, :, 0] - hf[:, :, 0]), axis=1) / np.abs(hf[:, 0, 0])
+    fe_err = np.max(np.abs(hm[:, :, 1] - hf[:, :, 1]) / np.abs(hf[:, :, 1]), axis=1)
     line = {
         "metric": "aggregate particle-steps/s of a batched ensemble of independent P3 runs on 1 MI355X",
         "value": out["ensemble"]["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -126,5 +156,14 @@ def run(args):
                    "members": specs},
         "sequential": out["sequential"], "ensemble_vs_sequential": out["ensemble"]["value"] / out["sequential"]["value"],
         "max_rel_energy_change": out["ensemble"]["max_rel_energy_change"],
+        "threads": out["threads"],
+        "energy_error_vs_fp64": {
+            "steps": args.fp64_steps,
+            "definition": "per member: max over steps of |E_mixed - E_fp64| / |E_fp64(0)| (total energy), "
+                          "max over steps of |W_mixed - W_fp64| / |W_fp64| (field energy); FP64 = the same member "
+                          "stepped by the FP64 GPU path from the same initial state",
+            "total_energy_max": float(te_err.max()), "field_energy_max": float(fe_err.max()),
+            "total_energy_per_member": [float(f"{e:.3e}") for e in te_err],
+            "field_energy_per_member": [float(f"{e:.3e}") for e in fe_err]},
     }
     print(json.dumps(line))
