@@ -32,7 +32,13 @@ Beside the headline: the 64 x 64 Landau workload in the P1 form (plain
 FP64, 409,600 particles, quadratic, dt 0.1, Picard tolerance 1e-10), and the
 CPU baselines (the reference's own P3 step, compiled from its sources,
 serial and with OpenMP on the host's cores, and its serial P1 step, each on
-a bounded sample of the same workload).
+a bounded sample of the same workload), and the batched ensemble
+(`ensemble`: --ensemble-members uneven Landau/Weibel runs drawn from a seed,
+mixed precision + predictor, every ensemble step one graph launch, timed
+over --ensemble-steps steps, with the same members stepped one by one for
+the ratio and each member's energies against its own FP64 run over 5 steps;
+unstaggered_pic_amd/ensemble_bench.py, which `--form ensemble` runs alone
+at any step count).
 
 One step = one upic_step_p3 (upic_step_p1) call.  The timed region covers
 exactly --steps steps bracketed by device synchronisation; inputs are
@@ -512,6 +518,11 @@ def main():
     ap.add_argument("--p1-ppc", type=int, default=100)
     ap.add_argument("--no-p1", action="store_true", help="skip the P1 Landau leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-members", type=int, default=64,
+                    help="members of the batched-ensemble sub-object of the default run (0: none)")
+    ap.add_argument("--ensemble-steps", type=int, default=20,
+                    help="timed ensemble steps of that sub-object (the sequential baselines: 5, "
+                         "the per-member FP64 comparison: 5)")
     ap.add_argument("--graphs", type=int, default=1,
                     help="1: each P3 step's outer solve as one device-driven hipGraph (0: host-driven loop)")
     ap.add_argument("--timing-steps", type=int, default=4,
@@ -605,6 +616,17 @@ def main():
         s1 = leg_summary(r1, args)
         s1["config"] = {"workload": workload_name("landau_p1", r1["p"].nx, r1["ppc"]), "particles": r1["np"]}
         out["p1_landau"] = s1
+    if code == 3 and args.form == "all" and args.ensemble_members > 0:
+        from unstaggered_pic_amd import ensemble_bench
+        e = ensemble_bench.measure(args.ensemble_members, args.ensemble_steps, 2, 5, 5)
+        out["ensemble"] = {k: e[k] for k in (
+            "metric", "value", "unit", "steps", "ms_per_step", "ensemble_vs_sequential",
+            "ensemble_vs_sequential_host_loop", "max_rel_energy_change", "energy_error_vs_fp64")}
+        out["ensemble"]["config"] = {"workload": e["config"]["workload"], "particles": e["config"]["particles"],
+                                     "members": len(e["config"]["members"]), "member_draw": "seed 2024",
+                                     "form": "mixed precision + predictor, one graph launch per ensemble step"}
+        out["ensemble"]["sequential_ms_per_step"] = e["sequential"]["ms_per_step"]
+        out["ensemble"]["energy_error_vs_fp64"].pop("definition", None)
     if not args.no_cpu_baseline:
         if code == 3:
             cpu = cpu_baseline_p3(args.nx, workload=args.workload)

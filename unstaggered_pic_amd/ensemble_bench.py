@@ -6,11 +6,12 @@ cubic splines at dt = 0.05), mixed precision with the physics-predictor
 guess and the selective particle refinement on, each on its own context
 with graph-replayed field solves.
 
-The same members are timed three times from the same state: stepped one
-after the other (one member on the GPU at a time, each step its own
-device-driven graph), advanced together by upic_ensemble_step with host
-threads (graphs off), and advanced together by upic_ensemble_step with
-graphs on (every ensemble step one hipGraph launch).  Then the mixed and the
+The same members are timed four times from the same state: stepped one
+after the other with the host-driven outer loop (graphs off), stepped one
+after the other with each step its own device-driven graph, advanced
+together by upic_ensemble_step with host threads (graphs off), and advanced
+together by upic_ensemble_step with graphs on (every ensemble step one
+hipGraph launch).  Then the mixed and the
 FP64 forms of every member run --fp64-steps steps from the same initial
 state for the per-member energy error.  Prints one JSON line: aggregate
 particle-steps/s of each, the ratio of the graph ensemble to the
@@ -74,8 +75,13 @@ class _Host:
 
 
 def run(args):
+    """bench.py --form ensemble: the measurement as the bench's one JSON line."""
+    print(json.dumps(measure(args.members, args.steps, args.warmup, args.baseline_steps, args.fp64_steps)))
+
+
+def measure(members, steps_timed, warmup, baseline_steps, fp64_steps):
     from unstaggered_pic_amd import lib
-    specs = member_specs(args.members)
+    specs = member_specs(members)
     loads = []
     for i, sp in enumerate(specs):
         loads.append(build(lib, sp))
@@ -99,47 +105,52 @@ def run(args):
         return ms
 
     out = {}
-    for mode in ("sequential", "threads", "ensemble"):
-        ms = fresh(graphs=mode != "threads")
+    for mode in ("sequential_host_loop", "sequential", "threads", "ensemble"):
+        ms = fresh(graphs=mode in ("sequential", "ensemble"))
         print(f"# {mode}: {len(ms)} members on the GPU", file=sys.stderr, flush=True)
         sims = [s for _, s in ms]
-        for _ in range(args.warmup):
-            lib.ensemble_step(sims, threads=16) if mode != "sequential" else [s.step() for s in sims]
+        one_by_one = mode.startswith("sequential")
+        for _ in range(warmup):
+            [s.step() for s in sims] if one_by_one else lib.ensemble_step(sims, threads=16)
         e0 = [s.diagnostics(1).total_energy for s in sims]
         for c, _ in ms:
             c.sync()
-        steps = args.steps if mode == "ensemble" else min(args.steps, args.baseline_steps)
+        steps = steps_timed if mode == "ensemble" else min(steps_timed, baseline_steps)
         t0 = time.perf_counter()
-        for _ in range(steps):
-            if mode != "sequential":
+        for k in range(steps):
+            if not one_by_one:
                 lib.ensemble_step(sims, threads=16)
             else:
                 for s in sims:
                     s.step()
+            if (k + 1) % 100 == 0:
+                print(f"# {mode}: step {k + 1}", file=sys.stderr, flush=True)
         for c, _ in ms:
             c.sync()
         el = time.perf_counter() - t0
-        drift = max(abs(s.diagnostics(1).total_energy - a) / abs(a) for s, a in zip(sims, e0))
+        drifts = [abs(s.diagnostics(1).total_energy - a) / abs(a) for s, a in zip(sims, e0)]
         print(f"# {mode}: {1e3 * el / steps:.1f} ms per ensemble step", file=sys.stderr, flush=True)
         out[mode] = {"value": total * steps / el, "ms_per_step": 1e3 * el / steps, "steps": steps,
-                     "max_rel_energy_change": drift}
+                     "max_rel_energy_change": max(drifts)}
+        if mode == "ensemble":
+            out[mode]["rel_energy_change_per_member"] = [float(f"{d:.3e}") for d in drifts]
         for c, s in ms:
             s.close()
             c.close()
     # per-member energy error against the FP64 form of the same member: both
-    # ensembles from the same initial state for --fp64-steps steps
+    # ensembles from the same initial state for fp64_steps steps
     hist = {}
     for form in ("mixed", "fp64"):
         ms = fresh(graphs=True, fp64=form == "fp64")
         sims = [s for _, s in ms]
         h = [[(d.total_energy, d.field_energy)] for d in (s.diagnostics(1) for s in sims)]
-        for _ in range(args.fp64_steps):
+        for _ in range(fp64_steps):
             lib.ensemble_step(sims, threads=16)
             for k, s in enumerate(sims):
                 d = s.diagnostics(1)
                 h[k].append((d.total_energy, d.field_energy))
         hist[form] = np.array(h)
-        print(f"# {form}: {args.fp64_steps} steps for the energy comparison", file=sys.stderr, flush=True)
+        print(f"# {form}: {fp64_steps} steps for the energy comparison", file=sys.stderr, flush=True)
         for c, s in ms:
             s.close()
             c.close()
@@ -148,17 +159,19 @@ def run(args):
     fe_err = np.max(np.abs(hm[:, :, 1] - hf[:, :, 1]) / np.abs(hf[:, :, 1]), axis=1)
     line = {
         "metric": "aggregate particle-steps/s of a batched ensemble of independent P3 runs on 1 MI355X",
-        "value": out["ensemble"]["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": out["ensemble"]["ms_per_step"], "higher_is_better": True,
+        "value": out["ensemble"]["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": steps_timed,
+        "warmup": warmup, "ms_per_step": out["ensemble"]["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64",
         "data": "synthetic (seeded Landau and bi-Maxwellian loads)",
-        "config": {"workload": f"ensemble_{args.members}_members_p3_mixed_predictor", "particles": total,
+        "config": {"workload": f"ensemble_{members}_members_p3_mixed_predictor", "particles": total,
                    "members": specs},
         "sequential": out["sequential"], "ensemble_vs_sequential": out["ensemble"]["value"] / out["sequential"]["value"],
         "max_rel_energy_change": out["ensemble"]["max_rel_energy_change"],
-        "threads": out["threads"],
+        "rel_energy_change_per_member": out["ensemble"]["rel_energy_change_per_member"],
+        "threads": out["threads"], "sequential_host_loop": out["sequential_host_loop"],
+        "ensemble_vs_sequential_host_loop": out["ensemble"]["value"] / out["sequential_host_loop"]["value"],
         "energy_error_vs_fp64": {
-            "steps": args.fp64_steps,
+            "steps": fp64_steps,
             "definition": "per member: max over steps of |E_mixed - E_fp64| / |E_fp64(0)| (total energy), "
                           "max over steps of |W_mixed - W_fp64| / |W_fp64| (field energy); FP64 = the same member "
                           "stepped by the FP64 GPU path from the same initial state",
@@ -166,4 +179,4 @@ def run(args):
             "total_energy_per_member": [float(f"{e:.3e}") for e in te_err],
             "field_energy_per_member": [float(f"{e:.3e}") for e in fe_err]},
     }
-    print(json.dumps(line))
+    return line
