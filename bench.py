@@ -408,6 +408,9 @@ def physics(r):
            "avg_path_segments": float(np.mean([i.path.total_segments / max(i.path.n_particles, 1) for i in infos]))}
     if r["code"] == 3:
         out["map_evaluations"] = [i.map_evaluations for i in infos]
+        # steps run as one device-driven graph launch (the others took the
+        # host-driven loop: deposit-autotuner timings, refinement)
+        out["graph_launched_steps"] = sum(1 for i in infos if i.graph_launch > 0)
         out["mean_inner_iterations"] = float(np.mean([i.local_iterations_mean for i in infos]))
         # summed over the step's sweeps (the guess acts in the first)
         out["inner_iterations_per_particle_step"] = float(np.mean([i.local_iterations_total for i in infos])) / r["np"]

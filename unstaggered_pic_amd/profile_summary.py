@@ -38,21 +38,35 @@ def short(name):
     return name.split("(")[0]
 
 
+def _working(values):
+    """The dispatches that did work: a launch of the device-driven step whose
+    gate was closed returns at once and reads below 1% of the kernel's
+    largest value; it is left out of the means."""
+    top = max(values) if values else 0.0
+    return [v for v in values if v >= 0.01 * top] if top > 0.0 else values
+
+
 def counters(dirname):
     rows = list(csv.DictReader(open(_one(dirname, "*counter_collection.csv"))))
     agg = collections.defaultdict(list)
     for r in rows:
         agg[r["Kernel_Name"]].append(float(r["Counter_Value"]))
-    return agg
+    return {k: _working(v) for k, v in agg.items()}
 
 
 def counters_by_name(dirname):
-    """{kernel: {counter: (mean over the last 4 dispatches, dispatches)}}."""
+    """{kernel: {counter: (mean over the last 4 working dispatches, dispatches)}}."""
     rows = list(csv.DictReader(open(_one(dirname, "*counter_collection.csv"))))
     agg = collections.defaultdict(lambda: collections.defaultdict(list))
     for r in rows:
         agg[r["Kernel_Name"]][r["Counter_Name"]].append(float(r["Counter_Value"]))
-    return {k: {c: (sum(v[-4:]) / len(v[-4:]), len(v)) for c, v in d.items()} for k, d in agg.items()}
+    out = {}
+    for k, d in agg.items():
+        out[k] = {}
+        for c, v in d.items():
+            w = _working(v)
+            out[k][c] = (sum(w[-4:]) / len(w[-4:]), len(w))
+    return out
 
 
 def trace_stats(dirname, after, nth_last=1):
@@ -63,11 +77,14 @@ def trace_stats(dirname, after, nth_last=1):
     marks = [i for i, r in enumerate(rows) if after in r["Kernel_Name"]]
     tail = rows[marks[-min(nth_last, len(marks))] + 1:] if marks else rows
     agg = collections.defaultdict(lambda: [0, 0])
+    durs = collections.defaultdict(list)
     for r in tail:
+        d = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
         agg[r["Kernel_Name"]][0] += 1
-        agg[r["Kernel_Name"]][1] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        agg[r["Kernel_Name"]][1] += d
+        durs[r["Kernel_Name"]].append(d)
     span = int(tail[-1]["End_Timestamp"]) - int(tail[0]["Start_Timestamp"]) if tail else 0
-    return agg, span
+    return agg, span, {k: _working(v) for k, v in durs.items()}
 
 
 def main():
@@ -90,17 +107,22 @@ def main():
     a = ap.parse_args()
     mean_ns = {}
     if a.after:
-        agg, span = trace_stats(a.trace, a.after, a.nth_last)
-        mean_ns = {short(k): t / n for k, (n, t) in agg.items()}
+        agg, span, work = trace_stats(a.trace, a.after, a.nth_last)
+        # mean duration of the launches that did work (gated-off launches of
+        # the device-driven step return at once)
+        mean_ns = {short(k): sum(v) / len(v) for k, v in work.items()}
         busy = sum(v[1] for v in agg.values())
         where = "the last" if a.nth_last == 1 else f"the {a.nth_last}th-last"
         print(f"Dispatches after {where} `{a.after}`: GPU busy {busy / 1e6:.2f} ms over a span of {span / 1e6:.2f} ms"
               + (f" ({a.steps} steps: {busy / 1e6 / a.steps:.2f} ms busy per step)" if a.steps else "") + ".")
         print()
-        print("| kernel | calls | total ms | average us | % of busy |" + (" ms per step |" if a.steps else ""))
+        print("| kernel | calls (gated off) | total ms | average us (working calls) | % of busy |"
+              + (" ms per step |" if a.steps else ""))
         print("|---|---|---|---|---|" + ("---|" if a.steps else ""))
         for k, (n, t) in sorted(agg.items(), key=lambda kv: -kv[1][1])[:a.top]:
-            line = f"| `{short(k)}` | {n} | {t / 1e6:.2f} | {t / n / 1e3:.1f} | {100.0 * t / busy:.1f} |"
+            nw = len(work[k])
+            calls = f"{n} ({n - nw})" if nw < n else f"{n}"
+            line = f"| `{short(k)}` | {calls} | {t / 1e6:.2f} | {mean_ns[short(k)] / 1e3:.1f} | {100.0 * t / busy:.1f} |"
             if a.steps:
                 line += f" {t / 1e6 / a.steps:.3f} |"
             print(line)
