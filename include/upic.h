@@ -351,6 +351,9 @@ typedef struct upic_host_state {
 
 upic_status upic_sim_create(upic_ctx* ctx, int64_t np, upic_sim** out);
 upic_status upic_sim_destroy(upic_sim* sim);
+/* Host arrays -> the resident state (NULL arrays keep their values).  Waits
+ * for the context's stream first: work still queued there may touch the
+ * same buffers. */
 upic_status upic_sim_upload(upic_sim* sim, const upic_host_state* st);
 upic_status upic_sim_download(upic_sim* sim, upic_host_state* st);
 /* P1 picard_step_orbit (P1:1509-1552). */
@@ -362,12 +365,20 @@ upic_status upic_step_p3(upic_sim* sim, upic_step_info* info);
 upic_status upic_step_p2(upic_sim* sim, upic_step_info* info);
 /* No reference counterpart (the reference runs one simulation per process):
  * one step of each of n independent simulations -- any codes, decks and
- * sizes, each on its own context (its own stream) -- advanced concurrently
- * by min(n, threads) host threads (threads <= 0: 16), so that small members
- * fill the GPU together.  infos (n entries) and status (n entries) may be
- * NULL.  Returns UPIC_OK when every member stepped, else the first member
- * status that is not UPIC_OK (each member's error text is on its context);
- * UPIC_EINVAL when two members share a context. */
+ * sizes, each on its own context (its own stream) -- advanced concurrently,
+ * so that small members fill the GPU together.  When every member is a P3
+ * simulation whose context has graphs on (upic_ctx_set_graphs), the
+ * members that can take the device-driven step (deposit autotuner settled)
+ * are one captured hipGraph, one branch per member, launched once and
+ * waited on once (info.graph_launch = 2; a member that needs more
+ * evaluations than its branch holds continues on its own stream, one that
+ * needs the selective refinement takes the host loop), and the others take
+ * the host-driven step one after the other; otherwise min(n, threads) host
+ * threads (threads <= 0: 16) step the members.
+ * infos (n entries) and status (n entries) may be NULL.  Returns UPIC_OK
+ * when every member stepped, else the first member status that is not
+ * UPIC_OK (each member's error text is on its context); UPIC_EINVAL when
+ * two members share a context. */
 upic_status upic_ensemble_step(upic_sim* const* sims, int32_t n, int32_t threads, upic_step_info* infos,
                                upic_status* status);
 /* compute_diagnostics (P1:1057-1087, P3:3521-3574, P2:898-962) on the current
