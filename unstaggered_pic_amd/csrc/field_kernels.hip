@@ -510,7 +510,7 @@ void launch_outer_atol(hipStream_t s, const double* sums, int N, double dV, doub
 // (anderson_gamma, dense_solve.hpp).
 __global__ void k_anderson_gamma(int m, const double* __restrict__ gram, double reg, double* __restrict__ ctl) {
     if (threadIdx.x != 0) return;
-    anderson_gamma(m, gram, reg, ctl);
+    anderson_gamma<8>(m, gram, reg, ctl);  // anderson_memory <= 8
 }
 void launch_anderson_gamma(hipStream_t s, int m, const double* gram, double reg, double* ctl) {
     hipLaunchKernelGGL(k_anderson_gamma, dim3(1), dim3(64), 0, s, m, gram, reg, ctl);

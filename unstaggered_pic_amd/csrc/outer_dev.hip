@@ -377,7 +377,7 @@ __global__ __launch_bounds__(256) void k_outer_gram(OuterDev* __restrict__ o, Ou
         cf.contr[2 * o->contr_it] = sums[0];
         cf.contr[2 * o->contr_it + 1] = sums[1];
     }
-    if (m > 0) anderson_gamma(m, sums + 2, cf.reg, o->ctl);
+    if (m > 0) anderson_gamma<kOuterMaxMemory>(m, sums + 2, cf.reg, o->ctl);
 }
 
 // Pass 2a: cand = X_last + damping R_last - sum_j gamma_j (dX_j + damping
