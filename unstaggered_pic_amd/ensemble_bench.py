@@ -117,9 +117,10 @@ def measure(members, steps_timed, warmup, baseline_steps, fp64_steps):
             c.sync()
         steps = steps_timed if mode == "ensemble" else min(steps_timed, baseline_steps)
         t0 = time.perf_counter()
+        launched = 0
         for k in range(steps):
             if not one_by_one:
-                lib.ensemble_step(sims, threads=16)
+                launched += sum(1 for i in lib.ensemble_step(sims, threads=16) if i.graph_launch == 2)
             else:
                 for s in sims:
                     s.step()
@@ -131,7 +132,7 @@ def measure(members, steps_timed, warmup, baseline_steps, fp64_steps):
         drifts = [abs(s.diagnostics(1).total_energy - a) / abs(a) for s, a in zip(sims, e0)]
         print(f"# {mode}: {1e3 * el / steps:.1f} ms per ensemble step", file=sys.stderr, flush=True)
         out[mode] = {"value": total * steps / el, "ms_per_step": 1e3 * el / steps, "steps": steps,
-                     "max_rel_energy_change": max(drifts)}
+                     "max_rel_energy_change": max(drifts), "graph_launched": launched}
         if mode == "ensemble":
             out[mode]["rel_energy_change_per_member"] = [float(f"{d:.3e}") for d in drifts]
         for c, s in ms:
@@ -160,6 +161,7 @@ def measure(members, steps_timed, warmup, baseline_steps, fp64_steps):
     line = {
         "metric": "aggregate particle-steps/s of a batched ensemble of independent P3 runs on 1 MI355X",
         "value": out["ensemble"]["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": steps_timed,
+        "graph_launched_member_steps": out["ensemble"]["graph_launched"],
         "warmup": warmup, "ms_per_step": out["ensemble"]["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64",
         "data": "synthetic (seeded Landau and bi-Maxwellian loads)",
