@@ -40,6 +40,7 @@ def run(args, prec, learned):
     n = N * args.ppc
     x, v, q, m = lib.load_particles(p, bench.workload_load(args.ppc), n)
     ctx = lib.Context(p)
+    ctx.set_graphs(args.graphs)
     sim = lib.Simulation(ctx, n)
     h = bench.HostArrays(n, N)
     h.x[:], h.v[:], h.q[:], h.m[:] = x, v, q, m
@@ -105,7 +106,10 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--nx", type=int, default=256)
     ap.add_argument("--ppc", type=int, default=256)
-    ap.add_argument("--collect-steps", type=int, default=40)
+    ap.add_argument("--collect-steps", type=int, default=200,
+                    help="steps whose final sweeps write the network's training rows (mixed + learned)")
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="1: each step's outer solve as one device-driven graph launch (0: host-driven loop)")
     ap.add_argument("--out", default="weibel_history")
     ap.add_argument("--fp64-history", default=None,
                     help="reuse the FP64 history CSV of an earlier run of the same deck instead of rerunning it")
