@@ -524,7 +524,7 @@ def main():
     ap.add_argument("--ensemble-members", type=int, default=64,
                     help="members of the batched-ensemble sub-object of the default run (0: none)")
     ap.add_argument("--ensemble-steps", type=int, default=20,
-                    help="timed ensemble steps of that sub-object (after 14 warmup steps; the sequential "
+                    help="timed ensemble steps of that sub-object (after 18 warmup steps; the sequential "
                          "baselines: 5, the per-member FP64 comparison: 5)")
     ap.add_argument("--graphs", type=int, default=1,
                     help="1: each P3 step's outer solve as one device-driven hipGraph (0: host-driven loop)")
@@ -621,10 +621,11 @@ def main():
         out["p1_landau"] = s1
     if code == 3 and args.form == "all" and args.ensemble_members > 0:
         from unstaggered_pic_amd import ensemble_bench
-        # 14 warmup steps: each member's deposit autotuner times its six
-        # variants twice (12 steps on the host-driven path) before its steps
-        # join the ensemble graph
-        e = ensemble_bench.measure(args.ensemble_members, args.ensemble_steps, 14, 5, 5)
+        # 18 warmup steps: each member's deposit autotuner times its eight
+        # variants twice (one timing per step: the later evaluations of a
+        # step deposit incrementally) on the host-driven path before its
+        # steps join the ensemble graph
+        e = ensemble_bench.measure(args.ensemble_members, args.ensemble_steps, 18, 5, 5)
         out["ensemble"] = {k: e[k] for k in (
             "metric", "value", "unit", "steps", "graph_launched_member_steps", "ms_per_step", "ensemble_vs_sequential",
             "ensemble_vs_sequential_host_loop", "max_rel_energy_change", "energy_error_vs_fp64")}

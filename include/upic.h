@@ -484,13 +484,15 @@ upic_status upic_sim_guess_params(upic_sim* sim, float* params);          /* UPI
 upic_status upic_sim_set_guess_params(upic_sim* sim, const float* params); /* and in (installs the network) */
 /* ---- Deposit autotuner (mixed precision) --------------------------------- */
 /* No reference counterpart (the reference deposits inside its per-particle
- * loop, P3:5380-5409).  The mixed-precision current deposit has six
+ * loop, P3:5380-5409).  The mixed-precision current deposit has eight
  * variants: 0 particle order with a wavefront reduction of the common
  * stencil window into an 8 x 32-node LDS tile, 1 one workgroup per 64
- * cells, 2 eight workgroups per 64 cells, 3 particle order with
- * per-particle atomics, 4 and 5 the wavefront reduction into 4 x 64 and
- * 16 x 16-node tiles; 1 and 2 need a freshly cell-sorted particle order and
- * ny % 64 == 0.  By default (variant -1) the
+ * cells of a row, 2 eight workgroups per 64 cells of a row, 3 particle
+ * order with per-particle atomics, 4 and 5 the wavefront reduction into
+ * 4 x 64 and 16 x 16-node tiles, 6 and 7 as 1 and 2 on 8 x 8-cell blocks;
+ * 1, 2, 6 and 7 need a freshly cell-sorted particle order (1 and 2:
+ * ny % 64 == 0; 6 and 7: nx, ny multiples of 8 and at least 16).  By
+ * default (variant -1) the
  * first map evaluations time every usable variant twice, one variant per
  * evaluation, and then keep the fastest, separately for sorted and unsorted
  * particle orders.  A variant >= 0 pins it (an unusable pin runs variant 0).

@@ -295,10 +295,10 @@ def test_c4_ensemble_members_vs_oracle(orc, graphs):
     4 to 16 particles per cell) advanced together by upic_ensemble_step in
     the mixed form with the predictor guess and the selective refinement on;
     each member against its own FP64 oracle run over 5 steps (graphs 0,
-    host threads) or 12 steps (graphs 1: once the deposit autotuners have
-    timed their variants -- one timing per step, in the step's first
-    evaluation -- every step of the whole ensemble is one graph launch),
-    mixed bounds."""
+    host threads) or 16 steps (graphs 1: once the deposit autotuners have
+    timed their six usable variants twice -- one timing per step, in the
+    step's first evaluation -- every step of the whole ensemble is one graph
+    launch), mixed bounds."""
     lib = gpu()
     members = []
     for kind, side, ppc, vth, amp, seed in (("landau", 16, 8, 0.05, 5e-3, 3), ("weibel", 32, 4, 0.2, 0.0, 4),
@@ -317,7 +317,7 @@ def test_c4_ensemble_members_vs_oracle(orc, graphs):
         ctx, sim = upload(lib, pm, st)
         ctx.set_graphs(graphs)
         members.append((p64, st, ctx, sim))
-    steps = 12 if graphs else 5
+    steps = 16 if graphs else 5
     print(f"c4 ensemble: graphs {graphs}, {steps} steps", flush=True)
     rows = [[m[3].diagnostics(1)] for m in members]
     infos = [[] for _ in members]

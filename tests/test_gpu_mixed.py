@@ -157,14 +157,14 @@ def test_mixed_vs_fp64_gpu_history():
 
 @pytest.mark.parametrize("split", [1, 0])
 def test_deposit_variants_agree_and_autotune(split):
-    """64x64 grid, 16 particles per cell, sorted every step: four steps with
-    each deposit variant (kernels and tile shapes) pinned reach the same
-    fixed point (total energy
-    1e-6, field energy 1e-4, positions 1e-5 of the box), and the autotuner
-    settles on a usable variant.  Without knot splitting the sort leaves the
-    particles in plain cell order, which the cell-mapped variants (1, 2) need;
-    with it, crossing orbits are grouped after the others and 1, 2 fall back
-    to variant 0."""
+    """64x64 grid, 16 particles per cell, sorted every step: six steps with
+    each deposit variant (kernels, tile shapes and cell blocks) pinned reach
+    the same fixed point (total energy 1e-6, field energy 1e-4, positions
+    1e-5 of the box), and the autotuner settles on a usable variant.
+    Without knot splitting the sort leaves the particles in plain cell
+    order, which the cell-mapped variants (1, 2 on row segments, 6, 7 on
+    8 x 8 blocks) need; with it, crossing orbits are grouped after the
+    others and those fall back to variant 0."""
     lib = gpu()
     p64 = make_params(code=CODE_P3, nx=64, ny=64, nz=1, Lx=8 * np.pi, Ly=8 * np.pi, Lz=1.0, dt=0.05,
                       spline_order=3, orbit_quad_order=4, nyquist_projection=1, nonlinear_rtol=1e-11,
@@ -179,13 +179,14 @@ def test_deposit_variants_agree_and_autotune(split):
     st.x[:], st.v[:], st.q[:], st.m[:] = x, v, q, m
     ctx = lib.Context(pm)
     res = {}
-    for variant in (-1, 0, 1, 2, 3, 4, 5):
+    variants = tuple(range(8))
+    for variant in (-1,) + variants:
         sim = lib.Simulation(ctx, n)
         sim.upload(st.copy())
         sim.init_fields(1.0)
         sim.set_deposit_variant(variant)
         rows = [sim.diagnostics(1)]
-        for _ in range(4):
+        for _ in range(6):
             assert sim.step().converged
             rows.append(sim.diagnostics(1))
         out = State(n, st.N)
@@ -198,10 +199,10 @@ def test_deposit_variants_agree_and_autotune(split):
     if split:
         assert chosen[1] in (0, 3, 4, 5), chosen
     else:
-        assert chosen[0] in (0, 1, 2, 3, 4, 5), chosen
+        assert chosen[0] in variants, chosen
     e0, x0 = res[0]
     assert np.max(np.abs(e0[:, 0] - e0[0, 0])) / e0[0, 0] < 1e-6
-    for variant in (-1, 1, 2, 3, 4, 5):
+    for variant in (-1,) + variants[1:]:
         e, xv = res[variant]
         assert np.max(np.abs(e[:, 0] - e0[:, 0]) / e0[:, 0]) < 1e-6, variant
         assert np.max(np.abs(e[:, 1] - e0[:, 1]) / e0[:, 1]) < 1e-4, variant

@@ -1041,31 +1041,39 @@ __global__ __launch_bounds__(256) void k_deposit_delta_mixed(MGrid g, QuadF qd, 
     block_partials<2, 1>(sums, maxs, 0, a.part);
 }
 
-// Cell-mapped deposit (cell-sorted particles, ny a multiple of 64): a
-// workgroup owns 64 consecutive cells of one grid row; lane l of every wave
-// takes cell c0 + l, and the kCellWaves waves of the kCellSlices workgroups
-// (grid y) of that row segment split each cell's particles.
-// The 64 lanes of an LDS atomic then address 64 different stencil windows
-// instead of one (particles of one cell share their stencil), which removes
-// the 64-way LDS bank serialisation of the particle-order deposit.  The LDS
-// tile spans rows i0-3..i0+4 and columns j0-3..j0+68 of the workgroup's row.
+// Cell-mapped deposit (cell-sorted particles): a workgroup owns a block of
+// 64 cells, bw rows by 64 / bw columns (bw = 1: 64 consecutive cells of one
+// grid row; bw = 8: an 8 x 8 block); lane l of every wave takes one cell of
+// the block, and the kCellWaves waves of the kCellSlices workgroups (grid y)
+// of that block split each cell's particles.  The 64 lanes of an LDS atomic
+// then address 64 different stencil windows instead of one (particles of one
+// cell share their stencil), which removes the 64-way LDS bank serialisation
+// of the particle-order deposit.  The LDS tile spans rows i0-3..i0+bw+4 and
+// columns j0-3..j0+64/bw+4 of the block.  The square block flushes about
+// half as many nodes per cell to the mesh as the row segment (each node of
+// a block's tile is added to the mesh with one FP64 atomic per component,
+// and neighbouring blocks repeat the nodes they share).
 constexpr int kCellWaves = 4, kCellSlices = 8, kCTR = 8, kCTC = 72;
+constexpr int kCTiles = kCTR * kCTC;  // LDS nodes, either shape (8 x 72 or 16 x 16)
 
 template <int R>
 __global__ __launch_bounds__(256) void k_deposit_cells_mixed(MGrid g, QuadF qd, int split, P3MArgs a,
-                                                             const int* __restrict__ cell_end) {
+                                                             const int* __restrict__ cell_end, int bw) {
     if (UPIC_GATE_CLOSED(a.gate)) return;
-    __shared__ float tile[kCTR * kCTC * 3];
-    const int c0 = blockIdx.x * 64;
-    const int i0 = c0 / g.ny, j0 = c0 - (c0 / g.ny) * g.ny;
+    __shared__ float tile[kCTiles * 3];
+    const int bh = 64 / bw;  // block columns
+    const int nbj = g.ny / bh;
+    const int i0 = (blockIdx.x / nbj) * bw, j0 = (blockIdx.x - (blockIdx.x / nbj) * nbj) * bh;
+    // tile rows and columns: the block, 3 before and 5 after
+    const int tr = bw == 1 ? kCTR : bw + 8, tc = bw == 1 ? kCTC : bh + 8;
     const int ox = wrapn(i0 - 3, g.nx), oy = wrapn(j0 - 3, g.ny);
-    for (int e = threadIdx.x; e < kCTR * kCTC * 3; e += blockDim.x) tile[e] = 0.0f;
+    for (int e = threadIdx.x; e < kCTiles * 3; e += blockDim.x) tile[e] = 0.0f;
     __syncthreads();
     const int lane = threadIdx.x & 63, sub = threadIdx.x >> 6;
-    const int c = c0 + lane;
+    const int ci = i0 + lane / bh, cj = j0 + lane - (lane / bh) * bh;
+    const int c = ci * g.ny + cj;
     const int64_t kb = c == 0 ? 0 : cell_end[c - 1], ke = cell_end[c];
     double cross = 0.0, segs = 0.0, mseg = 0.0;
-    const int ci = i0, cj = j0 + lane;
     const int stride = kCellWaves * gridDim.y;
     for (int64_t p = kb + sub + kCellWaves * blockIdx.y; p < ke; p += stride) {
         if (a.reason[p] != 0) continue;
@@ -1092,8 +1100,8 @@ __global__ __launch_bounds__(256) void k_deposit_cells_mixed(MGrid g, QuadF qd, 
                     int dy = ib - oy;
                     if (dy < 0) dy += g.ny;
                     const float w = W[aa][bb] * qs;
-                    if (dx < kCTR && dy < kCTC) {
-                        float* t = tile + 3 * (dx * kCTC + dy);
+                    if (dx < tr && dy < tc) {
+                        float* t = tile + 3 * (dx * tc + dy);
                         atomicAdd(t, w * vb[0]);
                         atomicAdd(t + 1, w * vb[1]);
                         atomicAdd(t + 2, w * vb[2]);
@@ -1150,11 +1158,11 @@ __global__ __launch_bounds__(256) void k_deposit_cells_mixed(MGrid g, QuadF qd, 
         mseg = fmax(mseg, (double)nseg);
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < kCTR * kCTC * 3; e += blockDim.x) {
+    for (int e = threadIdx.x; e < tr * tc * 3; e += blockDim.x) {
         const float v = tile[e];
         if (v != 0.0f) {
             const int comp = e % 3, node = e / 3;
-            const int dx = node / kCTC, dy = node - (node / kCTC) * kCTC;
+            const int dx = node / tc, dy = node - (node / tc) * tc;
             if (dx >= g.nx || dy >= g.ny) continue;
             const int ia = (ox + dx) % g.nx, ib = (oy + dy) % g.ny;
             unsafeAtomicAdd(&a.J_out[(size_t)comp * g.N + ia * g.ny + ib], (double)v);
@@ -1553,9 +1561,11 @@ void launch_deposit_delta_mixed(hipStream_t s, const GridDev& g, const QuadF& q,
 // 0: wavefront-reduced deposit on an 8 x 32 tile, 1 and 2: cell-mapped
 // deposits (1 or kCellSlices workgroups per 64 cells), 3: per-lane atomics,
 // 4 and 5: the wavefront-reduced deposit on 4 x 64 and 16 x 16 tiles
-int deposit_variants() { return 6; }
+int deposit_variants() { return 8; }
 bool deposit_variant_usable(const GridDev& g, int variant, bool cells_fresh) {
     if (variant == 1 || variant == 2) return cells_fresh && g.ny % 64 == 0 && g.nx >= kCTR;
+    // 8 x 8 blocks; the 16 x 16 tile must not wrap onto itself
+    if (variant == 6 || variant == 7) return cells_fresh && g.nx % 8 == 0 && g.ny % 8 == 0 && g.nx >= 16 && g.ny >= 16;
     return true;
 }
 
@@ -1583,18 +1593,19 @@ void launch_deposit_mixed(hipStream_t s, const GridDev& g, const QuadF& q, int s
         launch_finish_partials(s, a.part, mblk(a.np), 0, 0, nullptr, nullptr, a.stats);
         return;
     }
-    if (variant == 1 || variant == 2) {
+    if (variant == 1 || variant == 2 || variant == 6 || variant == 7) {
         const int nb = g.N / 64;
         // the partial slots of nb * slices workgroups must fit the
         // per-particle-block buffer (sized for ceil(np / 256) blocks)
-        const int want = variant == 1 ? 1 : kCellSlices;
+        const int want = (variant == 1 || variant == 6) ? 1 : kCellSlices;
         const int sl = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)mblk(a.np) / std::max(nb, 1)));
+        const int bw = variant >= 6 ? 8 : 1;
         const dim3 gr(nb, sl), bl(64 * kCellWaves);
         switch (g.order) {
-            case 1: hipLaunchKernelGGL(k_deposit_cells_mixed<1>, gr, bl, 0, s, mg, q, split, a, cell_end); break;
-            case 2: hipLaunchKernelGGL(k_deposit_cells_mixed<2>, gr, bl, 0, s, mg, q, split, a, cell_end); break;
-            case 3: hipLaunchKernelGGL(k_deposit_cells_mixed<3>, gr, bl, 0, s, mg, q, split, a, cell_end); break;
-            case 4: hipLaunchKernelGGL(k_deposit_cells_mixed<4>, gr, bl, 0, s, mg, q, split, a, cell_end); break;
+            case 1: hipLaunchKernelGGL(k_deposit_cells_mixed<1>, gr, bl, 0, s, mg, q, split, a, cell_end, bw); break;
+            case 2: hipLaunchKernelGGL(k_deposit_cells_mixed<2>, gr, bl, 0, s, mg, q, split, a, cell_end, bw); break;
+            case 3: hipLaunchKernelGGL(k_deposit_cells_mixed<3>, gr, bl, 0, s, mg, q, split, a, cell_end, bw); break;
+            case 4: hipLaunchKernelGGL(k_deposit_cells_mixed<4>, gr, bl, 0, s, mg, q, split, a, cell_end, bw); break;
             default: break;
         }
         launch_finish_partials(s, a.part, (int64_t)nb * sl, 0, 0, nullptr, nullptr, a.stats);
