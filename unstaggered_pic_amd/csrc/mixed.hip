@@ -1261,9 +1261,16 @@ __global__ __launch_bounds__(256) void k_deposit_wave_mixed(MGrid g, QuadF qd, i
     // the window of the first pending lane, sums W (masked to that window's
     // lanes) times q v / dV over the wavefront through LDS, one lane per
     // (node, component), and adds the sums to the tile.  A wavefront of
-    // cell-ordered particles holds one or two windows; lanes still pending
-    // after kRounds rounds use per-lane atomics.
-    constexpr int kRounds = 2;
+    // cell-ordered particles holds one or two windows, one of the group of
+    // knot-crossing orbits several (one per cell it spans); lanes still
+    // pending after kRounds rounds use per-lane atomics.  Eight rounds rather
+    // than two: on cold counter-streaming beams (1024^2, 64 per cell) the
+    // per-lane LDS float atomics of the crossing group cost 10.4 against 7.1
+    // ms per deposit, and their order-dependent FP32 sums left the outer
+    // solve 7.7 instead of 5 evaluations per step at the FP32 tolerance floor
+    // (232 -> 148 ms per step); the Weibel and Landau decks are unchanged or
+    // faster.
+    constexpr int kRounds = 8;
     float* rw = red[wv] + lane * ROW;
     rw[NW] = sv[0];
     rw[NW + 1] = sv[1];
