@@ -1287,11 +1287,16 @@ __global__ __launch_bounds__(256) void k_deposit_wave_mixed(MGrid g, QuadF qd, i
 #pragma unroll
             for (int bb = 0; bb <= R; ++bb) rw[aa * (R + 1) + bb] = in ? W[aa][bb] : 0.0f;
         wave_sync_lds();
-        // column (node, component) sums over the 64 lanes, 3 (R+1)^2 columns
+        // column (node, component) sums over the window's lanes (in lane
+        // order; the other lanes hold zero weights, which add nothing), 3
+        // (R+1)^2 columns
         for (int col = lane; col < 3 * NW; col += 64) {
             const int node = col / 3, comp = col - 3 * (col / 3);
             float acc = 0.0f;
-            for (int l = 0; l < 64; ++l) acc += red[wv][l * ROW + node] * red[wv][l * ROW + NW + comp];
+            for (unsigned long long mk = grp; mk; mk &= mk - 1ull) {
+                const int l = __ffsll((long long)mk) - 1;
+                acc += red[wv][l * ROW + node] * red[wv][l * ROW + NW + comp];
+            }
             if (acc != 0.0f) {
                 const int ia = wrap1n(lgx + node / (R + 1), g.nx), ib = wrap1n(lgy + node % (R + 1), g.ny);
                 tilef_add1(g, T, a.J_out, ia, ib, comp, acc);
