@@ -44,9 +44,24 @@ def test_trace_stats_after_marker(tmp_path):
         w.writerow(["Kernel_Name", "Start_Timestamp", "End_Timestamp"])
         for r in rows:
             w.writerow(r)
-    agg, span = profile_summary.trace_stats(str(tmp_path), "k_mlp")
+    agg, span, work = profile_summary.trace_stats(str(tmp_path), "k_mlp")
     assert agg == {"k_a": [2, 15], "k_b": [1, 20]}
     assert span == 45
+    assert work == {"k_a": [10, 5], "k_b": [20]}
+
+
+def test_gated_launches_left_out_of_means(tmp_path):
+    """A launch whose gate was closed returns at once: below 1% of the
+    kernel's largest value it does not enter the means."""
+    path = tmp_path / "run_counter_collection.csv"
+    with open(path, "w", newline="") as fh:
+        w = csv.writer(fh)
+        w.writerow(["Kernel_Name", "Counter_Name", "Counter_Value"])
+        for v in (100.0, 0.5, 102.0):
+            w.writerow(["k", "SQ_INSTS_VALU", v])
+    d = profile_summary.counters_by_name(str(tmp_path))
+    assert d["k"]["SQ_INSTS_VALU"] == (101.0, 2)
+    assert profile_summary.counters(str(tmp_path))["k"] == [100.0, 102.0]
 
 
 def test_counters_by_name_averages_last_dispatches(tmp_path):
