@@ -523,6 +523,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ensemble-members", type=int, default=64,
                     help="members of the batched-ensemble sub-object of the default run (0: none)")
+    ap.add_argument("--ensemble-guess", choices=["predictor", "learned"], default="predictor",
+                    help="the ensemble members' inner start (learned: one network per load kind)")
     ap.add_argument("--ensemble-steps", type=int, default=20,
                     help="timed ensemble steps of that sub-object (after 18 warmup steps; the sequential "
                          "baselines: 5, the per-member FP64 comparison: 5)")

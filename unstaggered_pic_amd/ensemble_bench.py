@@ -76,10 +76,46 @@ class _Host:
 
 def run(args):
     """bench.py --form ensemble: the measurement as the bench's one JSON line."""
-    print(json.dumps(measure(args.members, args.steps, args.warmup, args.baseline_steps, args.fp64_steps)))
+    print(json.dumps(measure(args.members, args.steps, args.warmup, args.baseline_steps, args.fp64_steps,
+                             guess=getattr(args, "ensemble_guess", "predictor"))))
 
 
-def measure(members, steps_timed, warmup, baseline_steps, fp64_steps):
+def train_per_kind(lib, specs, loads, collect_steps=40, rows_per_step=4096, epochs=15):
+    """One network per load kind (Landau, Weibel): rows from collect_steps
+    steps of the kind's largest member, trained on the device; returns
+    {kind: parameters} and the seconds spent."""
+    t0 = time.perf_counter()
+    nets = {}
+    for kind in ("landau", "weibel"):
+        ks = [k for k, sp in enumerate(specs) if sp["kind"] == kind]
+        if not ks:
+            continue
+        k = max(ks, key=lambda j: loads[j][1])
+        p, n, (x, v, q, m) = loads[k]
+        ctx = lib.Context(p)
+        sim = lib.Simulation(ctx, n)
+        h = _Host(n, p.nx * p.ny)
+        h.x[:], h.v[:], h.q[:], h.m[:] = x, v, q, m
+        sim.upload(h)
+        sim.init_fields(1.0)
+        stride = max(1, n // rows_per_step)
+        per = (n + stride - 1) // stride
+        sim.collect_guess_data(stride, per * collect_steps)
+        for _ in range(collect_steps):
+            sim.step()
+        sim.collect_guess_data(0, 0)
+        sim.train_guess(epochs=epochs, batch=4096, lr=2e-3, seed=5)
+        nets[kind] = sim.guess_params()
+        sim.close()
+        ctx.close()
+        print(f"# trained the {kind} network on member {k} ({n} particles)", file=sys.stderr, flush=True)
+    return nets, time.perf_counter() - t0
+
+
+def measure(members, steps_timed, warmup, baseline_steps, fp64_steps, guess="predictor"):
+    """guess: "predictor" (the physics predictor as every member's inner
+    start) or "learned" (predictor + network: one network per load kind,
+    trained on that kind's largest member and installed in all of them)."""
     from unstaggered_pic_amd import lib
     specs = member_specs(members)
     loads = []
@@ -88,6 +124,7 @@ def measure(members, steps_timed, warmup, baseline_steps, fp64_steps):
         if (i + 1) % 8 == 0 or i + 1 == len(specs):
             print(f"# loaded {i + 1}/{len(specs)} members", file=sys.stderr, flush=True)
     total = sum(n for _, n, _ in loads)
+    nets, train_s = train_per_kind(lib, specs, loads) if guess == "learned" else ({}, 0.0)
 
     def fresh(graphs=True, fp64=False):
         ms = []
@@ -101,6 +138,9 @@ def measure(members, steps_timed, warmup, baseline_steps, fp64_steps):
             h.x[:], h.v[:], h.q[:], h.m[:] = x, v, q, m
             sim.upload(h)
             sim.init_fields(1.0)
+            if nets and not fp64:
+                sim.set_guess_params(nets[specs[k]["kind"]])
+                sim.set_guess_mode(2)
             ms.append((ctx, sim))
         return ms
 
@@ -165,7 +205,10 @@ def measure(members, steps_timed, warmup, baseline_steps, fp64_steps):
         "warmup": warmup, "ms_per_step": out["ensemble"]["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64",
         "data": "synthetic (seeded Landau and bi-Maxwellian loads)",
-        "config": {"workload": f"ensemble_{members}_members_p3_mixed_predictor", "particles": total,
+        "config": {"workload": f"ensemble_{members}_members_p3_mixed_{guess}", "particles": total,
+                   "guess": guess if guess == "predictor" else
+                   "learned: one network per load kind, trained on its largest member (40 collection steps, "
+                   f"15 epochs; {train_s:.1f} s outside the timed region)",
                    "members": specs},
         "sequential": out["sequential"], "ensemble_vs_sequential": out["ensemble"]["value"] / out["sequential"]["value"],
         "max_rel_energy_change": out["ensemble"]["max_rel_energy_change"],

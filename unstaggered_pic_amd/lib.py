@@ -463,7 +463,7 @@ class Simulation:
         self.ctx._check(lib().upic_sim_set_guess_mode(self.h, int(mode)))
 
     def set_deposit_variant(self, variant):
-        """Pin the mixed-precision deposit variant (0-3) or re-enable the
+        """Pin the mixed-precision deposit variant (0-7) or re-enable the
         autotuner (-1)."""
         self.ctx._check(lib().upic_sim_set_deposit_variant(self.h, int(variant)))
 
