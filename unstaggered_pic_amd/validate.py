@@ -60,6 +60,7 @@ def run(args, prec, learned):
     t0 = time.perf_counter()
     k = 0
     if learned:
+        sim.set_guess_mode(1)  # the collection steps start from the physics predictor
         stride = max(1, n // 16384)
         per = (n + stride - 1) // stride
         sim.collect_guess_data(stride, per * args.collect_steps)
