@@ -729,7 +729,11 @@ __device__ __forceinline__ void map_particle(const MGrid& g, const QuadF& qd, in
                                              const float gin[3], int it0, bool defer, const float feat[12],
                                              const float dph[3], bool has_a0, float a0x, float a0y, float a0z,
                                              const DepTileF& T, double rsum[2], double rmax[2], int& nseg_total) {
-    const int nsub = a.layout ? (int)a.layout[p] : 1;
+    // Only the fused sweep runs per-particle substep layouts; the others
+    // (and the retry sweep) are launched for the uniform layout, so their
+    // substep loop, start-point gather and substep-path recording compile
+    // away (they had cost the later cubic sweep 244 VGPRs and 14%).
+    const int nsub = FUSED && a.layout ? (int)a.layout[p] : 1;
     const float h = (float)(a.dt / (double)nsub);
     const int c0 = a.cell[p];
     int ci = c0 / g.ny, cj = c0 - (c0 / g.ny) * g.ny;
@@ -821,15 +825,17 @@ __device__ __forceinline__ void map_particle(const MGrid& g, const QuadF& qd, in
                 }
             }
         }
-        if (a.write_out && a.sub.x0) {
-            // the substep's start point and vbar, as FP64 positions (the
-            // FP64 view's -L/2 + (cell + offset) h)
-            const int64_t j = a.sub.off[p] + s, n = a.sub.n;
-            a.sub.x0[j] = a.lo_x + ((double)ci + (double)tx) * a.hx;
-            a.sub.x0[n + j] = a.lo_y + ((double)cj + (double)ty) * a.hy;
-            a.sub.x0[2 * n + j] = a.lo_z + (double)tz * a.Lz;
+        if constexpr (FUSED) {
+            if (a.write_out && a.sub.x0) {
+                // the substep's start point and vbar, as FP64 positions (the
+                // FP64 view's -L/2 + (cell + offset) h)
+                const int64_t j = a.sub.off[p] + s, n = a.sub.n;
+                a.sub.x0[j] = a.lo_x + ((double)ci + (double)tx) * a.hx;
+                a.sub.x0[n + j] = a.lo_y + ((double)cj + (double)ty) * a.hy;
+                a.sub.x0[2 * n + j] = a.lo_z + (double)tz * a.Lz;
 #pragma unroll
-            for (int d = 0; d < 3; ++d) a.sub.vbar[d * n + j] = (double)vacc[d];
+                for (int d = 0; d < 3; ++d) a.sub.vbar[d * n + j] = (double)vacc[d];
+            }
         }
         const float dux = h * vacc[0] * g.ihx, duy = h * vacc[1] * g.ihy;
         if constexpr (FUSED) {
@@ -883,7 +889,7 @@ __device__ __forceinline__ void p3_map_mixed_body(const MGrid& g, const QuadF& q
     float feat[12] = {}, dph[3] = {0.0f, 0.0f, 0.0f}, lg[3] = {0.0f, 0.0f, 0.0f}, a0s[3] = {0.0f, 0.0f, 0.0f};
     constexpr bool PRED = GM == 1 || GM == 2;
     if constexpr (PRED || COLLECT) {
-        if (p < a.np && !a.layout) {
+        if (p < a.np && (!FUSED || !a.layout)) {
             const int c0 = a.cell[p];
             float v0[3];
             ld3f(a.v, a.np, p, v0);
@@ -907,9 +913,11 @@ __device__ __forceinline__ void p3_map_mixed_body(const MGrid& g, const QuadF& q
 #pragma unroll
             for (int d = 0; d < 3; ++d) gin[d] += dph[d] + (GM == 2 ? lg[d] : 0.0f);
         }
+        // (the unfused sweeps never run a substep layout)
+        const bool uniform = !FUSED || !a.layout;
         map_particle<R, COLLECT, FUSED, !FUSED && GM != 2>(g, qd, split, a, p, gin, 0,
-                                                           a.retry_n != nullptr && !a.layout, feat, dph,
-                                        (PRED || COLLECT || GM == 3) && !a.layout, a0s[0], a0s[1], a0s[2], T,
+                                                           a.retry_n != nullptr && uniform, feat, dph,
+                                        (PRED || COLLECT || GM == 3) && uniform, a0s[0], a0s[1], a0s[2], T,
                                         rsum, rmax, nseg_total);
     }
     if constexpr (FUSED) {
