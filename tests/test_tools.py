@@ -51,8 +51,8 @@ def test_trace_stats_after_marker(tmp_path):
 
 
 def test_gated_launches_left_out_of_means(tmp_path):
-    """A launch whose gate was closed returns at once: below 1% of the
-    kernel's largest value it does not enter the means."""
+    """A launch whose gate was closed returns at once: below 5% of the
+    kernel's median it does not enter the means."""
     path = tmp_path / "run_counter_collection.csv"
     with open(path, "w", newline="") as fh:
         w = csv.writer(fh)
