@@ -80,30 +80,53 @@ struct AxF {
     int fl;
     float w[R + 1], dw[R + 1];  // dw already divided by h
 };
-// The last weight is the complement of the others and the last derivative
-// weight minus their sum: the FP32 closed forms round with a bias (2/3 rounds
-// up, so the cubic weights sum to 1 + 4e-8 on average), and a partition of
-// unity off by a bias scales every gather and deposit coherently, which
-// shows up as an energy drift; the complement leaves only unbiased rounding.
+// One weight is the complement of the others and its derivative weight
+// minus their sum: the FP32 closed forms round with a bias (2/3 rounds up,
+// so the cubic weights sum to 1 + 4e-8 on average), and a partition of unity
+// off by a bias scales every gather and deposit coherently, which shows up as
+// an energy drift; the complement leaves only unbiased rounding.
+//
+// Quadratic and cubic take the weights as polynomials of the in-cell
+// fraction u (the distance past the stencil's centre knot, in [0, 1)): the
+// same B-spline pieces bsp_f selects by |r|, without evaluating and selecting
+// every piece per node; the complement goes to a middle weight, the largest.
+// Other degrees evaluate bsp_f per node, with the last weight the complement.
 template <int R>
 __device__ __forceinline__ void axis_f(float t, float ih, AxF<R>& s) {
     const float f = floorf(t - 0.5f * (float)(R - 1));
     s.fl = (int)f;
-    const float r0 = t - f;
-    float sw = 0.0f, sdw = 0.0f;
+    if constexpr (R == 3) {
+        // nodes at r = 1+u, u, u-1, u-2 from the particle
+        const float u = (t - f) - 1.0f, v = 1.0f - u;
+        const float u2 = u * u, v2 = v * v;
+        const float w0 = v2 * v * (1.0f / 6.0f), w3 = u2 * u * (1.0f / 6.0f);
+        const float w1 = (2.0f / 3.0f) - u2 + 0.5f * u2 * u;
+        const float d0 = -0.5f * v2, d1 = u * (1.5f * u - 2.0f), d3 = 0.5f * u2;
+        s.w[0] = w0; s.w[1] = w1; s.w[2] = 1.0f - (w0 + w1 + w3); s.w[3] = w3;
+        s.dw[0] = d0 * ih; s.dw[1] = d1 * ih; s.dw[2] = -(d0 + d1 + d3) * ih; s.dw[3] = d3 * ih;
+    } else if constexpr (R == 2) {
+        // nodes at r = u+1/2, u-1/2, u-3/2
+        const float u = (t - f) - 0.5f, v = 1.0f - u;
+        const float w0 = 0.5f * v * v, w2 = 0.5f * u * u;
+        s.w[0] = w0; s.w[1] = 1.0f - (w0 + w2); s.w[2] = w2;
+        s.dw[0] = -v * ih; s.dw[1] = (v - u) * ih; s.dw[2] = u * ih;
+    } else {
+        const float r0 = t - f;
+        float sw = 0.0f, sdw = 0.0f;
 #pragma unroll
-    for (int a = 0; a < R; ++a) {
-        float w, dw;
-        bsp_f<R>(r0 - (float)a, w, dw);
-        s.w[a] = w;
-        s.dw[a] = dw;
-        sw += w;
-        sdw += dw;
+        for (int a = 0; a < R; ++a) {
+            float w, dw;
+            bsp_f<R>(r0 - (float)a, w, dw);
+            s.w[a] = w;
+            s.dw[a] = dw;
+            sw += w;
+            sdw += dw;
+        }
+        s.w[R] = 1.0f - sw;
+        s.dw[R] = -sdw;
+#pragma unroll
+        for (int a = 0; a <= R; ++a) s.dw[a] *= ih;
     }
-    s.w[R] = 1.0f - sw;
-    s.dw[R] = -sdw;
-#pragma unroll
-    for (int a = 0; a <= R; ++a) s.dw[a] *= ih;
 }
 
 // Knot crossings of the two in-plane axes in increasing s (the merge of
