@@ -474,7 +474,7 @@ upic_status upic_sim_train_guess(upic_sim* sim, int32_t epochs, int32_t batch, d
  * inner tolerance, which slows some outer solves (weak net currents, the
  * first steps of a run whose network guesses pass at their first
  * iteration): when two steps in a row that start from the extrapolated
- * current 2 J^n - J^{n-1} (every step but a run's first) need more than
+ * current (every step but a run's first) need more than
  * four outer evaluations, every evaluation starts from mode 1 or 2 for the
  * next 16 steps, and the vbar start is then tried again (each relapse
  * doubles the pause, up to 1024 steps).  Mode 0 keeps the reference's v^n
