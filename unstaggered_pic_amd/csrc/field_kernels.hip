@@ -7,46 +7,55 @@
 namespace upic {
 
 // ---------------------------------------------------------------------------
-// FFT: one workgroup transforms LPB lines of length n staged in LDS
-// (bit-reversed load, log2 n radix-2 Danielson-Lanczos stages, store), the
+// FFT: one workgroup transforms 2^llpb lines of length n = 2^logn staged in
+// LDS (bit-reversed load, logn radix-2 Danielson-Lanczos stages, store), the
 // algorithm of fft1d (P1:424-454) with tabulated twiddles.  Lines along a
-// strided axis are loaded line-fastest so neighbouring lanes touch
-// neighbouring addresses.
+// strided axis (stride 2^lstride) are loaded line-fastest so neighbouring
+// lanes touch neighbouring addresses.  Every size is a power of two (the
+// grid's sides are), so the index arithmetic is shifts and masks.  TWL: the
+// n/2 twiddles are staged in LDS with the lines; each stage then waits on
+// LDS only, not on a global load (the stages are a chain of barriers, and
+// the 256^2 transforms are latency-bound).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_fft_axis(double* __restrict__ data, int N, int n, int logn, int stride,
-                                                  int lines, int lpb, const double* __restrict__ tw, int inverse,
+template <bool TWL>
+__global__ __launch_bounds__(256) void k_fft_axis(double* __restrict__ data, int N, int logn, int lstride, int lines,
+                                                  int llpb, const double* __restrict__ tw, int inverse,
                                                   const int* __restrict__ gate) {
     if (UPIC_GATE_CLOSED(gate)) return;
-    extern __shared__ double lds[];  // lpb * n complex
+    extern __shared__ double lds[];  // lpb * n complex (+ n/2 twiddles)
     double2* a = reinterpret_cast<double2*>(lds);
+    const int n = 1 << logn, lpb = 1 << llpb;
+    const int total = lpb << logn;
+    const double2* twg = reinterpret_cast<const double2*>(tw);
+    double2* twl = a + total;
+    if constexpr (TWL)
+        for (int e = threadIdx.x; e < (n >> 1); e += blockDim.x) twl[e] = twg[e];
     const int field = blockIdx.y;
     double2* base_ptr = reinterpret_cast<double2*>(data) + (int64_t)field * N;
-    const int line0 = blockIdx.x * lpb;
-    const int total = lpb * n;
+    const int line0 = blockIdx.x << llpb;
+    const int smask = (1 << lstride) - 1;
     // load with bit reversal
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
         int lb, t;
-        if (stride == 1) { lb = e / n; t = e - lb * n; }
-        else { t = e / lpb; lb = e - t * lpb; }
+        if (lstride == 0) { lb = e >> logn; t = e & (n - 1); }
+        else { t = e >> llpb; lb = e & (lpb - 1); }
         const int l = line0 + lb;
         if (l < lines) {
-            const int base = (l / stride) * (n * stride) + (l % stride);
+            const int base = ((l >> lstride) << (logn + lstride)) + (l & smask);
             const int r = (int)(__brev((unsigned)t) >> (32 - logn));
-            a[lb * n + r] = base_ptr[base + t * stride];
+            a[(lb << logn) + r] = base_ptr[base + (t << lstride)];
         }
     }
     __syncthreads();
-    const int halfn = n >> 1;
-    const int nbf = lpb * halfn;
+    const int nbf = total >> 1;
     for (int s = 1; s <= logn; ++s) {
-        const int len = 1 << s, half = len >> 1;
-        const int tstep = n >> s;  // twiddle index stride
+        const int hs = s - 1;  // log2 of the half length
         for (int bi = threadIdx.x; bi < nbf; bi += blockDim.x) {
-            const int lb = bi / halfn, j = bi - lb * halfn;
-            const int grp = j / half, pos = j - grp * half;
-            const int i0 = lb * n + grp * len + pos, i1 = i0 + half;
-            double wr = tw[2 * (pos * tstep)], wi = tw[2 * (pos * tstep) + 1];
-            if (inverse) wi = -wi;
+            const int lb = bi >> (logn - 1), j = bi & ((n >> 1) - 1);
+            const int grp = j >> hs, pos = j & ((1 << hs) - 1);
+            const int i0 = (lb << logn) + (grp << s) + pos, i1 = i0 + (1 << hs);
+            const double2 w = TWL ? twl[pos << (logn - s)] : twg[pos << (logn - s)];
+            const double wr = w.x, wi = inverse ? -w.y : w.y;
             const double2 u = a[i0], x = a[i1];
             const double2 v = make_double2(x.x * wr - x.y * wi, x.x * wi + x.y * wr);
             a[i0] = make_double2(u.x + v.x, u.y + v.y);
@@ -57,14 +66,14 @@ __global__ __launch_bounds__(256) void k_fft_axis(double* __restrict__ data, int
     const double sc = inverse ? 1.0 / (double)n : 1.0;
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
         int lb, t;
-        if (stride == 1) { lb = e / n; t = e - lb * n; }
-        else { t = e / lpb; lb = e - t * lpb; }
+        if (lstride == 0) { lb = e >> logn; t = e & (n - 1); }
+        else { t = e >> llpb; lb = e & (lpb - 1); }
         const int l = line0 + lb;
         if (l < lines) {
-            const int base = (l / stride) * (n * stride) + (l % stride);
-            double2 v = a[lb * n + t];
+            const int base = ((l >> lstride) << (logn + lstride)) + (l & smask);
+            double2 v = a[(lb << logn) + t];
             if (inverse) { v.x *= sc; v.y *= sc; }
-            base_ptr[base + t * stride] = v;
+            base_ptr[base + (t << lstride)] = v;
         }
     }
 }
@@ -73,18 +82,23 @@ void launch_fft(hipStream_t s, const FFTPlan& plan, double* data, int batch, boo
     const int nx = plan.n[0], ny = plan.n[1], nz = plan.n[2];
     const int N = nx * ny * nz;
     const int strides[3] = {ny * nz, nz, 1};
+    auto log2i = [](int v) { int k = 0; while ((1 << k) < v) ++k; return k; };
     // z, y, x order (P1:456-485)
     for (int ax = 2; ax >= 0; --ax) {
         const int n = plan.n[ax];
         if (n <= 1) continue;
-        int logn = 0;
-        while ((1 << logn) < n) ++logn;
+        const int logn = log2i(n);
         const int lines = N / n;
         const int lpb = n >= 512 ? 1 : 512 / n;
         const int blocks = (lines + lpb - 1) / lpb;
         const size_t lds = (size_t)lpb * n * sizeof(double2);
-        hipLaunchKernelGGL(k_fft_axis, dim3(blocks, batch), dim3(256), lds, s, data, N, n, logn, strides[ax], lines, lpb,
-                           plan.tw[ax], inverse ? 1 : 0, current_gate());
+        const size_t lds_tw = lds + (size_t)(n / 2) * sizeof(double2);
+        if (lds_tw <= 64 * 1024)
+            hipLaunchKernelGGL(k_fft_axis<true>, dim3(blocks, batch), dim3(256), lds_tw, s, data, N, logn,
+                               log2i(strides[ax]), lines, log2i(lpb), plan.tw[ax], inverse ? 1 : 0, current_gate());
+        else
+            hipLaunchKernelGGL(k_fft_axis<false>, dim3(blocks, batch), dim3(256), lds, s, data, N, logn,
+                               log2i(strides[ax]), lines, log2i(lpb), plan.tw[ax], inverse ? 1 : 0, current_gate());
     }
 }
 
@@ -332,8 +346,11 @@ void launch_spectral_continuity(hipStream_t s, const SpecArgs& sp, double dt, co
 // each thread sums every kProjThreads-th element of its line and the line's
 // partial sums are added in a fixed order.  ROWS (stride 1): a line's
 // threads are adjacent and read adjacent elements; otherwise adjacent
-// threads take adjacent lines, which are adjacent in memory.
-constexpr int kProjLines = 16, kProjThreads = 16;
+// threads take adjacent lines, which are adjacent in memory.  Four lines of
+// 64 threads: a 256^2 field is 64 workgroups per component, each thread
+// four elements of its line (16 lines of 16 threads had made 16 workgroups
+// of sixteen serial loads per thread, 8.8 us per launch).
+constexpr int kProjLines = 4, kProjThreads = 64;
 template <bool ROWS>
 // src: the field read (the same as f for the in-place pass); each element
 // is read and written by the same thread.
