@@ -936,9 +936,12 @@ __device__ __forceinline__ void p3_map_mixed_body(const MGrid& g, const QuadF& q
 #pragma unroll
             for (int d = 0; d < 3; ++d) gin[d] += dph[d] + (GM == 2 ? lg[d] : 0.0f);
         }
-        // (the unfused sweeps never run a substep layout)
+        // (the unfused sweeps never run a substep layout; they all fold the
+        // end-point gather into the orbit's node loop: with the network in
+        // the same sweep that also keeps it at 238 VGPRs without scratch,
+        // against 256 and 48 bytes of scratch for the separate gather)
         const bool uniform = !FUSED || !a.layout;
-        map_particle<R, COLLECT, FUSED, !FUSED && GM != 2>(g, qd, split, a, p, gin, 0,
+        map_particle<R, COLLECT, FUSED, !FUSED>(g, qd, split, a, p, gin, 0,
                                                            a.retry_n != nullptr && uniform, feat, dph,
                                         (PRED || COLLECT || GM == 3) && uniform, a0s[0], a0s[1], a0s[2], T,
                                         rsum, rmax, nseg_total);
