@@ -373,8 +373,10 @@ upic_status upic_step_p2(upic_sim* sim, upic_step_info* info);
  * waited on once (info.graph_launch = 2; a member that needs more
  * evaluations than its branch holds continues on its own stream, one that
  * needs the selective refinement takes the host loop), and the others take
- * the host-driven step one after the other; otherwise min(n, threads) host
- * threads (threads <= 0: 16) step the members.
+ * the host-driven step; the members' host work before and after the graph
+ * (step start, predictor, completion, host-driven steps) runs on
+ * min(n, threads) host threads (threads <= 0: 16).  Otherwise that many host
+ * threads step the members.
  * infos (n entries) and status (n entries) may be NULL.  Returns UPIC_OK
  * when every member stepped, else the first member status that is not
  * UPIC_OK (each member's error text is on its context); UPIC_EINVAL when

@@ -169,6 +169,8 @@ struct upic_sim {
     struct OuterGraph { uint64_t key; hipGraph_t graph; hipGraphExec_t exec; };
     std::vector<OuterGraph> outer_graphs;
     int outer_slots = 0, outer_step_slots = 0;
+    // the last step needed more evaluations than its graph held
+    bool slots_short = false;
     bool capturing = false;  // a step is being captured (field solves are recorded inline)
     hipEvent_t ens_ev = nullptr;  // orders the member's stream and an ensemble launch
     uint64_t serial = 0;          // unique per simulation of the process (ensemble graph keys)
@@ -1471,12 +1473,19 @@ bool device_complete(upic_sim* s, upic_step_info* info) {
         s->vdep_used = false;
         return false;
     }
-    // slots of the next steps' graphs: grow to this step's evaluations and
-    // one spare at once, shrink by one when two or more went unused
+    // slots of the next steps' graphs: this step's evaluations and one
+    // spare, grown when two steps in a row needed more than the graph held
+    // (a single outlier's extra evaluations ran after the graph) and shrunk
+    // at once when two or more went unused.  A spare slot is not free: its
+    // gated launches run one after the other, about 1 ms per slot for a
+    // small ensemble member, and a slot count held high after an outlier had
+    // cost an ensemble of 64 members 20-40 ms per step.
     const int slots = s->outer_step_slots;
     int next = slots;
-    if (o.maps + 1 > slots) next = o.maps + 1;
-    else if (o.maps + 2 < slots) next = slots - 1;
+    const bool short_now = o.maps + 1 > slots;
+    if (short_now && s->slots_short) next = o.maps + 1;
+    else if (o.maps + 2 < slots) next = o.maps + 1;
+    s->slots_short = short_now;
     if (c->graph_slots == 0) s->outer_slots = std::min(kOuterSlotsMax, std::max(kOuterSlotsMin, next));
     s->step_maps = o.maps;
     s->vdep_used = fast && s->guess_mode >= 1 && s->vdep_off_steps == 0 && o.maps > 1;
@@ -2002,12 +2011,37 @@ EnsembleCache& ensemble_cache() {
 }
 constexpr size_t kEnsembleGraphCache = 8;
 
-void ensemble_step_graph(upic_sim* const* sims, int32_t n, upic_step_info* infos, upic_status* rc) {
+// Runs fn(i) for the members i in [0, n) on up to `threads` host threads
+// (each member has its own context and stream, so members do not share
+// host or device state outside the graph cache).
+template <class F>
+void for_members(int32_t n, int32_t threads, F&& fn) {
+    std::atomic<int32_t> next{0};
+    auto work = [&] {
+        for (;;) {
+            const int32_t i = next.fetch_add(1);
+            if (i >= n) break;
+            fn(i);
+        }
+    };
+    const int32_t nt = std::max<int32_t>(1, std::min<int32_t>(n, threads > 0 ? threads : 16));
+    std::vector<std::thread> pool;
+    for (int32_t t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+}
+
+void ensemble_step_graph(upic_sim* const* sims, int32_t n, int32_t threads, upic_step_info* infos,
+                         upic_status* rc) {
     std::vector<char> ok((size_t)n, 0), dev((size_t)n, 0), ext((size_t)n, 0);
     std::vector<uint64_t> keys((size_t)n, 0);
-    for (int32_t i = 0; i < n; ++i) {
+    // the members' step starts (sort, kept start state, spectra, tolerance)
+    // and predictors, enqueued from several host threads: from one, the
+    // host's launch rate had kept the GPU waiting between ensemble steps
+    for_members(n, threads, [&](int32_t i) {
         upic_sim* m = sims[i];
         rc[i] = (upic_status)guarded(m->c, [&] {
+            HIPCHK(hipSetDevice(m->c->device));
             ext[i] = p3_step_begin(m);
             ok[i] = 1;
             if (device_outer_ok(m)) {
@@ -2015,7 +2049,7 @@ void ensemble_step_graph(upic_sim* const* sims, int32_t n, upic_step_info* infos
                 dev[i] = 1;
             }
         });
-    }
+    });
     std::vector<int32_t> members;
     for (int32_t i = 0; i < n; ++i)
         if (ok[i] && dev[i]) members.push_back(i);
@@ -2094,11 +2128,12 @@ void ensemble_step_graph(upic_sim* const* sims, int32_t n, upic_step_info* infos
             }
         }
     }
-    for (int32_t i = 0; i < n; ++i) {
-        if (!ok[i]) continue;
+    for_members(n, threads, [&](int32_t i) {
+        if (!ok[i]) return;
         upic_sim* m = sims[i];
         upic_step_info* inf = infos ? infos + i : nullptr;
         rc[i] = (upic_status)guarded(m->c, [&] {
+            HIPCHK(hipSetDevice(m->c->device));
             if (dev[i] && device_complete(m, inf)) {
                 if (inf) inf->graph_launch = 2;
             } else {
@@ -2106,7 +2141,7 @@ void ensemble_step_graph(upic_sim* const* sims, int32_t n, upic_step_info* infos
             }
             p3_step_end(m, inf, ext[i] != 0);
         });
-    }
+    });
 }
 
 extern "C" {
@@ -2332,24 +2367,14 @@ upic_status upic_ensemble_step(upic_sim* const* sims, int32_t n, int32_t threads
     bool graph_path = n > 0;
     for (int32_t i = 0; i < n; ++i) graph_path = graph_path && sims[i]->c->p.code == 3 && sims[i]->c->graphs;
     if (graph_path) {
-        ensemble_step_graph(sims, n, infos, rc.data());
+        ensemble_step_graph(sims, n, threads, infos, rc.data());
     } else {
-        std::atomic<int32_t> next{0};
-        auto work = [&] {
-            for (;;) {
-                const int32_t i = next.fetch_add(1);
-                if (i >= n) break;
-                upic_sim* m = sims[i];
-                upic_step_info* inf = infos ? infos + i : nullptr;
-                const int code = m->c->p.code;
-                rc[(size_t)i] = code == 1 ? upic_step_p1(m, inf) : code == 2 ? upic_step_p2(m, inf) : upic_step_p3(m, inf);
-            }
-        };
-        const int32_t nt = std::max<int32_t>(1, std::min<int32_t>(n, threads > 0 ? threads : 16));
-        std::vector<std::thread> pool;
-        for (int32_t t = 1; t < nt; ++t) pool.emplace_back(work);
-        work();
-        for (auto& t : pool) t.join();
+        for_members(n, threads, [&](int32_t i) {
+            upic_sim* m = sims[i];
+            upic_step_info* inf = infos ? infos + i : nullptr;
+            const int code = m->c->p.code;
+            rc[(size_t)i] = code == 1 ? upic_step_p1(m, inf) : code == 2 ? upic_step_p2(m, inf) : upic_step_p3(m, inf);
+        });
     }
     upic_status first = UPIC_OK;
     for (int32_t i = 0; i < n; ++i) {
