@@ -315,12 +315,10 @@ void timed(upic_ctx* c, const char* name, F&& launch) {
 
 // real fields -> spectra; spectra -> real (destroys the spectra)
 inline void to_spectra(upic_ctx* c, const double* re, double* zh, int batch) {
-    launch_real_to_complex(c->stream, re, zh, (int64_t)batch * c->N());
-    launch_fft(c->stream, c->plan, zh, batch, false);
+    launch_fft(c->stream, c->plan, zh, batch, false, re, nullptr);
 }
 inline void from_spectra(upic_ctx* c, double* zh, double* re, int batch) {
-    launch_fft(c->stream, c->plan, zh, batch, true);
-    launch_complex_real(c->stream, zh, re, (int64_t)batch * c->N());
+    launch_fft(c->stream, c->plan, zh, batch, true, nullptr, re);
 }
 
 }  // namespace upic_host

@@ -15,11 +15,15 @@ namespace upic {
 // grid's sides are), so the index arithmetic is shifts and masks.  TWL: the
 // n/2 twiddles are staged in LDS with the lines; each stage then waits on
 // LDS only, not on a global load (the stages are a chain of barriers, and
-// the 256^2 transforms are latency-bound).
+// the 256^2 transforms are latency-bound).  re_in (first pass of a forward
+// transform): the lines are read from real fields, imaginary parts zero;
+// re_out (last pass of an inverse transform): only the real parts are
+// stored, into real fields; either saves the separate conversion pass.
 // ---------------------------------------------------------------------------
 template <bool TWL>
 __global__ __launch_bounds__(256) void k_fft_axis(double* __restrict__ data, int N, int logn, int lstride, int lines,
                                                   int llpb, const double* __restrict__ tw, int inverse,
+                                                  const double* __restrict__ re_in, double* __restrict__ re_out,
                                                   const int* __restrict__ gate) {
     if (UPIC_GATE_CLOSED(gate)) return;
     extern __shared__ double lds[];  // lpb * n complex (+ n/2 twiddles)
@@ -43,7 +47,8 @@ __global__ __launch_bounds__(256) void k_fft_axis(double* __restrict__ data, int
         if (l < lines) {
             const int base = ((l >> lstride) << (logn + lstride)) + (l & smask);
             const int r = (int)(__brev((unsigned)t) >> (32 - logn));
-            a[(lb << logn) + r] = base_ptr[base + (t << lstride)];
+            a[(lb << logn) + r] = re_in ? make_double2(re_in[(int64_t)field * N + base + (t << lstride)], 0.0)
+                                        : base_ptr[base + (t << lstride)];
         }
     }
     __syncthreads();
@@ -73,20 +78,34 @@ __global__ __launch_bounds__(256) void k_fft_axis(double* __restrict__ data, int
             const int base = ((l >> lstride) << (logn + lstride)) + (l & smask);
             double2 v = a[(lb << logn) + t];
             if (inverse) { v.x *= sc; v.y *= sc; }
-            base_ptr[base + (t << lstride)] = v;
+            if (re_out) re_out[(int64_t)field * N + base + (t << lstride)] = v.x;
+            else base_ptr[base + (t << lstride)] = v;
         }
     }
 }
 
-void launch_fft(hipStream_t s, const FFTPlan& plan, double* data, int batch, bool inverse) {
+void launch_fft(hipStream_t s, const FFTPlan& plan, double* data, int batch, bool inverse, const double* re_in,
+                double* re_out) {
     const int nx = plan.n[0], ny = plan.n[1], nz = plan.n[2];
     const int N = nx * ny * nz;
     const int strides[3] = {ny * nz, nz, 1};
     auto log2i = [](int v) { int k = 0; while ((1 << k) < v) ++k; return k; };
+    int last = -1;
+    for (int ax = 0; ax < 3; ++ax)
+        if (plan.n[ax] > 1) { last = ax; break; }
+    if (last < 0) {  // a single point: the transform is the identity
+        if (re_in) launch_real_to_complex(s, re_in, data, (int64_t)batch * N);
+        if (re_out) launch_complex_real(s, data, re_out, (int64_t)batch * N);
+        return;
+    }
+    bool first = true;
     // z, y, x order (P1:456-485)
     for (int ax = 2; ax >= 0; --ax) {
         const int n = plan.n[ax];
         if (n <= 1) continue;
+        const double* rin = first ? re_in : nullptr;
+        double* rout = ax == last ? re_out : nullptr;
+        first = false;
         const int logn = log2i(n);
         const int lines = N / n;
         const int lpb = n >= 512 ? 1 : 512 / n;
@@ -95,10 +114,12 @@ void launch_fft(hipStream_t s, const FFTPlan& plan, double* data, int batch, boo
         const size_t lds_tw = lds + (size_t)(n / 2) * sizeof(double2);
         if (lds_tw <= 64 * 1024)
             hipLaunchKernelGGL(k_fft_axis<true>, dim3(blocks, batch), dim3(256), lds_tw, s, data, N, logn,
-                               log2i(strides[ax]), lines, log2i(lpb), plan.tw[ax], inverse ? 1 : 0, current_gate());
+                               log2i(strides[ax]), lines, log2i(lpb), plan.tw[ax], inverse ? 1 : 0, rin, rout,
+                               current_gate());
         else
             hipLaunchKernelGGL(k_fft_axis<false>, dim3(blocks, batch), dim3(256), lds, s, data, N, logn,
-                               log2i(strides[ax]), lines, log2i(lpb), plan.tw[ax], inverse ? 1 : 0, current_gate());
+                               log2i(strides[ax]), lines, log2i(lpb), plan.tw[ax], inverse ? 1 : 0, rin, rout,
+                               current_gate());
     }
 }
 

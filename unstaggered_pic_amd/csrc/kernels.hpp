@@ -268,7 +268,11 @@ struct FFTPlan {
 };
 // In-place complex FFT over all three axes (z, y, x order as P1:456-485) of
 // `batch` consecutive fields of N complex values each.
-void launch_fft(hipStream_t s, const FFTPlan& plan, double* data, int batch, bool inverse);
+// re_in (forward): read the first pass from real fields; re_out (inverse):
+// store the last pass's real parts into real fields (data then holds an
+// intermediate pass).
+void launch_fft(hipStream_t s, const FFTPlan& plan, double* data, int batch, bool inverse,
+                const double* re_in = nullptr, double* re_out = nullptr);
 // re/im packing
 void launch_real_to_complex(hipStream_t s, const double* re, double* z, int64_t n);
 void launch_complex_real(hipStream_t s, const double* z, double* re, int64_t n);

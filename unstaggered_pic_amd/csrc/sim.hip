@@ -102,12 +102,10 @@ struct upic_sim {
     // set after a uniform-layout sweep of the current step: the next sweep's
     // inner solves start from vdep (guess modes 1 and 2)
     bool vdep_valid = false;
-    // node currents of the two previous steps (mixed precision): the outer
-    // solve starts from their extrapolation when valid
+    // node current of the previous step (mixed precision): the outer solve
+    // starts from the linear extrapolation 2 J^n - J^{n-1} when valid
     double* Jprev = nullptr;
-    double* Jprev2 = nullptr;
     bool jprev_valid = false;
-    bool jprev2_valid = false;
     // deferred second Picard iterations of the sweep (count, indices, guesses,
     // starting guesses)
     int* retry_n = nullptr;
@@ -208,8 +206,7 @@ void sim_free(upic_sim* s) {
     double** dbl[] = {&s->x, &s->v, &s->v_prev, &s->P, &s->vbar_last, &s->q, &s->m, &s->phi, &s->psi, &s->rho,
                       &s->A, &s->U, &s->J, &s->S0, &s->S1, &s->zs, &s->AG, &s->NR, &s->Jb[0], &s->Jb[1], &s->gtmp,
                       &s->vg[0], &s->vg[1], &s->xo, &s->vo, &s->Po, &s->vbo, &s->guess, &s->resid, &s->cand,
-                      &s->trial, &s->Jacc, &s->Jtry, &s->Jfield, &s->ptmp, &s->ptmp1, &s->fixed_ion, &s->Jprev,
-                      &s->Jprev2};
+                      &s->trial, &s->Jacc, &s->Jtry, &s->Jfield, &s->ptmp, &s->ptmp1, &s->fixed_ion, &s->Jprev};
     for (double** p : dbl) dfree(*p);
     for (int i = 0; i < 9; ++i) { dfree(s->xh[i]); dfree(s->rh[i]); }
     for (int i = 0; i < 8; ++i) { dfree(s->dX[i]); dfree(s->dR[i]); }
@@ -983,20 +980,18 @@ bool anderson_launch(upic_sim* s, History& H, int memory, double damping, double
 }
 
 // The outer solve's initial macro current: J^n (the reference's, P3:5930)
-// or, mixed precision with the uniform layout (fast), the extrapolation of
-// the last steps' converged currents: quadratic, 3 J^n - 3 J^{n-1} + J^{n-2},
-// once three are kept, linear before.  On the Weibel deck (256^2, 256 per
-// cell, dt = 0.05) the quadratic start leaves the outer solve at 4 map
-// evaluations per step, with the converged residual 2.1e-8 instead of
-// 5.3e-8 of the current's norm and fewer particles moving between the later
-// sweeps (11.5 -> 11.1 ms per step with the physics predictor).
+// or, mixed precision with the uniform layout (fast), the linear
+// extrapolation 2 J^n - J^{n-1} of the last two converged currents.  The
+// quadratic one, 3 J^n - 3 J^{n-1} + J^{n-2}, cut the initial residual on
+// the Weibel deck (256^2, 256 per cell) from 9e-3 to 2e-3 of the current's
+// norm and the step by 3%, but left cold Landau runs (v_th 0.02-0.04 c,
+// quadratic splines, dt 0.1) with total-energy drifts of 3-5e-6 over 500
+// steps against 1-5e-7: the outer solve stops at the same tolerance, with
+// an error that followed the extrapolation's.
 void launch_current_predictor(upic_sim* s, bool fast) {
     upic_ctx* c = s->c;
     const int64_t n3 = 3ll * c->N();
-    if (fast && s->jprev2_valid) {
-        LinComb le{3, {3.0, -3.0, 1.0}, {s->J, s->Jprev, s->Jprev2}};
-        launch_lincomb(c->stream, n3, le, s->guess);
-    } else if (fast && s->jprev_valid) {
+    if (fast && s->jprev_valid) {
         LinComb le{2, {2.0, -1.0}, {s->J, s->Jprev}};
         launch_lincomb(c->stream, n3, le, s->guess);
     } else {
@@ -1004,14 +999,11 @@ void launch_current_predictor(upic_sim* s, bool fast) {
     }
 }
 
-// The committed step's current joins the history (before the commit
-// overwrites J).
+// The committed step's current becomes J^{n-1} of the next (before the
+// commit overwrites J).
 void shift_current_history(upic_sim* s) {
     upic_ctx* c = s->c;
-    const int64_t n3 = 3ll * c->N();
-    if (s->jprev_valid) launch_copy(c->stream, n3, s->Jprev, s->Jprev2);
-    launch_copy(c->stream, n3, s->J, s->Jprev);
-    s->jprev2_valid = s->jprev_valid;
+    launch_copy(c->stream, 3ll * c->N(), s->J, s->Jprev);
     s->jprev_valid = true;
 }
 
@@ -1329,10 +1321,7 @@ void enqueue_outer(upic_sim* s, int slots, bool with_init) {
     }
     launch_outer_commit_gate(c->stream, s->outer, s->commit_gate);
     const GateScope gs(s->commit_gate);
-    if (s->mixed) {
-        launch_copy(c->stream, 3ll * N, s->Jprev, s->Jprev2);
-        launch_copy(c->stream, 3ll * N, s->J, s->Jprev);
-    }
+    if (s->mixed) launch_copy(c->stream, 3ll * N, s->J, s->Jprev);
     commit_fields(s, s->Jtry);
     HIPCHK(hipGetLastError());
 }
@@ -1542,7 +1531,6 @@ bool device_complete(upic_sim* s, upic_step_info* info) {
         std::swap(s->ms.vb, s->mo.vb);
         s->hist_perm_pending = false;
         s->fp64_stale = true;
-        s->jprev2_valid = s->jprev_valid;  // the graph's commit shifted the history
         s->jprev_valid = true;
     } else {
         std::swap(s->v_prev, s->v);
@@ -2227,7 +2215,6 @@ upic_status upic_sim_create(upic_ctx* c, int64_t np, upic_sim** out) {
             s->ftmp2 = dalloc<float>(np);
             s->vdep = dalloc<float>(n3);
             s->Jprev = dalloc<double>(3 * (size_t)N);
-            s->Jprev2 = dalloc<double>(3 * (size_t)N);
             s->retry_n = dalloc<int>(1);
             s->retry_idx = dalloc<int>(np);
             s->retry_gs = dalloc<float>(n3);
@@ -2264,7 +2251,7 @@ upic_status upic_sim_upload(upic_sim* s, const upic_host_state* st) {
     if (!s || !st || st->np != s->np) return UPIC_EINVAL;
     upic_ctx* c = s->c;
     return (upic_status)guarded(c, [&] {
-        s->jprev_valid = s->jprev2_valid = false;
+        s->jprev_valid = false;
         s->fix_scale = 0.0;  // charges may change: the deterministic deposit rescales
         fp64_view(s);  // arrays the caller leaves out keep their current values
         // the copies below run on the null stream, which does not wait for
@@ -2387,7 +2374,7 @@ upic_status upic_ensemble_step(upic_sim* const* sims, int32_t n, int32_t threads
 static upic_status init_fields(upic_sim* s, double n0, const double* A_host, const double* U_host) {
     upic_ctx* c = s->c;
     return (upic_status)guarded(c, [&] {
-        s->jprev_valid = s->jprev2_valid = false;
+        s->jprev_valid = false;
         const upic_params& p = c->p;
         const int N = c->N();
         const int64_t np = s->np;
@@ -2689,7 +2676,7 @@ upic_status upic_sim_substep_paths(upic_sim* s, int64_t* n_total, int64_t* offse
 
 upic_status upic_sim_device_state(upic_sim* s, upic_host_state* d) {
     if (!s || !d) return UPIC_EINVAL;
-    s->jprev_valid = s->jprev2_valid = false;  // the caller may write the node current
+    s->jprev_valid = false;  // the caller may write the node current
     const int rc = guarded(s->c, [&] { fp64_view(s); });
     if (rc != UPIC_OK) return (upic_status)rc;
     std::memset(d, 0, sizeof(*d));
