@@ -142,6 +142,13 @@ struct AxisSt {
     double dw[R + 1];  // d/dx (already divided by h)
 };
 
+//
+// Quadratic and cubic evaluate, for each node, the one piece its distance
+// r falls in (node a's |r| lies in a fixed piece for every particle) with
+// the same operations bspline_wd applies to that piece, so the weights are
+// the same bits without its per-node compare-and-select of every piece
+// (the one exception: a particle exactly on a knot, where one cubic weight
+// comes from the adjacent piece, equal to within one rounding).
 template <int R>
 __device__ __forceinline__ void axis_stencil(double x, int n, double L, double h, AxisSt<R>& s) {
     const double u = (x + 0.5 * L) / h;
@@ -149,13 +156,39 @@ __device__ __forceinline__ void axis_stencil(double x, int n, double L, double h
     int b = i_left % n;
     if (b < 0) b += n;
     s.base = b;
+    if constexpr (R == 3) {
+        // r = 1+f, f, f-1, f-2 with f in [0, 1): pieces 2, 1, 1, 2
+        const double a0 = u - (double)i_left, a1 = u - (double)(i_left + 1);
+        const double a2 = fabs(u - (double)(i_left + 2)), a3 = fabs(u - (double)(i_left + 3));
+        const double z0 = 2.0 - a0, z3 = 2.0 - a3;
+        s.w[0] = z0 * z0 * z0 / 6.0;
+        s.dw[0] = -0.5 * z0 * z0 / h;
+        s.w[1] = (2.0 / 3.0) - a1 * a1 + 0.5 * a1 * a1 * a1;
+        s.dw[1] = (-2.0 * a1 + 1.5 * a1 * a1) / h;
+        s.w[2] = (2.0 / 3.0) - a2 * a2 + 0.5 * a2 * a2 * a2;
+        s.dw[2] = -(-2.0 * a2 + 1.5 * a2 * a2) / h;
+        s.w[3] = z3 * z3 * z3 / 6.0;
+        s.dw[3] = 0.5 * z3 * z3 / h;
+    } else if constexpr (R == 2) {
+        // r = 1/2+f, f-1/2, f-3/2 with f in [0, 1): pieces 2, 1, 2
+        const double r0 = u - (double)i_left, r1 = u - (double)(i_left + 1);
+        const double a2 = fabs(u - (double)(i_left + 2));
+        const double z0 = 1.5 - r0, z2 = 1.5 - a2;
+        s.w[0] = 0.5 * z0 * z0;
+        s.dw[0] = -z0 / h;
+        s.w[1] = 0.75 - fabs(r1) * fabs(r1);
+        s.dw[1] = -2.0 * r1 / h;
+        s.w[2] = 0.5 * z2 * z2;
+        s.dw[2] = z2 / h;
+    } else {
 #pragma unroll
-    for (int a = 0; a <= R; ++a) {
-        const double r = u - (double)(i_left + a);
-        double w, dw;
-        bspline_wd<R>(r, w, dw);
-        s.w[a] = w;
-        s.dw[a] = dw / h;
+        for (int a = 0; a <= R; ++a) {
+            const double r = u - (double)(i_left + a);
+            double w, dw;
+            bspline_wd<R>(r, w, dw);
+            s.w[a] = w;
+            s.dw[a] = dw / h;
+        }
     }
 }
 
